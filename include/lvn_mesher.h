@@ -1,0 +1,171 @@
+/*
+ * lvn_mesher.h — C ABI of the MI355X (gfx950) dual-contouring chunk mesher.
+ *
+ * Drop-in replacement for leven's OpenCL compute module.  Every entry point
+ * names the reference interface it replaces (paths relative to the leven
+ * source tree, leven/src/...).  Plain pointers and sizes only; no HIP or torch
+ * types cross this boundary.  All functions return LVN_OK (0) or a negative
+ * LVN_ERR_* code, like the reference's CL_SUCCESS / negative CL error
+ * convention (src/compute.h:12, src/compute_local.h:194-204).
+ *
+ * Threading: one lvn_ctx per host thread.  The global state set by lvn_init /
+ * lvn_set_noise_seed / lvn_store_csg_op is shared by all contexts and must not
+ * be changed while another thread uses a context (the reference is likewise
+ * single-threaded: src/volume.cpp:68-113).
+ */
+#ifndef LVN_MESHER_H
+#define LVN_MESHER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes -------------------------------------------------------- */
+#define LVN_OK              0
+#define LVN_ERR_INVALID    (-1)   /* bad argument (null pointer, unsupported size, material > 255) */
+#define LVN_ERR_NOMEM      (-2)   /* device allocation failed */
+#define LVN_ERR_CAPACITY   (-3)   /* caller buffer too small; the needed counts are still written */
+#define LVN_ERR_DEVICE     (-4)   /* HIP runtime error */
+#define LVN_ERR_STATE      (-5)   /* lvn_init not called / library shut down */
+#define LVN_ERR_OVERFLOW   (-6)   /* a chunk exceeded a per-chunk device capacity */
+
+/* ---- materials (src/volume_materials.h:8-9) ------------------------------ */
+#define LVN_MATERIAL_NONE  200
+#define LVN_MATERIAL_AIR   201
+
+/* ---- POD types, byte-compatible with the reference ------------------------ */
+
+/* = CSGOperationInfo (src/compute.h:16-24) = CSGOperation (cl/apply_csg_operation.cl:5-14); 48 bytes.
+ * Units as the reference caller produces them (src/clipmap.cpp:1628-1632): voxel units. */
+typedef struct lvn_csg_op {
+    int32_t type;        /* 0 = add, 1 = subtract */
+    int32_t brush;       /* 0 = cube (RenderShape_Cube), 1 = sphere (RenderShape_Sphere) */
+    int32_t material;    /* 0..255; subtract ops conventionally carry LVN_MATERIAL_AIR */
+    float   rotate_y;    /* radians, applied to the cube's xz plane (pR, cl/apply_csg_operation.cl:57-60) */
+    float   origin[4];   /* xyz used */
+    float   dims[4];     /* cube: half extents xyz; sphere: radius in dims[0] */
+} lvn_csg_op;
+
+/* = MeshVertex (src/render_types.h:24-38); 48 bytes. */
+typedef struct lvn_vertex {
+    float pos[4];        /* world units, w = 1 */
+    float normal[4];     /* mean of edge normals (not renormalised), w = 0 */
+    float colour[4];     /* LOD colour rgb, a = dominant material */
+} lvn_vertex;
+
+/* = MeshTriangle (src/render_types.h:42-58); 12 bytes. */
+typedef struct lvn_triangle {
+    int32_t idx[3];
+} lvn_triangle;
+
+/* = SeamNodeInfo (src/compute.h:26-31); 48 bytes. */
+typedef struct lvn_seam_node {
+    int32_t local_min_and_material[4];  /* voxel xyz in the chunk, w = (dominant << 8) | corner mask */
+    float   pos[4];
+    float   normal[4];
+} lvn_seam_node;
+
+/* Density function selector.  The reference compiles its density function in
+ * (DensityFunc, cl/noise.cl:225-268); here it is a runtime descriptor. */
+#define LVN_FIELD_HEIGHTFIELD     0   /* d = y - amp * fbm2(x, z) */
+#define LVN_FIELD_TERRAIN_CAVES   1   /* d = max(y - amp * fbm2(x, z), fbm3(x, y, z) - threshold) */
+#define LVN_FIELD_LEVEN_TERRAIN   2   /* leven's own: d = y - 900 * Terrain(x, z) (cl/noise.cl:205-223) */
+
+typedef struct lvn_field_desc {
+    int32_t kind;
+    int32_t height_octaves;     /* BasicFractal (cl/noise.cl:8-36) over snoise2 */
+    float   height_frequency;
+    float   height_lacunarity;
+    float   height_gain;
+    float   height_amplitude;   /* voxels */
+    int32_t cave_octaves;       /* the same fBm over snoise3 (cl/simplex.cl:159-230) */
+    float   cave_frequency;
+    float   cave_lacunarity;
+    float   cave_gain;
+    float   cave_threshold;
+    int32_t reserved;
+} lvn_field_desc;
+
+/* Context flags. */
+#define LVN_CTX_FP16_DENSITY  1u   /* store the density field as sign-preserving FP16 */
+#define LVN_CTX_PRECISE_QEF   2u   /* compensated FP32 QEF sums + FP64 re-solve of ill-conditioned QEFs */
+
+typedef struct lvn_ctx lvn_ctx;
+
+/* ---- global state (src/compute.h:35-40) ---------------------------------- */
+
+/* Replaces Compute_Initialise(noiseSeed, defaultMaterial, numCSGBrushes), src/compute.cpp:195-234.
+ * field == NULL selects the 64^3 heightfield of the plain CPU workload
+ * (6 octaves, f = 1/128, lacunarity 2, gain 0.5, amplitude 24). */
+int lvn_init(const lvn_field_desc* field, int32_t noise_seed, uint32_t default_material);
+/* Replaces Compute_Shutdown, src/compute.cpp:238-241. */
+int lvn_shutdown(void);
+/* Replaces Compute_SetNoiseSeed, src/compute_density_field.cpp:129-134. */
+int lvn_set_noise_seed(int32_t noise_seed);
+/* Replaces Compute_StoreCSGOperation(op, AABB), src/compute_density_field.cpp:312-317.
+ * aabb_min/aabb_max are world units (AABB, src/aabb.h). */
+int lvn_store_csg_op(const lvn_csg_op* op, const int32_t aabb_min[3], const int32_t aabb_max[3]);
+/* Replaces Compute_ClearCSGOperations, src/compute_density_field.cpp:321-326. */
+int lvn_clear_csg_ops(void);
+/* Copies the 256x256 permutation table in use (row = texel y) into out[65536]. */
+int lvn_get_perm_table(uint8_t* out);
+
+/* ---- mesh generation context (src/compute.h:46-77) ------------------------ */
+
+/* Replaces Compute_MeshGenContext::create(voxelsPerChunk), src/compute.cpp:605-610.
+ * voxels_per_chunk: power of two in [4, 64]. */
+int  lvn_ctx_create(int32_t voxels_per_chunk, uint32_t flags, lvn_ctx** out);
+void lvn_ctx_destroy(lvn_ctx* ctx);
+/* Replaces Compute_MeshGenContext::voxelsPerChunk, src/compute.cpp:612-615. */
+int  lvn_ctx_voxels_per_chunk(const lvn_ctx* ctx);
+
+/* Replaces Compute_MeshGenContext::applyCSGOperations, src/compute.cpp:617-623
+ * (-> Compute_ApplyCSGOperations, src/compute_csg.cpp:224-242).  min/size in world units. */
+int lvn_apply_csg(lvn_ctx* ctx, const lvn_csg_op* ops, int32_t n, const int32_t min[3], int32_t size);
+/* Replaces Compute_MeshGenContext::freeChunkOctree, src/compute_octree.cpp:379-387. */
+int lvn_free_chunk(lvn_ctx* ctx, const int32_t min[3], int32_t size);
+/* Replaces Compute_MeshGenContext::isChunkEmpty, src/compute_density_field.cpp:278-299.
+ * *out = 1 when the chunk's field has at least one sign-changing edge.  (The reference
+ * returns that same value under the name isEmpty; the C++ wrapper keeps its name.) */
+int lvn_chunk_has_surface(lvn_ctx* ctx, const int32_t min[3], int32_t size, int32_t* out);
+/* Replaces Compute_MeshGenContext::generateChunkMesh, src/compute_octree.cpp:351-375.
+ * Writes up to the caps; returns LVN_ERR_CAPACITY (with the needed counts in
+ * *nv / *nt / *ns) when any output does not fit.  tri holds 3 indices per triangle. */
+int lvn_generate_chunk_mesh(lvn_ctx* ctx, const int32_t min[3], int32_t size,
+                            lvn_vertex* v, int32_t v_cap, int32_t* nv,
+                            int32_t* tri, int32_t tri_cap, int32_t* nt,
+                            lvn_seam_node* s, int32_t s_cap, int32_t* ns);
+
+/* ---- test hook: intermediate stages of one chunk --------------------------- */
+
+/* Buffers for lvn_debug_chunk_stages.  F = voxels_per_chunk + 2, H = voxels_per_chunk + 1.
+ * Any pointer may be NULL. Counts are always written. */
+typedef struct lvn_stage_dump {
+    float*   density;            /* F^3, x fastest (FP16 contexts: widened) */
+    uint8_t* materials;          /* F^3 */
+    int32_t* edge_ids;           /* edge_cap: ((x | y<<S | z<<2S) << 2) | axis, ascending */
+    float*   edge_data;          /* 4 per edge: normal.xyz, t */
+    int32_t  edge_cap;
+    int32_t  num_edges;
+    int32_t* voxel_codes;        /* voxel_cap each, ascending x + N*y + N*N*z order */
+    int32_t* voxel_edge_masks;
+    int32_t* voxel_materials;    /* (dominant << 8) | corner mask */
+    float*   voxel_positions;    /* 4 per voxel: solved vertex, world units, w = 1 */
+    float*   voxel_normals;      /* 4 per voxel */
+    int32_t  voxel_cap;
+    int32_t  num_voxels;
+} lvn_stage_dump;
+
+/* Runs the plain pipeline for one fresh chunk (no caches) with the given CSG
+ * batch and copies out each stage.  Not a reference interface: exists so that
+ * parity tests can compare every stage. */
+int lvn_debug_chunk_stages(lvn_ctx* ctx, const int32_t min[3], int32_t size,
+                           const lvn_csg_op* ops, int32_t nops, lvn_stage_dump* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LVN_MESHER_H */

@@ -1,0 +1,140 @@
+"""ctypes mirrors of the POD types in include/lvn_mesher.h.
+
+Byte layouts equal the reference's structs: CSGOperationInfo (src/compute.h:16-24),
+MeshVertex (src/render_types.h:24-38), SeamNodeInfo (src/compute.h:26-31).
+Importing this module loads nothing.
+"""
+import ctypes as C
+
+import numpy as np
+
+LVN_OK = 0
+LVN_ERR_INVALID = -1
+LVN_ERR_NOMEM = -2
+LVN_ERR_CAPACITY = -3
+LVN_ERR_DEVICE = -4
+LVN_ERR_STATE = -5
+LVN_ERR_OVERFLOW = -6
+
+ERROR_NAMES = {
+    LVN_OK: "LVN_OK",
+    LVN_ERR_INVALID: "LVN_ERR_INVALID",
+    LVN_ERR_NOMEM: "LVN_ERR_NOMEM",
+    LVN_ERR_CAPACITY: "LVN_ERR_CAPACITY",
+    LVN_ERR_DEVICE: "LVN_ERR_DEVICE",
+    LVN_ERR_STATE: "LVN_ERR_STATE",
+    LVN_ERR_OVERFLOW: "LVN_ERR_OVERFLOW",
+}
+
+MATERIAL_NONE = 200
+MATERIAL_AIR = 201
+
+FIELD_HEIGHTFIELD = 0
+FIELD_TERRAIN_CAVES = 1
+FIELD_LEVEN_TERRAIN = 2
+
+CTX_FP16_DENSITY = 1
+CTX_PRECISE_QEF = 2
+
+
+class CsgOp(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32),
+        ("brush", C.c_int32),
+        ("material", C.c_int32),
+        ("rotate_y", C.c_float),
+        ("origin", C.c_float * 4),
+        ("dims", C.c_float * 4),
+    ]
+
+
+class Vertex(C.Structure):
+    _fields_ = [("pos", C.c_float * 4), ("normal", C.c_float * 4), ("colour", C.c_float * 4)]
+
+
+class SeamNode(C.Structure):
+    _fields_ = [
+        ("local_min_and_material", C.c_int32 * 4),
+        ("pos", C.c_float * 4),
+        ("normal", C.c_float * 4),
+    ]
+
+
+class FieldDesc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("height_octaves", C.c_int32),
+        ("height_frequency", C.c_float),
+        ("height_lacunarity", C.c_float),
+        ("height_gain", C.c_float),
+        ("height_amplitude", C.c_float),
+        ("cave_octaves", C.c_int32),
+        ("cave_frequency", C.c_float),
+        ("cave_lacunarity", C.c_float),
+        ("cave_gain", C.c_float),
+        ("cave_threshold", C.c_float),
+        ("reserved", C.c_int32),
+    ]
+
+
+class StageDump(C.Structure):
+    _fields_ = [
+        ("density", C.POINTER(C.c_float)),
+        ("materials", C.POINTER(C.c_uint8)),
+        ("edge_ids", C.POINTER(C.c_int32)),
+        ("edge_data", C.POINTER(C.c_float)),
+        ("edge_cap", C.c_int32),
+        ("num_edges", C.c_int32),
+        ("voxel_codes", C.POINTER(C.c_int32)),
+        ("voxel_edge_masks", C.POINTER(C.c_int32)),
+        ("voxel_materials", C.POINTER(C.c_int32)),
+        ("voxel_positions", C.POINTER(C.c_float)),
+        ("voxel_normals", C.POINTER(C.c_float)),
+        ("voxel_cap", C.c_int32),
+        ("num_voxels", C.c_int32),
+    ]
+
+
+assert C.sizeof(CsgOp) == 48
+assert C.sizeof(Vertex) == 48
+assert C.sizeof(SeamNode) == 48
+assert C.sizeof(FieldDesc) == 48
+
+# numpy views of the same layouts
+CSG_OP_DTYPE = np.dtype(
+    [("type", "<i4"), ("brush", "<i4"), ("material", "<i4"), ("rotate_y", "<f4"),
+     ("origin", "<f4", 4), ("dims", "<f4", 4)]
+)
+VERTEX_DTYPE = np.dtype([("pos", "<f4", 4), ("normal", "<f4", 4), ("colour", "<f4", 4)])
+SEAM_DTYPE = np.dtype([("local_min_and_material", "<i4", 4), ("pos", "<f4", 4), ("normal", "<f4", 4)])
+assert CSG_OP_DTYPE.itemsize == 48 and VERTEX_DTYPE.itemsize == 48 and SEAM_DTYPE.itemsize == 48
+
+
+def field_desc(kind=FIELD_HEIGHTFIELD, height_octaves=6, height_frequency=1.0 / 128.0,
+               height_lacunarity=2.0, height_gain=0.5, height_amplitude=24.0,
+               cave_octaves=5, cave_frequency=1.0 / 64.0, cave_lacunarity=2.0, cave_gain=0.5,
+               cave_threshold=0.3):
+    return FieldDesc(kind, height_octaves, height_frequency, height_lacunarity, height_gain,
+                     height_amplitude, cave_octaves, cave_frequency, cave_lacunarity, cave_gain,
+                     cave_threshold, 0)
+
+
+def ops_array(ops):
+    """numpy structured array (CSG_OP_DTYPE) -> contiguous ctypes pointer-compatible buffer."""
+    arr = np.ascontiguousarray(ops, dtype=CSG_OP_DTYPE)
+    return arr
+
+
+def ptr(arr, ctype):
+    """ctypes pointer to a contiguous numpy array (or None)."""
+    if arr is None:
+        return None
+    assert arr.flags["C_CONTIGUOUS"]
+    return arr.ctypes.data_as(C.POINTER(ctype))
+
+
+def vptr(arr):
+    if arr is None:
+        return None
+    assert arr.flags["C_CONTIGUOUS"]
+    return C.c_void_p(arr.ctypes.data)

@@ -1,0 +1,49 @@
+// lvn_kernels.h — launch interface of the gfx950 kernels (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lvn_shared.h"
+
+namespace lvn {
+
+// Arguments common to every stage kernel (passed by value).
+struct KArgs {
+    char* arena;             // chunk slot 0; slot i starts at arena + i * L.slot_bytes
+    SlotLayout L;
+    Geometry G;
+    Caps caps;
+    const ChunkDesc* chunks;
+    Counters* counters;
+    const DevOp* ops;
+    const int* op_index;
+    FieldParams F;
+    const uint8_t* perm;     // 65,536 bytes
+    const float4* grads;     // 16 decoded gradients
+    int defmat;
+    int fp16;                // density stored as sign-preserving FP16
+    int ysplit;              // density kernel: y segments per column
+    int pad;
+};
+
+enum ScanKind { kScanEdges = 0, kScanVoxels = 1, kScanQuads = 2, kScanSeams = 3 };
+
+// Stage launchers: `chunks` chunk slots [0, chunks) processed by one launch each.
+void launch_density(const KArgs& a, int chunks, hipStream_t s);
+void launch_csg(const KArgs& a, int chunks, hipStream_t s);
+void launch_edge_flags(const KArgs& a, int chunks, hipStream_t s);
+void launch_scan(const KArgs& a, int chunks, ScanKind kind, hipStream_t s);
+void launch_edge_compact(const KArgs& a, int chunks, hipStream_t s);
+void launch_edge_cross(const KArgs& a, int chunks, hipStream_t s);
+void launch_voxel_flags(const KArgs& a, int chunks, hipStream_t s);
+void launch_voxel_compact(const KArgs& a, int chunks, hipStream_t s);
+void launch_qef(const KArgs& a, int chunks, hipStream_t s);
+void launch_quad_flags(const KArgs& a, int chunks, hipStream_t s);
+void launch_quad_emit(const KArgs& a, int chunks, hipStream_t s);
+void launch_vertex_seam(const KArgs& a, int chunks, hipStream_t s);
+void launch_seam_emit(const KArgs& a, int chunks, hipStream_t s);
+
+// Whole pipeline for chunk slots [0, chunks).
+void launch_pipeline(const KArgs& a, int chunks, hipStream_t s);
+
+}  // namespace lvn

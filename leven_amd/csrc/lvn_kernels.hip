@@ -1,0 +1,840 @@
+// lvn_kernels.hip — the chunk-mesher stages as gfx950 kernels.
+//
+// Every kernel takes the chunk slot from blockIdx.y, so one launch serves one
+// chunk (plain form) or a whole batch (batched form).  Compaction is
+// deterministic: each stage writes a 64-bit ballot mask and a count per
+// 64-element segment, one workgroup per chunk scans the counts, and the
+// compacting kernel places element i at base[seg] + popcount(mask below i).
+// The element order is therefore the reference's linear order everywhere
+// (cl/density_field.cl:52-53, cl/octree.cl:153, cl/octree.cl:398-439).
+
+#include "lvn_device.hpp"
+#include "lvn_kernels.h"
+
+#include <hip/hip_fp16.h>
+
+namespace lvn {
+
+namespace {
+
+template <typename T>
+__device__ __forceinline__ T* slot_ptr(const KArgs& a, int chunk, uint64_t off) {
+    return reinterpret_cast<T*>(a.arena + (uint64_t)chunk * a.L.slot_bytes + off);
+}
+
+__device__ __forceinline__ int field_index(const Geometry& G, int x, int y, int z) {
+    return x + G.F * (y + G.F * z);
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
+
+// Dense lookup of a compacted element through its segment mask: index or -1.
+__device__ __forceinline__ int dense_index(const uint64_t* mask, const int* base, int i) {
+    const uint64_t m = mask[i >> 6];
+    const int lane = i & 63;
+    if (!((m >> lane) & 1ull)) return -1;
+    return base[i >> 6] + __popcll(m & lanemask_lt(lane));
+}
+
+// FP16 store that keeps the sign of every negative sample (an underflow to -0
+// would turn a solid sample into air under the reference's `density < 0` test,
+// cl/density_field.cl:34).
+__device__ __forceinline__ __half to_half_keep_sign(float d) {
+    __half h = __float2half_rn(d);
+    if (d < 0.f && !(__half2float(h) < 0.f)) h = __ushort_as_half((unsigned short)0x8001u);
+    return h;
+}
+
+// cl/shared_constants.cl:4-9 / cl/octree.cl:227-232
+__constant__ int kEdgeMap[12][2] = {{0, 4}, {1, 5}, {2, 6}, {3, 7}, {0, 2}, {1, 3},
+                                    {4, 6}, {5, 7}, {0, 1}, {2, 3}, {4, 5}, {6, 7}};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Density + default material (GenerateDefaultField, cl/density_field.cl:11-37)
+// ---------------------------------------------------------------------------
+// One thread per (x, z) column segment: the height term depends on x and z
+// only, so it is evaluated once and reused down the column.
+__global__ __launch_bounds__(512) void k_density(KArgs a) {
+    __shared__ NoiseLds T;
+    load_noise_tables(T, a.perm, a.grads);
+    const int chunk = blockIdx.y;
+    const Geometry& G = a.G;
+    const int F = G.F;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int seg_len = (F + a.ysplit - 1) / a.ysplit;
+    const int col = t % (F * F);
+    const int seg = t / (F * F);
+    if (seg >= a.ysplit) return;
+    const ChunkDesc cd = a.chunks[chunk];
+    const int x = col % F, z = col / F;
+    const float wx = (float)(x * cd.scale + cd.off[0]);
+    const float wz = (float)(z * cd.scale + cd.off[2]);
+    const float H = height_term(a.F, T, wx, wz);
+    float* dens = slot_ptr<float>(a, chunk, a.L.density);
+    __half* dens16 = slot_ptr<__half>(a, chunk, a.L.density);
+    uint8_t* mat = slot_ptr<uint8_t>(a, chunk, a.L.mat);
+    uint8_t* orig = slot_ptr<uint8_t>(a, chunk, a.L.orig);
+    const int y0 = seg * seg_len;
+    const int y1 = min(F, y0 + seg_len);
+    for (int y = y0; y < y1; ++y) {
+        const float wy = (float)(y * cd.scale + cd.off[1]);
+        const float d = combine(a.F, T, wy - H, wx, wy, wz);
+        const int idx = field_index(G, x, y, z);
+        if (a.fp16)
+            dens16[idx] = to_half_keep_sign(d);
+        else
+            dens[idx] = d;
+        const uint8_t m = d < 0.f ? (uint8_t)a.defmat : (uint8_t)kAir;
+        mat[idx] = m;
+        orig[idx] = m;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CSG material override (CSG_HermiteIndices + BrushMaterial,
+// cl/apply_csg_operation.cl:114-140,179-210): ops in order, the last op whose
+// brush contains a sample sets its material.  One workgroup per chunk walks
+// the ops in order over each op's bounding box of samples.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_csg(KArgs a) {
+    const int chunk = blockIdx.y;
+    const ChunkDesc cd = a.chunks[chunk];
+    if (cd.op_count == 0) return;
+    const int F = a.G.F;
+    uint8_t* mat = slot_ptr<uint8_t>(a, chunk, a.L.mat);
+    for (int k = 0; k < cd.op_count; ++k) {
+        const DevOp op = a.ops[a.op_index[cd.op_start + k]];
+        const float r = op.radius + 1.f;
+        const float inv = 1.f / (float)cd.scale;
+        int lo[3], hi[3];
+        const float o[3] = {op.ox, op.oy, op.oz};
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = max(0, (int)floorf((o[c] - r - (float)cd.off[c]) * inv));
+            hi[c] = min(F - 1, (int)ceilf((o[c] + r - (float)cd.off[c]) * inv));
+        }
+        const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+        if (nx > 0 && ny > 0 && nz > 0) {
+            const int total = nx * ny * nz;
+            const uint8_t newm = op.type == 0 ? (uint8_t)op.material : (uint8_t)kAir;
+            for (int i = threadIdx.x; i < total; i += blockDim.x) {
+                const int x = lo[0] + i % nx;
+                const int y = lo[1] + (i / nx) % ny;
+                const int z = lo[2] + i / (nx * ny);
+                const float px = (float)(x * cd.scale + cd.off[0]);
+                const float py = (float)(y * cd.scale + cd.off[1]);
+                const float pz = (float)(z * cd.scale + cd.off[2]);
+                if (brush_density(op, px, py, pz) <= 0.f) mat[field_index(a.G, x, y, z)] = newm;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Edge classification (FindFieldEdges, cl/density_field.cl:41-76): slot
+// s = 3 * (x + H y + H^2 z) + axis, sign change between the slot's sample and
+// its +axis neighbour.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_edge_flags(KArgs a) {
+    const int chunk = blockIdx.y;
+    const Geometry& G = a.G;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint8_t* mat = slot_ptr<uint8_t>(a, chunk, a.L.mat);
+    int flag = 0;
+    if (s < G.edge_slots) {
+        const int h = s / 3, axis = s - 3 * h;
+        const int x = h % G.H, r = h / G.H, y = r % G.H, z = r / G.H;
+        const int i0 = field_index(G, x, y, z);
+        const int i1 = i0 + (axis == 0 ? 1 : axis == 1 ? G.F : G.F * G.F);
+        flag = (mat[i0] != kAir) != (mat[i1] != kAir);
+    }
+    const uint64_t m = __ballot(flag);
+    const int seg = s >> 6;
+    if ((threadIdx.x & 63) == 0 && seg < G.edge_segs) {
+        slot_ptr<uint64_t>(a, chunk, a.L.e_mask)[seg] = m;
+        slot_ptr<int>(a, chunk, a.L.e_count)[seg] = __popcll(m);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of segment counts, one workgroup per chunk (replaces the
+// recursive Scan / ExclusiveScan with its blocking readback, src/compute.cpp:344-396).
+// ---------------------------------------------------------------------------
+// The stage is a template parameter: with a runtime selector hipcc (ROCm 7.2,
+// gfx950) kept the voxel counter's address for the seam branch's total.
+template <int KIND>
+__global__ __launch_bounds__(1024) void k_scan(KArgs a) {
+    __shared__ int buf[1024];
+    const int chunk = blockIdx.y;
+    Counters* cnt = a.counters + chunk;
+    const int* count;
+    int* base;
+    int n, cap, flag;
+    int* total;
+    if constexpr (KIND == kScanEdges) {
+        count = slot_ptr<int>(a, chunk, a.L.e_count);
+        base = slot_ptr<int>(a, chunk, a.L.e_base);
+        n = a.G.edge_segs;
+        total = &cnt->edges;
+        cap = a.caps.edges;
+        flag = kStatusEdgeOverflow;
+    } else if constexpr (KIND == kScanVoxels) {
+        count = slot_ptr<int>(a, chunk, a.L.v_count);
+        base = slot_ptr<int>(a, chunk, a.L.v_base);
+        n = a.G.voxel_segs;
+        total = &cnt->voxels;
+        cap = a.caps.voxels;
+        flag = kStatusVoxelOverflow;
+    } else if constexpr (KIND == kScanQuads) {
+        count = slot_ptr<int>(a, chunk, a.L.q_count);
+        base = slot_ptr<int>(a, chunk, a.L.q_base);
+        n = (3 * cnt->voxels + 63) >> 6;
+        total = &cnt->quads;
+        cap = a.caps.quads;
+        flag = kStatusQuadOverflow;
+    } else {
+        count = slot_ptr<int>(a, chunk, a.L.s_count);
+        base = slot_ptr<int>(a, chunk, a.L.s_base);
+        n = (cnt->voxels + 63) >> 6;
+        total = &cnt->seams;
+        cap = a.caps.seams;
+        flag = kStatusSeamOverflow;
+    }
+    const int tid = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b0 = min(n, tid * per), b1 = min(n, b0 + per);
+    int sum = 0;
+    for (int i = b0; i < b1; ++i) sum += count[i];
+    buf[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = tid >= off ? buf[tid - off] : 0;
+        __syncthreads();
+        buf[tid] += v;
+        __syncthreads();
+    }
+    int run = buf[tid] - sum;
+    for (int i = b0; i < b1; ++i) {
+        base[i] = run;
+        run += count[i];
+    }
+    if (tid == 1023) {
+        const int tot = buf[1023];
+        if (tot > cap) {
+            *total = cap;
+            cnt->status |= flag;
+        } else {
+            *total = tot;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Edge compaction (CompactEdges, cl/density_field.cl:80-92): the edge id is
+// ((x | y << S | z << 2S) << 2) | axis (cl/density_field.cl:63,74).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_edge_compact(KArgs a) {
+    const int chunk = blockIdx.y;
+    const Geometry& G = a.G;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= G.edge_slots) return;
+    const int i = dense_index(slot_ptr<uint64_t>(a, chunk, a.L.e_mask), slot_ptr<int>(a, chunk, a.L.e_base), s);
+    if (i < 0 || i >= a.caps.edges) return;
+    const int h = s / 3, axis = s - 3 * h;
+    const int x = h % G.H, r = h / G.H, y = r % G.H, z = r / G.H;
+    slot_ptr<int>(a, chunk, a.L.e_ids)[i] = ((x | (y << G.shift) | (z << (2 * G.shift))) << 2) | axis;
+}
+
+// ---------------------------------------------------------------------------
+// Edge crossing + normal
+// ---------------------------------------------------------------------------
+
+// FindEdgeIntersectionInfo, cl/density_field.cl:96-151: first arg-min of |d|
+// over t = k/16, then a central-difference normal with h = 0.001.  The height
+// term of samples that share (x, z) is evaluated once; the two end samples are
+// the stored field samples when the field is FP32.
+__device__ float4 density_crossing(const KArgs& a, const NoiseLds& T, const ChunkDesc& cd, int x, int y, int z,
+                                   int axis, bool ends_known, float d_start, float d_end) {
+    const FieldParams& Fp = a.F;
+    const int w0x = x * cd.scale + cd.off[0], w0y = y * cd.scale + cd.off[1], w0z = z * cd.scale + cd.off[2];
+    const float p0x = (float)w0x, p0y = (float)w0y, p0z = (float)w0z;
+    const float p1x = (float)(w0x + (axis == 0 ? cd.scale : 0));
+    const float p1y = (float)(w0y + (axis == 1 ? cd.scale : 0));
+    const float p1z = (float)(w0z + (axis == 2 ? cd.scale : 0));
+    const float colH = axis == 1 ? height_term(Fp, T, p0x, p0z) : 0.f;
+    float minV = 3.402823466e+38f, tbest = 0.f, Hbest = colH, cur = 0.f;
+    for (int k = 0; k <= 16; ++k) {
+        const float px = p0x + (p1x - p0x) * cur;
+        const float py = p0y + (p1y - p0y) * cur;
+        const float pz = p0z + (p1z - p0z) * cur;
+        float d, H = colH;
+        if (ends_known && k == 0) {
+            d = fabsf(d_start);
+        } else if (ends_known && k == 16) {
+            d = fabsf(d_end);
+        } else {
+            if (axis != 1) H = height_term(Fp, T, px, pz);
+            d = fabsf(combine(Fp, T, py - H, px, py, pz));
+        }
+        if (d < minV) {
+            tbest = cur;
+            minV = d;
+            Hbest = H;
+        }
+        cur = cur + 0.0625f;
+    }
+    const float px = p0x + (p1x - p0x) * tbest;
+    const float py = p0y + (p1y - p0y) * tbest;
+    const float pz = p0z + (p1z - p0z) * tbest;
+    const float h = 0.001f;
+    // The height at the arg-min sample is only known when it was evaluated there.
+    const bool hb_known = axis == 1 || !(ends_known && (tbest == 0.f || tbest == 1.f));
+    const float H0 = hb_known ? Hbest : height_term(Fp, T, px, pz);
+    const float Hxp = height_term(Fp, T, px + h, pz);
+    const float Hxm = height_term(Fp, T, px - h, pz);
+    const float Hzp = height_term(Fp, T, px, pz + h);
+    const float Hzm = height_term(Fp, T, px, pz - h);
+    const float dx = combine(Fp, T, py - Hxp, px + h, py, pz) - combine(Fp, T, py - Hxm, px - h, py, pz);
+    const float dy = combine(Fp, T, (py + h) - H0, px, py + h, pz) - combine(Fp, T, (py - h) - H0, px, py - h, pz);
+    const float dz = combine(Fp, T, py - Hzp, px, py, pz + h) - combine(Fp, T, py - Hzm, px, py, pz - h);
+    const float3 n = normalize_or_zero(dx, dy, dz);
+    return make_float4(n.x, n.y, n.z, tbest);
+}
+
+// FindEdgeIntersectionInfo of the CSG program, cl/apply_csg_operation.cl:443-477,
+// with BrushZeroCrossing (:86-110) and BrushNormal (:144-175).
+__device__ float4 brush_crossing(const KArgs& a, const ChunkDesc& cd, int x, int y, int z, int axis) {
+    const int w0x = x * cd.scale + cd.off[0], w0y = y * cd.scale + cd.off[1], w0z = z * cd.scale + cd.off[2];
+    const float p0x = (float)w0x, p0y = (float)w0y, p0z = (float)w0z;
+    const float p1x = (float)(w0x + (axis == 0 ? cd.scale : 0));
+    const float p1y = (float)(w0y + (axis == 1 ? cd.scale : 0));
+    const float p1z = (float)(w0z + (axis == 2 ? cd.scale : 0));
+    const DevOp* ops = a.ops;
+    const int* idx = a.op_index + cd.op_start;
+    float minD = 3.402823466e+38f, crossing = 0.f;
+    for (float t = 0.f; t <= 1.f; t = t + (1.f / 16.f)) {
+        const float px = p0x + (p1x - p0x) * t;
+        const float py = p0y + (p1y - p0y) * t;
+        const float pz = p0z + (p1z - p0z) * t;
+        for (int k = 0; k < cd.op_count; ++k) {
+            const float d = fabsf(brush_density(ops[idx[k]], px, py, pz));
+            if (d < minD) {
+                crossing = t;
+                minD = d;
+            }
+        }
+    }
+    const float px = p0x + (p1x - p0x) * crossing;
+    const float py = p0y + (p1y - p0y) * crossing;
+    const float pz = p0z + (p1z - p0z) * crossing;
+    const float h = 0.001f;
+    float3 n = make_float3(0.f, 0.f, 0.f);
+    for (int k = 0; k < cd.op_count; ++k) {
+        const DevOp op = ops[idx[k]];
+        if (brush_density(op, px, py, pz) > 0.f) continue;
+        const float dx = brush_density(op, px + h, py, pz) - brush_density(op, px - h, py, pz);
+        const float dy = brush_density(op, px, py + h, pz) - brush_density(op, px, py - h, pz);
+        const float dz = brush_density(op, px, py, pz + h) - brush_density(op, px, py, pz - h);
+        const float flip = op.type == 0 ? 1.f : -1.f;
+        const float3 u = normalize_or_zero(dx, dy, dz);
+        n = make_float3(flip * u.x, flip * u.y, flip * u.z);
+    }
+    return make_float4(n.x, n.y, n.z, crossing);
+}
+
+// One thread per compacted edge.  An edge with an endpoint whose material a
+// brush changed takes brush data (src/compute_csg.cpp:97-217); otherwise the
+// density crossing (fresh field) or its data from the cached edge list.
+__global__ __launch_bounds__(256) void k_edge_cross(KArgs a) {
+    __shared__ NoiseLds T;
+    const int chunk = blockIdx.y;
+    const int E = a.counters[chunk].edges;
+    if ((int)(blockIdx.x * blockDim.x) >= E) return;
+    load_noise_tables(T, a.perm, a.grads);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E) return;
+    const Geometry& G = a.G;
+    const ChunkDesc cd = a.chunks[chunk];
+    const int id = slot_ptr<int>(a, chunk, a.L.e_ids)[i];
+    const int axis = id & 3, hidx = id >> 2;
+    const int msk = (1 << G.shift) - 1;
+    const int x = hidx & msk, y = (hidx >> G.shift) & msk, z = (hidx >> (2 * G.shift)) & msk;
+    if (x >= G.H || y >= G.H || z >= G.H || axis > 2) {  // never produced by k_edge_compact
+        atomicOr(&a.counters[chunk].status, kStatusLookupMiss);
+        return;
+    }
+    const int i0 = field_index(G, x, y, z);
+    const int i1 = i0 + (axis == 0 ? 1 : axis == 1 ? G.F : G.F * G.F);
+    const uint8_t* mat = slot_ptr<uint8_t>(a, chunk, a.L.mat);
+    const uint8_t* orig = slot_ptr<uint8_t>(a, chunk, a.L.orig);
+    float4 out;
+    const bool touched = cd.op_count > 0 && (mat[i0] != orig[i0] || mat[i1] != orig[i1]);
+    if (touched) {
+        out = brush_crossing(a, cd, x, y, z, axis);
+    } else if (cd.mode == kModeCached) {
+        const int* oid = reinterpret_cast<const int*>(cd.old_ids);
+        const float4* odata = reinterpret_cast<const float4*>(cd.old_data);
+        int lo = 0, hi = cd.old_edges;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (oid[mid] < id) lo = mid + 1; else hi = mid;
+        }
+        if (lo < cd.old_edges && oid[lo] == id) {
+            out = odata[lo];
+        } else {
+            out = make_float4(0.f, 0.f, 0.f, 0.f);
+            atomicOr(&a.counters[chunk].status, kStatusLookupMiss);
+        }
+    } else {
+        const float* dens = slot_ptr<float>(a, chunk, a.L.density);
+        const bool ends = !a.fp16;
+        out = density_crossing(a, T, cd, x, y, z, axis, ends, ends ? dens[i0] : 0.f, ends ? dens[i1] : 0.f);
+    }
+    slot_ptr<float4>(a, chunk, a.L.e_data)[i] = out;
+}
+
+// ---------------------------------------------------------------------------
+// Active voxels (FindActiveVoxels + CompactVoxels, cl/octree.cl:142-223)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int corner_materials(const KArgs& a, const uint8_t* mat, int x, int y, int z, int* cm) {
+    const Geometry& G = a.G;
+    int corners = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        cm[i] = mat[field_index(G, x + ((i >> 2) & 1), y + ((i >> 1) & 1), z + (i & 1))];
+        corners |= (cm[i] == kAir ? 0 : 1) << i;
+    }
+    return corners;
+}
+
+// FindDominantMaterial with its insertion sort, cl/octree.cl:78-138.
+__device__ __forceinline__ int dominant_material(const int* m) {
+    int d[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) d[i] = m[i];
+#pragma unroll
+    for (int i = 1; i < 8; ++i) {
+        const int tmp = d[i];
+        int j = i;
+        while (j >= 1 && tmp < d[j - 1]) {
+            d[j] = d[j - 1];
+            --j;
+        }
+        d[j] = tmp;
+    }
+    int current = d[0], count = 1, maxCount = 0, maxMaterial = 0;
+    for (int i = 1; i < 8; ++i) {
+        const int v = d[i];
+        if (v == kAir || v == kNone) continue;
+        if (current != v) {
+            if (count > maxCount) {
+                maxCount = count;
+                maxMaterial = current;
+            }
+            current = v;
+            count = 1;
+        } else {
+            ++count;
+        }
+    }
+    if (count > maxCount) maxMaterial = current;
+    return maxMaterial;
+}
+
+__global__ __launch_bounds__(256) void k_voxel_flags(KArgs a) {
+    const int chunk = blockIdx.y;
+    const Geometry& G = a.G;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    int active = 0;
+    if (v < G.voxels) {
+        const int x = v % G.N, y = (v / G.N) % G.N, z = v / (G.N * G.N);
+        int cm[8];
+        const int corners = corner_materials(a, slot_ptr<uint8_t>(a, chunk, a.L.mat), x, y, z, cm);
+        active = corners != 0 && corners != 255;
+    }
+    const uint64_t m = __ballot(active);
+    const int seg = v >> 6;
+    if ((threadIdx.x & 63) == 0 && seg < G.voxel_segs) {
+        slot_ptr<uint64_t>(a, chunk, a.L.v_mask)[seg] = m;
+        slot_ptr<int>(a, chunk, a.L.v_count)[seg] = __popcll(m);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_voxel_compact(KArgs a) {
+    const int chunk = blockIdx.y;
+    const Geometry& G = a.G;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= G.voxels) return;
+    const int i = dense_index(slot_ptr<uint64_t>(a, chunk, a.L.v_mask), slot_ptr<int>(a, chunk, a.L.v_base), v);
+    if (i < 0 || i >= a.caps.voxels) return;
+    const int x = v % G.N, y = (v / G.N) % G.N, z = v / (G.N * G.N);
+    int cm[8];
+    const int corners = corner_materials(a, slot_ptr<uint8_t>(a, chunk, a.L.mat), x, y, z, cm);
+    int edges = 0;
+#pragma unroll
+    for (int e = 0; e < 12; ++e)
+        edges |= (((corners >> kEdgeMap[e][0]) & 1) != ((corners >> kEdgeMap[e][1]) & 1)) << e;
+    slot_ptr<int>(a, chunk, a.L.v_code)[i] = (int)code_for_position(x, y, z, G.depth);
+    slot_ptr<int>(a, chunk, a.L.v_emask)[i] = edges;
+    slot_ptr<int>(a, chunk, a.L.v_mat)[i] = (dominant_material(cm) << 8) | corners;
+}
+
+// ---------------------------------------------------------------------------
+// QEF build + solve (CreateLeafNodes + SolveQEFs, cl/octree.cl:236-331,
+// cl/qef.cl).  Edge data is found through the edge segment masks instead of a
+// cuckoo hash (cl/cuckoo.cl:73-104).
+// ---------------------------------------------------------------------------
+
+struct Q4 {
+    float x, y, z, w;
+};
+
+__device__ __forceinline__ float dot4(const Q4& a, const Q4& b) {
+    return ((a.x * b.x + a.y * b.y) + a.z * b.z) + a.w * b.w;
+}
+
+__device__ __forceinline__ void givens(float app, float apq, float aqq, float& c, float& s) {  // cl/qef.cl:31-42
+    const float tau = (aqq - app) / (2.f * apq);
+    const float stt = sqrtf(1.f + tau * tau);
+    const float tn = 1.f / ((tau >= 0.f) ? (tau + stt) : (tau - stt));
+    c = 1.f / sqrtf(1.f + tn * tn);
+    s = tn * c;
+}
+
+// svd_rotate, cl/qef.cl:58-86, for the pair (A, B); U/W are the other two
+// off-diagonal entries rotated with it (vtav[0][3-b], vtav[1-a][2]).
+#define LVN_ROTATE(AA, BB, AB, U, W, VA0, VB0, VA1, VB1, VA2, VB2)                   \
+    if (AB != 0.f) {                                                                   \
+        float c, s;                                                                    \
+        givens(AA, AB, BB, c, s);                                                      \
+        const float cc = c * c, ss = s * s;                                            \
+        const float mx = ((2.f * c) * s) * AB;                                         \
+        const float u0 = AA, w0 = BB;                                                  \
+        AA = (cc * u0 - mx) + ss * w0;                                                 \
+        BB = (ss * u0 + mx) + cc * w0;                                                 \
+        const float u1 = U, w1 = W;                                                    \
+        U = c * u1 - s * w1;                                                           \
+        W = s * u1 + c * w1;                                                           \
+        AB = 0.f;                                                                      \
+        float uu = VA0, ww = VB0;                                                      \
+        VA0 = c * uu - s * ww;                                                         \
+        VB0 = s * uu + c * ww;                                                         \
+        uu = VA1; ww = VB1;                                                            \
+        VA1 = c * uu - s * ww;                                                         \
+        VB1 = s * uu + c * ww;                                                         \
+        uu = VA2; ww = VB2;                                                            \
+        VA2 = c * uu - s * ww;                                                         \
+        VB2 = s * uu + c * ww;                                                         \
+    }
+
+__device__ __forceinline__ float invdet(float x) {  // svd_invdet, cl/qef.cl:107-109
+    return (fabsf(x) < 0.1f || fabsf(1.f / x) < 0.1f) ? 0.f : 1.f / x;
+}
+
+__global__ __launch_bounds__(256) void k_qef(KArgs a) {
+    const int chunk = blockIdx.y;
+    const int V = a.counters[chunk].voxels;
+    const int vi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (vi >= V) return;
+    const Geometry& G = a.G;
+    const ChunkDesc cd = a.chunks[chunk];
+    const uint32_t code = (uint32_t)slot_ptr<int>(a, chunk, a.L.v_code)[vi];
+    const int emask = slot_ptr<int>(a, chunk, a.L.v_emask)[vi];
+    const int3 p = position_for_code(code, G.depth);
+    const uint64_t* em = slot_ptr<uint64_t>(a, chunk, a.L.e_mask);
+    const int* eb = slot_ptr<int>(a, chunk, a.L.e_base);
+    const float4* ed = slot_ptr<float4>(a, chunk, a.L.e_data);
+    const float fscale = (float)cd.scale;
+
+    float ata0 = 0.f, ata1 = 0.f, ata2 = 0.f, ata3 = 0.f, ata4 = 0.f, ata5 = 0.f;
+    Q4 atb = {0.f, 0.f, 0.f, 0.f}, mass = {0.f, 0.f, 0.f, 0.f}, nsum = {0.f, 0.f, 0.f, 0.f};
+    for (int e = 0; e < 12; ++e) {
+        if (!((emask >> e) & 1)) continue;
+        const int e0 = kEdgeMap[e][0], e1 = kEdgeMap[e][1], axis = e >> 2;
+        const int hx = p.x + ((e0 >> 2) & 1), hy = p.y + ((e0 >> 1) & 1), hz = p.z + (e0 & 1);
+        const int slot = 3 * (hx + G.H * (hy + G.H * hz)) + axis;
+        const int k = dense_index(em, eb, slot);
+        if (k < 0 || k >= a.caps.edges) continue;
+        const float4 d = ed[k];
+        const float a0x = (float)hx, a0y = (float)hy, a0z = (float)hz;
+        const float a1x = (float)(p.x + ((e1 >> 2) & 1)), a1y = (float)(p.y + ((e1 >> 1) & 1)),
+                    a1z = (float)(p.z + (e1 & 1));
+        Q4 pos, n;
+        pos.x = fscale * (a0x + (a1x - a0x) * d.w);
+        pos.y = fscale * (a0y + (a1y - a0y) * d.w);
+        pos.z = fscale * (a0z + (a1z - a0z) * d.w);
+        pos.w = fscale * (0.f + (0.f - 0.f) * d.w);
+        n.x = d.x;
+        n.y = d.y;
+        n.z = d.z;
+        n.w = 0.f;
+        // qef_add_point, cl/qef.cl:170-191
+        ata0 = ata0 + n.x * n.x;
+        ata1 = ata1 + n.x * n.y;
+        ata2 = ata2 + n.x * n.z;
+        ata3 = ata3 + n.y * n.y;
+        ata4 = ata4 + n.y * n.z;
+        ata5 = ata5 + n.z * n.z;
+        const float b = dot4(pos, n);
+        atb.x = atb.x + n.x * b;
+        atb.y = atb.y + n.y * b;
+        atb.z = atb.z + n.z * b;
+        mass.x = mass.x + pos.x;
+        mass.y = mass.y + pos.y;
+        mass.z = mass.z + pos.z;
+        mass.w = mass.w + 1.f;
+        nsum.x = nsum.x + n.x;
+        nsum.y = nsum.y + n.y;
+        nsum.z = nsum.z + n.z;
+        nsum.w = (nsum.w + n.w) + 1.f;
+    }
+    // qef_create_from_points tail (cl/qef.cl:302), then qef_solve's divide by max(w, 1) (:244).
+    {
+        const float w = mass.w;
+        mass.x = mass.x / w;
+        mass.y = mass.y / w;
+        mass.z = mass.z / w;
+        mass.w = mass.w / w;
+        const float mw = fmaxf(mass.w, 1.f);
+        mass.x = mass.x / mw;
+        mass.y = mass.y / mw;
+        mass.z = mass.z / mw;
+        mass.w = mass.w / mw;
+    }
+    Q4 amp;
+    {
+        const Q4 row = {ata0, ata1, ata2, 0.f};
+        amp.x = atb.x - dot4(row, mass);
+        amp.y = atb.y - ((ata1 * mass.x + ata3 * mass.y) + ata4 * mass.z);
+        amp.z = atb.z - ((ata2 * mass.x + ata4 * mass.y) + ata5 * mass.z);
+        amp.w = atb.w - 0.f;
+    }
+    // svd_solve_sym, 10 cyclic sweeps (cl/qef.cl:88-105).
+    float v00 = ata0, v01 = ata1, v02 = ata2, v11 = ata3, v12 = ata4, v22 = ata5;
+    float V00 = 1.f, V01 = 0.f, V02 = 0.f, V10 = 0.f, V11 = 1.f, V12 = 0.f, V20 = 0.f, V21 = 0.f, V22 = 1.f;
+    for (int sweep = 0; sweep < 10; ++sweep) {
+        LVN_ROTATE(v00, v11, v01, v02, v12, V00, V01, V10, V11, V20, V21)
+        LVN_ROTATE(v00, v22, v02, v01, v12, V00, V02, V10, V12, V20, V22)
+        LVN_ROTATE(v11, v22, v12, v01, v02, V01, V02, V11, V12, V21, V22)
+    }
+    const float d0 = invdet(v00), d1 = invdet(v11), d2 = invdet(v22);
+    const float Vm[3][3] = {{V00, V01, V02}, {V10, V11, V12}, {V20, V21, V22}};
+    float x[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        Q4 row;
+        row.x = ((Vm[r][0] * d0) * Vm[0][0] + (Vm[r][1] * d1) * Vm[0][1]) + (Vm[r][2] * d2) * Vm[0][2];
+        row.y = ((Vm[r][0] * d0) * Vm[1][0] + (Vm[r][1] * d1) * Vm[1][1]) + (Vm[r][2] * d2) * Vm[1][2];
+        row.z = ((Vm[r][0] * d0) * Vm[2][0] + (Vm[r][1] * d1) * Vm[2][1]) + (Vm[r][2] * d2) * Vm[2][2];
+        row.w = 0.f;
+        x[r] = dot4(row, amp);
+    }
+    float4 out;
+    out.x = ((x[0] + mass.x) * 4.f) + (float)cd.min[0];
+    out.y = ((x[1] + mass.y) * 4.f) + (float)cd.min[1];
+    out.z = ((x[2] + mass.z) * 4.f) + (float)cd.min[2];
+    out.w = 1.f;
+    slot_ptr<float4>(a, chunk, a.L.v_pos)[vi] = out;
+    const float nw = nsum.w;
+    slot_ptr<float4>(a, chunk, a.L.v_norm)[vi] = make_float4(nsum.x / nw, nsum.y / nw, nsum.z / nw, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// Quads (GenerateMesh + CompactMeshTriangles, cl/octree.cl:335-464)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int quad_flag(const Geometry& G, int3 p, int corners, int axis) {
+    const int pa = axis == 0 ? p.y : axis == 1 ? p.z : p.x;
+    const int pb = axis == 0 ? p.z : axis == 1 ? p.x : p.y;
+    if (pa == G.N - 1 || pb == G.N - 1) return 0;  // isEdgeVoxel, cl/octree.cl:411-417
+    const int edge = axis * 4 + 3;
+    const int m1 = (corners >> kEdgeMap[edge][0]) & 1, m2 = (corners >> kEdgeMap[edge][1]) & 1;
+    return m1 != m2;
+}
+
+__global__ __launch_bounds__(256) void k_quad_flags(KArgs a) {
+    const int chunk = blockIdx.y;
+    const int V = a.counters[chunk].voxels;
+    const int nq = 3 * V;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int)(blockIdx.x * blockDim.x) >= nq) return;
+    int flag = 0;
+    if (q < nq) {
+        const int vi = q / 3, axis = q - 3 * vi;
+        const int3 p = position_for_code((uint32_t)slot_ptr<int>(a, chunk, a.L.v_code)[vi], a.G.depth);
+        flag = quad_flag(a.G, p, slot_ptr<int>(a, chunk, a.L.v_mat)[vi] & 0xff, axis);
+    }
+    const uint64_t m = __ballot(flag);
+    if ((threadIdx.x & 63) == 0 && (q >> 6) < ((nq + 63) >> 6)) {
+        slot_ptr<uint64_t>(a, chunk, a.L.q_mask)[q >> 6] = m;
+        slot_ptr<int>(a, chunk, a.L.q_count)[q >> 6] = __popcll(m);
+    }
+}
+
+// cl/octree.cl:376-381
+__constant__ int kEdgeNodeOffsets[3][4][3] = {{{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1}},
+                                              {{0, 0, 0}, {1, 0, 0}, {0, 0, 1}, {1, 0, 1}},
+                                              {{0, 0, 0}, {0, 1, 0}, {1, 0, 0}, {1, 1, 0}}};
+
+__global__ __launch_bounds__(256) void k_quad_emit(KArgs a) {
+    const int chunk = blockIdx.y;
+    const int V = a.counters[chunk].voxels;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 3 * V) return;
+    const int qi = dense_index(slot_ptr<uint64_t>(a, chunk, a.L.q_mask), slot_ptr<int>(a, chunk, a.L.q_base), q);
+    if (qi < 0 || qi >= a.caps.quads) return;
+    const Geometry& G = a.G;
+    const int vi = q / 3, axis = q - 3 * vi;
+    const int3 p = position_for_code((uint32_t)slot_ptr<int>(a, chunk, a.L.v_code)[vi], G.depth);
+    const int corners = slot_ptr<int>(a, chunk, a.L.v_mat)[vi] & 0xff;
+    const uint64_t* vm = slot_ptr<uint64_t>(a, chunk, a.L.v_mask);
+    const int* vb = slot_ptr<int>(a, chunk, a.L.v_base);
+    int nodes[4];
+    nodes[0] = vi;
+    bool ok = true;
+#pragma unroll
+    for (int n = 1; n < 4; ++n) {
+        const int x = p.x + kEdgeNodeOffsets[axis][n][0];
+        const int y = p.y + kEdgeNodeOffsets[axis][n][1];
+        const int z = p.z + kEdgeNodeOffsets[axis][n][2];
+        nodes[n] = dense_index(vm, vb, x + G.N * (y + G.N * z));
+        ok = ok && nodes[n] >= 0;
+    }
+    if (!ok) {  // cannot happen: all four voxels around a sign-changing interior edge are active
+        atomicOr(&a.counters[chunk].status, kStatusLookupMiss);
+        return;
+    }
+    const int edge = axis * 4 + 3;
+    const int flip = ((corners >> kEdgeMap[edge][0]) & 1) != 0 ? 1 : 0;  // ProcessEdge, cl/octree.cl:335-372
+    int* out = slot_ptr<int>(a, chunk, a.L.tris) + 6 * qi;
+    if (flip) {
+        out[0] = nodes[0]; out[1] = nodes[3]; out[2] = nodes[1];
+        out[3] = nodes[0]; out[4] = nodes[2]; out[5] = nodes[3];
+    } else {
+        out[0] = nodes[0]; out[1] = nodes[1]; out[2] = nodes[3];
+        out[3] = nodes[0]; out[4] = nodes[3]; out[5] = nodes[2];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Vertex buffer + seam flags (GenerateMeshVertexBuffer + FindSeamNodes,
+// cl/octree.cl:475-518)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_vertex_seam(KArgs a) {
+    const int chunk = blockIdx.y;
+    const int V = a.counters[chunk].voxels;
+    const int vi = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int)(blockIdx.x * blockDim.x) >= V) return;
+    int seam = 0;
+    if (vi < V) {
+        const ChunkDesc cd = a.chunks[chunk];
+        const int N = a.G.N;
+        const int3 p = position_for_code((uint32_t)slot_ptr<int>(a, chunk, a.L.v_code)[vi], a.G.depth);
+        seam = p.x == 0 || p.x == N - 1 || p.y == 0 || p.y == N - 1 || p.z == 0 || p.z == N - 1;
+        if (a.counters[chunk].quads > 0) {
+            const int m = slot_ptr<int>(a, chunk, a.L.v_mat)[vi];
+            float4* vout = slot_ptr<float4>(a, chunk, a.L.verts) + 3 * vi;
+            vout[0] = slot_ptr<float4>(a, chunk, a.L.v_pos)[vi];
+            vout[1] = slot_ptr<float4>(a, chunk, a.L.v_norm)[vi];
+            vout[2] = make_float4(cd.colour[0], cd.colour[1], cd.colour[2], (float)(m >> 8));
+        }
+    }
+    const uint64_t mk = __ballot(seam);
+    if ((threadIdx.x & 63) == 0 && (vi >> 6) < ((V + 63) >> 6)) {
+        slot_ptr<uint64_t>(a, chunk, a.L.s_mask)[vi >> 6] = mk;
+        slot_ptr<int>(a, chunk, a.L.s_count)[vi >> 6] = __popcll(mk);
+    }
+}
+
+// ExtractSeamNodeInfo, cl/octree.cl:529-551.
+__global__ __launch_bounds__(256) void k_seam_emit(KArgs a) {
+    const int chunk = blockIdx.y;
+    const int V = a.counters[chunk].voxels;
+    const int vi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (vi >= V) return;
+    const int si = dense_index(slot_ptr<uint64_t>(a, chunk, a.L.s_mask), slot_ptr<int>(a, chunk, a.L.s_base), vi);
+    if (si < 0 || si >= a.caps.seams) return;
+    const int3 p = position_for_code((uint32_t)slot_ptr<int>(a, chunk, a.L.v_code)[vi], a.G.depth);
+    int4* out = reinterpret_cast<int4*>(slot_ptr<float4>(a, chunk, a.L.seams) + 3 * si);
+    out[0] = make_int4(p.x, p.y, p.z, slot_ptr<int>(a, chunk, a.L.v_mat)[vi]);
+    float4* fo = slot_ptr<float4>(a, chunk, a.L.seams) + 3 * si;
+    fo[1] = slot_ptr<float4>(a, chunk, a.L.v_pos)[vi];
+    fo[2] = slot_ptr<float4>(a, chunk, a.L.v_norm)[vi];
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+static inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+void launch_density(const KArgs& a, int chunks, hipStream_t s) {
+    const long threads = (long)a.G.F * a.G.F * a.ysplit;
+    hipLaunchKernelGGL(k_density, dim3(ceil_div(threads, 512), chunks), dim3(512), 0, s, a);
+}
+void launch_csg(const KArgs& a, int chunks, hipStream_t s) {
+    hipLaunchKernelGGL(k_csg, dim3(1, chunks), dim3(1024), 0, s, a);
+}
+void launch_edge_flags(const KArgs& a, int chunks, hipStream_t s) {
+    hipLaunchKernelGGL(k_edge_flags, dim3(ceil_div(a.G.edge_slots, 256), chunks), dim3(256), 0, s, a);
+}
+void launch_scan(const KArgs& a, int chunks, ScanKind kind, hipStream_t s) {
+    const dim3 grid(1, chunks), block(1024);
+    switch (kind) {
+    case kScanEdges: hipLaunchKernelGGL(k_scan<kScanEdges>, grid, block, 0, s, a); break;
+    case kScanVoxels: hipLaunchKernelGGL(k_scan<kScanVoxels>, grid, block, 0, s, a); break;
+    case kScanQuads: hipLaunchKernelGGL(k_scan<kScanQuads>, grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL(k_scan<kScanSeams>, grid, block, 0, s, a); break;
+    }
+}
+void launch_edge_compact(const KArgs& a, int chunks, hipStream_t s) {
+    hipLaunchKernelGGL(k_edge_compact, dim3(ceil_div(a.G.edge_slots, 256), chunks), dim3(256), 0, s, a);
+}
+void launch_edge_cross(const KArgs& a, int chunks, hipStream_t s) {
+    hipLaunchKernelGGL(k_edge_cross, dim3(ceil_div(a.caps.edges, 256), chunks), dim3(256), 0, s, a);
+}
+void launch_voxel_flags(const KArgs& a, int chunks, hipStream_t s) {
+    hipLaunchKernelGGL(k_voxel_flags, dim3(ceil_div(a.G.voxels, 256), chunks), dim3(256), 0, s, a);
+}
+void launch_voxel_compact(const KArgs& a, int chunks, hipStream_t s) {
+    hipLaunchKernelGGL(k_voxel_compact, dim3(ceil_div(a.G.voxels, 256), chunks), dim3(256), 0, s, a);
+}
+void launch_qef(const KArgs& a, int chunks, hipStream_t s) {
+    hipLaunchKernelGGL(k_qef, dim3(ceil_div(a.caps.voxels, 256), chunks), dim3(256), 0, s, a);
+}
+void launch_quad_flags(const KArgs& a, int chunks, hipStream_t s) {
+    hipLaunchKernelGGL(k_quad_flags, dim3(ceil_div(3L * a.caps.voxels, 256), chunks), dim3(256), 0, s, a);
+}
+void launch_quad_emit(const KArgs& a, int chunks, hipStream_t s) {
+    hipLaunchKernelGGL(k_quad_emit, dim3(ceil_div(3L * a.caps.voxels, 256), chunks), dim3(256), 0, s, a);
+}
+void launch_vertex_seam(const KArgs& a, int chunks, hipStream_t s) {
+    hipLaunchKernelGGL(k_vertex_seam, dim3(ceil_div(a.caps.voxels, 256), chunks), dim3(256), 0, s, a);
+}
+void launch_seam_emit(const KArgs& a, int chunks, hipStream_t s) {
+    hipLaunchKernelGGL(k_seam_emit, dim3(ceil_div(a.caps.voxels, 256), chunks), dim3(256), 0, s, a);
+}
+
+void launch_pipeline(const KArgs& a, int chunks, hipStream_t s) {
+    launch_density(a, chunks, s);
+    launch_csg(a, chunks, s);
+    launch_edge_flags(a, chunks, s);
+    launch_scan(a, chunks, kScanEdges, s);
+    launch_edge_compact(a, chunks, s);
+    launch_edge_cross(a, chunks, s);
+    launch_voxel_flags(a, chunks, s);
+    launch_scan(a, chunks, kScanVoxels, s);
+    launch_voxel_compact(a, chunks, s);
+    launch_qef(a, chunks, s);
+    launch_quad_flags(a, chunks, s);
+    launch_scan(a, chunks, kScanQuads, s);
+    launch_quad_emit(a, chunks, s);
+    launch_vertex_seam(a, chunks, s);
+    launch_scan(a, chunks, kScanSeams, s);
+    launch_seam_emit(a, chunks, s);
+}
+
+}  // namespace lvn

@@ -1,0 +1,92 @@
+// lvn_runtime.h — host-side state of the chunk mesher runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "lvn_kernels.h"
+
+namespace lvn {
+
+// Global state set by lvn_init / lvn_set_noise_seed / lvn_store_csg_op
+// (the reference keeps the same as process globals: src/compute.cpp:168-191,
+// src/compute_density_field.cpp:23-24).
+struct Global {
+    bool init = false;
+    lvn_field_desc desc{};
+    FieldParams fp{};
+    int32_t seed = 0;
+    uint32_t defmat = 0;
+    std::vector<uint8_t> perm;
+    uint8_t* d_perm = nullptr;
+    float4* d_grads = nullptr;
+    std::vector<lvn_csg_op> ops;
+    std::vector<std::array<int32_t, 6>> aabbs;
+    std::mutex mu;
+};
+extern Global g_state;
+
+// Cache key: ivec4(min, size) as in src/compute_local.h:125,138.
+using Key = std::tuple<int32_t, int32_t, int32_t, int32_t>;
+inline Key make_key(const int32_t* min, int32_t size) { return Key(min[0], min[1], min[2], size); }
+
+// Device-resident density-field state (GPUDensityField, src/compute_local.h:113-123):
+// final materials and the sorted edge list with its hermite data.
+struct FieldEntry {
+    uint8_t* d_mat = nullptr;
+    int* d_ids = nullptr;
+    float4* d_data = nullptr;
+    int edges = 0;
+    int cap = 0;
+    int last_op = 0;
+    void release();
+};
+
+// A finished chunk (the reference caches its octree; the mesh is a pure
+// function of it, so the mesh is what is kept).
+struct MeshEntry {
+    std::vector<float> verts;   // 12 floats per vertex
+    std::vector<int> tris;      // 3 ints per triangle
+    std::vector<float> seams;   // 12 words per seam node
+};
+
+struct Ctx {
+    int N = 0;
+    uint32_t flags = 0;
+    Geometry G{};
+    Caps caps{};
+    SlotLayout L{};
+    int slots = 0;
+    hipStream_t stream = nullptr;
+    char* d_arena = nullptr;
+    ChunkDesc* d_chunks = nullptr;
+    Counters* d_counters = nullptr;
+    ChunkDesc* h_chunks = nullptr;
+    Counters* h_counters = nullptr;
+    DevOp* d_ops = nullptr;
+    DevOp* h_ops = nullptr;
+    int* d_op_index = nullptr;
+    int* h_op_index = nullptr;
+    int ops_cap = 0, idx_cap = 0;
+    std::map<Key, FieldEntry> fields;
+    std::map<Key, MeshEntry> meshes;
+
+    int allocate(int n, uint32_t flags, int nslots, int edge_cap, int voxel_cap);
+    int ensure_ops(int nops, int nidx);
+    KArgs kargs() const;
+    ~Ctx();
+};
+
+DevOp make_dev_op(const lvn_csg_op& op);
+Geometry make_geometry(int n);
+Caps make_caps(const Geometry& G, int edge_cap, int voxel_cap);
+SlotLayout make_layout(const Geometry& G, const Caps& c);
+void fill_chunk_desc(ChunkDesc& cd, const int32_t* min, int32_t size, int N);
+bool aabb_overlaps(const int32_t* mn, int32_t size, const std::array<int32_t, 6>& bb);
+
+}  // namespace lvn

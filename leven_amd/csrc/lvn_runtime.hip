@@ -1,0 +1,724 @@
+// lvn_runtime.hip — host runtime and C ABI of the chunk mesher.
+//
+// Mirrors leven's compute host layer (src/compute.cpp, src/compute_density_field.cpp,
+// src/compute_octree.cpp, src/compute_csg.cpp): global noise seed, stored CSG
+// operations, and per-context caches of density fields and finished chunks
+// keyed by (min, size).  Every stage runs on the GPU; the host only sequences
+// launches on one stream and copies results out (two synchronisations per
+// chunk instead of the reference's eight or more).
+
+#include "lvn_kernels.h"
+#include "lvn_runtime.h"
+
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#define LVN_HIP(call)                                                                                 \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess) {                                                                       \
+            fprintf(stderr, "lvn: HIP error %s at %s:%d (%s)\n", hipGetErrorString(e_), __FILE__,     \
+                    __LINE__, #call);                                                                 \
+            return LVN_ERR_DEVICE;                                                                    \
+        }                                                                                             \
+    } while (0)
+
+namespace lvn {
+
+Global g_state;
+
+// ---------------------------------------------------------------------------
+// Permutation table (CreateNoisePermutationLookupImage, src/compute_density_field.cpp:90-125)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// Ken Perlin's reference permutation; the reference's perm[512] holds it, then
+// the same list without its first six entries, then six zeros
+// (src/compute_density_field.cpp:28-52).
+const uint8_t kPerlin[256] = {
+    151, 160, 137, 91,  90,  15,  131, 13,  201, 95,  96,  53,  194, 233, 7,   225, 140, 36,  103, 30,  69,  142,
+    8,   99,  37,  240, 21,  10,  23,  190, 6,   148, 247, 120, 234, 75,  0,   26,  197, 62,  94,  252, 219, 203,
+    117, 35,  11,  32,  57,  177, 33,  88,  237, 149, 56,  87,  174, 20,  125, 136, 171, 168, 68,  175, 74,  165,
+    71,  134, 139, 48,  27,  166, 77,  146, 158, 231, 83,  111, 229, 122, 60,  211, 133, 230, 220, 105, 92,  41,
+    55,  46,  245, 40,  244, 102, 143, 54,  65,  25,  63,  161, 1,   216, 80,  73,  209, 76,  132, 187, 208, 89,
+    18,  169, 200, 196, 135, 130, 116, 188, 159, 86,  164, 100, 109, 198, 173, 186, 3,   64,  52,  217, 226, 250,
+    124, 123, 5,   202, 38,  147, 118, 126, 255, 82,  85,  212, 207, 206, 59,  227, 47,  16,  58,  17,  182, 189,
+    28,  42,  223, 183, 170, 213, 119, 248, 152, 2,   44,  154, 163, 70,  221, 153, 101, 155, 167, 43,  172, 9,
+    129, 22,  39,  253, 19,  98,  108, 110, 79,  113, 224, 232, 178, 185, 112, 104, 218, 246, 97,  228, 251, 34,
+    242, 193, 238, 210, 144, 12,  191, 179, 162, 241, 81,  51,  145, 235, 249, 14,  239, 107, 49,  192, 214, 31,
+    181, 199, 106, 157, 184, 84,  204, 176, 115, 121, 50,  45,  127, 4,   150, 254, 138, 236, 205, 93,  222, 114,
+    67,  29,  24,  72,  243, 141, 128, 195, 78,  66,  215, 61,  156, 180};
+
+// Perlin's 16 gradients (src/compute_density_field.cpp:60-63).
+const int kGrad3[16][3] = {{0, 1, 1},  {0, 1, -1},  {0, -1, 1}, {0, -1, -1}, {1, 0, 1},  {1, 0, -1},
+                           {-1, 0, 1}, {-1, 0, -1}, {1, 1, 0},  {1, -1, 0},  {-1, 1, 0}, {-1, -1, 0},
+                           {1, 0, -1}, {-1, 0, -1}, {0, -1, 1}, {0, 1, 1}};
+
+// Shuffle with a fixed generator (the reference's std::shuffle +
+// default_random_engine is library-defined): Fisher-Yates from index 511 down,
+// j = splitmix64 % (i + 1), state = (uint32)seed.
+void build_perm_table(int32_t seed, uint8_t* out) {
+    int sh[512];
+    for (int i = 0; i < 256; ++i) sh[i] = kPerlin[i];
+    for (int i = 0; i < 250; ++i) sh[256 + i] = kPerlin[6 + i];
+    for (int i = 506; i < 512; ++i) sh[i] = 0;
+    uint64_t st = (uint64_t)(uint32_t)seed;
+    for (int i = 511; i > 0; --i) {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        std::swap(sh[i], sh[z % (uint64_t)(i + 1)]);
+    }
+    for (int row = 0; row < 256; ++row)
+        for (int col = 0; col < 256; ++col) {
+            // NoiseHash (src/compute_density_field.cpp:68-86): one-at-a-time hash of the key bytes.
+            const uint32_t key = (((uint32_t)row << 24) | ((uint32_t)col << 16)) ^ (uint32_t)seed;
+            uint32_t h = 0;
+            for (int b = 0; b < 4; ++b) {
+                h += (key >> (8 * b)) & 0xffu;
+                h += h << 10;
+                h ^= h >> 6;
+            }
+            h += h << 3;
+            h ^= h >> 11;
+            h += h << 15;
+            out[row * 256 + col] = (uint8_t)sh[h & 0x1ffu];
+        }
+}
+
+// UNORM8 decode of the gradient bytes g*64+64, then *4-1 (cl/simplex.cl:124).
+void build_grads(float4* out) {
+    for (int i = 0; i < 16; ++i) {
+        float c[3];
+        for (int k = 0; k < 3; ++k) c[k] = ((float)(kGrad3[i][k] * 64 + 64) / 255.0f) * 4.0f - 1.0f;
+        out[i] = make_float4(c[0], c[1], c[2], 0.f);
+    }
+}
+
+FieldParams make_field_params(const lvn_field_desc& d) {
+    FieldParams f;
+    f.kind = d.kind;
+    f.hoct = d.height_octaves;
+    f.hfreq = d.height_frequency;
+    f.hlac = d.height_lacunarity;
+    f.hgain = d.height_gain;
+    f.hamp = d.height_amplitude;
+    f.coct = d.cave_octaves;
+    f.cfreq = d.cave_frequency;
+    f.clac = d.cave_lacunarity;
+    f.cgain = d.cave_gain;
+    f.cthr = d.cave_threshold;
+    // |snoise3| <= 32 * 4 * max_r (0.6 - r^2)^4 * r * |g| with |g| <= 1.4253: 3.811.
+    double amp = 1.0, sum = 0.0;
+    for (int i = 0; i < d.cave_octaves; ++i) {
+        sum += std::fabs(amp);
+        amp *= (double)d.cave_gain;
+    }
+    f.cave_cut = (float)((3.82 * sum + 1e-3) * 1.001 - (double)d.cave_threshold);
+    return f;
+}
+
+lvn_field_desc default_field() {
+    lvn_field_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.kind = LVN_FIELD_HEIGHTFIELD;
+    d.height_octaves = 6;
+    d.height_frequency = 1.f / 128.f;
+    d.height_lacunarity = 2.f;
+    d.height_gain = 0.5f;
+    d.height_amplitude = 24.f;
+    d.cave_octaves = 5;
+    d.cave_frequency = 1.f / 64.f;
+    d.cave_lacunarity = 2.f;
+    d.cave_gain = 0.5f;
+    d.cave_threshold = 0.3f;
+    return d;
+}
+
+void lod_colour(int size, float* rgb) {  // ColourForMinLeafSize, src/clipmap.cpp:329-352
+    switch (size / kClipmapLeafSize) {
+    case 1: rgb[0] = 0.3f; rgb[1] = 0.1f; rgb[2] = 0.f; break;
+    case 2: rgb[0] = 0.f; rgb[1] = 0.f; rgb[2] = 0.5f; break;
+    case 4: rgb[0] = 0.f; rgb[1] = 0.5f; rgb[2] = 0.5f; break;
+    case 8: rgb[0] = 0.5f; rgb[1] = 0.f; rgb[2] = 0.5f; break;
+    case 16: rgb[0] = 0.f; rgb[1] = 0.5f; rgb[2] = 0.f; break;
+    default: rgb[0] = 0.5f; rgb[1] = 0.f; rgb[2] = 0.f; break;
+    }
+}
+
+inline uint64_t align256(uint64_t v) { return (v + 255) & ~(uint64_t)255; }
+
+}  // namespace
+
+DevOp make_dev_op(const lvn_csg_op& op) {
+    DevOp d;
+    std::memset(&d, 0, sizeof(d));
+    d.type = op.type;
+    d.brush = op.brush;
+    d.material = op.material;
+    d.ox = op.origin[0];
+    d.oy = op.origin[1];
+    d.oz = op.origin[2];
+    d.dx = op.dims[0];
+    d.dy = op.dims[1];
+    d.dz = op.dims[2];
+    d.c = cosf(op.rotate_y);
+    d.s = sinf(op.rotate_y);
+    d.radius = op.brush == 1 ? std::fabs(op.dims[0])
+                             : std::sqrt(op.dims[0] * op.dims[0] + op.dims[1] * op.dims[1] + op.dims[2] * op.dims[2]);
+    return d;
+}
+
+Geometry make_geometry(int n) {
+    Geometry G;
+    G.N = n;
+    G.H = n + 1;
+    G.F = n + 2;
+    G.depth = 0;
+    while ((1 << G.depth) < n) ++G.depth;
+    G.shift = G.depth + 1;
+    G.field_samples = G.F * G.F * G.F;
+    G.edge_slots = 3 * G.H * G.H * G.H;
+    G.edge_segs = (G.edge_slots + 63) / 64;
+    G.voxels = n * n * n;
+    G.voxel_segs = std::max(1, G.voxels / 64);
+    return G;
+}
+
+Caps make_caps(const Geometry& G, int edge_cap, int voxel_cap) {
+    Caps c;
+    c.edges = edge_cap > 0 ? std::min(edge_cap, G.edge_slots) : G.edge_slots;
+    c.voxels = voxel_cap > 0 ? std::min(voxel_cap, G.voxels) : G.voxels;
+    c.quads = 3 * c.voxels;
+    c.seams = c.voxels;
+    c.edge_segs_q = (3 * c.voxels + 63) / 64;
+    c.seam_segs = (c.voxels + 63) / 64;
+    return c;
+}
+
+SlotLayout make_layout(const Geometry& G, const Caps& c) {
+    SlotLayout L;
+    uint64_t o = 0;
+    auto take = [&](uint64_t bytes) {
+        const uint64_t at = o;
+        o = align256(o + bytes);
+        return at;
+    };
+    const uint64_t fs = (uint64_t)G.field_samples;
+    L.density = take(fs * 4);
+    L.mat = take(fs);
+    L.orig = take(fs);
+    L.e_mask = take((uint64_t)G.edge_segs * 8);
+    L.e_count = take((uint64_t)G.edge_segs * 4);
+    L.e_base = take((uint64_t)G.edge_segs * 4);
+    L.e_ids = take((uint64_t)c.edges * 4);
+    L.e_data = take((uint64_t)c.edges * 16);
+    L.v_mask = take((uint64_t)G.voxel_segs * 8);
+    L.v_count = take((uint64_t)G.voxel_segs * 4);
+    L.v_base = take((uint64_t)G.voxel_segs * 4);
+    L.v_code = take((uint64_t)c.voxels * 4);
+    L.v_emask = take((uint64_t)c.voxels * 4);
+    L.v_mat = take((uint64_t)c.voxels * 4);
+    L.v_pos = take((uint64_t)c.voxels * 16);
+    L.v_norm = take((uint64_t)c.voxels * 16);
+    L.q_mask = take((uint64_t)c.edge_segs_q * 8);
+    L.q_count = take((uint64_t)c.edge_segs_q * 4);
+    L.q_base = take((uint64_t)c.edge_segs_q * 4);
+    L.tris = take((uint64_t)c.quads * 24);
+    L.s_mask = take((uint64_t)c.seam_segs * 8);
+    L.s_count = take((uint64_t)c.seam_segs * 4);
+    L.s_base = take((uint64_t)c.seam_segs * 4);
+    L.seams = take((uint64_t)c.seams * 48);
+    L.verts = take((uint64_t)c.voxels * 48);
+    L.slot_bytes = o;
+    return L;
+}
+
+void fill_chunk_desc(ChunkDesc& cd, const int32_t* min, int32_t size, int N) {
+    std::memset(&cd, 0, sizeof(cd));
+    for (int i = 0; i < 3; ++i) {
+        cd.min[i] = min[i];
+        cd.off[i] = min[i] / kLeafSizeScale;  // LeafScaleVec, src/compute.cpp:560-568
+    }
+    cd.size = size;
+    cd.scale = size / (N * kLeafSizeScale);  // src/compute_density_field.cpp:149
+    lod_colour(size, cd.colour);
+}
+
+bool aabb_overlaps(const int32_t* mn, int32_t size, const std::array<int32_t, 6>& bb) {  // src/aabb.h:26-35
+    return !(mn[0] + size < bb[0] || mn[1] + size < bb[1] || mn[2] + size < bb[2] || mn[0] > bb[3] ||
+             mn[1] > bb[4] || mn[2] > bb[5]);
+}
+
+int ensure_global() { return g_state.init ? LVN_OK : LVN_ERR_STATE; }
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+
+int Ctx::allocate(int n, uint32_t fl, int nslots, int edge_cap, int voxel_cap) {
+    N = n;
+    flags = fl;
+    G = make_geometry(n);
+    caps = make_caps(G, edge_cap, voxel_cap);
+    L = make_layout(G, caps);
+    slots = nslots;
+    LVN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    LVN_HIP(hipMalloc(&d_arena, L.slot_bytes * (uint64_t)slots));
+    LVN_HIP(hipMalloc(&d_chunks, sizeof(ChunkDesc) * slots));
+    LVN_HIP(hipMalloc(&d_counters, sizeof(Counters) * slots));
+    LVN_HIP(hipHostMalloc(&h_chunks, sizeof(ChunkDesc) * slots, hipHostMallocDefault));
+    LVN_HIP(hipHostMalloc(&h_counters, sizeof(Counters) * slots, hipHostMallocDefault));
+    return LVN_OK;
+}
+
+int Ctx::ensure_ops(int nops, int nidx) {
+    if (nops > ops_cap) {
+        if (d_ops) LVN_HIP(hipFree(d_ops));
+        if (h_ops) LVN_HIP(hipHostFree(h_ops));
+        ops_cap = std::max(nops, 2 * ops_cap);
+        LVN_HIP(hipMalloc(&d_ops, sizeof(DevOp) * ops_cap));
+        LVN_HIP(hipHostMalloc(&h_ops, sizeof(DevOp) * ops_cap, hipHostMallocDefault));
+    }
+    if (nidx > idx_cap) {
+        if (d_op_index) LVN_HIP(hipFree(d_op_index));
+        if (h_op_index) LVN_HIP(hipHostFree(h_op_index));
+        idx_cap = std::max(nidx, 2 * idx_cap);
+        LVN_HIP(hipMalloc(&d_op_index, sizeof(int) * idx_cap));
+        LVN_HIP(hipHostMalloc(&h_op_index, sizeof(int) * idx_cap, hipHostMallocDefault));
+    }
+    return LVN_OK;
+}
+
+KArgs Ctx::kargs() const {
+    KArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.arena = d_arena;
+    a.L = L;
+    a.G = G;
+    a.caps = caps;
+    a.chunks = d_chunks;
+    a.counters = d_counters;
+    a.ops = d_ops;
+    a.op_index = d_op_index;
+    a.F = g_state.fp;
+    a.perm = g_state.d_perm;
+    a.grads = g_state.d_grads;
+    a.defmat = (int)g_state.defmat;
+    a.fp16 = (flags & LVN_CTX_FP16_DENSITY) ? 1 : 0;
+    a.ysplit = 1;
+    return a;
+}
+
+Ctx::~Ctx() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& kv : fields) kv.second.release();
+    if (d_arena) (void)hipFree(d_arena);
+    if (d_chunks) (void)hipFree(d_chunks);
+    if (d_counters) (void)hipFree(d_counters);
+    if (d_ops) (void)hipFree(d_ops);
+    if (d_op_index) (void)hipFree(d_op_index);
+    if (h_chunks) (void)hipHostFree(h_chunks);
+    if (h_counters) (void)hipHostFree(h_counters);
+    if (h_ops) (void)hipHostFree(h_ops);
+    if (h_op_index) (void)hipHostFree(h_op_index);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void FieldEntry::release() {
+    if (d_mat) (void)hipFree(d_mat);
+    if (d_ids) (void)hipFree(d_ids);
+    if (d_data) (void)hipFree(d_data);
+    d_mat = nullptr;
+    d_ids = nullptr;
+    d_data = nullptr;
+}
+
+// Upload one chunk descriptor + its op batch to slot 0.
+static int upload_single(Ctx* c, const ChunkDesc& cd_in, const std::vector<lvn_csg_op>& ops) {
+    const int n = (int)ops.size();
+    int r = c->ensure_ops(std::max(n, 1), std::max(n, 1));
+    if (r) return r;
+    for (int i = 0; i < n; ++i) {
+        c->h_ops[i] = make_dev_op(ops[i]);
+        c->h_op_index[i] = i;
+    }
+    c->h_chunks[0] = cd_in;
+    c->h_chunks[0].op_start = 0;
+    c->h_chunks[0].op_count = n;
+    if (n) {
+        LVN_HIP(hipMemcpyAsync(c->d_ops, c->h_ops, sizeof(DevOp) * n, hipMemcpyHostToDevice, c->stream));
+        LVN_HIP(hipMemcpyAsync(c->d_op_index, c->h_op_index, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    }
+    LVN_HIP(hipMemcpyAsync(c->d_chunks, c->h_chunks, sizeof(ChunkDesc), hipMemcpyHostToDevice, c->stream));
+    LVN_HIP(hipMemsetAsync(c->d_counters, 0, sizeof(Counters), c->stream));
+    return LVN_OK;
+}
+
+// Field stages for slot 0: density (fresh) or cached materials, CSG batch, edges, crossings.
+static int run_field(Ctx* c, const int32_t* min, int32_t size, const std::vector<lvn_csg_op>& ops,
+                     const FieldEntry* cached) {
+    ChunkDesc cd;
+    fill_chunk_desc(cd, min, size, c->N);
+    if (cached) {
+        cd.mode = kModeCached;
+        cd.old_edges = cached->edges;
+        cd.old_ids = (uint64_t)(uintptr_t)cached->d_ids;
+        cd.old_data = (uint64_t)(uintptr_t)cached->d_data;
+    }
+    int r = upload_single(c, cd, ops);
+    if (r) return r;
+    KArgs a = c->kargs();
+    a.ysplit = 6;
+    if (cached) {
+        char* slot = c->d_arena;
+        LVN_HIP(hipMemcpyAsync(slot + c->L.mat, cached->d_mat, c->G.field_samples, hipMemcpyDeviceToDevice, c->stream));
+        LVN_HIP(hipMemcpyAsync(slot + c->L.orig, cached->d_mat, c->G.field_samples, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        launch_density(a, 1, c->stream);
+    }
+    if (!ops.empty()) launch_csg(a, 1, c->stream);
+    launch_edge_flags(a, 1, c->stream);
+    launch_scan(a, 1, kScanEdges, c->stream);
+    launch_edge_compact(a, 1, c->stream);
+    launch_edge_cross(a, 1, c->stream);
+    LVN_HIP(hipGetLastError());
+    return LVN_OK;
+}
+
+static int run_mesh(Ctx* c) {
+    KArgs a = c->kargs();
+    launch_voxel_flags(a, 1, c->stream);
+    launch_scan(a, 1, kScanVoxels, c->stream);
+    launch_voxel_compact(a, 1, c->stream);
+    launch_qef(a, 1, c->stream);
+    launch_quad_flags(a, 1, c->stream);
+    launch_scan(a, 1, kScanQuads, c->stream);
+    launch_quad_emit(a, 1, c->stream);
+    launch_vertex_seam(a, 1, c->stream);
+    launch_scan(a, 1, kScanSeams, c->stream);
+    launch_seam_emit(a, 1, c->stream);
+    LVN_HIP(hipGetLastError());
+    return LVN_OK;
+}
+
+static int read_counters(Ctx* c, Counters* out) {
+    LVN_HIP(hipMemcpyAsync(c->h_counters, c->d_counters, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+    LVN_HIP(hipStreamSynchronize(c->stream));
+    *out = c->h_counters[0];
+    return LVN_OK;
+}
+
+// Copy slot 0's materials + edge list into a cache entry (StoreDensityField,
+// src/compute_density_field.cpp:303-308).
+static int store_field(Ctx* c, const Key& key, int edges, int last_op) {
+    FieldEntry& f = c->fields[key];
+    if (!f.d_mat) LVN_HIP(hipMalloc(&f.d_mat, c->G.field_samples));
+    if (edges > f.cap) {
+        if (f.d_ids) LVN_HIP(hipFree(f.d_ids));
+        if (f.d_data) LVN_HIP(hipFree(f.d_data));
+        f.cap = std::max(edges, 1);
+        LVN_HIP(hipMalloc(&f.d_ids, sizeof(int) * f.cap));
+        LVN_HIP(hipMalloc(&f.d_data, 16 * (size_t)f.cap));
+    }
+    char* slot = c->d_arena;
+    LVN_HIP(hipMemcpyAsync(f.d_mat, slot + c->L.mat, c->G.field_samples, hipMemcpyDeviceToDevice, c->stream));
+    if (edges) {
+        LVN_HIP(hipMemcpyAsync(f.d_ids, slot + c->L.e_ids, sizeof(int) * edges, hipMemcpyDeviceToDevice, c->stream));
+        LVN_HIP(hipMemcpyAsync(f.d_data, slot + c->L.e_data, 16 * (size_t)edges, hipMemcpyDeviceToDevice, c->stream));
+    }
+    f.edges = edges;
+    f.last_op = last_op;
+    LVN_HIP(hipStreamSynchronize(c->stream));
+    return LVN_OK;
+}
+
+static std::vector<lvn_csg_op> stored_ops_for(const int32_t* min, int32_t size, int from) {
+    std::vector<lvn_csg_op> out;
+    for (size_t i = (size_t)std::max(from, 0); i < g_state.ops.size(); ++i)
+        if (aabb_overlaps(min, size, g_state.aabbs[i])) out.push_back(g_state.ops[i]);
+    return out;
+}
+
+// LoadDensityField (src/compute_density_field.cpp:235-274) into slot 0:
+// cached or fresh field, then the stored ops recorded since the field's last
+// update that overlap it; the field is cached only when ops were applied.
+static int load_field(Ctx* c, const int32_t* min, int32_t size, Counters* cnt) {
+    const Key key = make_key(min, size);
+    auto it = c->fields.find(key);
+    const FieldEntry* cached = it == c->fields.end() ? nullptr : &it->second;
+    const int from = cached ? cached->last_op : 0;
+    const std::vector<lvn_csg_op> ops = stored_ops_for(min, size, from);
+    int r = run_field(c, min, size, ops, cached);
+    if (r) return r;
+    r = read_counters(c, cnt);
+    if (r) return r;
+    if (cnt->status) return LVN_ERR_OVERFLOW;
+    if (!ops.empty()) {
+        r = store_field(c, key, cnt->edges, (int)g_state.ops.size());
+        if (r) return r;
+    }
+    return LVN_OK;
+}
+
+static int copy_out(Ctx* c, const Counters& k, MeshEntry* m) {
+    const int V = k.voxels, Q = k.quads, S = k.seams;
+    m->verts.resize(Q > 0 ? (size_t)V * 12 : 0);
+    m->tris.resize((size_t)Q * 6);
+    m->seams.resize((size_t)S * 12);
+    char* slot = c->d_arena;
+    if (!m->verts.empty())
+        LVN_HIP(hipMemcpyAsync(m->verts.data(), slot + c->L.verts, m->verts.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    if (!m->tris.empty())
+        LVN_HIP(hipMemcpyAsync(m->tris.data(), slot + c->L.tris, m->tris.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    if (!m->seams.empty())
+        LVN_HIP(hipMemcpyAsync(m->seams.data(), slot + c->L.seams, m->seams.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    LVN_HIP(hipStreamSynchronize(c->stream));
+    return LVN_OK;
+}
+
+}  // namespace lvn
+
+using namespace lvn;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int lvn_init(const lvn_field_desc* field, int32_t noise_seed, uint32_t default_material) {
+    if (default_material > 255) return LVN_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(g_state.mu);
+    g_state.desc = field ? *field : default_field();
+    if (g_state.desc.kind < 0 || g_state.desc.kind > 2 || g_state.desc.height_octaves < 0 ||
+        g_state.desc.cave_octaves < 0)
+        return LVN_ERR_INVALID;
+    g_state.fp = make_field_params(g_state.desc);
+    g_state.defmat = default_material;
+    if (!g_state.d_perm) {
+        LVN_HIP(hipMalloc(&g_state.d_perm, 65536));
+        LVN_HIP(hipMalloc(&g_state.d_grads, sizeof(float4) * 16));
+    }
+    g_state.perm.resize(65536);
+    build_perm_table(noise_seed, g_state.perm.data());
+    float4 grads[16];
+    build_grads(grads);
+    LVN_HIP(hipMemcpy(g_state.d_perm, g_state.perm.data(), 65536, hipMemcpyHostToDevice));
+    LVN_HIP(hipMemcpy(g_state.d_grads, grads, sizeof(grads), hipMemcpyHostToDevice));
+    g_state.seed = noise_seed;
+    g_state.init = true;
+    return LVN_OK;
+}
+
+int lvn_shutdown(void) {
+    std::lock_guard<std::mutex> lock(g_state.mu);
+    if (g_state.d_perm) (void)hipFree(g_state.d_perm);
+    if (g_state.d_grads) (void)hipFree(g_state.d_grads);
+    g_state.d_perm = nullptr;
+    g_state.d_grads = nullptr;
+    g_state.ops.clear();
+    g_state.aabbs.clear();
+    g_state.init = false;
+    return LVN_OK;
+}
+
+int lvn_set_noise_seed(int32_t seed) {
+    if (!g_state.init) return LVN_ERR_STATE;
+    build_perm_table(seed, g_state.perm.data());
+    LVN_HIP(hipMemcpy(g_state.d_perm, g_state.perm.data(), 65536, hipMemcpyHostToDevice));
+    g_state.seed = seed;
+    return LVN_OK;
+}
+
+int lvn_store_csg_op(const lvn_csg_op* op, const int32_t aabb_min[3], const int32_t aabb_max[3]) {
+    if (!op || !aabb_min || !aabb_max) return LVN_ERR_INVALID;
+    if (op->material < 0 || op->material > 255 || op->type < 0 || op->type > 1 || op->brush < 0 || op->brush > 1)
+        return LVN_ERR_INVALID;
+    g_state.ops.push_back(*op);
+    g_state.aabbs.push_back({aabb_min[0], aabb_min[1], aabb_min[2], aabb_max[0], aabb_max[1], aabb_max[2]});
+    return LVN_OK;
+}
+
+int lvn_clear_csg_ops(void) {
+    g_state.ops.clear();
+    g_state.aabbs.clear();
+    return LVN_OK;
+}
+
+int lvn_get_perm_table(uint8_t* out) {
+    if (!out) return LVN_ERR_INVALID;
+    if (!g_state.init) return LVN_ERR_STATE;
+    std::memcpy(out, g_state.perm.data(), 65536);
+    return LVN_OK;
+}
+
+int lvn_ctx_create(int32_t voxels_per_chunk, uint32_t flags, lvn_ctx** out) {
+    if (!out) return LVN_ERR_INVALID;
+    *out = nullptr;
+    if (!g_state.init) return LVN_ERR_STATE;
+    const int n = voxels_per_chunk;
+    if (n < 4 || n > 64 || (n & (n - 1))) return LVN_ERR_INVALID;
+    Ctx* c = new Ctx;
+    const int r = c->allocate(n, flags, 1, 0, 0);
+    if (r) {
+        delete c;
+        return r == LVN_ERR_DEVICE ? LVN_ERR_NOMEM : r;
+    }
+    *out = reinterpret_cast<lvn_ctx*>(c);
+    return LVN_OK;
+}
+
+void lvn_ctx_destroy(lvn_ctx* ctx) { delete reinterpret_cast<Ctx*>(ctx); }
+
+int lvn_ctx_voxels_per_chunk(const lvn_ctx* ctx) {
+    return ctx ? reinterpret_cast<const Ctx*>(ctx)->N : 0;
+}
+
+int lvn_free_chunk(lvn_ctx* ctx, const int32_t min[3], int32_t size) {
+    if (!ctx || !min) return LVN_ERR_INVALID;
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    c->meshes.erase(make_key(min, size));  // Compute_FreeChunkOctree erases the octree only
+    return LVN_OK;
+}
+
+int lvn_chunk_has_surface(lvn_ctx* ctx, const int32_t min[3], int32_t size, int32_t* out) {
+    if (!ctx || !min || !out) return LVN_ERR_INVALID;
+    if (!g_state.init) return LVN_ERR_STATE;
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    const Key key = make_key(min, size);
+    auto it = c->fields.find(key);
+    if (it != c->fields.end()) {
+        *out = it->second.edges > 0;
+        return LVN_OK;
+    }
+    // Compute_ChunkIsEmpty (src/compute_density_field.cpp:278-299): default field, no
+    // stored ops, and the field is cached (with no ops recorded as applied).
+    int r = run_field(c, min, size, {}, nullptr);
+    if (r) return r;
+    Counters k;
+    r = read_counters(c, &k);
+    if (r) return r;
+    if (k.status) return LVN_ERR_OVERFLOW;
+    r = store_field(c, key, k.edges, 0);
+    if (r) return r;
+    *out = k.edges > 0;
+    return LVN_OK;
+}
+
+int lvn_apply_csg(lvn_ctx* ctx, const lvn_csg_op* ops, int32_t n, const int32_t min[3], int32_t size) {
+    if (!ctx || !min || n < 0 || (n > 0 && !ops)) return LVN_ERR_INVALID;
+    if (!g_state.init) return LVN_ERR_STATE;
+    for (int i = 0; i < n; ++i)
+        if (ops[i].material < 0 || ops[i].material > 255 || ops[i].type < 0 || ops[i].type > 1 ||
+            ops[i].brush < 0 || ops[i].brush > 1)
+            return LVN_ERR_INVALID;
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    // Compute_ApplyCSGOperations (src/compute_csg.cpp:224-242): LoadDensityField, apply, store.
+    Counters k;
+    int r = load_field(c, min, size, &k);
+    if (r) return r;
+    const Key key = make_key(min, size);
+    r = store_field(c, key, k.edges, (int)g_state.ops.size());  // the field of the load step (slot 0)
+    if (r) return r;
+    if (n == 0) return LVN_OK;
+    std::vector<lvn_csg_op> batch(ops, ops + n);
+    r = run_field(c, min, size, batch, &c->fields[key]);
+    if (r) return r;
+    r = read_counters(c, &k);
+    if (r) return r;
+    if (k.status) return LVN_ERR_OVERFLOW;
+    return store_field(c, key, k.edges, (int)g_state.ops.size() + n);
+}
+
+int lvn_generate_chunk_mesh(lvn_ctx* ctx, const int32_t min[3], int32_t size, lvn_vertex* v, int32_t v_cap,
+                            int32_t* nv, int32_t* tri, int32_t tri_cap, int32_t* nt, lvn_seam_node* s,
+                            int32_t s_cap, int32_t* ns) {
+    if (!ctx || !min || !nv || !nt || !ns) return LVN_ERR_INVALID;
+    if (!g_state.init) return LVN_ERR_STATE;
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    *nv = *nt = *ns = 0;
+    const Key key = make_key(min, size);
+    auto it = c->meshes.find(key);
+    MeshEntry fresh;
+    const MeshEntry* m = nullptr;
+    if (it != c->meshes.end()) {
+        m = &it->second;  // LoadOctree cache hit (src/compute_octree.cpp:157-163)
+    } else {
+        Counters k;
+        int r = load_field(c, min, size, &k);
+        if (r) return r;
+        if (k.edges == 0) return LVN_OK;  // no octree, nothing cached (src/compute_octree.cpp:167-171)
+        r = run_mesh(c);
+        if (r) return r;
+        r = read_counters(c, &k);
+        if (r) return r;
+        if (k.status) return LVN_ERR_OVERFLOW;
+        r = copy_out(c, k, &fresh);
+        if (r) return r;
+        m = &(c->meshes[key] = std::move(fresh));
+    }
+    const int V = (int)(m->verts.size() / 12), T = (int)(m->tris.size() / 3), S = (int)(m->seams.size() / 12);
+    *nv = V;
+    *nt = T;
+    *ns = S;
+    if (V > v_cap || T > tri_cap || S > s_cap) return LVN_ERR_CAPACITY;
+    if (V && v) std::memcpy(v, m->verts.data(), m->verts.size() * 4);
+    if (T && tri) std::memcpy(tri, m->tris.data(), m->tris.size() * 4);
+    if (S && s) std::memcpy(s, m->seams.data(), m->seams.size() * 4);
+    return LVN_OK;
+}
+
+int lvn_debug_chunk_stages(lvn_ctx* ctx, const int32_t min[3], int32_t size, const lvn_csg_op* ops, int32_t nops,
+                           lvn_stage_dump* out) {
+    if (!ctx || !min || !out || nops < 0 || (nops > 0 && !ops)) return LVN_ERR_INVALID;
+    if (!g_state.init) return LVN_ERR_STATE;
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    std::vector<lvn_csg_op> batch(ops, ops + nops);
+    int r = run_field(c, min, size, batch, nullptr);
+    if (r) return r;
+    r = run_mesh(c);
+    if (r) return r;
+    Counters k;
+    r = read_counters(c, &k);
+    if (r) return r;
+    out->num_edges = k.edges;
+    out->num_voxels = k.voxels;
+    char* slot = c->d_arena;
+    const size_t fs = (size_t)c->G.field_samples;
+    if (out->density) {
+        if (c->flags & LVN_CTX_FP16_DENSITY) {
+            std::vector<__half> tmp(fs);
+            LVN_HIP(hipMemcpy(tmp.data(), slot + c->L.density, fs * 2, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < fs; ++i) out->density[i] = __half2float(tmp[i]);
+        } else {
+            LVN_HIP(hipMemcpy(out->density, slot + c->L.density, fs * 4, hipMemcpyDeviceToHost));
+        }
+    }
+    if (out->materials) LVN_HIP(hipMemcpy(out->materials, slot + c->L.mat, fs, hipMemcpyDeviceToHost));
+    const int ne = std::min(k.edges, out->edge_cap);
+    if (out->edge_ids && ne > 0) LVN_HIP(hipMemcpy(out->edge_ids, slot + c->L.e_ids, 4 * (size_t)ne, hipMemcpyDeviceToHost));
+    if (out->edge_data && ne > 0) LVN_HIP(hipMemcpy(out->edge_data, slot + c->L.e_data, 16 * (size_t)ne, hipMemcpyDeviceToHost));
+    const int nvx = std::min(k.voxels, out->voxel_cap);
+    if (nvx > 0) {
+        if (out->voxel_codes) LVN_HIP(hipMemcpy(out->voxel_codes, slot + c->L.v_code, 4 * (size_t)nvx, hipMemcpyDeviceToHost));
+        if (out->voxel_edge_masks) LVN_HIP(hipMemcpy(out->voxel_edge_masks, slot + c->L.v_emask, 4 * (size_t)nvx, hipMemcpyDeviceToHost));
+        if (out->voxel_materials) LVN_HIP(hipMemcpy(out->voxel_materials, slot + c->L.v_mat, 4 * (size_t)nvx, hipMemcpyDeviceToHost));
+        if (out->voxel_positions) LVN_HIP(hipMemcpy(out->voxel_positions, slot + c->L.v_pos, 16 * (size_t)nvx, hipMemcpyDeviceToHost));
+        if (out->voxel_normals) LVN_HIP(hipMemcpy(out->voxel_normals, slot + c->L.v_norm, 16 * (size_t)nvx, hipMemcpyDeviceToHost));
+    }
+    if (k.status) return LVN_ERR_OVERFLOW;
+    if (k.edges > out->edge_cap || k.voxels > out->voxel_cap) return LVN_ERR_CAPACITY;
+    return LVN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+// lvn_shared.h — POD types shared by the host runtime and the gfx950 kernels.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/lvn_mesher.h"
+
+namespace lvn {
+
+constexpr int kAir = LVN_MATERIAL_AIR;    // src/volume_materials.h:9
+constexpr int kNone = LVN_MATERIAL_NONE;  // src/volume_materials.h:8
+constexpr int kLeafSizeScale = 4;         // src/volume_constants.h:7-8
+constexpr int kClipmapLeafSize = 256;     // src/volume_constants.h:14-15
+
+// Device copy of lvn_field_desc plus the cave-term cut (see combine()).
+struct FieldParams {
+    int kind;
+    int hoct;
+    float hfreq, hlac, hgain, hamp;
+    int coct;
+    float cfreq, clac, cgain, cthr;
+    float cave_cut;
+};
+
+// Device form of one CSG operation: cos/sin of rotate_y precomputed on the
+// host, and the op's bounding radius for work culling.
+struct DevOp {
+    int type, brush, material, pad;
+    float ox, oy, oz, radius;
+    float dx, dy, dz, pad2;
+    float c, s, pad3, pad4;
+};
+
+// Geometry of a chunk (src/compute.cpp:245-278): N voxels per side,
+// H = N + 1 hermite points, F = N + 2 field samples, index shift S = log2(N) + 1.
+struct Geometry {
+    int N, H, F, shift, depth;
+    int field_samples;   // F^3
+    int edge_slots;      // 3 H^3
+    int edge_segs;       // ceil(edge_slots / 64)
+    int voxels;          // N^3
+    int voxel_segs;      // N^3 / 64 (>= 1)
+};
+
+// Per-slot capacities of the compacted arrays.
+struct Caps {
+    int edges;     // E
+    int voxels;    // V
+    int quads;     // Q (<= 3 V)
+    int seams;     // <= V
+    int edge_segs_q;   // ceil(3 V / 64)
+    int seam_segs;     // ceil(V / 64)
+};
+
+// Byte offsets of every array inside one chunk slot of the arena.
+struct SlotLayout {
+    uint64_t slot_bytes;
+    uint64_t density, mat, orig;
+    uint64_t e_mask, e_count, e_base, e_ids, e_data;
+    uint64_t v_mask, v_count, v_base, v_code, v_emask, v_mat, v_pos, v_norm;
+    uint64_t q_mask, q_count, q_base, tris;
+    uint64_t s_mask, s_count, s_base, seams, verts;
+};
+
+// Per-chunk request descriptor.
+struct ChunkDesc {
+    int min[3];
+    int size;
+    int off[3];      // min / 4 (C truncation), voxel units
+    int scale;       // size / (N * 4)
+    int op_start;    // into the op-index array
+    int op_count;
+    int mode;        // kModeFresh / kModeCached
+    int old_edges;   // cached mode: number of edges in the old list
+    float colour[4];
+    uint64_t old_ids;   // cached mode: device pointer to the old sorted edge ids
+    uint64_t old_data;  // cached mode: device pointer to the old edge data (float4)
+};
+
+constexpr int kModeFresh = 0;   // untouched edges take density crossings
+constexpr int kModeCached = 1;  // untouched edges keep their data from the old list
+
+// Per-chunk counters written by the kernels.
+struct Counters {
+    int edges, voxels, quads, seams;
+    int status;      // 0 ok, else LVN_ERR_OVERFLOW bits
+    int pad[3];
+};
+
+constexpr int kStatusEdgeOverflow = 1;
+constexpr int kStatusVoxelOverflow = 2;
+constexpr int kStatusQuadOverflow = 4;
+constexpr int kStatusSeamOverflow = 8;
+constexpr int kStatusLookupMiss = 16;
+
+}  // namespace lvn

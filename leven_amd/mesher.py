@@ -1,0 +1,201 @@
+"""Python binding of the C ABI in include/lvn_mesher.h (leven_amd/liblvn_mesher.so).
+
+Mirrors the reference's compute interface (src/compute.h:35-77):
+``init`` = Compute_Initialise, ``Context`` = Compute_MeshGenContext.  There is no
+CPU fallback: a missing or unloadable library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import (CSG_OP_DTYPE, ERROR_NAMES, LVN_ERR_CAPACITY, LVN_OK, SEAM_DTYPE, VERTEX_DTYPE, FieldDesc,
+                  StageDump, ptr, vptr)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblvn_mesher.so")
+
+# every symbol declared in include/lvn_mesher.h
+EXPORTS = [
+    "lvn_init", "lvn_shutdown", "lvn_set_noise_seed", "lvn_store_csg_op", "lvn_clear_csg_ops",
+    "lvn_get_perm_table", "lvn_ctx_create", "lvn_ctx_destroy", "lvn_ctx_voxels_per_chunk", "lvn_apply_csg",
+    "lvn_free_chunk", "lvn_chunk_has_surface", "lvn_generate_chunk_mesh", "lvn_debug_chunk_stages",
+]
+
+_lib = None
+
+
+class LvnError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what} failed: {ERROR_NAMES.get(code, code)}")
+        self.code = code
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m leven_amd.build` (no CPU fallback exists)")
+        L = C.CDLL(LIB_PATH)
+        i32, u32 = C.c_int32, C.c_uint32
+        vp = C.c_void_p
+        sig = {
+            "lvn_init": (i32, [C.POINTER(FieldDesc), i32, u32]),
+            "lvn_shutdown": (i32, []),
+            "lvn_set_noise_seed": (i32, [i32]),
+            "lvn_store_csg_op": (i32, [vp, vp, vp]),
+            "lvn_clear_csg_ops": (i32, []),
+            "lvn_get_perm_table": (i32, [vp]),
+            "lvn_ctx_create": (i32, [i32, u32, C.POINTER(vp)]),
+            "lvn_ctx_destroy": (None, [vp]),
+            "lvn_ctx_voxels_per_chunk": (i32, [vp]),
+            "lvn_apply_csg": (i32, [vp, vp, i32, vp, i32]),
+            "lvn_free_chunk": (i32, [vp, vp, i32]),
+            "lvn_chunk_has_surface": (i32, [vp, vp, i32, C.POINTER(i32)]),
+            "lvn_generate_chunk_mesh": (i32, [vp, vp, i32, vp, i32, C.POINTER(i32), vp, i32, C.POINTER(i32), vp,
+                                              i32, C.POINTER(i32)]),
+            "lvn_debug_chunk_stages": (i32, [vp, vp, i32, vp, i32, C.POINTER(StageDump)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code, what):
+    if code != LVN_OK:
+        raise LvnError(code, what)
+    return code
+
+
+def _i3(v):
+    return np.ascontiguousarray(v, dtype=np.int32).reshape(3)
+
+
+def init(field=None, seed=0, default_material=0):
+    """Compute_Initialise(noiseSeed, defaultMaterial, ...) with a field descriptor."""
+    check(lib().lvn_init(C.byref(field) if field is not None else None, int(seed), int(default_material)),
+          "lvn_init")
+
+
+def shutdown():
+    check(lib().lvn_shutdown(), "lvn_shutdown")
+
+
+def set_noise_seed(seed):
+    check(lib().lvn_set_noise_seed(int(seed)), "lvn_set_noise_seed")
+
+
+def store_csg_op(op, aabb_min, aabb_max):
+    o = np.ascontiguousarray(op, dtype=CSG_OP_DTYPE).reshape(1)
+    check(lib().lvn_store_csg_op(vptr(o), vptr(_i3(aabb_min)), vptr(_i3(aabb_max))), "lvn_store_csg_op")
+
+
+def clear_csg_ops():
+    check(lib().lvn_clear_csg_ops(), "lvn_clear_csg_ops")
+
+
+def perm_table():
+    out = np.zeros(65536, dtype=np.uint8)
+    check(lib().lvn_get_perm_table(vptr(out)), "lvn_get_perm_table")
+    return out
+
+
+class Context:
+    """Compute_MeshGenContext over the C ABI."""
+
+    def __init__(self, voxels_per_chunk=64, flags=0):
+        h = C.c_void_p()
+        check(lib().lvn_ctx_create(int(voxels_per_chunk), int(flags), C.byref(h)), "lvn_ctx_create")
+        self._h = h
+        self.n = voxels_per_chunk
+        self.f = voxels_per_chunk + 2
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lvn_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def voxels_per_chunk(self):
+        return lib().lvn_ctx_voxels_per_chunk(self._h)
+
+    def apply_csg(self, ops, min_xyz, size):
+        o = np.ascontiguousarray(ops, dtype=CSG_OP_DTYPE)
+        check(lib().lvn_apply_csg(self._h, vptr(o), len(o), vptr(_i3(min_xyz)), int(size)), "lvn_apply_csg")
+
+    def free_chunk(self, min_xyz, size):
+        check(lib().lvn_free_chunk(self._h, vptr(_i3(min_xyz)), int(size)), "lvn_free_chunk")
+
+    def has_surface(self, min_xyz, size):
+        out = C.c_int32()
+        check(lib().lvn_chunk_has_surface(self._h, vptr(_i3(min_xyz)), int(size), C.byref(out)),
+              "lvn_chunk_has_surface")
+        return bool(out.value)
+
+    def generate_chunk_mesh(self, min_xyz, size, v_cap=None, tri_cap=None, s_cap=None):
+        """Returns (vertices[VERTEX_DTYPE], triangles int32[T,3], seams[SEAM_DTYPE])."""
+        n3 = self.n ** 3
+        v_cap = n3 if v_cap is None else v_cap
+        tri_cap = 6 * n3 if tri_cap is None else tri_cap
+        s_cap = n3 if s_cap is None else s_cap
+        v = np.zeros(v_cap, dtype=VERTEX_DTYPE)
+        t = np.zeros((tri_cap, 3), dtype=np.int32)
+        s = np.zeros(s_cap, dtype=SEAM_DTYPE)
+        nv, nt, ns = C.c_int32(), C.c_int32(), C.c_int32()
+        rc = lib().lvn_generate_chunk_mesh(self._h, vptr(_i3(min_xyz)), int(size), vptr(v), v_cap, C.byref(nv),
+                                           vptr(t), tri_cap, C.byref(nt), vptr(s), s_cap, C.byref(ns))
+        if rc == LVN_ERR_CAPACITY:
+            raise LvnError(rc, f"lvn_generate_chunk_mesh (needs {nv.value} vertices, {nt.value} triangles, "
+                               f"{ns.value} seam nodes)")
+        check(rc, "lvn_generate_chunk_mesh")
+        return v[:nv.value].copy(), t[:nt.value].copy(), s[:ns.value].copy()
+
+    def debug_stages(self, min_xyz, size, ops=None, edge_cap=None, voxel_cap=None):
+        """Every stage of one fresh chunk (density ... vertex positions)."""
+        f3 = self.f ** 3
+        h = self.n + 1
+        edge_cap = 3 * h ** 3 if edge_cap is None else edge_cap
+        voxel_cap = self.n ** 3 if voxel_cap is None else voxel_cap
+        bufs = dict(
+            density=np.zeros(f3, dtype=np.float32),
+            materials=np.zeros(f3, dtype=np.uint8),
+            edge_ids=np.zeros(edge_cap, dtype=np.int32),
+            edge_data=np.zeros((edge_cap, 4), dtype=np.float32),
+            voxel_codes=np.zeros(voxel_cap, dtype=np.int32),
+            voxel_edge_masks=np.zeros(voxel_cap, dtype=np.int32),
+            voxel_materials=np.zeros(voxel_cap, dtype=np.int32),
+            voxel_positions=np.zeros((voxel_cap, 4), dtype=np.float32),
+            voxel_normals=np.zeros((voxel_cap, 4), dtype=np.float32),
+        )
+        d = StageDump()
+        d.density = ptr(bufs["density"], C.c_float)
+        d.materials = ptr(bufs["materials"], C.c_uint8)
+        d.edge_ids = ptr(bufs["edge_ids"], C.c_int32)
+        d.edge_data = ptr(bufs["edge_data"], C.c_float)
+        d.edge_cap = edge_cap
+        d.voxel_codes = ptr(bufs["voxel_codes"], C.c_int32)
+        d.voxel_edge_masks = ptr(bufs["voxel_edge_masks"], C.c_int32)
+        d.voxel_materials = ptr(bufs["voxel_materials"], C.c_int32)
+        d.voxel_positions = ptr(bufs["voxel_positions"], C.c_float)
+        d.voxel_normals = ptr(bufs["voxel_normals"], C.c_float)
+        d.voxel_cap = voxel_cap
+        o = None if ops is None or len(ops) == 0 else np.ascontiguousarray(ops, dtype=CSG_OP_DTYPE)
+        check(lib().lvn_debug_chunk_stages(self._h, vptr(_i3(min_xyz)), int(size), vptr(o),
+                                           0 if o is None else len(o), C.byref(d)), "lvn_debug_chunk_stages")
+        e, v = d.num_edges, d.num_voxels
+        return dict(
+            density=bufs["density"], materials=bufs["materials"],
+            edge_ids=bufs["edge_ids"][:e].copy(), edge_data=bufs["edge_data"][:e].copy(),
+            voxel_codes=bufs["voxel_codes"][:v].copy(), voxel_edge_masks=bufs["voxel_edge_masks"][:v].copy(),
+            voxel_materials=bufs["voxel_materials"][:v].copy(),
+            voxel_positions=bufs["voxel_positions"][:v].copy(), voxel_normals=bufs["voxel_normals"][:v].copy(),
+        )

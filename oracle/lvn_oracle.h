@@ -1,0 +1,112 @@
+/*
+ * lvn_oracle.h — CPU oracle of the dual-contouring chunk mesher.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a scalar C++ restatement of leven's
+ * OpenCL chunk-generation pipeline (leven/cl/ and leven/src/compute_*.cpp),
+ * used as the parity checker by tests/, by __graft_entry__.smoke() and by
+ * bench.py's cpu_baseline leg.  The product (leven_amd/liblvn_mesher.so) never
+ * links, loads or calls it.
+ *
+ * Arithmetic contract (shared with the HIP kernels, which are written
+ * independently): IEEE FP32, round-to-nearest, no contraction
+ * (-ffp-contract=off), correctly rounded sqrt and division, the operation
+ * order given in each function's comment.  Unsuffixed double constants of the
+ * reference (0.5, 0.6, 2.0 in cl/simplex.cl, cl/qef.cl) are restated as float
+ * constants.  OpenCL builtins are restated as:
+ *   dot(a,b)     = ((a.x*b.x + a.y*b.y) + a.z*b.z) [+ a.w*b.w]
+ *   length(v)    = sqrt(dot(v,v))
+ *   normalize(v) = v == 0 ? v : v / length(v)      (per component division)
+ *   mix(a,b,t)   = a + (b - a) * t
+ *   rsqrt(x)     = 1 / sqrt(x)
+ *   pow(f, -1)   = 1 / f
+ *
+ * Parity pinning of each function is listed in oracle/README.md.
+ */
+#ifndef LVN_ORACLE_H
+#define LVN_ORACLE_H
+
+#include <stdint.h>
+#include "../include/lvn_mesher.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A1: 256x256 permutation table (row = texel y), seeded. */
+void  orc_perm_table(int32_t seed, uint8_t* out65536);
+/* The reference's base array perm[512] (src/compute_density_field.cpp:28-52) as initialised. */
+void  orc_base_perm512(int32_t* out512);
+/* The shuffled copy used to fill the table. */
+void  orc_shuffled_perm512(int32_t seed, int32_t* out512);
+uint32_t orc_noise_hash(int32_t x, int32_t y, int32_t seed);
+uint32_t orc_oat_hash(const uint8_t* bytes, int32_t n);
+void     orc_splitmix64(uint64_t state, int32_t n, uint64_t* out);
+/* The three gradient component values a UNORM8 texel decodes to (byte 0, 64, 128). */
+void  orc_grad_values(float* out3);
+
+/* A3 / A4 primitives. */
+float orc_snoise2(const uint8_t* perm, float x, float y);
+float orc_snoise3(const uint8_t* perm, float x, float y, float z);
+float orc_fbm2(const uint8_t* perm, int32_t octaves, float frequency, float lacunarity, float gain, float x, float y);
+float orc_fbm3(const uint8_t* perm, int32_t octaves, float frequency, float lacunarity, float gain,
+               float x, float y, float z);
+float orc_density(const lvn_field_desc* desc, const uint8_t* perm, float x, float y, float z);
+double orc_density_f64(const lvn_field_desc* desc, const uint8_t* perm, double x, double y, double z);
+float orc_brush_density(const lvn_csg_op* op, float x, float y, float z);
+
+/* Octree codes (cl/octree.cl:34-72). */
+uint32_t orc_code_for_position(int32_t x, int32_t y, int32_t z, int32_t node_depth, int32_t max_depth);
+void     orc_position_for_code(uint32_t code, int32_t max_depth, int32_t* xyz);
+
+/* A11: QEF solve of explicit hermite data (positions/normals 4 floats each). */
+void orc_qef_solve_points_f32(const float* positions, const float* normals, int32_t count, float* out4);
+void orc_qef_solve_points_f64(const double* positions, const double* normals, int32_t count, double* out4);
+
+/* ---- one chunk ---------------------------------------------------------- */
+typedef struct orc_chunk orc_chunk;
+
+orc_chunk* orc_chunk_new(int32_t voxels_per_chunk);
+void       orc_chunk_free(orc_chunk* c);
+
+/* Default field for (min, size) followed by one CSG batch (ops as given, already
+ * filtered by the caller).  Replaces GenerateDefaultDensityField + FindDefaultEdges
+ * + ApplyCSGOperations (src/compute_density_field.cpp:138-231, src/compute_csg.cpp:11-220). */
+int orc_chunk_generate(orc_chunk* c, const lvn_field_desc* desc, const uint8_t* perm,
+                       const int32_t min[3], int32_t size, uint32_t default_material,
+                       const lvn_csg_op* ops, int32_t nops);
+/* Same, but the density samples (F^3) are supplied instead of evaluated. */
+int orc_chunk_generate_from_density(orc_chunk* c, const lvn_field_desc* desc, const uint8_t* perm,
+                                    const int32_t min[3], int32_t size, uint32_t default_material,
+                                    const float* density);
+/* Another CSG batch on the current field (Compute_ApplyCSGOperations). */
+int orc_chunk_apply_csg(orc_chunk* c, const lvn_csg_op* ops, int32_t nops);
+/* Octree + mesh (ConstructOctreeFromField, GenerateMeshFromOctree, GatherSeamNodesFromOctree). */
+int orc_chunk_build(orc_chunk* c);
+
+/* counts[0..5] = edges, voxels, quads, exported vertices, exported triangles, seam nodes */
+void orc_chunk_counts(const orc_chunk* c, int32_t* counts);
+void orc_chunk_get_density(const orc_chunk* c, float* out);
+void orc_chunk_get_materials(const orc_chunk* c, uint8_t* out);
+void orc_chunk_get_edges(const orc_chunk* c, int32_t* ids, float* data4);
+void orc_chunk_get_voxels(const orc_chunk* c, int32_t* codes, int32_t* masks, int32_t* materials);
+void orc_chunk_get_positions(const orc_chunk* c, float* pos4, float* normal4);
+void orc_chunk_get_positions_f64(const orc_chunk* c, double* pos4);
+void orc_chunk_get_triangles(const orc_chunk* c, int32_t* tri3);
+void orc_chunk_get_vertices(const orc_chunk* c, lvn_vertex* v);
+void orc_chunk_get_seams(const orc_chunk* c, lvn_seam_node* s);
+
+/* ---- many chunks, timed (CPU baseline) ------------------------------------ */
+/* Meshes n chunks, one chunk per thread on `threads` threads.  Stored ops are filtered per
+ * chunk by AABB overlap like LoadDensityField (src/compute_density_field.cpp:255-263);
+ * op_aabbs holds 6 ints per op (min xyz, max xyz).  seconds[i] = wall time of chunk i.
+ * totals[0..3] += edges, voxels, triangles, non-empty chunks. Returns wall seconds. */
+double orc_mesh_chunks(const lvn_field_desc* desc, const uint8_t* perm, int32_t voxels_per_chunk,
+                       int32_t n, const int32_t* mins, const int32_t* sizes, uint32_t default_material,
+                       const lvn_csg_op* ops, int32_t nops, const int32_t* op_aabbs,
+                       int32_t threads, double* seconds, int64_t* totals);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,50 @@
+"""The C-ABI library loads and exports every symbol include/lvn_mesher.h declares (CPU only)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from leven_amd import abi, mesher
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "lvn_mesher.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lvn_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declarations_are_bound():
+    assert declared_functions() == sorted(mesher.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(mesher.LIB_PATH):
+        pytest.skip("library not built (python -m leven_amd.build)")
+    L = C.CDLL(mesher.LIB_PATH)
+    for name in declared_functions():
+        assert hasattr(L, name), name
+
+
+def test_struct_layouts_match_reference_sizes():
+    # CSGOperationInfo, MeshVertex, SeamNodeInfo are 48 bytes in the reference
+    assert C.sizeof(abi.CsgOp) == 48
+    assert C.sizeof(abi.Vertex) == 48
+    assert C.sizeof(abi.SeamNode) == 48
+    assert C.sizeof(abi.FieldDesc) == 48
+
+
+def test_calls_before_init_fail_cleanly():
+    if not os.path.exists(mesher.LIB_PATH):
+        pytest.skip("library not built")
+    L = mesher.lib()
+    h = C.c_void_p()
+    # no GPU work: the state check comes first
+    if L.lvn_get_perm_table(None) != abi.LVN_ERR_INVALID:
+        pytest.fail("null output must be rejected")
+    state = L.lvn_ctx_create(64, 0, C.byref(h))
+    assert state in (abi.LVN_ERR_STATE, abi.LVN_OK)
+    if state == abi.LVN_OK:
+        L.lvn_ctx_destroy(h)

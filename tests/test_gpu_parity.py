@@ -1,0 +1,214 @@
+"""Parity of the HIP pipeline with the CPU oracle, through the C ABI (needs a gfx950 GPU).
+
+Bar: bit-exact density, materials, edge list, edge data, active-voxel set, codes,
+masks, materials and triangle index buffers; vertex positions within 1e-4 voxel
+(they are bit-exact too when both sides follow the same FP32 operation order).
+"""
+import numpy as np
+import pytest
+
+from leven_amd import abi, mesher, workloads
+from leven_amd.abi import CTX_FP16_DENSITY, MATERIAL_AIR
+
+pytestmark = pytest.mark.gpu
+
+VOXEL_TOL = 1e-4  # voxel units
+
+
+def setup_field(desc, seed, default_material=0):
+    mesher.init(desc, seed, default_material)
+    mesher.clear_csg_ops()
+
+
+def oracle_chunk(oracle, desc, seed, n, mn, size, ops=None, default_material=0):
+    perm = oracle.perm_table(seed)
+    return oracle.Chunk(n).generate(desc, perm, mn, size, default_material=default_material, ops=ops).build()
+
+
+def same_bits(a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return a.shape == b.shape and (a.view(np.uint32) == b.view(np.uint32)).all()
+
+
+def compare_stages(g, c, n, size):
+    assert same_bits(g["density"], c.density()), "density differs"
+    assert (g["materials"] == c.materials()).all(), "materials differ"
+    ids, data = c.edges()
+    assert len(g["edge_ids"]) == len(ids) and (g["edge_ids"] == ids).all(), "edge list differs"
+    assert same_bits(g["edge_data"], data), "edge crossing/normal data differs"
+    codes, masks, mats = c.voxels()
+    assert (g["voxel_codes"] == codes).all() and (g["voxel_edge_masks"] == masks).all()
+    assert (g["voxel_materials"] == mats).all()
+    pos, nrm = c.positions()
+    voxel = 4.0 * (size // (n * 4))  # world units per voxel at this LOD
+    err = np.abs(g["voxel_positions"][:, :3] - pos[:, :3]).max() / voxel if len(pos) else 0.0
+    assert err <= VOXEL_TOL, f"vertex error {err} voxel"
+    assert same_bits(g["voxel_positions"], pos), "vertex positions not bit-exact"
+    assert same_bits(g["voxel_normals"], nrm)
+
+
+CASES = [
+    # name, desc, seed, n, min, size
+    ("heightfield_n16", workloads.heightfield_desc(), 1, 16, (64, -32, 128), 64),
+    ("heightfield_lod2_n16", workloads.heightfield_desc(), 1, 16, (0, -128, 0), 256),
+    ("caves_n32", workloads.terrain_desc(), 2, 32, (128, -256, 0), 128),
+    ("leven_terrain_n16", abi.field_desc(kind=abi.FIELD_LEVEN_TERRAIN), 93923590, 16, (0, 2320, 0), 64),
+    ("plain_cpu_n64", workloads.heightfield_desc(), 1, 64, (0, -128, 0), 256),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_stages_bit_exact(oracle, case):
+    name, desc, seed, n, mn, size = case
+    setup_field(desc, seed)
+    assert (mesher.perm_table() == oracle.perm_table(seed)).all()
+    ctx = mesher.Context(n)
+    g = ctx.debug_stages(mn, size)
+    c = oracle_chunk(oracle, desc, seed, n, mn, size)
+    assert c.counts()["edges"] > 0, "case must have a surface"
+    compare_stages(g, c, n, size)
+
+
+def test_caves_full_chunk_bit_exact(oracle):
+    """A 64^3 underground chunk of the 3D terrain (cave term everywhere)."""
+    desc = workloads.terrain_desc()
+    setup_field(desc, 2)
+    g = mesher.Context(64).debug_stages((256, -512, 0), 256)
+    c = oracle_chunk(oracle, desc, 2, 64, (256, -512, 0), 256)
+    assert c.counts()["voxels"] > 1000
+    compare_stages(g, c, 64, 256)
+
+
+def test_topology_given_same_density(oracle):
+    """Feed the GPU's density field to the oracle: the edge list, active set and
+    triangles must be identical."""
+    desc = workloads.terrain_desc()
+    setup_field(desc, 2)
+    n, mn, size = 32, (0, -128, 128), 128
+    ctx = mesher.Context(n)
+    g = ctx.debug_stages(mn, size)
+    c = oracle.Chunk(n).generate_from_density(desc, oracle.perm_table(2), mn, size, g["density"]).build()
+    ids, _ = c.edges()
+    assert (g["edge_ids"] == ids).all()
+    codes, masks, mats = c.voxels()
+    assert (g["voxel_codes"] == codes).all() and (g["voxel_materials"] == mats).all()
+    v, t, s = ctx.generate_chunk_mesh(mn, size)
+    assert (t == c.triangles()).all()
+
+
+def _ops_for_chunk(w, mn, size):
+    keep = [i for i, bb in enumerate(w["aabbs"])
+            if not (mn[0] + size < bb[0] or mn[1] + size < bb[1] or mn[2] + size < bb[2]
+                    or mn[0] > bb[3] or mn[1] > bb[4] or mn[2] > bb[5])]
+    return w["ops"][keep]
+
+
+def test_csg_fresh_chunk_bit_exact(oracle):
+    """terrain_512 chunks with the brushes that overlap them (LoadDensityField's filter)."""
+    w = workloads.terrain_512()
+    setup_field(w["desc"], w["seed"])
+    ctx = mesher.Context(64)
+    tested = 0
+    for mn in [(256, -256, 256), (768, 0, 512), (0, -256, 1024), (1280, 0, 1536)]:
+        ops = _ops_for_chunk(w, mn, 256)
+        if len(ops) == 0:
+            continue
+        g = ctx.debug_stages(mn, 256, ops)
+        c = oracle_chunk(oracle, w["desc"], w["seed"], 64, mn, 256, ops=ops)
+        compare_stages(g, c, 64, 256)
+        tested += 1
+    assert tested >= 2
+
+
+def test_generate_chunk_mesh_matches_oracle(oracle):
+    w = workloads.plain_cpu()
+    setup_field(w["desc"], w["seed"])
+    ctx = mesher.Context(64)
+    mn, size = tuple(w["mins"][0]), int(w["sizes"][0])
+    v, t, s = ctx.generate_chunk_mesh(mn, size)
+    c = oracle_chunk(oracle, w["desc"], w["seed"], 64, mn, size)
+    assert len(v) == c.counts()["vertices"] and len(t) == c.counts()["triangles"]
+    assert (t == c.triangles()).all()
+    assert (v.view(np.uint32) == c.vertices().view(np.uint32)).all()
+    assert (s.view(np.uint32) == c.seams().view(np.uint32)).all()
+    # cached second call returns the same
+    v2, t2, s2 = ctx.generate_chunk_mesh(mn, size)
+    assert (t2 == t).all() and (v2.view(np.uint32) == v.view(np.uint32)).all()
+
+
+def test_capacity_error_reports_counts():
+    w = workloads.plain_cpu()
+    setup_field(w["desc"], w["seed"])
+    ctx = mesher.Context(64)
+    with pytest.raises(mesher.LvnError) as e:
+        ctx.generate_chunk_mesh(tuple(w["mins"][0]), 256, v_cap=100, tri_cap=100, s_cap=10)
+    assert e.value.code == abi.LVN_ERR_CAPACITY
+
+
+def test_empty_chunks():
+    setup_field(workloads.heightfield_desc(), 1)
+    ctx = mesher.Context(16)
+    for mn in [(0, 4096, 0), (0, -4096, 0)]:  # all air, all solid
+        v, t, s = ctx.generate_chunk_mesh(mn, 64)
+        assert len(v) == len(t) == len(s) == 0
+        assert not ctx.has_surface(mn, 64)
+
+
+def test_stored_ops_and_cache_semantics(oracle):
+    """Stored ops are applied when a chunk is first generated; a cached chunk ignores
+    ops stored later until freeChunkOctree; applyCSGOperations edits a cached field."""
+    w = workloads.terrain_512()
+    desc, seed = w["desc"], w["seed"]
+    setup_field(desc, seed)
+    perm = oracle.perm_table(seed)
+    ctx = mesher.Context(64)
+    mn, size = (256, -256, 256), 256
+    ops = _ops_for_chunk(w, mn, size)
+    assert len(ops) >= 2
+    first, second = ops[:1], ops[1:2]
+    bb = workloads.op_aabb(first[0])
+    mesher.store_csg_op(first[0], bb[:3], bb[3:])
+    _, t1, _ = ctx.generate_chunk_mesh(mn, size)
+    c1 = oracle.Chunk(64).generate(desc, perm, mn, size, ops=first).build()
+    assert (t1 == c1.triangles()).all()
+    # a later stored op does not change the cached chunk ...
+    bb2 = workloads.op_aabb(second[0])
+    mesher.store_csg_op(second[0], bb2[:3], bb2[3:])
+    _, t2, _ = ctx.generate_chunk_mesh(mn, size)
+    assert (t2 == t1).all()
+    # ... until it is freed: the cached field then takes the new op as its own batch
+    ctx.free_chunk(mn, size)
+    v3, t3, _ = ctx.generate_chunk_mesh(mn, size)
+    c3 = oracle.Chunk(64).generate(desc, perm, mn, size, ops=first).apply_csg(second).build()
+    assert (t3 == c3.triangles()).all()
+    assert (v3.view(np.uint32) == c3.vertices().view(np.uint32)).all()
+    # direct edit (Compute_ApplyCSGOperations) on the cached field
+    extra = workloads.make_ops(3, seed=77, extent=64, y_range=(-60, -40))[0]
+    extra["origin"][:, 0] += 64
+    extra["origin"][:, 2] += 64
+    ctx.apply_csg(extra, mn, size)
+    ctx.free_chunk(mn, size)
+    v4, t4, _ = ctx.generate_chunk_mesh(mn, size)
+    c4 = oracle.Chunk(64).generate(desc, perm, mn, size, ops=first).apply_csg(second).apply_csg(extra).build()
+    assert (t4 == c4.triangles()).all()
+    assert (v4.view(np.uint32) == c4.vertices().view(np.uint32)).all()
+    mesher.clear_csg_ops()
+
+
+def test_fp16_density_keeps_topology(oracle):
+    """FP16 storage: sign-preserving rounding keeps materials, edges and triangles equal
+    to the FP32 form; density within half precision."""
+    desc = workloads.terrain_desc()
+    setup_field(desc, 2)
+    n, mn, size = 32, (0, -128, 128), 128
+    g32 = mesher.Context(n).debug_stages(mn, size)
+    g16 = mesher.Context(n, flags=CTX_FP16_DENSITY).debug_stages(mn, size)
+    assert (g16["materials"] == g32["materials"]).all()
+    assert (g16["edge_ids"] == g32["edge_ids"]).all()
+    rel = np.abs(g16["density"] - g32["density"]) / np.maximum(np.abs(g32["density"]), 1e-3)
+    assert (rel[np.abs(g32["density"]) > 1e-3] <= 2.0 ** -11).all()
+    neg = g32["density"] < 0
+    assert (g16["density"][neg] < 0).all()
+    # crossings are re-evaluated in FP32, so the hermite data stays bit-exact
+    assert same_bits(g16["edge_data"], g32["edge_data"])
