@@ -1,0 +1,159 @@
+"""Benchmark: 64^3 chunks meshed per second on one MI355X, batched vs plain.
+
+    python bench.py [--steps K] [--warmup W] [--workload terrain_512]
+
+A step is one batched pass of the whole pipeline over every chunk of the
+workload (default: the 512-chunk 3D terrain with 256 CSG brushes), inputs staged
+in HBM before the timed region.  The plain per-chunk form (one chunk per launch
+sequence, synchronised per chunk) is timed over the same chunks; the CPU
+baseline is the oracle on the host's cores.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from leven_amd import mesher, workloads  # noqa: E402
+
+METRIC = "64^3 chunks meshed/sec (batched vs plain, same GPU) + p99 chunk latency"
+PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 (packed) vector peak
+PEAK_HBM_GBS = 8000.0
+
+# FP32 operations per evaluation, counted from the kernel source (add/sub/mul/div/sqrt = 1)
+FLOPS_SNOISE2 = 49
+FLOPS_SNOISE3 = 90
+
+
+def flops_height(desc):
+    return 3 + (FLOPS_SNOISE2 + 5) * desc.height_octaves
+
+
+def flops_cave(desc):
+    return 5 + (FLOPS_SNOISE3 + 6) * desc.cave_octaves
+
+
+def stage_flops(w, desc):
+    """Algorithmic FLOPs of the density and edge-crossing stages from the kernels' work counters."""
+    fh, fc = flops_height(desc), flops_cave(desc)
+    density = w["density_height"] * fh + w["density_cave"] * fc + w["samples"] * 2
+    cross = w["cross_height"] * fh + w["cross_cave"] * fc + w["edges"] * 200 + w["cross_brush"] * 20
+    return density, cross
+
+
+def cpu_baseline(wl, threads):
+    from oracle import py_oracle
+    perm = py_oracle.perm_table(wl["seed"])
+    wall, secs, totals = py_oracle.mesh_chunks(wl["desc"], perm, 64, wl["mins"], wl["sizes"], 0, wl["ops"],
+                                               wl["aabbs"], threads=threads)
+    n = len(wl["sizes"])
+    return dict(value=n / wall, unit="chunks/s", cores=threads, kind="port",
+                sample=f"all {n} chunks of {wl['name']} on {threads} threads, oracle (C++ restatement, "
+                       f"FP32, one chunk per thread): {wall:.2f} s wall, p50 {np.percentile(secs, 50) * 1e3:.1f} ms, "
+                       f"p99 {np.percentile(secs, 99) * 1e3:.1f} ms per chunk")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="terrain_512", choices=sorted(workloads.WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--plain-chunks", type=int, default=0, help="chunks for the plain form (0 = all)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (the build is single-GPU); got --gpus {args.gpus}", file=sys.stderr)
+        sys.exit(2)
+
+    wl = workloads.WORKLOADS[args.workload]()
+    desc = wl["desc"]
+    n = len(wl["sizes"])
+    mesher.init(desc, wl["seed"], wl["default_material"])
+    mesher.clear_csg_ops()
+    for op, bb in zip(wl["ops"], wl["aabbs"]):
+        mesher.store_csg_op(op, bb[:3], bb[3:])
+
+    # ---- batched form ---------------------------------------------------------
+    batch = mesher.Batch(64, n, edge_cap=262144, voxel_cap=131072, flags=wl["flags"])
+    batch.set_chunks(wl["mins"], wl["sizes"])
+    for _ in range(args.warmup):
+        batch.run()
+    batch.wait()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run()
+    batch.wait()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    batched = n * args.steps / elapsed
+    counts = batch.counts()
+    assert (counts[:, 4] == 0).all(), "a chunk overflowed its slot capacity"
+
+    stage_ms = batch.run_timed()
+    work = batch.work()
+    d_flops, c_flops = stage_flops(work, desc)
+    dom = max(("density", "edge_cross"), key=lambda k: stage_ms[k])
+    dom_flops = d_flops if dom == "density" else c_flops
+    achieved = dom_flops / (stage_ms[dom] * 1e-3) / 1e12
+
+    # ---- plain per-chunk form ---------------------------------------------------
+    plain = mesher.Batch(64, 1, edge_cap=262144, voxel_cap=131072, flags=wl["flags"])
+    m = n if args.plain_chunks <= 0 else min(n, args.plain_chunks)
+    idx = np.linspace(0, n - 1, m).astype(int)
+    plain.set_chunks(wl["mins"][:1], wl["sizes"][:1])
+    plain.run()
+    plain.wait()
+    lat = []
+    t0 = time.perf_counter()
+    for i in idx:
+        ts = time.perf_counter()
+        plain.set_chunks(wl["mins"][i:i + 1], wl["sizes"][i:i + 1])
+        plain.run()
+        plain.wait()
+        lat.append(time.perf_counter() - ts)
+    plain_rate = m / (time.perf_counter() - t0)
+    lat = np.array(lat) * 1e3
+
+    out = {
+        "metric": METRIC,
+        "value": round(batched, 2),
+        "unit": "chunks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl["name"], "chunks": n, "voxels_per_chunk": 64,
+                   "field": "max(y - 24 fbm2, fbm3 - 0.3), 5+5 octaves" if desc.kind == 1 else "heightfield",
+                   "csg_ops": int(len(wl["ops"])), "form": "batched (one launch per stage over all chunks, hipGraph)"},
+        "plain_chunks_per_s": round(plain_rate, 2),
+        "batched_over_plain": round(batched / plain_rate, 2),
+        "plain_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                             "p99": round(float(np.percentile(lat, 99)), 3)},
+        "totals": {"edges": int(counts[:, 0].sum()), "voxels": int(counts[:, 1].sum()),
+                   "triangles": int(counts[:, 2].sum()), "nonempty_chunks": int((counts[:, 2] > 0).sum())},
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "roofline": {"bound": "valu", "kernel": "k_" + dom, "achieved": round(achieved, 3),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": None, "flops_per_launch": int(dom_flops),
+                     "launch_ms": round(stage_ms[dom], 4)},
+    }
+    if not args.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        out["cpu_baseline"] = cpu_baseline(wl, threads)
+        out["batched_over_cpu"] = round(batched / out["cpu_baseline"]["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

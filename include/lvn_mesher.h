@@ -138,6 +138,48 @@ int lvn_generate_chunk_mesh(lvn_ctx* ctx, const int32_t min[3], int32_t size,
                             int32_t* tri, int32_t tri_cap, int32_t* nt,
                             lvn_seam_node* s, int32_t s_cap, int32_t* ns);
 
+/* ---- batched form: many chunks per launch, device-resident outputs ---------- */
+
+/* One chunk request: min in world units, size = 256 * 2^LOD (src/clipmap.cpp:378). */
+typedef struct lvn_chunk_req {
+    int32_t min[3];
+    int32_t size;
+} lvn_chunk_req;
+
+typedef struct lvn_batch lvn_batch;
+
+/* A batch of up to max_chunks chunk slots in HBM.  edge_cap / voxel_cap bound the
+ * compacted edges / active voxels of one chunk (0 = the worst case for the chunk
+ * size); a chunk that exceeds them reports LVN_ERR_OVERFLOW in its status.
+ * No reference counterpart: the reference meshes one chunk per call
+ * (GenerateMeshDataForNode, src/clipmap.cpp:356-427). */
+int  lvn_batch_create(int32_t voxels_per_chunk, int32_t max_chunks, int32_t edge_cap, int32_t voxel_cap,
+                      uint32_t flags, lvn_batch** out);
+void lvn_batch_destroy(lvn_batch* b);
+/* Stage n chunk requests; each chunk takes the stored CSG ops whose AABB overlaps it,
+ * in store order (LoadDensityField's filter, src/compute_density_field.cpp:255-263).
+ * Each chunk is meshed as a fresh chunk (no field or octree cache). */
+int  lvn_batch_set_chunks(lvn_batch* b, int32_t n, const lvn_chunk_req* reqs);
+/* Enqueue the whole pipeline for the staged chunks: one launch per stage covering
+ * every chunk, replayed from a captured hipGraph.  Asynchronous. */
+int  lvn_batch_run(lvn_batch* b);
+int  lvn_batch_wait(lvn_batch* b);
+/* Same as lvn_batch_run + wait without the graph, timing each stage with HIP events
+ * on the batch stream: stage_ms[0..15] in pipeline order (see lvn_batch_stage_name). */
+int  lvn_batch_run_timed(lvn_batch* b, float* stage_ms, int32_t n_stages);
+const char* lvn_batch_stage_name(int32_t stage);
+/* Per chunk: 5 ints = edges, active voxels, triangles, seam nodes, status (waits). */
+int  lvn_batch_counts(lvn_batch* b, int32_t* counts);
+/* Copy chunk i's mesh out with generateChunkMesh's export rules (vertices only when
+ * triangles exist, src/compute_octree.cpp:227-232). */
+int  lvn_batch_get_mesh(lvn_batch* b, int32_t i, lvn_vertex* v, int32_t v_cap, int32_t* nv,
+                        int32_t* tri, int32_t tri_cap, int32_t* nt, lvn_seam_node* s, int32_t s_cap, int32_t* ns);
+/* Runs the pipeline once more with work counters on and returns the algorithmic work
+ * the kernels performed: work[0] height-term and work[1] cave-term evaluations in the
+ * density stage, work[2] / work[3] the same in the edge-crossing stage, work[4] brush
+ * evaluations in the edge-crossing stage, work[5] density samples, work[6] edges. */
+int  lvn_batch_work(lvn_batch* b, uint64_t* work);
+
 /* ---- test hook: intermediate stages of one chunk --------------------------- */
 
 /* Buffers for lvn_debug_chunk_stages.  F = voxels_per_chunk + 2, H = voxels_per_chunk + 1.

@@ -24,9 +24,17 @@ struct KArgs {
     int fp16;                // density stored as sign-preserving FP16
     int ysplit;              // density kernel: y segments per column
     int pad;
+    unsigned long long* work;  // optional work counters (see lvn_batch_work), else null
 };
 
 enum ScanKind { kScanEdges = 0, kScanVoxels = 1, kScanQuads = 2, kScanSeams = 3 };
+
+void launch_clear_counters(Counters* c, int n, hipStream_t s);
+
+constexpr int kNumStages = 16;
+extern const char* const kStageNames[kNumStages];
+// Launch stage `i` (0..kNumStages-1) of the pipeline.
+void launch_stage(const KArgs& a, int chunks, int i, hipStream_t s);
 
 // Stage launchers: `chunks` chunk slots [0, chunks) processed by one launch each.
 void launch_density(const KArgs& a, int chunks, hipStream_t s);
@@ -43,7 +51,7 @@ void launch_quad_emit(const KArgs& a, int chunks, hipStream_t s);
 void launch_vertex_seam(const KArgs& a, int chunks, hipStream_t s);
 void launch_seam_emit(const KArgs& a, int chunks, hipStream_t s);
 
-// Whole pipeline for chunk slots [0, chunks).
+// Whole pipeline for chunk slots [0, chunks), counters cleared first.
 void launch_pipeline(const KArgs& a, int chunks, hipStream_t s);
 
 }  // namespace lvn

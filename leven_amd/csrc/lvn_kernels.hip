@@ -78,9 +78,11 @@ __global__ __launch_bounds__(512) void k_density(KArgs a) {
     uint8_t* orig = slot_ptr<uint8_t>(a, chunk, a.L.orig);
     const int y0 = seg * seg_len;
     const int y1 = min(F, y0 + seg_len);
+    unsigned caves = 0;
     for (int y = y0; y < y1; ++y) {
         const float wy = (float)(y * cd.scale + cd.off[1]);
         const float d = combine(a.F, T, wy - H, wx, wy, wz);
+        if (a.work) caves += (a.F.kind == LVN_FIELD_TERRAIN_CAVES && !((wy - H) > a.F.cave_cut)) ? 1u : 0u;
         const int idx = field_index(G, x, y, z);
         if (a.fp16)
             dens16[idx] = to_half_keep_sign(d);
@@ -89,6 +91,11 @@ __global__ __launch_bounds__(512) void k_density(KArgs a) {
         const uint8_t m = d < 0.f ? (uint8_t)a.defmat : (uint8_t)kAir;
         mat[idx] = m;
         orig[idx] = m;
+    }
+    if (a.work) {
+        atomicAdd(&a.work[0], 1ull);
+        atomicAdd(&a.work[1], (unsigned long long)caves);
+        atomicAdd(&a.work[5], (unsigned long long)(y1 > y0 ? y1 - y0 : 0));
     }
 }
 
@@ -255,15 +262,17 @@ __global__ __launch_bounds__(256) void k_edge_compact(KArgs a) {
 // over t = k/16, then a central-difference normal with h = 0.001.  The height
 // term of samples that share (x, z) is evaluated once; the two end samples are
 // the stored field samples when the field is FP32.
-__device__ float4 density_crossing(const KArgs& a, const NoiseLds& T, const ChunkDesc& cd, int x, int y, int z,
-                                   int axis, bool ends_known, float d_start, float d_end) {
+__device__ __forceinline__ float4 density_crossing(const KArgs& a, const NoiseLds& T, const ChunkDesc& cd, int x, int y, int z,
+                                   int axis, bool ends_known, float d_start, float d_end, unsigned& nh, unsigned& nc) {
     const FieldParams& Fp = a.F;
+    const bool caves = Fp.kind == LVN_FIELD_TERRAIN_CAVES;
     const int w0x = x * cd.scale + cd.off[0], w0y = y * cd.scale + cd.off[1], w0z = z * cd.scale + cd.off[2];
     const float p0x = (float)w0x, p0y = (float)w0y, p0z = (float)w0z;
     const float p1x = (float)(w0x + (axis == 0 ? cd.scale : 0));
     const float p1y = (float)(w0y + (axis == 1 ? cd.scale : 0));
     const float p1z = (float)(w0z + (axis == 2 ? cd.scale : 0));
     const float colH = axis == 1 ? height_term(Fp, T, p0x, p0z) : 0.f;
+    nh += axis == 1 ? 1u : 0u;
     float minV = 3.402823466e+38f, tbest = 0.f, Hbest = colH, cur = 0.f;
     for (int k = 0; k <= 16; ++k) {
         const float px = p0x + (p1x - p0x) * cur;
@@ -275,7 +284,11 @@ __device__ float4 density_crossing(const KArgs& a, const NoiseLds& T, const Chun
         } else if (ends_known && k == 16) {
             d = fabsf(d_end);
         } else {
-            if (axis != 1) H = height_term(Fp, T, px, pz);
+            if (axis != 1) {
+                H = height_term(Fp, T, px, pz);
+                ++nh;
+            }
+            nc += (caves && !((py - H) > Fp.cave_cut)) ? 1u : 0u;
             d = fabsf(combine(Fp, T, py - H, px, py, pz));
         }
         if (d < minV) {
@@ -299,13 +312,18 @@ __device__ float4 density_crossing(const KArgs& a, const NoiseLds& T, const Chun
     const float dx = combine(Fp, T, py - Hxp, px + h, py, pz) - combine(Fp, T, py - Hxm, px - h, py, pz);
     const float dy = combine(Fp, T, (py + h) - H0, px, py + h, pz) - combine(Fp, T, (py - h) - H0, px, py - h, pz);
     const float dz = combine(Fp, T, py - Hzp, px, py, pz + h) - combine(Fp, T, py - Hzm, px, py, pz - h);
+    nh += hb_known ? 4u : 5u;
+    if (caves) {
+        nc += !((py - Hxp) > Fp.cave_cut) + !((py - Hxm) > Fp.cave_cut) + !(((py + h) - H0) > Fp.cave_cut) +
+              !(((py - h) - H0) > Fp.cave_cut) + !((py - Hzp) > Fp.cave_cut) + !((py - Hzm) > Fp.cave_cut);
+    }
     const float3 n = normalize_or_zero(dx, dy, dz);
     return make_float4(n.x, n.y, n.z, tbest);
 }
 
 // FindEdgeIntersectionInfo of the CSG program, cl/apply_csg_operation.cl:443-477,
 // with BrushZeroCrossing (:86-110) and BrushNormal (:144-175).
-__device__ float4 brush_crossing(const KArgs& a, const ChunkDesc& cd, int x, int y, int z, int axis) {
+__device__ __forceinline__ float4 brush_crossing(const KArgs& a, const ChunkDesc& cd, int x, int y, int z, int axis) {
     const int w0x = x * cd.scale + cd.off[0], w0y = y * cd.scale + cd.off[1], w0z = z * cd.scale + cd.off[2];
     const float p0x = (float)w0x, p0y = (float)w0y, p0z = (float)w0z;
     const float p1x = (float)(w0x + (axis == 0 ? cd.scale : 0));
@@ -347,7 +365,7 @@ __device__ float4 brush_crossing(const KArgs& a, const ChunkDesc& cd, int x, int
 // One thread per compacted edge.  An edge with an endpoint whose material a
 // brush changed takes brush data (src/compute_csg.cpp:97-217); otherwise the
 // density crossing (fresh field) or its data from the cached edge list.
-__global__ __launch_bounds__(256) void k_edge_cross(KArgs a) {
+__global__ __launch_bounds__(512) void k_edge_cross(KArgs a) {
     __shared__ NoiseLds T;
     const int chunk = blockIdx.y;
     const int E = a.counters[chunk].edges;
@@ -390,7 +408,16 @@ __global__ __launch_bounds__(256) void k_edge_cross(KArgs a) {
     } else {
         const float* dens = slot_ptr<float>(a, chunk, a.L.density);
         const bool ends = !a.fp16;
-        out = density_crossing(a, T, cd, x, y, z, axis, ends, ends ? dens[i0] : 0.f, ends ? dens[i1] : 0.f);
+        unsigned nh = 0, nc = 0;
+        out = density_crossing(a, T, cd, x, y, z, axis, ends, ends ? dens[i0] : 0.f, ends ? dens[i1] : 0.f, nh, nc);
+        if (a.work) {
+            atomicAdd(&a.work[2], (unsigned long long)nh);
+            atomicAdd(&a.work[3], (unsigned long long)nc);
+        }
+    }
+    if (a.work) {
+        atomicAdd(&a.work[6], 1ull);
+        if (touched) atomicAdd(&a.work[4], (unsigned long long)(17 * cd.op_count));
     }
     slot_ptr<float4>(a, chunk, a.L.e_data)[i] = out;
 }
@@ -765,11 +792,22 @@ __global__ __launch_bounds__(256) void k_seam_emit(KArgs a) {
     fo[2] = slot_ptr<float4>(a, chunk, a.L.v_norm)[vi];
 }
 
+static inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+// Zero the per-chunk counters (a kernel rather than a memset node, so that a
+// captured graph clears them correctly on every replay).
+__global__ __launch_bounds__(256) void k_clear_counters(Counters* c, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c[i] = Counters{0, 0, 0, 0, 0, {0, 0, 0}};
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 
-static inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
+void launch_clear_counters(Counters* c, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_counters, dim3(ceil_div(n, 256)), dim3(256), 0, s, c, n);
+}
 
 void launch_density(const KArgs& a, int chunks, hipStream_t s) {
     const long threads = (long)a.G.F * a.G.F * a.ysplit;
@@ -794,7 +832,7 @@ void launch_edge_compact(const KArgs& a, int chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_edge_compact, dim3(ceil_div(a.G.edge_slots, 256), chunks), dim3(256), 0, s, a);
 }
 void launch_edge_cross(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_edge_cross, dim3(ceil_div(a.caps.edges, 256), chunks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_edge_cross, dim3(ceil_div(a.caps.edges, 512), chunks), dim3(512), 0, s, a);
 }
 void launch_voxel_flags(const KArgs& a, int chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_voxel_flags, dim3(ceil_div(a.G.voxels, 256), chunks), dim3(256), 0, s, a);
@@ -818,7 +856,33 @@ void launch_seam_emit(const KArgs& a, int chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_seam_emit, dim3(ceil_div(a.caps.voxels, 256), chunks), dim3(256), 0, s, a);
 }
 
+const char* const kStageNames[kNumStages] = {
+    "density", "csg", "edge_flags", "scan_edges", "edge_compact", "edge_cross", "voxel_flags", "scan_voxels",
+    "voxel_compact", "qef", "quad_flags", "scan_quads", "quad_emit", "vertex_seam", "scan_seams", "seam_emit"};
+
+void launch_stage(const KArgs& a, int chunks, int i, hipStream_t s) {
+    switch (i) {
+    case 0: launch_density(a, chunks, s); break;
+    case 1: launch_csg(a, chunks, s); break;
+    case 2: launch_edge_flags(a, chunks, s); break;
+    case 3: launch_scan(a, chunks, kScanEdges, s); break;
+    case 4: launch_edge_compact(a, chunks, s); break;
+    case 5: launch_edge_cross(a, chunks, s); break;
+    case 6: launch_voxel_flags(a, chunks, s); break;
+    case 7: launch_scan(a, chunks, kScanVoxels, s); break;
+    case 8: launch_voxel_compact(a, chunks, s); break;
+    case 9: launch_qef(a, chunks, s); break;
+    case 10: launch_quad_flags(a, chunks, s); break;
+    case 11: launch_scan(a, chunks, kScanQuads, s); break;
+    case 12: launch_quad_emit(a, chunks, s); break;
+    case 13: launch_vertex_seam(a, chunks, s); break;
+    case 14: launch_scan(a, chunks, kScanSeams, s); break;
+    default: launch_seam_emit(a, chunks, s); break;
+    }
+}
+
 void launch_pipeline(const KArgs& a, int chunks, hipStream_t s) {
+    launch_clear_counters(a.counters, chunks, s);
     launch_density(a, chunks, s);
     launch_csg(a, chunks, s);
     launch_edge_flags(a, chunks, s);

@@ -82,6 +82,19 @@ struct Ctx {
     ~Ctx();
 };
 
+// A batch of chunk slots processed by one launch per stage (the batched form).
+struct Batch {
+    Ctx ctx;
+    int n = 0;                       // staged chunks
+    hipGraphExec_t exec = nullptr;   // captured pipeline for `graph_n` chunks
+    hipGraph_t graph = nullptr;
+    int graph_n = -1;
+    const void* graph_ops = nullptr;   // op arrays the graph was captured with
+    const void* graph_index = nullptr;
+    unsigned long long* d_work = nullptr;
+    ~Batch();
+};
+
 DevOp make_dev_op(const lvn_csg_op& op);
 Geometry make_geometry(int n);
 Caps make_caps(const Geometry& G, int edge_cap, int voxel_cap);

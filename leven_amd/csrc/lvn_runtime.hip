@@ -360,7 +360,7 @@ static int upload_single(Ctx* c, const ChunkDesc& cd_in, const std::vector<lvn_c
         LVN_HIP(hipMemcpyAsync(c->d_op_index, c->h_op_index, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
     }
     LVN_HIP(hipMemcpyAsync(c->d_chunks, c->h_chunks, sizeof(ChunkDesc), hipMemcpyHostToDevice, c->stream));
-    LVN_HIP(hipMemsetAsync(c->d_counters, 0, sizeof(Counters), c->stream));
+    launch_clear_counters(c->d_counters, 1, c->stream);
     return LVN_OK;
 }
 
@@ -484,6 +484,49 @@ static int copy_out(Ctx* c, const Counters& k, MeshEntry* m) {
         LVN_HIP(hipMemcpyAsync(m->seams.data(), slot + c->L.seams, m->seams.size() * 4, hipMemcpyDeviceToHost, c->stream));
     LVN_HIP(hipStreamSynchronize(c->stream));
     return LVN_OK;
+}
+
+Batch::~Batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (d_work) (void)hipFree(d_work);
+}
+
+// Stage chunk descriptors and the AABB-filtered stored-op lists of `n` requests.
+static int stage_batch(Batch* b, int n, const lvn_chunk_req* reqs) {
+    Ctx* c = &b->ctx;
+    std::vector<int> index;
+    index.reserve((size_t)n * 4);
+    const int nops = (int)g_state.ops.size();
+    int r = c->ensure_ops(std::max(nops, 1), 1);
+    if (r) return r;
+    for (int i = 0; i < nops; ++i) c->h_ops[i] = make_dev_op(g_state.ops[i]);
+    for (int i = 0; i < n; ++i) {
+        ChunkDesc& cd = c->h_chunks[i];
+        fill_chunk_desc(cd, reqs[i].min, reqs[i].size, c->N);
+        cd.op_start = (int)index.size();
+        for (int k = 0; k < nops; ++k)
+            if (aabb_overlaps(reqs[i].min, reqs[i].size, g_state.aabbs[k])) index.push_back(k);
+        cd.op_count = (int)index.size() - cd.op_start;
+    }
+    r = c->ensure_ops(std::max(nops, 1), std::max((int)index.size(), 1));
+    if (r) return r;
+    for (int i = 0; i < nops; ++i) c->h_ops[i] = make_dev_op(g_state.ops[i]);
+    if (!index.empty()) std::memcpy(c->h_op_index, index.data(), index.size() * sizeof(int));
+    if (nops) LVN_HIP(hipMemcpyAsync(c->d_ops, c->h_ops, sizeof(DevOp) * nops, hipMemcpyHostToDevice, c->stream));
+    if (!index.empty())
+        LVN_HIP(hipMemcpyAsync(c->d_op_index, c->h_op_index, sizeof(int) * index.size(), hipMemcpyHostToDevice,
+                               c->stream));
+    LVN_HIP(hipMemcpyAsync(c->d_chunks, c->h_chunks, sizeof(ChunkDesc) * n, hipMemcpyHostToDevice, c->stream));
+    LVN_HIP(hipStreamSynchronize(c->stream));
+    b->n = n;
+    return LVN_OK;
+}
+
+static KArgs batch_args(Batch* b) {
+    KArgs a = b->ctx.kargs();
+    a.ysplit = b->n >= 64 ? 1 : 6;  // small batches split columns to fill the GPU
+    return a;
 }
 
 }  // namespace lvn
@@ -675,6 +718,146 @@ int lvn_generate_chunk_mesh(lvn_ctx* ctx, const int32_t min[3], int32_t size, lv
     if (V && v) std::memcpy(v, m->verts.data(), m->verts.size() * 4);
     if (T && tri) std::memcpy(tri, m->tris.data(), m->tris.size() * 4);
     if (S && s) std::memcpy(s, m->seams.data(), m->seams.size() * 4);
+    return LVN_OK;
+}
+
+int lvn_batch_create(int32_t voxels_per_chunk, int32_t max_chunks, int32_t edge_cap, int32_t voxel_cap,
+                     uint32_t flags, lvn_batch** out) {
+    if (!out) return LVN_ERR_INVALID;
+    *out = nullptr;
+    if (!g_state.init) return LVN_ERR_STATE;
+    const int n = voxels_per_chunk;
+    if (n < 4 || n > 64 || (n & (n - 1)) || max_chunks < 1 || edge_cap < 0 || voxel_cap < 0) return LVN_ERR_INVALID;
+    Batch* b = new Batch;
+    int r = b->ctx.allocate(n, flags, max_chunks, edge_cap, voxel_cap);
+    if (!r && hipMalloc(&b->d_work, sizeof(unsigned long long) * 8) != hipSuccess) r = LVN_ERR_NOMEM;
+    if (r) {
+        delete b;
+        return r == LVN_ERR_DEVICE ? LVN_ERR_NOMEM : r;
+    }
+    *out = reinterpret_cast<lvn_batch*>(b);
+    return LVN_OK;
+}
+
+void lvn_batch_destroy(lvn_batch* b) { delete reinterpret_cast<Batch*>(b); }
+
+int lvn_batch_set_chunks(lvn_batch* bh, int32_t n, const lvn_chunk_req* reqs) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (!b || n < 0 || (n > 0 && !reqs) || n > b->ctx.slots) return LVN_ERR_INVALID;
+    if (!g_state.init) return LVN_ERR_STATE;
+    for (int i = 0; i < n; ++i)
+        if (reqs[i].size < b->ctx.N * kLeafSizeScale || reqs[i].size % (b->ctx.N * kLeafSizeScale)) return LVN_ERR_INVALID;
+    return stage_batch(b, n, reqs);
+}
+
+int lvn_batch_run(lvn_batch* bh) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (!b) return LVN_ERR_INVALID;
+    if (b->n == 0) return LVN_OK;
+    Ctx* c = &b->ctx;
+    if (b->graph_n != b->n || b->graph_ops != c->d_ops || b->graph_index != c->d_op_index) {
+        if (b->exec) LVN_HIP(hipGraphExecDestroy(b->exec));
+        if (b->graph) LVN_HIP(hipGraphDestroy(b->graph));
+        b->exec = nullptr;
+        b->graph = nullptr;
+        const KArgs a = batch_args(b);
+        LVN_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        launch_pipeline(a, b->n, c->stream);
+        LVN_HIP(hipStreamEndCapture(c->stream, &b->graph));
+        LVN_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->graph_n = b->n;
+        b->graph_ops = c->d_ops;
+        b->graph_index = c->d_op_index;
+    }
+    LVN_HIP(hipGraphLaunch(b->exec, c->stream));
+    return LVN_OK;
+}
+
+int lvn_batch_wait(lvn_batch* bh) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (!b) return LVN_ERR_INVALID;
+    LVN_HIP(hipStreamSynchronize(b->ctx.stream));
+    return LVN_OK;
+}
+
+const char* lvn_batch_stage_name(int32_t stage) {
+    return stage >= 0 && stage < kNumStages ? kStageNames[stage] : nullptr;
+}
+
+int lvn_batch_run_timed(lvn_batch* bh, float* stage_ms, int32_t n_stages) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (!b || !stage_ms || n_stages < kNumStages) return LVN_ERR_INVALID;
+    Ctx* c = &b->ctx;
+    const KArgs a = batch_args(b);
+    hipEvent_t ev[kNumStages + 1];
+    for (auto& e : ev) LVN_HIP(hipEventCreate(&e));
+    launch_clear_counters(c->d_counters, std::max(b->n, 1), c->stream);
+    for (int i = 0; i < kNumStages; ++i) {
+        LVN_HIP(hipEventRecord(ev[i], c->stream));
+        if (b->n) launch_stage(a, b->n, i, c->stream);
+    }
+    LVN_HIP(hipEventRecord(ev[kNumStages], c->stream));
+    LVN_HIP(hipEventSynchronize(ev[kNumStages]));
+    LVN_HIP(hipGetLastError());
+    for (int i = 0; i < kNumStages; ++i) LVN_HIP(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
+    for (auto& e : ev) LVN_HIP(hipEventDestroy(e));
+    return LVN_OK;
+}
+
+int lvn_batch_work(lvn_batch* bh, uint64_t* work) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (!b || !work) return LVN_ERR_INVALID;
+    Ctx* c = &b->ctx;
+    KArgs a = batch_args(b);
+    a.work = b->d_work;
+    LVN_HIP(hipMemsetAsync(b->d_work, 0, sizeof(unsigned long long) * 8, c->stream));
+    if (b->n) launch_pipeline(a, b->n, c->stream);
+    unsigned long long h[8];
+    LVN_HIP(hipMemcpyAsync(h, b->d_work, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    LVN_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < 7; ++i) work[i] = h[i];
+    return LVN_OK;
+}
+
+int lvn_batch_counts(lvn_batch* bh, int32_t* counts) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (!b || !counts) return LVN_ERR_INVALID;
+    Ctx* c = &b->ctx;
+    if (b->n == 0) return LVN_OK;
+    LVN_HIP(hipMemcpyAsync(c->h_counters, c->d_counters, sizeof(Counters) * b->n, hipMemcpyDeviceToHost, c->stream));
+    LVN_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < b->n; ++i) {
+        const Counters& k = c->h_counters[i];
+        counts[5 * i + 0] = k.edges;
+        counts[5 * i + 1] = k.voxels;
+        counts[5 * i + 2] = 2 * k.quads;
+        counts[5 * i + 3] = k.seams;
+        counts[5 * i + 4] = k.status ? LVN_ERR_OVERFLOW : LVN_OK;
+    }
+    return LVN_OK;
+}
+
+int lvn_batch_get_mesh(lvn_batch* bh, int32_t i, lvn_vertex* v, int32_t v_cap, int32_t* nv, int32_t* tri,
+                       int32_t tri_cap, int32_t* nt, lvn_seam_node* s, int32_t s_cap, int32_t* ns) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (!b || i < 0 || i >= b->n || !nv || !nt || !ns) return LVN_ERR_INVALID;
+    Ctx* c = &b->ctx;
+    Counters k;
+    LVN_HIP(hipMemcpyAsync(&c->h_counters[i], c->d_counters + i, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+    LVN_HIP(hipStreamSynchronize(c->stream));
+    k = c->h_counters[i];
+    if (k.status) return LVN_ERR_OVERFLOW;
+    const int V = k.edges > 0 && k.quads > 0 ? k.voxels : 0;
+    const int T = k.edges > 0 ? 2 * k.quads : 0;
+    const int S = k.edges > 0 ? k.seams : 0;
+    *nv = V;
+    *nt = T;
+    *ns = S;
+    if (V > v_cap || T > tri_cap || S > s_cap) return LVN_ERR_CAPACITY;
+    const char* slot = c->d_arena + (uint64_t)i * c->L.slot_bytes;
+    if (V && v) LVN_HIP(hipMemcpy(v, slot + c->L.verts, sizeof(lvn_vertex) * (size_t)V, hipMemcpyDeviceToHost));
+    if (T && tri) LVN_HIP(hipMemcpy(tri, slot + c->L.tris, 12 * (size_t)T, hipMemcpyDeviceToHost));
+    if (S && s) LVN_HIP(hipMemcpy(s, slot + c->L.seams, sizeof(lvn_seam_node) * (size_t)S, hipMemcpyDeviceToHost));
     return LVN_OK;
 }
 

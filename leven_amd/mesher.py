@@ -20,6 +20,8 @@ EXPORTS = [
     "lvn_init", "lvn_shutdown", "lvn_set_noise_seed", "lvn_store_csg_op", "lvn_clear_csg_ops",
     "lvn_get_perm_table", "lvn_ctx_create", "lvn_ctx_destroy", "lvn_ctx_voxels_per_chunk", "lvn_apply_csg",
     "lvn_free_chunk", "lvn_chunk_has_surface", "lvn_generate_chunk_mesh", "lvn_debug_chunk_stages",
+    "lvn_batch_create", "lvn_batch_destroy", "lvn_batch_set_chunks", "lvn_batch_run", "lvn_batch_wait",
+    "lvn_batch_run_timed", "lvn_batch_stage_name", "lvn_batch_counts", "lvn_batch_get_mesh", "lvn_batch_work",
 ]
 
 _lib = None
@@ -56,6 +58,17 @@ def lib():
             "lvn_generate_chunk_mesh": (i32, [vp, vp, i32, vp, i32, C.POINTER(i32), vp, i32, C.POINTER(i32), vp,
                                               i32, C.POINTER(i32)]),
             "lvn_debug_chunk_stages": (i32, [vp, vp, i32, vp, i32, C.POINTER(StageDump)]),
+            "lvn_batch_create": (i32, [i32, i32, i32, i32, u32, C.POINTER(vp)]),
+            "lvn_batch_destroy": (None, [vp]),
+            "lvn_batch_set_chunks": (i32, [vp, i32, vp]),
+            "lvn_batch_run": (i32, [vp]),
+            "lvn_batch_wait": (i32, [vp]),
+            "lvn_batch_run_timed": (i32, [vp, vp, i32]),
+            "lvn_batch_stage_name": (C.c_char_p, [i32]),
+            "lvn_batch_counts": (i32, [vp, vp]),
+            "lvn_batch_get_mesh": (i32, [vp, i32, vp, i32, C.POINTER(i32), vp, i32, C.POINTER(i32), vp, i32,
+                                         C.POINTER(i32)]),
+            "lvn_batch_work": (i32, [vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -199,3 +212,74 @@ class Context:
             voxel_materials=bufs["voxel_materials"][:v].copy(),
             voxel_positions=bufs["voxel_positions"][:v].copy(), voxel_normals=bufs["voxel_normals"][:v].copy(),
         )
+
+
+NUM_STAGES = 16
+
+
+def chunk_reqs(mins, sizes):
+    mins = np.ascontiguousarray(mins, dtype=np.int32).reshape(-1, 3)
+    sizes = np.ascontiguousarray(sizes, dtype=np.int32).reshape(-1, 1)
+    return np.ascontiguousarray(np.hstack([mins, sizes]), dtype=np.int32)
+
+
+class Batch:
+    """The batched form: many chunk slots in HBM, one launch per stage, graph replay."""
+
+    def __init__(self, voxels_per_chunk=64, max_chunks=512, edge_cap=0, voxel_cap=0, flags=0):
+        h = C.c_void_p()
+        check(lib().lvn_batch_create(int(voxels_per_chunk), int(max_chunks), int(edge_cap), int(voxel_cap),
+                                     int(flags), C.byref(h)), "lvn_batch_create")
+        self._h = h
+        self.n = 0
+        self.vpc = voxels_per_chunk
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lvn_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_chunks(self, mins, sizes):
+        reqs = chunk_reqs(mins, sizes)
+        check(lib().lvn_batch_set_chunks(self._h, len(reqs), vptr(reqs)), "lvn_batch_set_chunks")
+        self.n = len(reqs)
+
+    def run(self):
+        check(lib().lvn_batch_run(self._h), "lvn_batch_run")
+
+    def wait(self):
+        check(lib().lvn_batch_wait(self._h), "lvn_batch_wait")
+
+    def run_timed(self):
+        ms = np.zeros(NUM_STAGES, dtype=np.float32)
+        check(lib().lvn_batch_run_timed(self._h, vptr(ms), NUM_STAGES), "lvn_batch_run_timed")
+        names = [lib().lvn_batch_stage_name(i).decode() for i in range(NUM_STAGES)]
+        return dict(zip(names, ms.astype(float).tolist()))
+
+    def work(self):
+        w = np.zeros(8, dtype=np.uint64)
+        check(lib().lvn_batch_work(self._h, vptr(w)), "lvn_batch_work")
+        keys = ["density_height", "density_cave", "cross_height", "cross_cave", "cross_brush", "samples", "edges"]
+        return {k: int(v) for k, v in zip(keys, w)}
+
+    def counts(self):
+        """int32 [n, 5]: edges, voxels, triangles, seams, status."""
+        out = np.zeros((max(self.n, 1), 5), dtype=np.int32)
+        check(lib().lvn_batch_counts(self._h, vptr(out)), "lvn_batch_counts")
+        return out[: self.n]
+
+    def get_mesh(self, i):
+        k = self.counts()[i]
+        v = np.zeros(max(int(k[1]), 1), dtype=VERTEX_DTYPE)
+        t = np.zeros((max(int(k[2]), 1), 3), dtype=np.int32)
+        s = np.zeros(max(int(k[3]), 1), dtype=SEAM_DTYPE)
+        nv, nt, ns = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().lvn_batch_get_mesh(self._h, int(i), vptr(v), len(v), C.byref(nv), vptr(t), len(t), C.byref(nt),
+                                       vptr(s), len(s), C.byref(ns)), "lvn_batch_get_mesh")
+        return v[:nv.value].copy(), t[:nt.value].copy(), s[:ns.value].copy()
