@@ -143,6 +143,7 @@ def main():
         "totals": {"edges": int(counts[:, 0].sum()), "voxels": int(counts[:, 1].sum()),
                    "triangles": int(counts[:, 2].sum()), "nonempty_chunks": int((counts[:, 2] > 0).sum())},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "work": work,
         "roofline": {"bound": "valu", "kernel": "k_" + dom, "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": None, "flops_per_launch": int(dom_flops),

@@ -168,7 +168,8 @@ int  lvn_batch_wait(lvn_batch* b);
  * on the batch stream: stage_ms[0..15] in pipeline order (see lvn_batch_stage_name). */
 int  lvn_batch_run_timed(lvn_batch* b, float* stage_ms, int32_t n_stages);
 const char* lvn_batch_stage_name(int32_t stage);
-/* Per chunk: 5 ints = edges, active voxels, triangles, seam nodes, status (waits). */
+/* Per chunk: 6 ints = edges, active voxels, triangles, seam nodes, status, and the
+ * vertices re-solved in FP64 by the precise QEF (LVN_CTX_PRECISE_QEF).  Waits. */
 int  lvn_batch_counts(lvn_batch* b, int32_t* counts);
 /* Copy chunk i's mesh out with generateChunkMesh's export rules (vertices only when
  * triangles exist, src/compute_octree.cpp:227-232). */

@@ -23,7 +23,7 @@ struct KArgs {
     int defmat;
     int fp16;                // density stored as sign-preserving FP16
     int ysplit;              // density kernel: y segments per column
-    int pad;
+    int precise;             // QEF: compensated FP32 sums + FP64 re-solve
     unsigned long long* work;  // optional work counters (see lvn_batch_work), else null
 };
 

@@ -26,6 +26,25 @@ __device__ __forceinline__ int field_index(const Geometry& G, int x, int y, int 
     return x + G.F * (y + G.F * z);
 }
 
+// Division by H or F (runtime values) as a multiply-high: exact for n < 2^21
+// since (n * ceil(2^32/d)) / 2^32 - n/d < n / 2^32 < 1/d for d < 2^11.
+__device__ __forceinline__ int div_magic(int n, uint32_t magic) { return (int)__umulhi((uint32_t)n, magic); }
+
+// Hermite point h = x + H (y + H z) -> (x, y, z).
+__device__ __forceinline__ void decode_hermite(const Geometry& G, int h, int& x, int& y, int& z) {
+    const int r = div_magic(h, G.magic_h);
+    x = h - r * G.H;
+    z = div_magic(r, G.magic_h);
+    y = r - z * G.H;
+}
+
+// Voxel v = x + N (y + N z) with N = 2^depth.
+__device__ __forceinline__ void decode_voxel(const Geometry& G, int v, int& x, int& y, int& z) {
+    x = v & (G.N - 1);
+    y = (v >> G.depth) & (G.N - 1);
+    z = v >> (2 * G.depth);
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
 
 // Dense lookup of a compacted element through its segment mask: index or -1.
@@ -64,11 +83,12 @@ __global__ __launch_bounds__(512) void k_density(KArgs a) {
     const int F = G.F;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int seg_len = (F + a.ysplit - 1) / a.ysplit;
-    const int col = t % (F * F);
-    const int seg = t / (F * F);
+    const int seg = div_magic(div_magic(t, G.magic_f), G.magic_f);
+    const int col = t - seg * F * F;
     if (seg >= a.ysplit) return;
     const ChunkDesc cd = a.chunks[chunk];
-    const int x = col % F, z = col / F;
+    const int z = div_magic(col, G.magic_f);
+    const int x = col - z * F;
     const float wx = (float)(x * cd.scale + cd.off[0]);
     const float wz = (float)(z * cd.scale + cd.off[2]);
     const float H = height_term(a.F, T, wx, wz);
@@ -152,7 +172,8 @@ __global__ __launch_bounds__(256) void k_edge_flags(KArgs a) {
     int flag = 0;
     if (s < G.edge_slots) {
         const int h = s / 3, axis = s - 3 * h;
-        const int x = h % G.H, r = h / G.H, y = r % G.H, z = r / G.H;
+        int x, y, z;
+        decode_hermite(G, h, x, y, z);
         const int i0 = field_index(G, x, y, z);
         const int i1 = i0 + (axis == 0 ? 1 : axis == 1 ? G.F : G.F * G.F);
         flag = (mat[i0] != kAir) != (mat[i1] != kAir);
@@ -250,7 +271,8 @@ __global__ __launch_bounds__(256) void k_edge_compact(KArgs a) {
     const int i = dense_index(slot_ptr<uint64_t>(a, chunk, a.L.e_mask), slot_ptr<int>(a, chunk, a.L.e_base), s);
     if (i < 0 || i >= a.caps.edges) return;
     const int h = s / 3, axis = s - 3 * h;
-    const int x = h % G.H, r = h / G.H, y = r % G.H, z = r / G.H;
+    int x, y, z;
+    decode_hermite(G, h, x, y, z);
     slot_ptr<int>(a, chunk, a.L.e_ids)[i] = ((x | (y << G.shift) | (z << (2 * G.shift))) << 2) | axis;
 }
 
@@ -477,7 +499,8 @@ __global__ __launch_bounds__(256) void k_voxel_flags(KArgs a) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     int active = 0;
     if (v < G.voxels) {
-        const int x = v % G.N, y = (v / G.N) % G.N, z = v / (G.N * G.N);
+        int x, y, z;
+        decode_voxel(G, v, x, y, z);
         int cm[8];
         const int corners = corner_materials(a, slot_ptr<uint8_t>(a, chunk, a.L.mat), x, y, z, cm);
         active = corners != 0 && corners != 255;
@@ -497,7 +520,8 @@ __global__ __launch_bounds__(256) void k_voxel_compact(KArgs a) {
     if (v >= G.voxels) return;
     const int i = dense_index(slot_ptr<uint64_t>(a, chunk, a.L.v_mask), slot_ptr<int>(a, chunk, a.L.v_base), v);
     if (i < 0 || i >= a.caps.voxels) return;
-    const int x = v % G.N, y = (v / G.N) % G.N, z = v / (G.N * G.N);
+    int x, y, z;
+    decode_voxel(G, v, x, y, z);
     int cm[8];
     const int corners = corner_materials(a, slot_ptr<uint8_t>(a, chunk, a.L.mat), x, y, z, cm);
     int edges = 0;
@@ -670,6 +694,208 @@ __global__ __launch_bounds__(256) void k_qef(KArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Precision form of the QEF (LVN_CTX_PRECISE_QEF).  The normal equations are
+// formed about the mass point, A = sum n n^T and r = sum n (n . (p - m)) (the
+// same least-squares problem as ATb - ATA m, without its cancellation), with
+// every product split exactly by an FMA (TwoProduct) and summed with Neumaier
+// compensation.  The Jacobi pseudo-inverse runs in FP32; a vertex whose
+// condition number exceeds 1e6, or with an eigenvalue within 1e-3 (relative) of
+// the truncation thresholds 0.1 / 10, is re-solved entirely in FP64.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void neumaier(float& s, float& c, float x) {
+    const float t = s + x;
+    c += fabsf(s) >= fabsf(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+}
+
+__device__ __forceinline__ void add_product(float& s, float& c, float a, float b) {
+    const float p = a * b;
+    const float e = fmaf(a, b, -p);  // exact error of the product
+    neumaier(s, c, p);
+    c += e;
+}
+
+template <typename T>
+__device__ __forceinline__ void jacobi_rotate(T& aa, T& bb, T& ab, T& u, T& w, T& va0, T& vb0, T& va1, T& vb1, T& va2,
+                                              T& vb2) {
+    if (ab == (T)0) return;
+    const T tau = (bb - aa) / ((T)2 * ab);
+    const T stt = sqrt((T)1 + tau * tau);
+    const T tn = (T)1 / ((tau >= (T)0) ? (tau + stt) : (tau - stt));
+    const T c = (T)1 / sqrt((T)1 + tn * tn);
+    const T s = tn * c;
+    const T cc = c * c, ss = s * s, mx = (((T)2 * c) * s) * ab;
+    const T u0 = aa, w0 = bb;
+    aa = (cc * u0 - mx) + ss * w0;
+    bb = (ss * u0 + mx) + cc * w0;
+    const T u1 = u, w1 = w;
+    u = c * u1 - s * w1;
+    w = s * u1 + c * w1;
+    ab = (T)0;
+    T x = va0, y = vb0;
+    va0 = c * x - s * y;
+    vb0 = s * x + c * y;
+    x = va1; y = vb1;
+    va1 = c * x - s * y;
+    vb1 = s * x + c * y;
+    x = va2; y = vb2;
+    va2 = c * x - s * y;
+    vb2 = s * x + c * y;
+}
+
+// 10 cyclic Jacobi sweeps + truncated pseudo-inverse applied to r (cl/qef.cl:88-144).
+template <typename T>
+__device__ __forceinline__ void qef_pinv_solve(const T A[6], const T r[3], T x[3], T sig[3]) {
+    T v00 = A[0], v01 = A[1], v02 = A[2], v11 = A[3], v12 = A[4], v22 = A[5];
+    T V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int sweep = 0; sweep < 10; ++sweep) {
+        jacobi_rotate<T>(v00, v11, v01, v02, v12, V[0][0], V[0][1], V[1][0], V[1][1], V[2][0], V[2][1]);
+        jacobi_rotate<T>(v00, v22, v02, v01, v12, V[0][0], V[0][2], V[1][0], V[1][2], V[2][0], V[2][2]);
+        jacobi_rotate<T>(v11, v22, v12, v01, v02, V[0][1], V[0][2], V[1][1], V[1][2], V[2][1], V[2][2]);
+    }
+    sig[0] = v00;
+    sig[1] = v11;
+    sig[2] = v22;
+    const T tol = (T)0.1;
+    T d[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        d[i] = (fabs(sig[i]) < tol || fabs((T)1 / sig[i]) < tol) ? (T)0 : (T)1 / sig[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        T acc = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            acc += ((V[i][0] * d[0]) * V[j][0] + (V[i][1] * d[1]) * V[j][1] + (V[i][2] * d[2]) * V[j][2]) * r[j];
+        x[i] = acc;
+    }
+}
+
+__device__ __forceinline__ bool near_threshold(float s) {
+    const float a = fabsf(s);
+    return fabsf(a - 0.1f) < 1e-4f || fabsf(a - 10.f) < 1e-2f;
+}
+
+__global__ __launch_bounds__(256) void k_qef_precise(KArgs a) {
+    const int chunk = blockIdx.y;
+    const int V = a.counters[chunk].voxels;
+    const int vi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (vi >= V) return;
+    const Geometry& G = a.G;
+    const ChunkDesc cd = a.chunks[chunk];
+    const uint32_t code = (uint32_t)slot_ptr<int>(a, chunk, a.L.v_code)[vi];
+    const int emask = slot_ptr<int>(a, chunk, a.L.v_emask)[vi];
+    const int3 p = position_for_code(code, G.depth);
+    const uint64_t* em = slot_ptr<uint64_t>(a, chunk, a.L.e_mask);
+    const int* eb = slot_ptr<int>(a, chunk, a.L.e_base);
+    const float4* ed = slot_ptr<float4>(a, chunk, a.L.e_data);
+    const float fscale = (float)cd.scale;
+
+    float px[12], py[12], pz[12], nx[12], ny[12], nz[12];
+    bool ok[12];
+    float cnt = 0.f;
+    float sx = 0.f, cx = 0.f, sy = 0.f, cy = 0.f, sz = 0.f, cz = 0.f;
+    float nsx = 0.f, nsy = 0.f, nsz = 0.f;
+#pragma unroll
+    for (int e = 0; e < 12; ++e) {
+        ok[e] = false;
+        px[e] = py[e] = pz[e] = nx[e] = ny[e] = nz[e] = 0.f;
+        if (!((emask >> e) & 1)) continue;
+        const int e0 = kEdgeMap[e][0], e1 = kEdgeMap[e][1], axis = e >> 2;
+        const int hx = p.x + ((e0 >> 2) & 1), hy = p.y + ((e0 >> 1) & 1), hz = p.z + (e0 & 1);
+        const int k = dense_index(em, eb, 3 * (hx + G.H * (hy + G.H * hz)) + axis);
+        if (k < 0 || k >= a.caps.edges) continue;
+        const float4 d = ed[k];
+        const float a0x = (float)hx, a0y = (float)hy, a0z = (float)hz;
+        const float a1x = (float)(p.x + ((e1 >> 2) & 1)), a1y = (float)(p.y + ((e1 >> 1) & 1)),
+                    a1z = (float)(p.z + (e1 & 1));
+        ok[e] = true;
+        px[e] = fscale * (a0x + (a1x - a0x) * d.w);
+        py[e] = fscale * (a0y + (a1y - a0y) * d.w);
+        pz[e] = fscale * (a0z + (a1z - a0z) * d.w);
+        nx[e] = d.x;
+        ny[e] = d.y;
+        nz[e] = d.z;
+        cnt += 1.f;
+        neumaier(sx, cx, px[e]);
+        neumaier(sy, cy, py[e]);
+        neumaier(sz, cz, pz[e]);
+        nsx = nsx + d.x;
+        nsy = nsy + d.y;
+        nsz = nsz + d.z;
+    }
+    const float mx = (sx + cx) / cnt, my = (sy + cy) / cnt, mz = (sz + cz) / cnt;
+    float As[6] = {0, 0, 0, 0, 0, 0}, Ac[6] = {0, 0, 0, 0, 0, 0}, rs[3] = {0, 0, 0}, rc[3] = {0, 0, 0};
+#pragma unroll
+    for (int e = 0; e < 12; ++e) {
+        if (!ok[e]) continue;
+        add_product(As[0], Ac[0], nx[e], nx[e]);
+        add_product(As[1], Ac[1], nx[e], ny[e]);
+        add_product(As[2], Ac[2], nx[e], nz[e]);
+        add_product(As[3], Ac[3], ny[e], ny[e]);
+        add_product(As[4], Ac[4], ny[e], nz[e]);
+        add_product(As[5], Ac[5], nz[e], nz[e]);
+        const float b = fmaf(nx[e], px[e] - mx, fmaf(ny[e], py[e] - my, nz[e] * (pz[e] - mz)));
+        add_product(rs[0], rc[0], nx[e], b);
+        add_product(rs[1], rc[1], ny[e], b);
+        add_product(rs[2], rc[2], nz[e], b);
+    }
+    float A[6], r[3], x[3], sig[3];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) A[i] = As[i] + Ac[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r[i] = rs[i] + rc[i];
+    qef_pinv_solve<float>(A, r, x, sig);
+    const float smax = fmaxf(fmaxf(fabsf(sig[0]), fabsf(sig[1])), fabsf(sig[2]));
+    const float smin = fminf(fminf(fabsf(sig[0]), fabsf(sig[1])), fabsf(sig[2]));
+    float4 out;
+    if (smin == 0.f || smax > 1e6f * smin || near_threshold(sig[0]) || near_threshold(sig[1]) ||
+        near_threshold(sig[2])) {
+        double n = 0, mxd = 0, myd = 0, mzd = 0;
+#pragma unroll
+        for (int e = 0; e < 12; ++e) {
+            if (!ok[e]) continue;
+            n += 1.0;
+            mxd += (double)px[e];
+            myd += (double)py[e];
+            mzd += (double)pz[e];
+        }
+        mxd /= n;
+        myd /= n;
+        mzd /= n;
+        double Ad[6] = {0, 0, 0, 0, 0, 0}, rd[3] = {0, 0, 0}, xd[3], sd[3];
+#pragma unroll
+        for (int e = 0; e < 12; ++e) {
+            if (!ok[e]) continue;
+            const double a0 = nx[e], a1 = ny[e], a2 = nz[e];
+            Ad[0] += a0 * a0;
+            Ad[1] += a0 * a1;
+            Ad[2] += a0 * a2;
+            Ad[3] += a1 * a1;
+            Ad[4] += a1 * a2;
+            Ad[5] += a2 * a2;
+            const double b = a0 * ((double)px[e] - mxd) + a1 * ((double)py[e] - myd) + a2 * ((double)pz[e] - mzd);
+            rd[0] += a0 * b;
+            rd[1] += a1 * b;
+            rd[2] += a2 * b;
+        }
+        qef_pinv_solve<double>(Ad, rd, xd, sd);
+        out = make_float4((float)((xd[0] + mxd) * 4.0 + (double)cd.min[0]),
+                          (float)((xd[1] + myd) * 4.0 + (double)cd.min[1]),
+                          (float)((xd[2] + mzd) * 4.0 + (double)cd.min[2]), 1.f);
+        atomicAdd(&a.counters[chunk].resolved, 1);
+    } else {
+        out = make_float4(((x[0] + mx) * 4.f) + (float)cd.min[0], ((x[1] + my) * 4.f) + (float)cd.min[1],
+                          ((x[2] + mz) * 4.f) + (float)cd.min[2], 1.f);
+    }
+    slot_ptr<float4>(a, chunk, a.L.v_pos)[vi] = out;
+    // the vertex normal is the reference's unnormalised mean of the edge normals
+    const float nw = cnt;
+    slot_ptr<float4>(a, chunk, a.L.v_norm)[vi] = make_float4(nsx / nw, nsy / nw, nsz / nw, 0.f);
+}
+
+// ---------------------------------------------------------------------------
 // Quads (GenerateMesh + CompactMeshTriangles, cl/octree.cl:335-464)
 // ---------------------------------------------------------------------------
 
@@ -798,7 +1024,7 @@ static inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) 
 // captured graph clears them correctly on every replay).
 __global__ __launch_bounds__(256) void k_clear_counters(Counters* c, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) c[i] = Counters{0, 0, 0, 0, 0, {0, 0, 0}};
+    if (i < n) c[i] = Counters{0, 0, 0, 0, 0, 0, {0, 0}};
 }
 
 // ---------------------------------------------------------------------------
@@ -841,7 +1067,10 @@ void launch_voxel_compact(const KArgs& a, int chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_voxel_compact, dim3(ceil_div(a.G.voxels, 256), chunks), dim3(256), 0, s, a);
 }
 void launch_qef(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_qef, dim3(ceil_div(a.caps.voxels, 256), chunks), dim3(256), 0, s, a);
+    if (a.precise)
+        hipLaunchKernelGGL(k_qef_precise, dim3(ceil_div(a.caps.voxels, 256), chunks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_qef, dim3(ceil_div(a.caps.voxels, 256), chunks), dim3(256), 0, s, a);
 }
 void launch_quad_flags(const KArgs& a, int chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_quad_flags, dim3(ceil_div(3L * a.caps.voxels, 256), chunks), dim3(256), 0, s, a);

@@ -191,6 +191,8 @@ Geometry make_geometry(int n) {
     G.edge_segs = (G.edge_slots + 63) / 64;
     G.voxels = n * n * n;
     G.voxel_segs = std::max(1, G.voxels / 64);
+    G.magic_h = (uint32_t)((((uint64_t)1 << 32) + G.H - 1) / G.H);
+    G.magic_f = (uint32_t)((((uint64_t)1 << 32) + G.F - 1) / G.F);
     return G;
 }
 
@@ -316,6 +318,7 @@ KArgs Ctx::kargs() const {
     a.defmat = (int)g_state.defmat;
     a.fp16 = (flags & LVN_CTX_FP16_DENSITY) ? 1 : 0;
     a.ysplit = 1;
+    a.precise = (flags & LVN_CTX_PRECISE_QEF) ? 1 : 0;
     return a;
 }
 
@@ -828,11 +831,12 @@ int lvn_batch_counts(lvn_batch* bh, int32_t* counts) {
     LVN_HIP(hipStreamSynchronize(c->stream));
     for (int i = 0; i < b->n; ++i) {
         const Counters& k = c->h_counters[i];
-        counts[5 * i + 0] = k.edges;
-        counts[5 * i + 1] = k.voxels;
-        counts[5 * i + 2] = 2 * k.quads;
-        counts[5 * i + 3] = k.seams;
-        counts[5 * i + 4] = k.status ? LVN_ERR_OVERFLOW : LVN_OK;
+        counts[6 * i + 0] = k.edges;
+        counts[6 * i + 1] = k.voxels;
+        counts[6 * i + 2] = 2 * k.quads;
+        counts[6 * i + 3] = k.seams;
+        counts[6 * i + 4] = k.status ? LVN_ERR_OVERFLOW : LVN_OK;
+        counts[6 * i + 5] = k.resolved;
     }
     return LVN_OK;
 }

@@ -40,6 +40,8 @@ struct Geometry {
     int edge_segs;       // ceil(edge_slots / 64)
     int voxels;          // N^3
     int voxel_segs;      // N^3 / 64 (>= 1)
+    uint32_t magic_h;    // ceil(2^32 / H): n / H == umulhi(n, magic_h) for n < 2^21
+    uint32_t magic_f;    // ceil(2^32 / F)
 };
 
 // Per-slot capacities of the compacted arrays.
@@ -84,7 +86,8 @@ constexpr int kModeCached = 1;  // untouched edges keep their data from the old 
 struct Counters {
     int edges, voxels, quads, seams;
     int status;      // 0 ok, else LVN_ERR_OVERFLOW bits
-    int pad[3];
+    int resolved;    // precise QEF: vertices re-solved in FP64
+    int pad[2];
 };
 
 constexpr int kStatusEdgeOverflow = 1;

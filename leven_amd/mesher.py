@@ -269,8 +269,8 @@ class Batch:
         return {k: int(v) for k, v in zip(keys, w)}
 
     def counts(self):
-        """int32 [n, 5]: edges, voxels, triangles, seams, status."""
-        out = np.zeros((max(self.n, 1), 5), dtype=np.int32)
+        """int32 [n, 6]: edges, voxels, triangles, seams, status, FP64 re-solved vertices."""
+        out = np.zeros((max(self.n, 1), 6), dtype=np.int32)
         check(lib().lvn_batch_counts(self._h, vptr(out)), "lvn_batch_counts")
         return out[: self.n]
 
