@@ -25,11 +25,17 @@ struct KArgs {
     int ysplit;              // density kernel: y segments per column
     int precise;             // QEF: compensated FP32 sums + FP64 re-solve
     unsigned long long* work;  // optional work counters (see lvn_batch_work), else null
+    int* queue;              // kNumQueues work-queue heads, zeroed by launch_clear_counters
+    int* item_prefix;        // chunks + 1 edge-item offsets (k_edge_items)
+    int cus;                 // compute units of the device
+    int pad2;
 };
+
+constexpr int kNumQueues = 8;
 
 enum ScanKind { kScanEdges = 0, kScanVoxels = 1, kScanQuads = 2, kScanSeams = 3 };
 
-void launch_clear_counters(Counters* c, int n, hipStream_t s);
+void launch_clear_counters(const KArgs& a, int n, hipStream_t s);
 
 constexpr int kNumStages = 16;
 extern const char* const kStageNames[kNumStages];

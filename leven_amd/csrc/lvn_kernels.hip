@@ -13,6 +13,8 @@
 
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+
 namespace lvn {
 
 namespace {
@@ -75,18 +77,13 @@ __constant__ int kEdgeMap[12][2] = {{0, 4}, {1, 5}, {2, 6}, {3, 7}, {0, 2}, {1, 
 // ---------------------------------------------------------------------------
 // One thread per (x, z) column segment: the height term depends on x and z
 // only, so it is evaluated once and reused down the column.
-__global__ __launch_bounds__(512) void k_density(KArgs a) {
-    __shared__ NoiseLds T;
-    load_noise_tables(T, a.perm, a.grads);
-    const int chunk = blockIdx.y;
+__device__ __forceinline__ void density_column(const KArgs& a, const NoiseLds& T, int chunk, int t) {
     const Geometry& G = a.G;
     const int F = G.F;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int seg_len = (F + a.ysplit - 1) / a.ysplit;
     const int seg = div_magic(div_magic(t, G.magic_f), G.magic_f);
     const int col = t - seg * F * F;
-    if (seg >= a.ysplit) return;
-    const ChunkDesc cd = a.chunks[chunk];
+    const ChunkDesc& cd = a.chunks[chunk];
     const int z = div_magic(col, G.magic_f);
     const int x = col - z * F;
     const float wx = (float)(x * cd.scale + cd.off[0]);
@@ -116,6 +113,32 @@ __global__ __launch_bounds__(512) void k_density(KArgs a) {
         atomicAdd(&a.work[0], 1ull);
         atomicAdd(&a.work[1], (unsigned long long)caves);
         atomicAdd(&a.work[5], (unsigned long long)(y1 > y0 ? y1 - y0 : 0));
+    }
+}
+
+// Wave-level work queue: lane 0 takes the next 64-element item, all lanes get it.
+__device__ __forceinline__ int next_item(int* queue) {
+    int item = 0;
+    if ((threadIdx.x & 63) == 0) item = atomicAdd(queue, 1);
+    return __shfl(item, 0);
+}
+
+// Persistent: the grid fills the machine once (two 1024-thread workgroups per CU,
+// bounded by the 64 KiB LDS table), each workgroup loads the noise tables once,
+// and every wave pulls 64-column items of any chunk until the queue is empty.
+__global__ __launch_bounds__(1024) void k_density(KArgs a, int chunks) {
+    __shared__ NoiseLds T;
+    load_noise_tables(T, a.perm, a.grads);
+    const int per_chunk = a.G.F * a.G.F * a.ysplit;
+    const int items_per_chunk = (per_chunk + 63) >> 6;
+    const int total = items_per_chunk * chunks;
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        const int item = next_item(&a.queue[0]);
+        if (item >= total) break;
+        const int chunk = item / items_per_chunk;
+        const int t = (item - chunk * items_per_chunk) * 64 + lane;
+        if (t < per_chunk) density_column(a, T, chunk, t);
     }
 }
 
@@ -384,19 +407,12 @@ __device__ __forceinline__ float4 brush_crossing(const KArgs& a, const ChunkDesc
     return make_float4(n.x, n.y, n.z, crossing);
 }
 
-// One thread per compacted edge.  An edge with an endpoint whose material a
-// brush changed takes brush data (src/compute_csg.cpp:97-217); otherwise the
-// density crossing (fresh field) or its data from the cached edge list.
-__global__ __launch_bounds__(512) void k_edge_cross(KArgs a) {
-    __shared__ NoiseLds T;
-    const int chunk = blockIdx.y;
-    const int E = a.counters[chunk].edges;
-    if ((int)(blockIdx.x * blockDim.x) >= E) return;
-    load_noise_tables(T, a.perm, a.grads);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= E) return;
+// One compacted edge.  An edge with an endpoint whose material a brush changed
+// takes brush data (src/compute_csg.cpp:97-217); otherwise the density crossing
+// (fresh field) or its data from the cached edge list.
+__device__ __forceinline__ void cross_edge(const KArgs& a, const NoiseLds& T, int chunk, int i) {
     const Geometry& G = a.G;
-    const ChunkDesc cd = a.chunks[chunk];
+    const ChunkDesc& cd = a.chunks[chunk];
     const int id = slot_ptr<int>(a, chunk, a.L.e_ids)[i];
     const int axis = id & 3, hidx = id >> 2;
     const int msk = (1 << G.shift) - 1;
@@ -442,6 +458,51 @@ __global__ __launch_bounds__(512) void k_edge_cross(KArgs a) {
         if (touched) atomicAdd(&a.work[4], (unsigned long long)(17 * cd.op_count));
     }
     slot_ptr<float4>(a, chunk, a.L.e_data)[i] = out;
+}
+
+// Items of 64 edges over all chunks: item_prefix[c] = sum over earlier chunks of
+// ceil(E / 64) (k_edge_items), item_prefix[chunks] = total.
+__global__ __launch_bounds__(1024) void k_edge_items(KArgs a, int chunks) {
+    __shared__ int buf[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < chunks; base += 1024) {
+        const int c = base + threadIdx.x;
+        const int v = c < chunks ? (a.counters[c].edges + 63) >> 6 : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int u = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += u;
+            __syncthreads();
+        }
+        if (c < chunks) a.item_prefix[c] = carry + buf[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.item_prefix[chunks] = carry;
+}
+
+// Persistent like k_density: waves pull 64-edge items of any chunk.
+__global__ __launch_bounds__(1024) void k_edge_cross(KArgs a, int chunks) {
+    __shared__ NoiseLds T;
+    load_noise_tables(T, a.perm, a.grads);
+    const int total = a.item_prefix[chunks];
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        const int item = next_item(&a.queue[1]);
+        if (item >= total) break;
+        int lo = 0, hi = chunks;  // last chunk with item_prefix[c] <= item
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (a.item_prefix[mid] <= item) lo = mid; else hi = mid;
+        }
+        const int i = (item - a.item_prefix[lo]) * 64 + lane;
+        if (i < a.counters[lo].edges) cross_edge(a, T, lo, i);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1022,22 +1083,29 @@ static inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) 
 
 // Zero the per-chunk counters (a kernel rather than a memset node, so that a
 // captured graph clears them correctly on every replay).
-__global__ __launch_bounds__(256) void k_clear_counters(Counters* c, int n) {
+__global__ __launch_bounds__(256) void k_clear_counters(Counters* c, int n, int* queue) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) c[i] = Counters{0, 0, 0, 0, 0, 0, {0, 0}};
+    if (i < kNumQueues) queue[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 
-void launch_clear_counters(Counters* c, int n, hipStream_t s) {
-    hipLaunchKernelGGL(k_clear_counters, dim3(ceil_div(n, 256)), dim3(256), 0, s, c, n);
+void launch_clear_counters(const KArgs& a, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_counters, dim3(ceil_div(std::max(n, kNumQueues), 256)), dim3(256), 0, s,
+                       a.counters, n, a.queue);
+}
+
+// Two 1024-thread workgroups per CU (the LDS table limit), fewer when the work is small.
+static unsigned persistent_blocks(const KArgs& a, long items) {
+    return (unsigned)std::max(1L, std::min((long)2 * a.cus, (items + 15) / 16));
 }
 
 void launch_density(const KArgs& a, int chunks, hipStream_t s) {
-    const long threads = (long)a.G.F * a.G.F * a.ysplit;
-    hipLaunchKernelGGL(k_density, dim3(ceil_div(threads, 512), chunks), dim3(512), 0, s, a);
+    const long items = (((long)a.G.F * a.G.F * a.ysplit + 63) / 64) * chunks;
+    hipLaunchKernelGGL(k_density, dim3(persistent_blocks(a, items)), dim3(1024), 0, s, a, chunks);
 }
 void launch_csg(const KArgs& a, int chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_csg, dim3(1, chunks), dim3(1024), 0, s, a);
@@ -1058,7 +1126,9 @@ void launch_edge_compact(const KArgs& a, int chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_edge_compact, dim3(ceil_div(a.G.edge_slots, 256), chunks), dim3(256), 0, s, a);
 }
 void launch_edge_cross(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_edge_cross, dim3(ceil_div(a.caps.edges, 512), chunks), dim3(512), 0, s, a);
+    hipLaunchKernelGGL(k_edge_items, dim3(1), dim3(1024), 0, s, a, chunks);
+    // the item count is on the device; size the grid for 512 edge items per chunk
+    hipLaunchKernelGGL(k_edge_cross, dim3(persistent_blocks(a, 512L * chunks)), dim3(1024), 0, s, a, chunks);
 }
 void launch_voxel_flags(const KArgs& a, int chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_voxel_flags, dim3(ceil_div(a.G.voxels, 256), chunks), dim3(256), 0, s, a);
@@ -1111,7 +1181,7 @@ void launch_stage(const KArgs& a, int chunks, int i, hipStream_t s) {
 }
 
 void launch_pipeline(const KArgs& a, int chunks, hipStream_t s) {
-    launch_clear_counters(a.counters, chunks, s);
+    launch_clear_counters(a, chunks, s);
     launch_density(a, chunks, s);
     launch_csg(a, chunks, s);
     launch_edge_flags(a, chunks, s);

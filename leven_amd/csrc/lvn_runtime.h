@@ -73,6 +73,9 @@ struct Ctx {
     int* d_op_index = nullptr;
     int* h_op_index = nullptr;
     int ops_cap = 0, idx_cap = 0;
+    int* d_queue = nullptr;
+    int* d_item_prefix = nullptr;
+    int cus = 0;
     std::map<Key, FieldEntry> fields;
     std::map<Key, MeshEntry> meshes;
 

@@ -280,6 +280,11 @@ int Ctx::allocate(int n, uint32_t fl, int nslots, int edge_cap, int voxel_cap) {
     LVN_HIP(hipMalloc(&d_counters, sizeof(Counters) * slots));
     LVN_HIP(hipHostMalloc(&h_chunks, sizeof(ChunkDesc) * slots, hipHostMallocDefault));
     LVN_HIP(hipHostMalloc(&h_counters, sizeof(Counters) * slots, hipHostMallocDefault));
+    LVN_HIP(hipMalloc(&d_queue, sizeof(int) * kNumQueues));
+    LVN_HIP(hipMalloc(&d_item_prefix, sizeof(int) * (slots + 1)));
+    int dev = 0;
+    LVN_HIP(hipGetDevice(&dev));
+    LVN_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     return LVN_OK;
 }
 
@@ -319,6 +324,9 @@ KArgs Ctx::kargs() const {
     a.fp16 = (flags & LVN_CTX_FP16_DENSITY) ? 1 : 0;
     a.ysplit = 1;
     a.precise = (flags & LVN_CTX_PRECISE_QEF) ? 1 : 0;
+    a.queue = d_queue;
+    a.item_prefix = d_item_prefix;
+    a.cus = cus;
     return a;
 }
 
@@ -330,6 +338,8 @@ Ctx::~Ctx() {
     if (d_counters) (void)hipFree(d_counters);
     if (d_ops) (void)hipFree(d_ops);
     if (d_op_index) (void)hipFree(d_op_index);
+    if (d_queue) (void)hipFree(d_queue);
+    if (d_item_prefix) (void)hipFree(d_item_prefix);
     if (h_chunks) (void)hipHostFree(h_chunks);
     if (h_counters) (void)hipHostFree(h_counters);
     if (h_ops) (void)hipHostFree(h_ops);
@@ -363,7 +373,7 @@ static int upload_single(Ctx* c, const ChunkDesc& cd_in, const std::vector<lvn_c
         LVN_HIP(hipMemcpyAsync(c->d_op_index, c->h_op_index, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
     }
     LVN_HIP(hipMemcpyAsync(c->d_chunks, c->h_chunks, sizeof(ChunkDesc), hipMemcpyHostToDevice, c->stream));
-    launch_clear_counters(c->d_counters, 1, c->stream);
+    launch_clear_counters(c->kargs(), 1, c->stream);
     return LVN_OK;
 }
 
@@ -794,7 +804,7 @@ int lvn_batch_run_timed(lvn_batch* bh, float* stage_ms, int32_t n_stages) {
     const KArgs a = batch_args(b);
     hipEvent_t ev[kNumStages + 1];
     for (auto& e : ev) LVN_HIP(hipEventCreate(&e));
-    launch_clear_counters(c->d_counters, std::max(b->n, 1), c->stream);
+    launch_clear_counters(a, std::max(b->n, 1), c->stream);
     for (int i = 0; i < kNumStages; ++i) {
         LVN_HIP(hipEventRecord(ev[i], c->stream));
         if (b->n) launch_stage(a, b->n, i, c->stream);

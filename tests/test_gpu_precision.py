@@ -48,7 +48,9 @@ def _errors(oracle, wl, pick, flags):
 @pytest.mark.parametrize("name", ["terrain_512", "precision"])
 def test_precise_qef_vertex_error_vs_fp64(oracle, name):
     wl = workloads.WORKLOADS[name]()
-    pick = [9, 77, 130, 200, 259, 333]
+    # chunk index = x + 8 * y + 64 * z over the 8x8x8 block; y = 3, 4 holds the surface
+    # (and every brush), y = 1 is deep in the caves
+    pick = [25, 100, 155, 230, 285, 357, 9, 333]
     plain_err, _, _ = _errors(oracle, wl, pick, 0)
     prec_err, resolved, verts = _errors(oracle, wl, pick, CTX_PRECISE_QEF | CTX_FP16_DENSITY)
     print(f"\n{name}: {verts} vertices; FP32 plain max {plain_err.max():.3e} p99.9 "
