@@ -58,6 +58,45 @@ def cpu_baseline(wl, threads):
                        f"p99 {np.percentile(secs, 99) * 1e3:.1f} ms per chunk")
 
 
+def serve(args):
+    """Serving form: replay the Poisson stream in real time against a server with a
+    64 GB HBM budget; latency = completion - submission on the server's clock."""
+    wl = workloads.serving(duration=args.serve_seconds)
+    mesher.init(wl["desc"], wl["seed"], wl["default_material"])
+    mesher.clear_csg_ops()
+    for op, bb in zip(wl["ops"], wl["aabbs"]):
+        mesher.store_csg_op(op, bb[:3], bb[3:])
+    server = mesher.Server(64, hbm_budget=64 << 30, max_batch=64, lanes=3)
+    events = wl["events"]
+    t0 = time.perf_counter()
+    for t, kind, payload in events:
+        while time.perf_counter() - t0 < t:
+            pass
+        if kind == "mesh":
+            server.submit_mesh(payload[0], payload[1])
+        else:
+            server.submit_edit(payload[0], payload[1])
+    server.drain()
+    wall = time.perf_counter() - t0
+    res = server.poll(len(events) + 16)
+    mesh = res[res["kind"] == 0]
+    assert (mesh["status"] == 0).all(), "a served chunk failed"
+    lat = (mesh["complete_s"] - mesh["submit_s"]) * 1e3
+    miss = mesh["cache_hit"] == 0
+    st = server.stats()
+    server.close()
+    return {"requests": int(len(res)), "mesh_requests": int(len(mesh)), "edits": int((res["kind"] == 1).sum()),
+            "offered_rate_per_s": wl["rate"], "seconds": round(wall, 3),
+            "throughput_requests_per_s": round(len(res) / wall, 1),
+            "chunks_meshed": st["chunks_meshed"], "batches": st["batches"], "max_batch_seen": st["max_batch_seen"],
+            "cache_hits": st["cache_hits"], "remeshed_after_edit": st["remeshed_after_edit"],
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+                           "max": round(float(lat.max()), 3)},
+            "miss_latency_ms": {"p50": round(float(np.percentile(lat[miss], 50)), 3),
+                                "p99": round(float(np.percentile(lat[miss], 99)), 3)} if miss.any() else None,
+            "hbm_budget_gb": 64}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +105,7 @@ def main():
     ap.add_argument("--workload", default="terrain_512", choices=sorted(workloads.WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--plain-chunks", type=int, default=0, help="chunks for the plain form (0 = all)")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="length of the served request stream (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the build is single-GPU); got --gpus {args.gpus}", file=sys.stderr)
@@ -149,6 +189,9 @@ def main():
                      "traffic": None, "flops_per_launch": int(dom_flops),
                      "launch_ms": round(stage_ms[dom], 4)},
     }
+    if args.serve_seconds > 0:
+        out["serving"] = serve(args)
+        out["p99_chunk_latency_ms"] = out["serving"]["latency_ms"]["p99"]
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(wl, threads)

@@ -181,6 +181,57 @@ int  lvn_batch_get_mesh(lvn_batch* b, int32_t i, lvn_vertex* v, int32_t v_cap, i
  * evaluations in the edge-crossing stage, work[5] density samples, work[6] edges. */
 int  lvn_batch_work(lvn_batch* b, uint64_t* work);
 
+/* ---- serving form: continuous batching with device-resident chunk state ---- */
+
+#define LVN_REQ_MESH  0   /* mesh the chunk (min, size) */
+#define LVN_REQ_EDIT  1   /* store a CSG op and invalidate the chunks its AABB overlaps */
+
+typedef struct lvn_request {
+    int32_t    kind;
+    int32_t    min[3];       /* mesh: chunk min, world units */
+    int32_t    size;         /* mesh: chunk size, world units */
+    int32_t    aabb_min[3];  /* edit: op AABB, world units (CalcCSGOperationBounds) */
+    int32_t    aabb_max[3];
+    int32_t    pad;
+    lvn_csg_op op;           /* edit */
+} lvn_request;
+
+typedef struct lvn_result {
+    int64_t ticket;
+    int32_t kind;
+    int32_t status;          /* LVN_OK or a negative LVN_ERR_* */
+    int32_t nv, nt, ns;      /* mesh: exported vertices, triangles, seam nodes */
+    int32_t cache_hit;       /* 1 when served from the device-resident cache */
+    double  submit_s;        /* host steady clock, seconds */
+    double  complete_s;
+} lvn_result;
+
+typedef struct lvn_server lvn_server;
+
+/* A server whose chunk slots (each holding one chunk's field and mesh in HBM) fit in
+ * hbm_budget bytes.  A scheduler thread admits waiting requests into free slots (evicting
+ * least-recently-used finished chunks), forms batches of up to max_batch chunks and keeps
+ * up to `lanes` batches in flight on separate streams.  Each chunk is meshed as a fresh
+ * chunk with the stored ops overlapping it, so a result does not depend on what else is
+ * in flight.  No reference counterpart: the reference meshes chunks one at a time on one
+ * task thread (src/volume.cpp:68-113). */
+int  lvn_server_create(int32_t voxels_per_chunk, uint64_t hbm_budget, int32_t max_batch, int32_t lanes,
+                       int32_t edge_cap, int32_t voxel_cap, uint32_t flags, lvn_server** out);
+void lvn_server_destroy(lvn_server* s);
+/* Non-blocking.  An edit completes when it is recorded; later mesh requests see it. */
+int  lvn_server_submit(lvn_server* s, const lvn_request* req, int64_t* ticket);
+/* Moves up to cap completed results into out (completion order); *n = number moved. */
+int  lvn_server_poll(lvn_server* s, lvn_result* out, int32_t cap, int32_t* n);
+/* Blocks until every submitted request has completed. */
+int  lvn_server_drain(lvn_server* s);
+/* Copies the current mesh of a completed mesh request's chunk (LVN_ERR_STATE if it has
+ * since been evicted or invalidated by an edit). */
+int  lvn_server_get_mesh(lvn_server* s, int64_t ticket, lvn_vertex* v, int32_t v_cap, int32_t* nv,
+                         int32_t* tri, int32_t tri_cap, int32_t* nt, lvn_seam_node* sn, int32_t s_cap, int32_t* ns);
+/* stats[0..7] = slots, batches launched, chunks meshed, cache hits, evictions,
+ * results re-meshed because an edit arrived in flight, cached chunks, max batch seen. */
+int  lvn_server_stats(lvn_server* s, int64_t* stats);
+
 /* ---- test hook: intermediate stages of one chunk --------------------------- */
 
 /* Buffers for lvn_debug_chunk_stages.  F = voxels_per_chunk + 2, H = voxels_per_chunk + 1.

@@ -100,6 +100,42 @@ assert C.sizeof(Vertex) == 48
 assert C.sizeof(SeamNode) == 48
 assert C.sizeof(FieldDesc) == 48
 
+REQ_MESH = 0
+REQ_EDIT = 1
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("min", C.c_int32 * 3),
+        ("size", C.c_int32),
+        ("aabb_min", C.c_int32 * 3),
+        ("aabb_max", C.c_int32 * 3),
+        ("pad", C.c_int32),
+        ("op", CsgOp),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("ticket", C.c_int64),
+        ("kind", C.c_int32),
+        ("status", C.c_int32),
+        ("nv", C.c_int32),
+        ("nt", C.c_int32),
+        ("ns", C.c_int32),
+        ("cache_hit", C.c_int32),
+        ("submit_s", C.c_double),
+        ("complete_s", C.c_double),
+    ]
+
+
+assert C.sizeof(Request) == 96
+assert C.sizeof(Result) == 48
+
+RESULT_DTYPE = np.dtype([("ticket", "<i8"), ("kind", "<i4"), ("status", "<i4"), ("nv", "<i4"), ("nt", "<i4"),
+                         ("ns", "<i4"), ("cache_hit", "<i4"), ("submit_s", "<f8"), ("complete_s", "<f8")])
+
 # numpy views of the same layouts
 CSG_OP_DTYPE = np.dtype(
     [("type", "<i4"), ("brush", "<i4"), ("material", "<i4"), ("rotate_y", "<f4"),

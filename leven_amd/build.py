@@ -13,7 +13,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblvn_mesher.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["lvn_kernels.hip", "lvn_runtime.hip"]
+SOURCES = ["lvn_kernels.hip", "lvn_runtime.hip", "lvn_server.hip"]
 
 # -ffp-contract=off: the kernels follow the reference's operation order without
 # fused multiply-adds, so results are bit-identical to the IEEE FP32 restatement.

@@ -29,6 +29,7 @@ struct KArgs {
     int* item_prefix;        // chunks + 1 edge-item offsets (k_edge_items)
     int cus;                 // compute units of the device
     int pad2;
+    const int* slot_map;     // batch entry -> arena slot, or null for the identity
 };
 
 constexpr int kNumQueues = 8;

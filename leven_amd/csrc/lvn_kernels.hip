@@ -19,9 +19,11 @@ namespace lvn {
 
 namespace {
 
+// Batch entry `chunk` lives in arena slot slot_map[chunk] (identity without a map).
 template <typename T>
 __device__ __forceinline__ T* slot_ptr(const KArgs& a, int chunk, uint64_t off) {
-    return reinterpret_cast<T*>(a.arena + (uint64_t)chunk * a.L.slot_bytes + off);
+    const uint64_t slot = a.slot_map ? (uint64_t)a.slot_map[chunk] : (uint64_t)chunk;
+    return reinterpret_cast<T*>(a.arena + slot * a.L.slot_bytes + off);
 }
 
 __device__ __forceinline__ int field_index(const Geometry& G, int x, int y, int z) {

@@ -1,0 +1,528 @@
+// lvn_server.hip — serving form: continuous batching over device-resident chunk slots.
+//
+// The batch arena's slots hold each served chunk's field and mesh in HBM; the
+// number of slots is what the HBM budget admits.  A scheduler thread takes
+// submitted requests, answers cache hits at once, gives each miss a free slot
+// (evicting the least-recently-used finished chunk), and launches batches of
+// misses on one of several lanes (stream + descriptors + counters + work
+// queues) as soon as a lane is free, so batches overlap on the device and new
+// requests join the next batch instead of waiting for a queue to drain.
+
+#include "lvn_kernels.h"
+#include "lvn_runtime.h"
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace lvn {
+
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Pending {
+    int64_t ticket;
+    lvn_request req;
+    double submit;
+};
+
+struct Job {
+    int64_t ticket;
+    Key key;
+    double submit;
+};
+
+struct Lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    ChunkDesc* d_chunks = nullptr;
+    ChunkDesc* h_chunks = nullptr;
+    Counters* d_counters = nullptr;
+    Counters* h_counters = nullptr;
+    int* d_slot_map = nullptr;
+    int* h_slot_map = nullptr;
+    int* d_queue = nullptr;
+    int* d_item_prefix = nullptr;
+    DevOp* d_ops = nullptr;
+    DevOp* h_ops = nullptr;
+    int ops_cap = 0;
+    int* d_op_index = nullptr;
+    int* h_op_index = nullptr;
+    int idx_cap = 0;
+    bool busy = false;
+    std::vector<Job> jobs;
+    std::vector<Key> keys;
+    std::vector<uint64_t> versions;
+};
+
+struct Entry {
+    int slot = -1;
+    bool valid = false;       // the slot holds this chunk's current mesh
+    bool in_flight = false;   // a lane is meshing it
+    uint64_t version = 0;     // bumped by every edit overlapping the chunk
+    uint64_t last_use = 0;
+    int nv = 0, nt = 0, ns = 0, status = 0;
+    std::vector<Job> waiters; // requests for this chunk that arrived while it was in flight
+};
+
+}  // namespace
+
+struct Server {
+    Ctx arena;
+    int max_batch = 64;
+    std::vector<Lane> lanes;
+
+    std::mutex mu;                 // incoming, completed, outstanding, stop
+    std::condition_variable cv;
+    std::deque<Pending> incoming;
+    std::deque<lvn_result> completed;
+    int64_t next_ticket = 1;
+    int64_t outstanding = 0;
+    bool stop = false;
+    int error = LVN_OK;
+
+    std::mutex cache_mu;           // cache + ticket_key (read by lvn_server_get_mesh)
+    std::map<Key, Entry> cache;
+    std::unordered_map<int64_t, Key> ticket_key;
+
+    // scheduler-thread state
+    std::vector<int> free_slots;
+    std::deque<Job> waiting;       // misses not yet admitted
+    std::vector<lvn_csg_op> ops;
+    std::vector<std::array<int32_t, 6>> aabbs;
+    uint64_t tick = 0;
+    int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    std::thread worker;
+
+    int init_lane(Lane& L);
+    void run();
+    void finish(int64_t ticket, int kind, int status, const Entry* e, bool hit, double submit);
+    void admit_and_launch();
+    void poll_lanes();
+    void handle(const Pending& p);
+    int launch(Lane& L, const std::vector<Job>& batch, const std::vector<int>& slots);
+    ~Server();
+};
+
+#define LVN_TRY(call)                                    \
+    do {                                                 \
+        if ((call) != hipSuccess) return LVN_ERR_DEVICE; \
+    } while (0)
+
+int Server::init_lane(Lane& L) {
+    const int B = max_batch;
+    LVN_TRY(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+    LVN_TRY(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
+    LVN_TRY(hipMalloc(&L.d_chunks, sizeof(ChunkDesc) * B));
+    LVN_TRY(hipHostMalloc(&L.h_chunks, sizeof(ChunkDesc) * B, hipHostMallocDefault));
+    LVN_TRY(hipMalloc(&L.d_counters, sizeof(Counters) * B));
+    LVN_TRY(hipHostMalloc(&L.h_counters, sizeof(Counters) * B, hipHostMallocDefault));
+    LVN_TRY(hipMalloc(&L.d_slot_map, sizeof(int) * B));
+    LVN_TRY(hipHostMalloc(&L.h_slot_map, sizeof(int) * B, hipHostMallocDefault));
+    LVN_TRY(hipMalloc(&L.d_queue, sizeof(int) * kNumQueues));
+    LVN_TRY(hipMalloc(&L.d_item_prefix, sizeof(int) * (B + 1)));
+    return LVN_OK;
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        stop = true;
+    }
+    cv.notify_all();
+    if (worker.joinable()) worker.join();
+    for (Lane& L : lanes) {
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
+        if (L.done) (void)hipEventDestroy(L.done);
+        if (L.d_chunks) (void)hipFree(L.d_chunks);
+        if (L.h_chunks) (void)hipHostFree(L.h_chunks);
+        if (L.d_counters) (void)hipFree(L.d_counters);
+        if (L.h_counters) (void)hipHostFree(L.h_counters);
+        if (L.d_slot_map) (void)hipFree(L.d_slot_map);
+        if (L.h_slot_map) (void)hipHostFree(L.h_slot_map);
+        if (L.d_queue) (void)hipFree(L.d_queue);
+        if (L.d_item_prefix) (void)hipFree(L.d_item_prefix);
+        if (L.d_ops) (void)hipFree(L.d_ops);
+        if (L.h_ops) (void)hipHostFree(L.h_ops);
+        if (L.d_op_index) (void)hipFree(L.d_op_index);
+        if (L.h_op_index) (void)hipHostFree(L.h_op_index);
+        if (L.stream) (void)hipStreamDestroy(L.stream);
+    }
+}
+
+void Server::finish(int64_t ticket, int kind, int status, const Entry* e, bool hit, double submit) {
+    lvn_result r;
+    std::memset(&r, 0, sizeof(r));
+    r.ticket = ticket;
+    r.kind = kind;
+    r.status = status;
+    if (e) {
+        r.nv = e->nv;
+        r.nt = e->nt;
+        r.ns = e->ns;
+    }
+    r.cache_hit = hit ? 1 : 0;
+    r.submit_s = submit;
+    r.complete_s = now_s();
+    std::lock_guard<std::mutex> lock(mu);
+    completed.push_back(r);
+    --outstanding;
+    cv.notify_all();
+}
+
+// An edit stores the op and invalidates every chunk whose AABB it overlaps; a mesh
+// request is a cache hit, joins an in-flight mesh of the same chunk, or waits.
+void Server::handle(const Pending& p) {
+    if (p.req.kind == LVN_REQ_EDIT) {
+        ops.push_back(p.req.op);
+        aabbs.push_back({p.req.aabb_min[0], p.req.aabb_min[1], p.req.aabb_min[2], p.req.aabb_max[0],
+                         p.req.aabb_max[1], p.req.aabb_max[2]});
+        {
+            std::lock_guard<std::mutex> lock(cache_mu);
+            for (auto& kv : cache) {
+                const int32_t mn[3] = {std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)};
+                if (!aabb_overlaps(mn, std::get<3>(kv.first), aabbs.back())) continue;
+                Entry& e = kv.second;
+                ++e.version;
+                if (e.valid) {
+                    e.valid = false;
+                    if (!e.in_flight) {
+                        free_slots.push_back(e.slot);
+                        e.slot = -1;
+                    }
+                }
+            }
+        }
+        finish(p.ticket, LVN_REQ_EDIT, LVN_OK, nullptr, false, p.submit);
+        return;
+    }
+    const Key key = make_key(p.req.min, p.req.size);
+    {
+        std::lock_guard<std::mutex> lock(cache_mu);
+        ticket_key[p.ticket] = key;
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            Entry& e = it->second;
+            if (e.valid && !e.in_flight) {
+                e.last_use = ++tick;
+                ++stats[3];
+                finish(p.ticket, LVN_REQ_MESH, e.status, &e, true, p.submit);
+                return;
+            }
+            if (e.in_flight) {
+                e.waiters.push_back(Job{p.ticket, key, p.submit});
+                return;
+            }
+        }
+    }
+    waiting.push_back(Job{p.ticket, key, p.submit});
+}
+
+int Server::launch(Lane& L, const std::vector<Job>& batch, const std::vector<int>& slots) {
+    const int n = (int)batch.size();
+    const int nops = (int)ops.size();
+    std::vector<int> index;
+    for (int i = 0; i < n; ++i) {
+        const int32_t mn[3] = {std::get<0>(batch[i].key), std::get<1>(batch[i].key), std::get<2>(batch[i].key)};
+        const int size = std::get<3>(batch[i].key);
+        ChunkDesc& cd = L.h_chunks[i];
+        fill_chunk_desc(cd, mn, size, arena.N);
+        cd.op_start = (int)index.size();
+        for (int k = 0; k < nops; ++k)
+            if (aabb_overlaps(mn, size, aabbs[k])) index.push_back(k);
+        cd.op_count = (int)index.size() - cd.op_start;
+        L.h_slot_map[i] = slots[i];
+    }
+    if (std::max(nops, 1) > L.ops_cap) {
+        if (L.d_ops) LVN_TRY(hipFree(L.d_ops));
+        if (L.h_ops) LVN_TRY(hipHostFree(L.h_ops));
+        L.ops_cap = std::max(2 * L.ops_cap, std::max(nops, 64));
+        LVN_TRY(hipMalloc(&L.d_ops, sizeof(DevOp) * L.ops_cap));
+        LVN_TRY(hipHostMalloc(&L.h_ops, sizeof(DevOp) * L.ops_cap, hipHostMallocDefault));
+    }
+    if (std::max((int)index.size(), 1) > L.idx_cap) {
+        if (L.d_op_index) LVN_TRY(hipFree(L.d_op_index));
+        if (L.h_op_index) LVN_TRY(hipHostFree(L.h_op_index));
+        L.idx_cap = std::max(2 * L.idx_cap, std::max((int)index.size(), 256));
+        LVN_TRY(hipMalloc(&L.d_op_index, sizeof(int) * L.idx_cap));
+        LVN_TRY(hipHostMalloc(&L.h_op_index, sizeof(int) * L.idx_cap, hipHostMallocDefault));
+    }
+    for (int k = 0; k < nops; ++k) L.h_ops[k] = make_dev_op(ops[k]);
+    if (!index.empty()) std::memcpy(L.h_op_index, index.data(), sizeof(int) * index.size());
+    if (nops) LVN_TRY(hipMemcpyAsync(L.d_ops, L.h_ops, sizeof(DevOp) * nops, hipMemcpyHostToDevice, L.stream));
+    if (!index.empty())
+        LVN_TRY(hipMemcpyAsync(L.d_op_index, L.h_op_index, sizeof(int) * index.size(), hipMemcpyHostToDevice,
+                               L.stream));
+    LVN_TRY(hipMemcpyAsync(L.d_chunks, L.h_chunks, sizeof(ChunkDesc) * n, hipMemcpyHostToDevice, L.stream));
+    LVN_TRY(hipMemcpyAsync(L.d_slot_map, L.h_slot_map, sizeof(int) * n, hipMemcpyHostToDevice, L.stream));
+    KArgs a = arena.kargs();
+    a.chunks = L.d_chunks;
+    a.counters = L.d_counters;
+    a.slot_map = L.d_slot_map;
+    a.queue = L.d_queue;
+    a.item_prefix = L.d_item_prefix;
+    a.ops = L.d_ops;
+    a.op_index = L.d_op_index;
+    a.ysplit = n >= 32 ? 1 : 6;
+    launch_pipeline(a, n, L.stream);
+    LVN_TRY(hipGetLastError());
+    LVN_TRY(hipMemcpyAsync(L.h_counters, L.d_counters, sizeof(Counters) * n, hipMemcpyDeviceToHost, L.stream));
+    LVN_TRY(hipEventRecord(L.done, L.stream));
+    L.busy = true;
+    L.jobs = batch;
+    ++stats[1];
+    stats[2] += n;
+    stats[7] = std::max<int64_t>(stats[7], n);
+    return LVN_OK;
+}
+
+// Give waiting misses slots (free, else the least-recently-used finished chunk) and
+// launch them on free lanes, oldest first.
+void Server::admit_and_launch() {
+    for (Lane& L : lanes) {
+        if (L.busy || waiting.empty()) continue;
+        std::vector<Job> batch;
+        std::vector<int> slots;
+        {
+            std::lock_guard<std::mutex> lock(cache_mu);
+            while (!waiting.empty() && (int)batch.size() < max_batch) {
+                Job j = waiting.front();
+                Entry& e = cache[j.key];
+                if (e.in_flight) {  // an earlier request in this pass took it
+                    e.waiters.push_back(j);
+                    waiting.pop_front();
+                    continue;
+                }
+                if (e.valid) {  // meshed while this request waited
+                    e.last_use = ++tick;
+                    ++stats[3];
+                    finish(j.ticket, LVN_REQ_MESH, e.status, &e, true, j.submit);
+                    waiting.pop_front();
+                    continue;
+                }
+                int slot = -1;
+                if (e.slot >= 0) {
+                    slot = e.slot;
+                } else if (!free_slots.empty()) {
+                    slot = free_slots.back();
+                    free_slots.pop_back();
+                } else {
+                    Entry* victim = nullptr;
+                    for (auto& kv : cache)
+                        if (kv.second.valid && !kv.second.in_flight && kv.second.slot >= 0 &&
+                            (!victim || kv.second.last_use < victim->last_use))
+                            victim = &kv.second;
+                    if (!victim) break;  // every slot is in flight: admit later
+                    slot = victim->slot;
+                    victim->slot = -1;
+                    victim->valid = false;
+                    ++stats[4];
+                }
+                e.slot = slot;
+                e.in_flight = true;
+                batch.push_back(j);
+                slots.push_back(slot);
+                waiting.pop_front();
+            }
+            L.versions.clear();
+            for (const Job& j : batch) L.versions.push_back(cache[j.key].version);
+        }
+        if (batch.empty()) continue;
+        const int r = launch(L, batch, slots);
+        if (r != LVN_OK) {
+            std::lock_guard<std::mutex> lock(mu);
+            error = r;
+        }
+    }
+}
+
+void Server::poll_lanes() {
+    for (Lane& L : lanes) {
+        if (!L.busy) continue;
+        if (hipEventQuery(L.done) != hipSuccess) continue;
+        std::vector<Job> requeue;
+        {
+            std::lock_guard<std::mutex> lock(cache_mu);
+            for (size_t i = 0; i < L.jobs.size(); ++i) {
+                const Job& j = L.jobs[i];
+                Entry& e = cache[j.key];
+                e.in_flight = false;
+                if (e.version != L.versions[i]) {
+                    // an edit arrived while this chunk was meshed: mesh it again
+                    ++stats[5];
+                    free_slots.push_back(e.slot);
+                    e.slot = -1;
+                    requeue.push_back(j);
+                    for (const Job& w : e.waiters) requeue.push_back(w);
+                    e.waiters.clear();
+                    continue;
+                }
+                const Counters& k = L.h_counters[i];
+                e.valid = true;
+                e.status = k.status ? LVN_ERR_OVERFLOW : LVN_OK;
+                e.nv = k.edges > 0 && k.quads > 0 ? k.voxels : 0;
+                e.nt = k.edges > 0 ? 2 * k.quads : 0;
+                e.ns = k.edges > 0 ? k.seams : 0;
+                e.last_use = ++tick;
+                finish(j.ticket, LVN_REQ_MESH, e.status, &e, false, j.submit);
+                for (const Job& w : e.waiters) finish(w.ticket, LVN_REQ_MESH, e.status, &e, true, w.submit);
+                e.waiters.clear();
+            }
+            stats[6] = 0;
+            for (auto& kv : cache) stats[6] += kv.second.valid ? 1 : 0;
+        }
+        for (auto it = requeue.rbegin(); it != requeue.rend(); ++it) waiting.push_front(*it);
+        L.busy = false;
+        L.jobs.clear();
+    }
+}
+
+void Server::run() {
+    for (;;) {
+        std::deque<Pending> batch;
+        {
+            std::unique_lock<std::mutex> lock(mu);
+            const bool busy = std::any_of(lanes.begin(), lanes.end(), [](const Lane& L) { return L.busy; });
+            if (incoming.empty() && waiting.empty() && !busy) {
+                if (stop) return;
+                cv.wait_for(lock, std::chrono::milliseconds(2));
+                continue;
+            }
+            batch.swap(incoming);
+        }
+        for (const Pending& p : batch) handle(p);
+        poll_lanes();
+        admit_and_launch();
+        const bool busy = std::any_of(lanes.begin(), lanes.end(), [](const Lane& L) { return L.busy; });
+        if (busy) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+
+}  // namespace lvn
+
+using namespace lvn;
+
+extern "C" {
+
+int lvn_server_create(int32_t voxels_per_chunk, uint64_t hbm_budget, int32_t max_batch, int32_t lanes,
+                      int32_t edge_cap, int32_t voxel_cap, uint32_t flags, lvn_server** out) {
+    if (!out) return LVN_ERR_INVALID;
+    *out = nullptr;
+    if (!g_state.init) return LVN_ERR_STATE;
+    const int n = voxels_per_chunk;
+    if (n < 4 || n > 64 || (n & (n - 1)) || max_batch < 1 || lanes < 1 || lanes > 8 || edge_cap < 0 ||
+        voxel_cap < 0)
+        return LVN_ERR_INVALID;
+    const SlotLayout L = make_layout(make_geometry(n), make_caps(make_geometry(n), edge_cap, voxel_cap));
+    const uint64_t nslots = hbm_budget / L.slot_bytes;
+    if (nslots < (uint64_t)max_batch) return LVN_ERR_NOMEM;
+    Server* s = new Server;
+    s->max_batch = max_batch;
+    int r = s->arena.allocate(n, flags, (int)std::min<uint64_t>(nslots, 1 << 20), edge_cap, voxel_cap);
+    if (!r) {
+        s->lanes.resize((size_t)lanes);
+        for (Lane& ln : s->lanes)
+            if (!r) r = s->init_lane(ln);
+    }
+    if (r) {
+        delete s;
+        return r == LVN_ERR_DEVICE ? LVN_ERR_NOMEM : r;
+    }
+    for (int i = s->arena.slots - 1; i >= 0; --i) s->free_slots.push_back(i);
+    s->stats[0] = s->arena.slots;
+    s->ops = g_state.ops;
+    s->aabbs = g_state.aabbs;
+    s->worker = std::thread([s] { s->run(); });
+    *out = reinterpret_cast<lvn_server*>(s);
+    return LVN_OK;
+}
+
+void lvn_server_destroy(lvn_server* sh) { delete reinterpret_cast<Server*>(sh); }
+
+int lvn_server_submit(lvn_server* sh, const lvn_request* req, int64_t* ticket) {
+    Server* s = reinterpret_cast<Server*>(sh);
+    if (!s || !req || !ticket) return LVN_ERR_INVALID;
+    if (req->kind == LVN_REQ_MESH) {
+        if (req->size < s->arena.N * kLeafSizeScale || req->size % (s->arena.N * kLeafSizeScale)) return LVN_ERR_INVALID;
+    } else if (req->kind == LVN_REQ_EDIT) {
+        const lvn_csg_op& o = req->op;
+        if (o.material < 0 || o.material > 255 || o.type < 0 || o.type > 1 || o.brush < 0 || o.brush > 1)
+            return LVN_ERR_INVALID;
+    } else {
+        return LVN_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (s->error) return s->error;
+    *ticket = s->next_ticket++;
+    s->incoming.push_back(Pending{*ticket, *req, now_s()});
+    ++s->outstanding;
+    s->cv.notify_all();
+    return LVN_OK;
+}
+
+int lvn_server_poll(lvn_server* sh, lvn_result* out, int32_t cap, int32_t* n) {
+    Server* s = reinterpret_cast<Server*>(sh);
+    if (!s || !n || cap < 0 || (cap > 0 && !out)) return LVN_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(s->mu);
+    int k = 0;
+    while (k < cap && !s->completed.empty()) {
+        out[k++] = s->completed.front();
+        s->completed.pop_front();
+    }
+    *n = k;
+    return s->error;
+}
+
+int lvn_server_drain(lvn_server* sh) {
+    Server* s = reinterpret_cast<Server*>(sh);
+    if (!s) return LVN_ERR_INVALID;
+    std::unique_lock<std::mutex> lock(s->mu);
+    s->cv.wait(lock, [s] { return s->outstanding == 0 || s->error != LVN_OK; });
+    return s->error;
+}
+
+int lvn_server_get_mesh(lvn_server* sh, int64_t ticket, lvn_vertex* v, int32_t v_cap, int32_t* nv, int32_t* tri,
+                        int32_t tri_cap, int32_t* nt, lvn_seam_node* sn, int32_t s_cap, int32_t* ns) {
+    Server* s = reinterpret_cast<Server*>(sh);
+    if (!s || !nv || !nt || !ns) return LVN_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(s->cache_mu);
+    auto tk = s->ticket_key.find(ticket);
+    if (tk == s->ticket_key.end()) return LVN_ERR_INVALID;
+    auto it = s->cache.find(tk->second);
+    if (it == s->cache.end() || !it->second.valid || it->second.in_flight || it->second.slot < 0) return LVN_ERR_STATE;
+    const Entry& e = it->second;
+    *nv = e.nv;
+    *nt = e.nt;
+    *ns = e.ns;
+    if (e.status) return e.status;
+    if (e.nv > v_cap || e.nt > tri_cap || e.ns > s_cap) return LVN_ERR_CAPACITY;
+    const Ctx& c = s->arena;
+    const char* slot = c.d_arena + (uint64_t)e.slot * c.L.slot_bytes;
+    if (e.nv && v && hipMemcpy(v, slot + c.L.verts, sizeof(lvn_vertex) * (size_t)e.nv, hipMemcpyDeviceToHost))
+        return LVN_ERR_DEVICE;
+    if (e.nt && tri && hipMemcpy(tri, slot + c.L.tris, 12 * (size_t)e.nt, hipMemcpyDeviceToHost)) return LVN_ERR_DEVICE;
+    if (e.ns && sn && hipMemcpy(sn, slot + c.L.seams, sizeof(lvn_seam_node) * (size_t)e.ns, hipMemcpyDeviceToHost))
+        return LVN_ERR_DEVICE;
+    return LVN_OK;
+}
+
+int lvn_server_stats(lvn_server* sh, int64_t* stats) {
+    Server* s = reinterpret_cast<Server*>(sh);
+    if (!s || !stats) return LVN_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(s->cache_mu);
+    for (int i = 0; i < 8; ++i) stats[i] = s->stats[i];
+    return LVN_OK;
+}
+
+}  // extern "C"

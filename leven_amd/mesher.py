@@ -9,8 +9,8 @@ import os
 
 import numpy as np
 
-from .abi import (CSG_OP_DTYPE, ERROR_NAMES, LVN_ERR_CAPACITY, LVN_OK, SEAM_DTYPE, VERTEX_DTYPE, FieldDesc,
-                  StageDump, ptr, vptr)
+from .abi import (CSG_OP_DTYPE, ERROR_NAMES, LVN_ERR_CAPACITY, LVN_OK, REQ_EDIT, REQ_MESH, RESULT_DTYPE, SEAM_DTYPE,
+                  VERTEX_DTYPE, CsgOp, FieldDesc, Request, StageDump, ptr, vptr)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblvn_mesher.so")
@@ -22,6 +22,8 @@ EXPORTS = [
     "lvn_free_chunk", "lvn_chunk_has_surface", "lvn_generate_chunk_mesh", "lvn_debug_chunk_stages",
     "lvn_batch_create", "lvn_batch_destroy", "lvn_batch_set_chunks", "lvn_batch_run", "lvn_batch_wait",
     "lvn_batch_run_timed", "lvn_batch_stage_name", "lvn_batch_counts", "lvn_batch_get_mesh", "lvn_batch_work",
+    "lvn_server_create", "lvn_server_destroy", "lvn_server_submit", "lvn_server_poll", "lvn_server_drain",
+    "lvn_server_get_mesh", "lvn_server_stats",
 ]
 
 _lib = None
@@ -69,6 +71,14 @@ def lib():
             "lvn_batch_get_mesh": (i32, [vp, i32, vp, i32, C.POINTER(i32), vp, i32, C.POINTER(i32), vp, i32,
                                          C.POINTER(i32)]),
             "lvn_batch_work": (i32, [vp, vp]),
+            "lvn_server_create": (i32, [i32, C.c_uint64, i32, i32, i32, i32, u32, C.POINTER(vp)]),
+            "lvn_server_destroy": (None, [vp]),
+            "lvn_server_submit": (i32, [vp, C.POINTER(Request), C.POINTER(C.c_int64)]),
+            "lvn_server_poll": (i32, [vp, vp, i32, C.POINTER(i32)]),
+            "lvn_server_drain": (i32, [vp]),
+            "lvn_server_get_mesh": (i32, [vp, C.c_int64, vp, i32, C.POINTER(i32), vp, i32, C.POINTER(i32), vp, i32,
+                                          C.POINTER(i32)]),
+            "lvn_server_stats": (i32, [vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -282,4 +292,75 @@ class Batch:
         nv, nt, ns = C.c_int32(), C.c_int32(), C.c_int32()
         check(lib().lvn_batch_get_mesh(self._h, int(i), vptr(v), len(v), C.byref(nv), vptr(t), len(t), C.byref(nt),
                                        vptr(s), len(s), C.byref(ns)), "lvn_batch_get_mesh")
+        return v[:nv.value].copy(), t[:nt.value].copy(), s[:ns.value].copy()
+
+
+class Server:
+    """The serving form: continuous batching over device-resident chunk slots."""
+
+    STATS = ["slots", "batches", "chunks_meshed", "cache_hits", "evictions", "remeshed_after_edit",
+             "cached_chunks", "max_batch_seen"]
+
+    def __init__(self, voxels_per_chunk=64, hbm_budget=64 << 30, max_batch=64, lanes=3, edge_cap=262144,
+                 voxel_cap=131072, flags=0):
+        h = C.c_void_p()
+        check(lib().lvn_server_create(int(voxels_per_chunk), int(hbm_budget), int(max_batch), int(lanes),
+                                      int(edge_cap), int(voxel_cap), int(flags), C.byref(h)), "lvn_server_create")
+        self._h = h
+        self.n = voxels_per_chunk
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lvn_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit_mesh(self, min_xyz, size):
+        r = Request()
+        r.kind = REQ_MESH
+        for i in range(3):
+            r.min[i] = int(min_xyz[i])
+        r.size = int(size)
+        t = C.c_int64()
+        check(lib().lvn_server_submit(self._h, C.byref(r), C.byref(t)), "lvn_server_submit")
+        return t.value
+
+    def submit_edit(self, op, aabb):
+        r = Request()
+        r.kind = REQ_EDIT
+        o = np.ascontiguousarray(op, dtype=CSG_OP_DTYPE).reshape(1)
+        C.memmove(C.byref(r.op), o.ctypes.data, 48)
+        for i in range(3):
+            r.aabb_min[i] = int(aabb[i])
+            r.aabb_max[i] = int(aabb[3 + i])
+        t = C.c_int64()
+        check(lib().lvn_server_submit(self._h, C.byref(r), C.byref(t)), "lvn_server_submit")
+        return t.value
+
+    def poll(self, cap=65536):
+        out = np.zeros(cap, dtype=RESULT_DTYPE)
+        n = C.c_int32()
+        check(lib().lvn_server_poll(self._h, vptr(out), cap, C.byref(n)), "lvn_server_poll")
+        return out[: n.value].copy()
+
+    def drain(self):
+        check(lib().lvn_server_drain(self._h), "lvn_server_drain")
+
+    def stats(self):
+        s = np.zeros(8, dtype=np.int64)
+        check(lib().lvn_server_stats(self._h, vptr(s)), "lvn_server_stats")
+        return dict(zip(self.STATS, (int(v) for v in s)))
+
+    def get_mesh(self, ticket, v_cap=131072, tri_cap=393216, s_cap=131072):
+        v = np.zeros(v_cap, dtype=VERTEX_DTYPE)
+        t = np.zeros((tri_cap, 3), dtype=np.int32)
+        s = np.zeros(s_cap, dtype=SEAM_DTYPE)
+        nv, nt, ns = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().lvn_server_get_mesh(self._h, int(ticket), vptr(v), v_cap, C.byref(nv), vptr(t), tri_cap,
+                                        C.byref(nt), vptr(s), s_cap, C.byref(ns)), "lvn_server_get_mesh")
         return v[:nv.value].copy(), t[:nt.value].copy(), s[:ns.value].copy()

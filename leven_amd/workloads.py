@@ -150,4 +150,63 @@ def clipmap(seed=4):
                 ops=ops, aabbs=aabbs, flags=0, camera=cam)
 
 
-WORKLOADS = {"plain_cpu": plain_cpu, "terrain_512": terrain_512, "precision": precision, "clipmap": clipmap}
+def serving(seed=5, duration=2.0, rate=4000.0, edit_fraction=0.05, speed_voxels=10.0):
+    """Poisson request stream (rate per second) along a seeded camera path at
+    speed_voxels voxels/s over the terrain_512 field with its 256 brushes.  A mesh
+    request picks a chunk uniformly from the 6-LOD clipmap selection around the camera
+    at its arrival time; an edit (edit_fraction of requests) drops a brush near the
+    camera and is followed, at the same time, by mesh requests for every selected chunk
+    its AABB overlaps (the forced re-mesh).  Returns the event list sorted by time:
+    (t, "mesh", (min, size)) or (t, "edit", (op, aabb))."""
+    rng = np.random.default_rng(seed)
+    base = terrain_512()
+    cam0 = np.array([rng.uniform(512, 1536), 0.0, rng.uniform(512, 1536)])
+    ang = rng.uniform(0, 2 * math.pi)
+    vel = np.array([math.cos(ang), 0.0, math.sin(ang)]) * speed_voxels * LEAF_SIZE_SCALE
+    R = LOD_MAX_NODE_SIZE
+    step = 0.25
+    selections = {}
+
+    def selection(t):
+        k = int(t / step)
+        if k not in selections:
+            cam = cam0 + vel * (k * step)
+            c = (int(cam[0]) // R) * R, (int(cam[2]) // R) * R
+            selections[k] = clipmap_select(cam, (c[0] - 2 * R, -R, c[1] - 2 * R), (c[0] + 2 * R, R, c[1] + 2 * R))
+        return selections[k]
+
+    events = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            break
+        sel = selection(t)
+        if rng.random() < edit_fraction:
+            cam = cam0 + vel * t
+            op = np.zeros(1, CSG_OP_DTYPE)[0]
+            is_add = rng.random() < 0.5
+            r = rng.uniform(2.0, 8.0)
+            c = cam / LEAF_SIZE_SCALE + rng.uniform(-32, 32, 3) * np.array([1, 0.5, 1])
+            op["type"] = 0 if is_add else 1
+            op["brush"] = int(rng.integers(0, 2))
+            op["material"] = int(rng.integers(1, 4)) if is_add else MATERIAL_AIR
+            op["origin"] = [c[0] + 0.5, c[1] + 0.5, c[2] + 0.5, 0.0]
+            op["dims"] = [r, r, r, 0.0] if op["brush"] == 1 else [r, rng.uniform(2, 8), rng.uniform(2, 8), 0.0]
+            bb = op_aabb(op)
+            events.append((t, "edit", (op, bb)))
+            for mn, size in sel:
+                if not (mn[0] + size < bb[0] or mn[1] + size < bb[1] or mn[2] + size < bb[2] or mn[0] > bb[3]
+                        or mn[1] > bb[4] or mn[2] > bb[5]):
+                    events.append((t, "mesh", (mn, size)))
+        else:
+            mn, size = sel[int(rng.integers(0, len(sel)))]
+            events.append((t, "mesh", (mn, size)))
+    return dict(name="serving", desc=base["desc"], seed=base["seed"], default_material=0, ops=base["ops"],
+                aabbs=base["aabbs"], events=events, duration=duration, rate=rate, flags=0,
+                mins=np.array([e[2][0] for e in events if e[1] == "mesh"], np.int32),
+                sizes=np.array([e[2][1] for e in events if e[1] == "mesh"], np.int32))
+
+
+WORKLOADS = {"plain_cpu": plain_cpu, "terrain_512": terrain_512, "precision": precision, "clipmap": clipmap,
+             "serving": serving}

@@ -1,0 +1,151 @@
+"""Serving form: continuous batching over device-resident chunk slots (needs a GPU).
+
+A served chunk's mesh must be identical whether it is requested alone or in a mix
+of concurrent requests (different batches, lanes and slots), identical to the
+batched form, and an edit overlapping a cached chunk must force a re-mesh that
+includes the new op.
+"""
+import numpy as np
+import pytest
+
+from leven_amd import mesher, workloads
+
+pytestmark = pytest.mark.gpu
+
+PROBES = [((256, -256, 256), 256), ((768, 0, 512), 256), ((0, -256, 1024), 256), ((512, -512, 0), 512)]
+
+
+def _setup():
+    w = workloads.terrain_512()
+    mesher.init(w["desc"], w["seed"], 0)
+    mesher.clear_csg_ops()
+    for op, bb in zip(w["ops"], w["aabbs"]):
+        mesher.store_csg_op(op, bb[:3], bb[3:])
+    return w
+
+
+def _same(a, b):
+    (v0, t0, s0), (v1, t1, s1) = a, b
+    return (len(t0) == len(t1) and (t0 == t1).all() and (v0.view(np.uint32) == v1.view(np.uint32)).all()
+            and (s0.view(np.uint32) == s1.view(np.uint32)).all())
+
+
+def test_served_alone_equals_in_mix_and_batched():
+    w = _setup()
+    alone = {}
+    s1 = mesher.Server(64, hbm_budget=4 << 30, max_batch=16, lanes=2)
+    for p in PROBES:
+        t = s1.submit_mesh(*p)
+        s1.drain()
+        alone[p] = s1.get_mesh(t)
+    assert sum(len(m[1]) > 0 for m in alone.values()) >= 3
+    s1.close()
+
+    # the mix: the 512 terrain chunks in shuffled order with the probes interleaved,
+    # repeats, and edits far away from every probe
+    rng = np.random.default_rng(0)
+    order = rng.permutation(len(w["mins"]))
+    s2 = mesher.Server(64, hbm_budget=32 << 30, max_batch=48, lanes=3)
+    far_ops, far_bb = workloads.make_ops(8, seed=11, extent=64, y_range=(-10, 10))
+    tickets = {}
+    for k, i in enumerate(order):
+        s2.submit_mesh(w["mins"][i], int(w["sizes"][i]))
+        if k % 97 == 0:
+            p = PROBES[(k // 97) % len(PROBES)]
+            tickets[p] = s2.submit_mesh(*p)
+        if k % 131 == 0:
+            j = (k // 131) % len(far_ops)
+            op = far_ops[j].copy()
+            op["origin"][0] += 6000.0
+            op["origin"][2] += 6000.0
+            s2.submit_edit(op, workloads.op_aabb(op))
+    s2.drain()
+    st = s2.stats()
+    assert st["batches"] > 1 and st["max_batch_seen"] > 1
+    for p, t in tickets.items():
+        assert _same(s2.get_mesh(t), alone[p]), f"probe {p} differs in the mix"
+    s2.close()
+
+    b = mesher.Batch(64, len(PROBES), edge_cap=262144, voxel_cap=131072)
+    b.set_chunks(np.array([p[0] for p in PROBES]), np.array([p[1] for p in PROBES]))
+    b.run()
+    b.wait()
+    for i, p in enumerate(PROBES):
+        assert _same(b.get_mesh(i), alone[p]), f"probe {p}: served != batched"
+    mesher.clear_csg_ops()
+
+
+def test_edit_forces_remesh(oracle):
+    w = _setup()
+    s = mesher.Server(64, hbm_budget=2 << 30, max_batch=8, lanes=2)
+    mn, size = PROBES[0]
+    t0 = s.submit_mesh(mn, size)
+    s.drain()
+    before = s.get_mesh(t0)
+    t_hit = s.submit_mesh(mn, size)
+    s.drain()
+    res = s.poll()
+    assert res[res["ticket"] == t_hit]["cache_hit"][0] == 1
+    # a subtracted sphere of radius 9 voxels at the chunk's centre column, 40 voxels down
+    op = np.zeros(1, dtype=workloads.CSG_OP_DTYPE)
+    op["type"], op["brush"], op["material"] = 1, 1, workloads.MATERIAL_AIR
+    op["origin"] = [mn[0] / 4 + 32.5, -40.5, mn[2] / 4 + 32.5, 0]
+    op["dims"] = [9.0, 9.0, 9.0, 0]
+    bb = workloads.op_aabb(op[0])
+    s.submit_edit(op, bb)
+    t1 = s.submit_mesh(mn, size)
+    s.drain()
+    res = s.poll()
+    assert res[res["ticket"] == t1]["cache_hit"][0] == 0
+    after = s.get_mesh(t1)
+    assert not _same(before, after)
+    # the oracle: the stored ops overlapping the chunk, in store order, then the edit
+    keep = [k for k, b2 in enumerate(w["aabbs"])
+            if not (mn[0] + size < b2[0] or mn[1] + size < b2[1] or mn[2] + size < b2[2]
+                    or mn[0] > b2[3] or mn[1] > b2[4] or mn[2] > b2[5])]
+    ops = np.concatenate([w["ops"][keep], op])
+    c = oracle.Chunk(64).generate(w["desc"], oracle.perm_table(w["seed"]), mn, size, ops=ops).build()
+    v, t, sn = after
+    assert (t == c.triangles()).all() and (v.view(np.uint32) == c.vertices().view(np.uint32)).all()
+    s.close()
+    mesher.clear_csg_ops()
+
+
+def test_eviction_under_small_budget():
+    """A budget of 40 slots for 512 distinct chunks: least-recently-used chunks are
+    evicted while others are in flight; every served count equals the batched form's
+    and the chunks still resident read back equal to it."""
+    w = _setup()
+    probe = mesher.Server(64, hbm_budget=1 << 30, max_batch=1, lanes=1)
+    slot_bytes = (1 << 30) // probe.stats()["slots"]
+    probe.close()
+    s = mesher.Server(64, hbm_budget=40 * slot_bytes + slot_bytes // 2, max_batch=16, lanes=2)
+    assert s.stats()["slots"] == 40
+    n = len(w["mins"])
+    order = np.concatenate([np.arange(n), np.arange(0, n, 7)])
+    tickets = [s.submit_mesh(w["mins"][i], int(w["sizes"][i])) for i in order]
+    s.drain()
+    res = s.poll(len(tickets) + 8)
+    assert len(res) == len(tickets) and (res["status"] == 0).all()
+    st = s.stats()
+    assert st["evictions"] > 0 and st["cached_chunks"] <= 40
+
+    b = mesher.Batch(64, n, edge_cap=262144, voxel_cap=131072)
+    b.set_chunks(w["mins"], w["sizes"])
+    b.run()
+    b.wait()
+    counts = b.counts()
+    by_ticket = {int(r["ticket"]): r for r in res}
+    for t, i in zip(tickets, order):
+        r = by_ticket[t]
+        e, nt = counts[i, 0], counts[i, 2]
+        assert r["nt"] == (nt if e > 0 else 0), f"chunk {i} triangles"
+        assert r["nv"] == (counts[i, 1] if e > 0 and nt > 0 else 0), f"chunk {i} vertices"
+        assert r["ns"] == (counts[i, 3] if e > 0 else 0), f"chunk {i} seams"
+    # request 8 chunks again (hits or re-meshes, all resident once drained) and read them back
+    again = [(s.submit_mesh(w["mins"][i], int(w["sizes"][i])), i) for i in range(0, n, 64)]
+    s.drain()
+    for t, i in again:
+        assert _same(s.get_mesh(t), b.get_mesh(int(i))), f"chunk {i} read back"
+    s.close()
+    mesher.clear_csg_ops()
