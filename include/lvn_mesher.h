@@ -165,7 +165,9 @@ int  lvn_batch_set_chunks(lvn_batch* b, int32_t n, const lvn_chunk_req* reqs);
 int  lvn_batch_run(lvn_batch* b);
 int  lvn_batch_wait(lvn_batch* b);
 /* Same as lvn_batch_run + wait without the graph, timing each stage with HIP events
- * on the batch stream: stage_ms[0..15] in pipeline order (see lvn_batch_stage_name). */
+ * on the batch stream: stage_ms[0 .. LVN_NUM_STAGES - 1] in pipeline order
+ * (see lvn_batch_stage_name); n_stages >= LVN_NUM_STAGES. */
+#define LVN_NUM_STAGES 12
 int  lvn_batch_run_timed(lvn_batch* b, float* stage_ms, int32_t n_stages);
 const char* lvn_batch_stage_name(int32_t stage);
 /* Per chunk: 6 ints = edges, active voxels, triangles, seam nodes, status, and the

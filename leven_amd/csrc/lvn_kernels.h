@@ -34,11 +34,11 @@ struct KArgs {
 
 constexpr int kNumQueues = 8;
 
-enum ScanKind { kScanEdges = 0, kScanVoxels = 1, kScanQuads = 2, kScanSeams = 3 };
+enum ScanKind { kScanEdges = 0, kScanVoxels = 1, kScanQuadsSeams = 2 };
 
 void launch_clear_counters(const KArgs& a, int n, hipStream_t s);
 
-constexpr int kNumStages = 16;
+constexpr int kNumStages = LVN_NUM_STAGES;
 extern const char* const kStageNames[kNumStages];
 // Launch stage `i` (0..kNumStages-1) of the pipeline.
 void launch_stage(const KArgs& a, int chunks, int i, hipStream_t s);
@@ -46,17 +46,14 @@ void launch_stage(const KArgs& a, int chunks, int i, hipStream_t s);
 // Stage launchers: `chunks` chunk slots [0, chunks) processed by one launch each.
 void launch_density(const KArgs& a, int chunks, hipStream_t s);
 void launch_csg(const KArgs& a, int chunks, hipStream_t s);
-void launch_edge_flags(const KArgs& a, int chunks, hipStream_t s);
+void launch_edge_rows(const KArgs& a, int chunks, hipStream_t s);
 void launch_scan(const KArgs& a, int chunks, ScanKind kind, hipStream_t s);
 void launch_edge_compact(const KArgs& a, int chunks, hipStream_t s);
 void launch_edge_cross(const KArgs& a, int chunks, hipStream_t s);
-void launch_voxel_flags(const KArgs& a, int chunks, hipStream_t s);
+void launch_voxel_rows(const KArgs& a, int chunks, hipStream_t s);
 void launch_voxel_compact(const KArgs& a, int chunks, hipStream_t s);
 void launch_qef(const KArgs& a, int chunks, hipStream_t s);
-void launch_quad_flags(const KArgs& a, int chunks, hipStream_t s);
-void launch_quad_emit(const KArgs& a, int chunks, hipStream_t s);
-void launch_vertex_seam(const KArgs& a, int chunks, hipStream_t s);
-void launch_seam_emit(const KArgs& a, int chunks, hipStream_t s);
+void launch_emit(const KArgs& a, int chunks, hipStream_t s);
 
 // Whole pipeline for chunk slots [0, chunks), counters cleared first.
 void launch_pipeline(const KArgs& a, int chunks, hipStream_t s);

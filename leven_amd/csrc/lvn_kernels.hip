@@ -34,14 +34,6 @@ __device__ __forceinline__ int field_index(const Geometry& G, int x, int y, int 
 // since (n * ceil(2^32/d)) / 2^32 - n/d < n / 2^32 < 1/d for d < 2^11.
 __device__ __forceinline__ int div_magic(int n, uint32_t magic) { return (int)__umulhi((uint32_t)n, magic); }
 
-// Hermite point h = x + H (y + H z) -> (x, y, z).
-__device__ __forceinline__ void decode_hermite(const Geometry& G, int h, int& x, int& y, int& z) {
-    const int r = div_magic(h, G.magic_h);
-    x = h - r * G.H;
-    z = div_magic(r, G.magic_h);
-    y = r - z * G.H;
-}
-
 // Voxel v = x + N (y + N z) with N = 2^depth.
 __device__ __forceinline__ void decode_voxel(const Geometry& G, int v, int& x, int& y, int& z) {
     x = v & (G.N - 1);
@@ -57,6 +49,39 @@ __device__ __forceinline__ int dense_index(const uint64_t* mask, const int* base
     const int lane = i & 63;
     if (!((m >> lane) & 1ull)) return -1;
     return base[i >> 6] + __popcll(m & lanemask_lt(lane));
+}
+
+// Work split of a chunk over the x dimension of its grid (blockIdx.y is the chunk).
+__device__ __forceinline__ int wave_in_chunk() { return (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); }
+__device__ __forceinline__ int waves_per_chunk() { return (int)(gridDim.x * (blockDim.x >> 6)); }
+__device__ __forceinline__ int thread_in_chunk() { return (int)(blockIdx.x * blockDim.x + threadIdx.x); }
+__device__ __forceinline__ int threads_per_chunk() { return (int)(gridDim.x * blockDim.x); }
+
+// Bits x + 1 of a 66-bit sample row (lo = x 0..63, hi = x 64, 65) moved to bit x.
+__device__ __forceinline__ uint64_t shift_next(uint64_t lo, uint64_t hi) { return (lo >> 1) | (hi << 63); }
+
+// Index of edge (x, y, z, axis) in the chunk's compacted edge list, or -1 when the
+// edge has no sign change.  The list is in slot order s = 3 (x + H y + H^2 z) + axis,
+// so the index is the row's base plus the edges of lower x in the row, plus the
+// lower axes at the same x.
+__device__ __forceinline__ int edge_index(int H, const uint64_t* rows, const int* base, int x, int y, int z, int axis) {
+    const int r = y + H * z;
+    const uint64_t ex = rows[4 * r], ey = rows[4 * r + 1], ez = rows[4 * r + 2];
+    if (x < 64) {
+        const uint64_t w = axis == 0 ? ex : axis == 1 ? ey : ez;
+        if (!((w >> x) & 1ull)) return -1;
+        const uint64_t lt = lanemask_lt(x);
+        int k = __popcll(ex & lt) + __popcll(ey & lt) + __popcll(ez & lt);
+        if (axis > 0) k += (int)((ex >> x) & 1ull);
+        if (axis > 1) k += (int)((ey >> x) & 1ull);
+        return base[r] + k;
+    }
+    const uint64_t hi = rows[4 * r + 3];  // x == 64: bit 0 x, bit 1 y, bit 2 z
+    if (!((hi >> axis) & 1ull)) return -1;
+    int k = __popcll(ex) + __popcll(ey) + __popcll(ez);
+    if (axis > 0) k += (int)(hi & 1ull);
+    if (axis > 1) k += (int)((hi >> 1) & 1ull);
+    return base[r] + k;
 }
 
 // FP16 store that keeps the sign of every negative sample (an underflow to -0
@@ -185,29 +210,73 @@ __global__ __launch_bounds__(1024) void k_csg(KArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Edge classification (FindFieldEdges, cl/density_field.cl:41-76): slot
-// s = 3 * (x + H y + H^2 z) + axis, sign change between the slot's sample and
-// its +axis neighbour.
+// Edge classification (FindFieldEdges, cl/density_field.cl:41-76) as bit rows.
+// k_solid_rows packs the post-CSG "not air" test of every sample into rows of
+// F bits (y, z fixed); k_edge_rows turns three neighbouring rows into the x, y
+// and z edge masks of a hermite row with a few 64-bit operations, so a chunk's
+// 3 H^3 edge slots are classified by H^2 threads.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_edge_flags(KArgs a) {
+constexpr int kRowsPerBlock = 16;  // sample rows staged per wave
+
+__global__ __launch_bounds__(256) void k_solid_rows(KArgs a) {
+    __shared__ uint32_t stage[4][kRowsPerBlock * 66 / 4];
     const int chunk = blockIdx.y;
     const Geometry& G = a.G;
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint8_t* mat = slot_ptr<uint8_t>(a, chunk, a.L.mat);
-    int flag = 0;
-    if (s < G.edge_slots) {
-        const int h = s / 3, axis = s - 3 * h;
-        int x, y, z;
-        decode_hermite(G, h, x, y, z);
-        const int i0 = field_index(G, x, y, z);
-        const int i1 = i0 + (axis == 0 ? 1 : axis == 1 ? G.F : G.F * G.F);
-        flag = (mat[i0] != kAir) != (mat[i1] != kAir);
+    const int F = G.F;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t* mat = slot_ptr<const uint32_t>(a, chunk, a.L.mat);
+    uint64_t* S = slot_ptr<uint64_t>(a, chunk, a.L.solid);
+    const uint8_t* bytes = reinterpret_cast<const uint8_t*>(stage[wave]);
+    const int blocks = (G.field_rows + kRowsPerBlock - 1) / kRowsPerBlock;
+    const int words_total = G.field_samples >> 2;  // F is even, so F^3 is a multiple of 8
+    const int block_words = (kRowsPerBlock * F) >> 2;
+    for (int b = wave_in_chunk(); b < blocks; b += waves_per_chunk()) {
+        // block b = rows [16 b, 16 b + 16) = bytes [16 F b, 16 F (b + 1)): 4-byte aligned
+        const int w0 = b * block_words;
+        for (int j = lane; j < block_words; j += 64)
+            if (w0 + j < words_total) stage[wave][j] = mat[w0 + j];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        const int rows = min(kRowsPerBlock, G.field_rows - b * kRowsPerBlock);
+        for (int k = 0; k < rows; ++k) {
+            const bool s0 = lane < F && bytes[k * F + lane] != kAir;
+            const bool s1 = lane + 64 < F && bytes[k * F + lane + 64] != kAir;
+            const uint64_t lo = __ballot(s0), hi = __ballot(s1);
+            if (lane == 0) {
+                S[2 * (b * kRowsPerBlock + k)] = lo;
+                S[2 * (b * kRowsPerBlock + k) + 1] = hi;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     }
-    const uint64_t m = __ballot(flag);
-    const int seg = s >> 6;
-    if ((threadIdx.x & 63) == 0 && seg < G.edge_segs) {
-        slot_ptr<uint64_t>(a, chunk, a.L.e_mask)[seg] = m;
-        slot_ptr<int>(a, chunk, a.L.e_count)[seg] = __popcll(m);
+}
+
+__global__ __launch_bounds__(256) void k_edge_rows(KArgs a) {
+    const int chunk = blockIdx.y;
+    const Geometry& G = a.G;
+    const int H = G.H, F = G.F;
+    const uint64_t* S = slot_ptr<const uint64_t>(a, chunk, a.L.solid);
+    uint64_t* E = slot_ptr<uint64_t>(a, chunk, a.L.e_rows);
+    int* count = slot_ptr<int>(a, chunk, a.L.e_count);
+    const uint64_t mlo = H >= 64 ? ~0ull : ((1ull << H) - 1ull);  // x in [0, H)
+    const uint64_t mhi = H > 64 ? 1ull : 0ull;                     // x = 64
+    for (int r = thread_in_chunk(); r < G.edge_rows; r += threads_per_chunk()) {
+        const int z = div_magic(r, G.magic_h), y = r - z * H;
+        const int s0 = y + F * z;
+        const uint64_t alo = S[2 * s0], ahi = S[2 * s0 + 1];                   // (y, z)
+        const uint64_t blo = S[2 * (s0 + 1)], bhi = S[2 * (s0 + 1) + 1];       // (y + 1, z)
+        const uint64_t clo = S[2 * (s0 + F)], chi = S[2 * (s0 + F) + 1];       // (y, z + 1)
+        const uint64_t xlo = (alo ^ shift_next(alo, ahi)) & mlo;
+        const uint64_t xhi = (ahi ^ (ahi >> 1)) & mhi;
+        const uint64_t ylo = (alo ^ blo) & mlo, yhi = (ahi ^ bhi) & mhi;
+        const uint64_t zlo = (alo ^ clo) & mlo, zhi = (ahi ^ chi) & mhi;
+        const uint64_t hi = xhi | (yhi << 1) | (zhi << 2);
+        E[4 * r] = xlo;
+        E[4 * r + 1] = ylo;
+        E[4 * r + 2] = zlo;
+        E[4 * r + 3] = hi;
+        count[r] = __popcll(xlo) + __popcll(ylo) + __popcll(zlo) + __popcll(hi);
     }
 }
 
@@ -215,51 +284,15 @@ __global__ __launch_bounds__(256) void k_edge_flags(KArgs a) {
 // Exclusive scan of segment counts, one workgroup per chunk (replaces the
 // recursive Scan / ExclusiveScan with its blocking readback, src/compute.cpp:344-396).
 // ---------------------------------------------------------------------------
-// The stage is a template parameter: with a runtime selector hipcc (ROCm 7.2,
-// gfx950) kept the voxel counter's address for the seam branch's total.
-template <int KIND>
-__global__ __launch_bounds__(1024) void k_scan(KArgs a) {
-    __shared__ int buf[1024];
-    const int chunk = blockIdx.y;
-    Counters* cnt = a.counters + chunk;
-    const int* count;
-    int* base;
-    int n, cap, flag;
-    int* total;
-    if constexpr (KIND == kScanEdges) {
-        count = slot_ptr<int>(a, chunk, a.L.e_count);
-        base = slot_ptr<int>(a, chunk, a.L.e_base);
-        n = a.G.edge_segs;
-        total = &cnt->edges;
-        cap = a.caps.edges;
-        flag = kStatusEdgeOverflow;
-    } else if constexpr (KIND == kScanVoxels) {
-        count = slot_ptr<int>(a, chunk, a.L.v_count);
-        base = slot_ptr<int>(a, chunk, a.L.v_base);
-        n = a.G.voxel_segs;
-        total = &cnt->voxels;
-        cap = a.caps.voxels;
-        flag = kStatusVoxelOverflow;
-    } else if constexpr (KIND == kScanQuads) {
-        count = slot_ptr<int>(a, chunk, a.L.q_count);
-        base = slot_ptr<int>(a, chunk, a.L.q_base);
-        n = (3 * cnt->voxels + 63) >> 6;
-        total = &cnt->quads;
-        cap = a.caps.quads;
-        flag = kStatusQuadOverflow;
-    } else {
-        count = slot_ptr<int>(a, chunk, a.L.s_count);
-        base = slot_ptr<int>(a, chunk, a.L.s_base);
-        n = (cnt->voxels + 63) >> 6;
-        total = &cnt->seams;
-        cap = a.caps.seams;
-        flag = kStatusSeamOverflow;
-    }
+// Exclusive scan of count[0, n) into base[0, n) by one 1024-thread workgroup;
+// returns the total to every thread.
+__device__ __forceinline__ int block_scan(const int* count, int* base, int n, int* buf) {
     const int tid = threadIdx.x;
     const int per = (n + 1023) / 1024;
     const int b0 = min(n, tid * per), b1 = min(n, b0 + per);
     int sum = 0;
     for (int i = b0; i < b1; ++i) sum += count[i];
+    __syncthreads();
     buf[tid] = sum;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
@@ -273,13 +306,30 @@ __global__ __launch_bounds__(1024) void k_scan(KArgs a) {
         base[i] = run;
         run += count[i];
     }
-    if (tid == 1023) {
-        const int tot = buf[1023];
-        if (tot > cap) {
-            *total = cap;
-            cnt->status |= flag;
-        } else {
-            *total = tot;
+    return buf[1023];
+}
+
+// Edge (kScanEdges) or voxel (kScanVoxels) scan: the stage is a template parameter
+// (with a runtime selector hipcc, ROCm 7.2, gfx950, kept one stage's counter
+// address for the other's total).
+template <int KIND>
+__global__ __launch_bounds__(1024) void k_scan(KArgs a) {
+    __shared__ int buf[1024];
+    const int chunk = blockIdx.y;
+    Counters* cnt = a.counters + chunk;
+    if constexpr (KIND == kScanEdges) {
+        const int t = block_scan(slot_ptr<const int>(a, chunk, a.L.e_count), slot_ptr<int>(a, chunk, a.L.e_base),
+                                 a.G.edge_rows, buf);
+        if (threadIdx.x == 0) {
+            cnt->edges = min(t, a.caps.edges);
+            if (t > a.caps.edges) atomicOr(&cnt->status, kStatusEdgeOverflow);
+        }
+    } else {
+        const int t = block_scan(slot_ptr<const int>(a, chunk, a.L.v_count), slot_ptr<int>(a, chunk, a.L.v_base),
+                                 a.G.voxel_segs, buf);
+        if (threadIdx.x == 0) {
+            cnt->voxels = min(t, a.caps.voxels);
+            if (t > a.caps.voxels) atomicOr(&cnt->status, kStatusVoxelOverflow);
         }
     }
 }
@@ -288,17 +338,40 @@ __global__ __launch_bounds__(1024) void k_scan(KArgs a) {
 // Edge compaction (CompactEdges, cl/density_field.cl:80-92): the edge id is
 // ((x | y << S | z << 2S) << 2) | axis (cl/density_field.cl:63,74).
 // ---------------------------------------------------------------------------
+// One lane per hermite row writes the ids of the row's edges in slot order (x,
+// then axis).  Consecutive lanes take rows of consecutive z at the same y, so a
+// horizontal surface keeps every lane of a wave busy.
 __global__ __launch_bounds__(256) void k_edge_compact(KArgs a) {
     const int chunk = blockIdx.y;
     const Geometry& G = a.G;
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= G.edge_slots) return;
-    const int i = dense_index(slot_ptr<uint64_t>(a, chunk, a.L.e_mask), slot_ptr<int>(a, chunk, a.L.e_base), s);
-    if (i < 0 || i >= a.caps.edges) return;
-    const int h = s / 3, axis = s - 3 * h;
-    int x, y, z;
-    decode_hermite(G, h, x, y, z);
-    slot_ptr<int>(a, chunk, a.L.e_ids)[i] = ((x | (y << G.shift) | (z << (2 * G.shift))) << 2) | axis;
+    const uint64_t* E = slot_ptr<const uint64_t>(a, chunk, a.L.e_rows);
+    const int* count = slot_ptr<const int>(a, chunk, a.L.e_count);
+    const int* base = slot_ptr<const int>(a, chunk, a.L.e_base);
+    int* ids = slot_ptr<int>(a, chunk, a.L.e_ids);
+    const int S = G.shift, cap = a.caps.edges;
+    for (int t = thread_in_chunk(); t < G.edge_rows; t += threads_per_chunk()) {
+        const int y = div_magic(t, G.magic_h), z = t - y * G.H;
+        const int r = y + G.H * z;
+        if (count[r] == 0) continue;
+        const uint64_t ex = E[4 * r], ey = E[4 * r + 1], ez = E[4 * r + 2], hi = E[4 * r + 3];
+        const int row_id = (y << S) | (z << (2 * S));
+        int k = base[r];
+        uint64_t any = ex | ey | ez;
+        while (any) {
+            const int x = (int)__builtin_ctzll(any);
+            any &= any - 1ull;
+            const int id = (x | row_id) << 2;
+            if ((ex >> x) & 1ull) { if (k < cap) ids[k] = id; ++k; }
+            if ((ey >> x) & 1ull) { if (k < cap) ids[k] = id | 1; ++k; }
+            if ((ez >> x) & 1ull) { if (k < cap) ids[k] = id | 2; ++k; }
+        }
+        const int id64 = (64 | row_id) << 2;  // x = 64 (N = 64 only)
+        for (int axis = 0; axis < 3; ++axis)
+            if ((hi >> axis) & 1ull) {
+                if (k < cap) ids[k] = id64 | axis;
+                ++k;
+            }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -556,44 +629,63 @@ __device__ __forceinline__ int dominant_material(const int* m) {
     return maxMaterial;
 }
 
-__global__ __launch_bounds__(256) void k_voxel_flags(KArgs a) {
+// A voxel is active when its 8 corners are not all solid or all air: over a
+// voxel row, the AND and OR of the four sample rows around it and their x + 1
+// shifts.  Segment `seg` of 64 voxels holds 64 / N voxel rows.
+__global__ __launch_bounds__(256) void k_voxel_rows(KArgs a) {
     const int chunk = blockIdx.y;
     const Geometry& G = a.G;
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    int active = 0;
-    if (v < G.voxels) {
-        int x, y, z;
-        decode_voxel(G, v, x, y, z);
-        int cm[8];
-        const int corners = corner_materials(a, slot_ptr<uint8_t>(a, chunk, a.L.mat), x, y, z, cm);
-        active = corners != 0 && corners != 255;
-    }
-    const uint64_t m = __ballot(active);
-    const int seg = v >> 6;
-    if ((threadIdx.x & 63) == 0 && seg < G.voxel_segs) {
-        slot_ptr<uint64_t>(a, chunk, a.L.v_mask)[seg] = m;
-        slot_ptr<int>(a, chunk, a.L.v_count)[seg] = __popcll(m);
+    const int N = G.N, F = G.F;
+    const uint64_t* S = slot_ptr<const uint64_t>(a, chunk, a.L.solid);
+    uint64_t* vmask = slot_ptr<uint64_t>(a, chunk, a.L.v_mask);
+    int* vcount = slot_ptr<int>(a, chunk, a.L.v_count);
+    const int per_seg = N >= 64 ? 1 : 64 / N;
+    const uint64_t nmask = N >= 64 ? ~0ull : ((1ull << N) - 1ull);
+    for (int seg = thread_in_chunk(); seg < G.voxel_segs; seg += threads_per_chunk()) {
+        uint64_t m = 0;
+        for (int k = 0; k < per_seg; ++k) {
+            const int vr = seg * per_seg + k;  // voxel row y + N z
+            const int y = vr & (N - 1), z = vr >> G.depth;
+            const int s0 = y + F * z;
+            const int rows[4] = {s0, s0 + 1, s0 + F, s0 + F + 1};
+            uint64_t all = ~0ull, any = 0ull;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const uint64_t lo = S[2 * rows[c]], hi = S[2 * rows[c] + 1];
+                const uint64_t nx = shift_next(lo, hi);
+                all &= lo & nx;
+                any |= lo | nx;
+            }
+            m |= ((any & ~all) & nmask) << (k * N);
+        }
+        vmask[seg] = m;
+        vcount[seg] = __popcll(m);
     }
 }
 
+// One lane per voxel segment writes the octree codes of its active voxels in
+// linear order; the corner materials, edge mask and dominant material are found
+// by the QEF kernel, which reads the corners anyway.  For N = 64 a segment is a
+// voxel row and consecutive lanes take consecutive z.
 __global__ __launch_bounds__(256) void k_voxel_compact(KArgs a) {
     const int chunk = blockIdx.y;
     const Geometry& G = a.G;
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= G.voxels) return;
-    const int i = dense_index(slot_ptr<uint64_t>(a, chunk, a.L.v_mask), slot_ptr<int>(a, chunk, a.L.v_base), v);
-    if (i < 0 || i >= a.caps.voxels) return;
-    int x, y, z;
-    decode_voxel(G, v, x, y, z);
-    int cm[8];
-    const int corners = corner_materials(a, slot_ptr<uint8_t>(a, chunk, a.L.mat), x, y, z, cm);
-    int edges = 0;
-#pragma unroll
-    for (int e = 0; e < 12; ++e)
-        edges |= (((corners >> kEdgeMap[e][0]) & 1) != ((corners >> kEdgeMap[e][1]) & 1)) << e;
-    slot_ptr<int>(a, chunk, a.L.v_code)[i] = (int)code_for_position(x, y, z, G.depth);
-    slot_ptr<int>(a, chunk, a.L.v_emask)[i] = edges;
-    slot_ptr<int>(a, chunk, a.L.v_mat)[i] = (dominant_material(cm) << 8) | corners;
+    const uint64_t* vmask = slot_ptr<const uint64_t>(a, chunk, a.L.v_mask);
+    const int* vbase = slot_ptr<const int>(a, chunk, a.L.v_base);
+    int* vcode = slot_ptr<int>(a, chunk, a.L.v_code);
+    const int cap = a.caps.voxels;
+    for (int t = thread_in_chunk(); t < G.voxel_segs; t += threads_per_chunk()) {
+        const int seg = G.N == 64 ? (t >> 6) + ((t & 63) << 6) : t;
+        uint64_t m = vmask[seg];
+        int i = vbase[seg];
+        while (m && i < cap) {
+            const int v = (seg << 6) + (int)__builtin_ctzll(m);
+            m &= m - 1ull;
+            int x, y, z;
+            decode_voxel(G, v, x, y, z);
+            vcode[i++] = (int)code_for_position(x, y, z, G.depth);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -648,18 +740,11 @@ __device__ __forceinline__ float invdet(float x) {  // svd_invdet, cl/qef.cl:107
     return (fabsf(x) < 0.1f || fabsf(1.f / x) < 0.1f) ? 0.f : 1.f / x;
 }
 
-__global__ __launch_bounds__(256) void k_qef(KArgs a) {
-    const int chunk = blockIdx.y;
-    const int V = a.counters[chunk].voxels;
-    const int vi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (vi >= V) return;
+__device__ __forceinline__ void qef_fast(const KArgs& a, int chunk, const ChunkDesc& cd, int3 p, int emask, float4& out,
+                                         float4& nrm) {
     const Geometry& G = a.G;
-    const ChunkDesc cd = a.chunks[chunk];
-    const uint32_t code = (uint32_t)slot_ptr<int>(a, chunk, a.L.v_code)[vi];
-    const int emask = slot_ptr<int>(a, chunk, a.L.v_emask)[vi];
-    const int3 p = position_for_code(code, G.depth);
-    const uint64_t* em = slot_ptr<uint64_t>(a, chunk, a.L.e_mask);
-    const int* eb = slot_ptr<int>(a, chunk, a.L.e_base);
+    const uint64_t* erows = slot_ptr<const uint64_t>(a, chunk, a.L.e_rows);
+    const int* eb = slot_ptr<const int>(a, chunk, a.L.e_base);
     const float4* ed = slot_ptr<float4>(a, chunk, a.L.e_data);
     const float fscale = (float)cd.scale;
 
@@ -669,8 +754,7 @@ __global__ __launch_bounds__(256) void k_qef(KArgs a) {
         if (!((emask >> e) & 1)) continue;
         const int e0 = kEdgeMap[e][0], e1 = kEdgeMap[e][1], axis = e >> 2;
         const int hx = p.x + ((e0 >> 2) & 1), hy = p.y + ((e0 >> 1) & 1), hz = p.z + (e0 & 1);
-        const int slot = 3 * (hx + G.H * (hy + G.H * hz)) + axis;
-        const int k = dense_index(em, eb, slot);
+        const int k = edge_index(G.H, erows, eb, hx, hy, hz, axis);
         if (k < 0 || k >= a.caps.edges) continue;
         const float4 d = ed[k];
         const float a0x = (float)hx, a0y = (float)hy, a0z = (float)hz;
@@ -746,14 +830,12 @@ __global__ __launch_bounds__(256) void k_qef(KArgs a) {
         row.w = 0.f;
         x[r] = dot4(row, amp);
     }
-    float4 out;
     out.x = ((x[0] + mass.x) * 4.f) + (float)cd.min[0];
     out.y = ((x[1] + mass.y) * 4.f) + (float)cd.min[1];
     out.z = ((x[2] + mass.z) * 4.f) + (float)cd.min[2];
     out.w = 1.f;
-    slot_ptr<float4>(a, chunk, a.L.v_pos)[vi] = out;
     const float nw = nsum.w;
-    slot_ptr<float4>(a, chunk, a.L.v_norm)[vi] = make_float4(nsum.x / nw, nsum.y / nw, nsum.z / nw, 0.f);
+    nrm = make_float4(nsum.x / nw, nsum.y / nw, nsum.z / nw, 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -840,18 +922,11 @@ __device__ __forceinline__ bool near_threshold(float s) {
     return fabsf(a - 0.1f) < 1e-4f || fabsf(a - 10.f) < 1e-2f;
 }
 
-__global__ __launch_bounds__(256) void k_qef_precise(KArgs a) {
-    const int chunk = blockIdx.y;
-    const int V = a.counters[chunk].voxels;
-    const int vi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (vi >= V) return;
+__device__ __forceinline__ void qef_precise(const KArgs& a, int chunk, const ChunkDesc& cd, int3 p, int emask, float4& out,
+                                            float4& nrm) {
     const Geometry& G = a.G;
-    const ChunkDesc cd = a.chunks[chunk];
-    const uint32_t code = (uint32_t)slot_ptr<int>(a, chunk, a.L.v_code)[vi];
-    const int emask = slot_ptr<int>(a, chunk, a.L.v_emask)[vi];
-    const int3 p = position_for_code(code, G.depth);
-    const uint64_t* em = slot_ptr<uint64_t>(a, chunk, a.L.e_mask);
-    const int* eb = slot_ptr<int>(a, chunk, a.L.e_base);
+    const uint64_t* erows = slot_ptr<const uint64_t>(a, chunk, a.L.e_rows);
+    const int* eb = slot_ptr<const int>(a, chunk, a.L.e_base);
     const float4* ed = slot_ptr<float4>(a, chunk, a.L.e_data);
     const float fscale = (float)cd.scale;
 
@@ -867,7 +942,7 @@ __global__ __launch_bounds__(256) void k_qef_precise(KArgs a) {
         if (!((emask >> e) & 1)) continue;
         const int e0 = kEdgeMap[e][0], e1 = kEdgeMap[e][1], axis = e >> 2;
         const int hx = p.x + ((e0 >> 2) & 1), hy = p.y + ((e0 >> 1) & 1), hz = p.z + (e0 & 1);
-        const int k = dense_index(em, eb, 3 * (hx + G.H * (hy + G.H * hz)) + axis);
+        const int k = edge_index(G.H, erows, eb, hx, hy, hz, axis);
         if (k < 0 || k >= a.caps.edges) continue;
         const float4 d = ed[k];
         const float a0x = (float)hx, a0y = (float)hy, a0z = (float)hz;
@@ -912,7 +987,6 @@ __global__ __launch_bounds__(256) void k_qef_precise(KArgs a) {
     qef_pinv_solve<float>(A, r, x, sig);
     const float smax = fmaxf(fmaxf(fabsf(sig[0]), fabsf(sig[1])), fabsf(sig[2]));
     const float smin = fminf(fminf(fabsf(sig[0]), fabsf(sig[1])), fabsf(sig[2]));
-    float4 out;
     if (smin == 0.f || smax > 1e6f * smin || near_threshold(sig[0]) || near_threshold(sig[1]) ||
         near_threshold(sig[2])) {
         double n = 0, mxd = 0, myd = 0, mzd = 0;
@@ -952,14 +1026,18 @@ __global__ __launch_bounds__(256) void k_qef_precise(KArgs a) {
         out = make_float4(((x[0] + mx) * 4.f) + (float)cd.min[0], ((x[1] + my) * 4.f) + (float)cd.min[1],
                           ((x[2] + mz) * 4.f) + (float)cd.min[2], 1.f);
     }
-    slot_ptr<float4>(a, chunk, a.L.v_pos)[vi] = out;
     // the vertex normal is the reference's unnormalised mean of the edge normals
     const float nw = cnt;
-    slot_ptr<float4>(a, chunk, a.L.v_norm)[vi] = make_float4(nsx / nw, nsy / nw, nsz / nw, 0.f);
+    nrm = make_float4(nsx / nw, nsy / nw, nsz / nw, 0.f);
 }
 
 // ---------------------------------------------------------------------------
-// Quads (GenerateMesh + CompactMeshTriangles, cl/octree.cl:335-464)
+// Per active voxel, one pass: corner materials and edge mask (CreateLeafNodes,
+// cl/octree.cl:236-283), dominant material (cl/octree.cl:78-138), the QEF vertex
+// written straight into the vertex buffer (GenerateMeshVertexBuffer,
+// cl/octree.cl:475-496), the voxel's quad flags (GenerateMesh, cl/octree.cl:398-439)
+// and seam flag (FindSeamNodes, cl/octree.cl:500-518).  Per 64-voxel segment the
+// quad count and the seam mask go to the quad and seam scans.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ int quad_flag(const Geometry& G, int3 p, int corners, int axis) {
@@ -971,22 +1049,75 @@ __device__ __forceinline__ int quad_flag(const Geometry& G, int3 p, int corners,
     return m1 != m2;
 }
 
-__global__ __launch_bounds__(256) void k_quad_flags(KArgs a) {
+template <bool kPrecise>
+__global__ __launch_bounds__(256) void k_qef(KArgs a) {
     const int chunk = blockIdx.y;
+    const Geometry& G = a.G;
     const int V = a.counters[chunk].voxels;
-    const int nq = 3 * V;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if ((int)(blockIdx.x * blockDim.x) >= nq) return;
-    int flag = 0;
-    if (q < nq) {
-        const int vi = q / 3, axis = q - 3 * vi;
-        const int3 p = position_for_code((uint32_t)slot_ptr<int>(a, chunk, a.L.v_code)[vi], a.G.depth);
-        flag = quad_flag(a.G, p, slot_ptr<int>(a, chunk, a.L.v_mat)[vi] & 0xff, axis);
+    const ChunkDesc cd = a.chunks[chunk];
+    const int* vcode = slot_ptr<const int>(a, chunk, a.L.v_code);
+    const uint8_t* mat = slot_ptr<const uint8_t>(a, chunk, a.L.mat);
+    int* vemask = slot_ptr<int>(a, chunk, a.L.v_emask);
+    int* vmat = slot_ptr<int>(a, chunk, a.L.v_mat);
+    float4* verts = slot_ptr<float4>(a, chunk, a.L.verts);
+    int* qcount = slot_ptr<int>(a, chunk, a.L.q_count);
+    uint64_t* smask = slot_ptr<uint64_t>(a, chunk, a.L.s_mask);
+    int* scount = slot_ptr<int>(a, chunk, a.L.s_count);
+    const int lane = threadIdx.x & 63;
+    const int N = G.N;
+    for (int base = wave_in_chunk() << 6; base < V; base += waves_per_chunk() << 6) {
+        const int vi = base + lane;
+        int qbits = 0, seam = 0;
+        if (vi < V) {
+            const int3 p = position_for_code((uint32_t)vcode[vi], G.depth);
+            int cm[8];
+            const int corners = corner_materials(a, mat, p.x, p.y, p.z, cm);
+            int emask = 0;
+#pragma unroll
+            for (int e = 0; e < 12; ++e)
+                emask |= (((corners >> kEdgeMap[e][0]) & 1) != ((corners >> kEdgeMap[e][1]) & 1)) << e;
+            const int m = (dominant_material(cm) << 8) | corners;
+            float4 pos, nrm;
+            if constexpr (kPrecise)
+                qef_precise(a, chunk, cd, p, emask, pos, nrm);
+            else
+                qef_fast(a, chunk, cd, p, emask, pos, nrm);
+#pragma unroll
+            for (int axis = 0; axis < 3; ++axis) qbits |= quad_flag(G, p, corners, axis) << axis;
+            seam = p.x == 0 || p.x == N - 1 || p.y == 0 || p.y == N - 1 || p.z == 0 || p.z == N - 1;
+            vemask[vi] = emask | (qbits << 12);
+            vmat[vi] = m;
+            verts[3 * vi] = pos;
+            verts[3 * vi + 1] = nrm;
+            verts[3 * vi + 2] = make_float4(cd.colour[0], cd.colour[1], cd.colour[2], (float)(m >> 8));
+        }
+        const int nq = __popc(qbits);
+        const uint64_t b0 = __ballot(nq & 1), b1 = __ballot(nq & 2), sm = __ballot(seam);
+        if (lane == 0) {
+            const int seg = base >> 6;
+            qcount[seg] = __popcll(b0) + 2 * __popcll(b1);
+            smask[seg] = sm;
+            scount[seg] = __popcll(sm);
+        }
     }
-    const uint64_t m = __ballot(flag);
-    if ((threadIdx.x & 63) == 0 && (q >> 6) < ((nq + 63) >> 6)) {
-        slot_ptr<uint64_t>(a, chunk, a.L.q_mask)[q >> 6] = m;
-        slot_ptr<int>(a, chunk, a.L.q_count)[q >> 6] = __popcll(m);
+}
+
+// Quad and seam scans of one chunk in one workgroup (the segment count is the
+// chunk's voxel count, known only on the device).
+__global__ __launch_bounds__(1024) void k_scan_quads_seams(KArgs a) {
+    __shared__ int buf[1024];
+    const int chunk = blockIdx.y;
+    Counters* cnt = a.counters + chunk;
+    const int n = (cnt->voxels + 63) >> 6;
+    const int q = block_scan(slot_ptr<const int>(a, chunk, a.L.q_count), slot_ptr<int>(a, chunk, a.L.q_base), n, buf);
+    if (threadIdx.x == 0) {
+        cnt->quads = min(q, a.caps.quads);
+        if (q > a.caps.quads) atomicOr(&cnt->status, kStatusQuadOverflow);
+    }
+    const int sn = block_scan(slot_ptr<const int>(a, chunk, a.L.s_count), slot_ptr<int>(a, chunk, a.L.s_base), n, buf);
+    if (threadIdx.x == 0) {
+        cnt->seams = min(sn, a.caps.seams);
+        if (sn > a.caps.seams) atomicOr(&cnt->status, kStatusSeamOverflow);
     }
 }
 
@@ -995,90 +1126,75 @@ __constant__ int kEdgeNodeOffsets[3][4][3] = {{{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, 
                                               {{0, 0, 0}, {1, 0, 0}, {0, 0, 1}, {1, 0, 1}},
                                               {{0, 0, 0}, {0, 1, 0}, {1, 0, 0}, {1, 1, 0}}};
 
-__global__ __launch_bounds__(256) void k_quad_emit(KArgs a) {
+// Quads (GenerateMesh + CompactMeshTriangles, cl/octree.cl:335-464) in (voxel,
+// axis) order, and seam nodes (ExtractSeamNodeInfo, cl/octree.cl:529-551) in
+// voxel order.
+__global__ __launch_bounds__(256) void k_emit(KArgs a) {
     const int chunk = blockIdx.y;
-    const int V = a.counters[chunk].voxels;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= 3 * V) return;
-    const int qi = dense_index(slot_ptr<uint64_t>(a, chunk, a.L.q_mask), slot_ptr<int>(a, chunk, a.L.q_base), q);
-    if (qi < 0 || qi >= a.caps.quads) return;
     const Geometry& G = a.G;
-    const int vi = q / 3, axis = q - 3 * vi;
-    const int3 p = position_for_code((uint32_t)slot_ptr<int>(a, chunk, a.L.v_code)[vi], G.depth);
-    const int corners = slot_ptr<int>(a, chunk, a.L.v_mat)[vi] & 0xff;
-    const uint64_t* vm = slot_ptr<uint64_t>(a, chunk, a.L.v_mask);
-    const int* vb = slot_ptr<int>(a, chunk, a.L.v_base);
-    int nodes[4];
-    nodes[0] = vi;
-    bool ok = true;
-#pragma unroll
-    for (int n = 1; n < 4; ++n) {
-        const int x = p.x + kEdgeNodeOffsets[axis][n][0];
-        const int y = p.y + kEdgeNodeOffsets[axis][n][1];
-        const int z = p.z + kEdgeNodeOffsets[axis][n][2];
-        nodes[n] = dense_index(vm, vb, x + G.N * (y + G.N * z));
-        ok = ok && nodes[n] >= 0;
-    }
-    if (!ok) {  // cannot happen: all four voxels around a sign-changing interior edge are active
-        atomicOr(&a.counters[chunk].status, kStatusLookupMiss);
-        return;
-    }
-    const int edge = axis * 4 + 3;
-    const int flip = ((corners >> kEdgeMap[edge][0]) & 1) != 0 ? 1 : 0;  // ProcessEdge, cl/octree.cl:335-372
-    int* out = slot_ptr<int>(a, chunk, a.L.tris) + 6 * qi;
-    if (flip) {
-        out[0] = nodes[0]; out[1] = nodes[3]; out[2] = nodes[1];
-        out[3] = nodes[0]; out[4] = nodes[2]; out[5] = nodes[3];
-    } else {
-        out[0] = nodes[0]; out[1] = nodes[1]; out[2] = nodes[3];
-        out[3] = nodes[0]; out[4] = nodes[3]; out[5] = nodes[2];
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Vertex buffer + seam flags (GenerateMeshVertexBuffer + FindSeamNodes,
-// cl/octree.cl:475-518)
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_vertex_seam(KArgs a) {
-    const int chunk = blockIdx.y;
     const int V = a.counters[chunk].voxels;
-    const int vi = blockIdx.x * blockDim.x + threadIdx.x;
-    if ((int)(blockIdx.x * blockDim.x) >= V) return;
-    int seam = 0;
-    if (vi < V) {
-        const ChunkDesc cd = a.chunks[chunk];
-        const int N = a.G.N;
-        const int3 p = position_for_code((uint32_t)slot_ptr<int>(a, chunk, a.L.v_code)[vi], a.G.depth);
-        seam = p.x == 0 || p.x == N - 1 || p.y == 0 || p.y == N - 1 || p.z == 0 || p.z == N - 1;
-        if (a.counters[chunk].quads > 0) {
-            const int m = slot_ptr<int>(a, chunk, a.L.v_mat)[vi];
-            float4* vout = slot_ptr<float4>(a, chunk, a.L.verts) + 3 * vi;
-            vout[0] = slot_ptr<float4>(a, chunk, a.L.v_pos)[vi];
-            vout[1] = slot_ptr<float4>(a, chunk, a.L.v_norm)[vi];
-            vout[2] = make_float4(cd.colour[0], cd.colour[1], cd.colour[2], (float)(m >> 8));
+    const int* vcode = slot_ptr<const int>(a, chunk, a.L.v_code);
+    const int* vemask = slot_ptr<const int>(a, chunk, a.L.v_emask);
+    const int* vmat = slot_ptr<const int>(a, chunk, a.L.v_mat);
+    const uint64_t* vm = slot_ptr<const uint64_t>(a, chunk, a.L.v_mask);
+    const int* vb = slot_ptr<const int>(a, chunk, a.L.v_base);
+    const int* qbase = slot_ptr<const int>(a, chunk, a.L.q_base);
+    const uint64_t* smask = slot_ptr<const uint64_t>(a, chunk, a.L.s_mask);
+    const int* sbase = slot_ptr<const int>(a, chunk, a.L.s_base);
+    const float4* verts = slot_ptr<const float4>(a, chunk, a.L.verts);
+    int* tris = slot_ptr<int>(a, chunk, a.L.tris);
+    float4* seams = slot_ptr<float4>(a, chunk, a.L.seams);
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lanemask_lt(lane);
+    for (int base = wave_in_chunk() << 6; base < V; base += waves_per_chunk() << 6) {
+        const int vi = base + lane, seg = base >> 6;
+        const int qbits = vi < V ? (vemask[vi] >> 12) & 7 : 0;
+        const int nq = __popc(qbits);
+        const uint64_t b0 = __ballot(nq & 1), b1 = __ballot(nq & 2);
+        const uint64_t sm = smask[seg];
+        const bool seam = (sm >> lane) & 1ull;
+        if (vi >= V || (!qbits && !seam)) continue;
+        const int3 p = position_for_code((uint32_t)vcode[vi], G.depth);
+        const int m = vmat[vi];
+        int qi = qbase[seg] + __popcll(b0 & lt) + 2 * __popcll(b1 & lt);
+        for (int axis = 0; axis < 3; ++axis) {
+            if (!((qbits >> axis) & 1)) continue;
+            int nodes[4];
+            nodes[0] = vi;
+            bool ok = true;
+#pragma unroll
+            for (int n = 1; n < 4; ++n) {
+                const int x = p.x + kEdgeNodeOffsets[axis][n][0];
+                const int y = p.y + kEdgeNodeOffsets[axis][n][1];
+                const int z = p.z + kEdgeNodeOffsets[axis][n][2];
+                nodes[n] = dense_index(vm, vb, x + G.N * (y + G.N * z));
+                ok = ok && nodes[n] >= 0;
+            }
+            if (!ok) {  // cannot happen: all four voxels around a sign-changing interior edge are active
+                atomicOr(&a.counters[chunk].status, kStatusLookupMiss);
+            } else if (qi < a.caps.quads) {
+                const int edge = axis * 4 + 3;
+                const int flip = ((m >> kEdgeMap[edge][0]) & 1) != 0 ? 1 : 0;  // ProcessEdge, cl/octree.cl:335-372
+                int* out = tris + 6 * qi;
+                if (flip) {
+                    out[0] = nodes[0]; out[1] = nodes[3]; out[2] = nodes[1];
+                    out[3] = nodes[0]; out[4] = nodes[2]; out[5] = nodes[3];
+                } else {
+                    out[0] = nodes[0]; out[1] = nodes[1]; out[2] = nodes[3];
+                    out[3] = nodes[0]; out[4] = nodes[3]; out[5] = nodes[2];
+                }
+            }
+            ++qi;
+        }
+        if (seam) {
+            const int si = sbase[seg] + __popcll(sm & lt);
+            if (si < a.caps.seams) {
+                reinterpret_cast<int4*>(seams)[3 * si] = make_int4(p.x, p.y, p.z, m);
+                seams[3 * si + 1] = verts[3 * vi];
+                seams[3 * si + 2] = verts[3 * vi + 1];
+            }
         }
     }
-    const uint64_t mk = __ballot(seam);
-    if ((threadIdx.x & 63) == 0 && (vi >> 6) < ((V + 63) >> 6)) {
-        slot_ptr<uint64_t>(a, chunk, a.L.s_mask)[vi >> 6] = mk;
-        slot_ptr<int>(a, chunk, a.L.s_count)[vi >> 6] = __popcll(mk);
-    }
-}
-
-// ExtractSeamNodeInfo, cl/octree.cl:529-551.
-__global__ __launch_bounds__(256) void k_seam_emit(KArgs a) {
-    const int chunk = blockIdx.y;
-    const int V = a.counters[chunk].voxels;
-    const int vi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (vi >= V) return;
-    const int si = dense_index(slot_ptr<uint64_t>(a, chunk, a.L.s_mask), slot_ptr<int>(a, chunk, a.L.s_base), vi);
-    if (si < 0 || si >= a.caps.seams) return;
-    const int3 p = position_for_code((uint32_t)slot_ptr<int>(a, chunk, a.L.v_code)[vi], a.G.depth);
-    int4* out = reinterpret_cast<int4*>(slot_ptr<float4>(a, chunk, a.L.seams) + 3 * si);
-    out[0] = make_int4(p.x, p.y, p.z, slot_ptr<int>(a, chunk, a.L.v_mat)[vi]);
-    float4* fo = slot_ptr<float4>(a, chunk, a.L.seams) + 3 * si;
-    fo[1] = slot_ptr<float4>(a, chunk, a.L.v_pos)[vi];
-    fo[2] = slot_ptr<float4>(a, chunk, a.L.v_norm)[vi];
 }
 
 static inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
@@ -1112,94 +1228,78 @@ void launch_density(const KArgs& a, int chunks, hipStream_t s) {
 void launch_csg(const KArgs& a, int chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_csg, dim3(1, chunks), dim3(1024), 0, s, a);
 }
-void launch_edge_flags(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_edge_flags, dim3(ceil_div(a.G.edge_slots, 256), chunks), dim3(256), 0, s, a);
+// Workgroups per chunk for a kernel that loops over `threads` work items of 256-thread
+// workgroups: all of them for a few chunks, about 8192 workgroups over a large batch.
+static unsigned chunk_blocks(int chunks, long threads) {
+    const long full = std::max(1L, (threads + 255) / 256);
+    const long share = std::max(1L, (8192L + chunks - 1) / chunks);
+    return (unsigned)std::min(full, share);
+}
+
+void launch_edge_rows(const KArgs& a, int chunks, hipStream_t s) {
+    const long solid_waves = (a.G.field_rows + kRowsPerBlock - 1) / kRowsPerBlock;
+    hipLaunchKernelGGL(k_solid_rows, dim3(chunk_blocks(chunks, 64L * solid_waves), chunks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_edge_rows, dim3(chunk_blocks(chunks, a.G.edge_rows), chunks), dim3(256), 0, s, a);
 }
 void launch_scan(const KArgs& a, int chunks, ScanKind kind, hipStream_t s) {
     const dim3 grid(1, chunks), block(1024);
     switch (kind) {
     case kScanEdges: hipLaunchKernelGGL(k_scan<kScanEdges>, grid, block, 0, s, a); break;
     case kScanVoxels: hipLaunchKernelGGL(k_scan<kScanVoxels>, grid, block, 0, s, a); break;
-    case kScanQuads: hipLaunchKernelGGL(k_scan<kScanQuads>, grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL(k_scan<kScanSeams>, grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL(k_scan_quads_seams, grid, block, 0, s, a); break;
     }
 }
 void launch_edge_compact(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_edge_compact, dim3(ceil_div(a.G.edge_slots, 256), chunks), dim3(256), 0, s, a);
+    const long groups = (a.G.edge_rows + 63) / 64;
+    hipLaunchKernelGGL(k_edge_compact, dim3(chunk_blocks(chunks, 64L * groups), chunks), dim3(256), 0, s, a);
 }
 void launch_edge_cross(const KArgs& a, int chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_edge_items, dim3(1), dim3(1024), 0, s, a, chunks);
     // the item count is on the device; size the grid for 512 edge items per chunk
     hipLaunchKernelGGL(k_edge_cross, dim3(persistent_blocks(a, 512L * chunks)), dim3(1024), 0, s, a, chunks);
 }
-void launch_voxel_flags(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_voxel_flags, dim3(ceil_div(a.G.voxels, 256), chunks), dim3(256), 0, s, a);
+void launch_voxel_rows(const KArgs& a, int chunks, hipStream_t s) {
+    hipLaunchKernelGGL(k_voxel_rows, dim3(chunk_blocks(chunks, a.G.voxel_segs), chunks), dim3(256), 0, s, a);
 }
 void launch_voxel_compact(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_voxel_compact, dim3(ceil_div(a.G.voxels, 256), chunks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_voxel_compact, dim3(chunk_blocks(chunks, a.G.voxel_segs), chunks), dim3(256), 0, s, a);
 }
+// The kernels below loop up to the chunk's voxel count, known only on the device.
 void launch_qef(const KArgs& a, int chunks, hipStream_t s) {
+    const dim3 grid(chunk_blocks(chunks, a.caps.voxels), chunks);
     if (a.precise)
-        hipLaunchKernelGGL(k_qef_precise, dim3(ceil_div(a.caps.voxels, 256), chunks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_qef<true>, grid, dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL(k_qef, dim3(ceil_div(a.caps.voxels, 256), chunks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_qef<false>, grid, dim3(256), 0, s, a);
 }
-void launch_quad_flags(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_quad_flags, dim3(ceil_div(3L * a.caps.voxels, 256), chunks), dim3(256), 0, s, a);
-}
-void launch_quad_emit(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_quad_emit, dim3(ceil_div(3L * a.caps.voxels, 256), chunks), dim3(256), 0, s, a);
-}
-void launch_vertex_seam(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_vertex_seam, dim3(ceil_div(a.caps.voxels, 256), chunks), dim3(256), 0, s, a);
-}
-void launch_seam_emit(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_seam_emit, dim3(ceil_div(a.caps.voxels, 256), chunks), dim3(256), 0, s, a);
+void launch_emit(const KArgs& a, int chunks, hipStream_t s) {
+    hipLaunchKernelGGL(k_emit, dim3(chunk_blocks(chunks, a.caps.voxels), chunks), dim3(256), 0, s, a);
 }
 
 const char* const kStageNames[kNumStages] = {
-    "density", "csg", "edge_flags", "scan_edges", "edge_compact", "edge_cross", "voxel_flags", "scan_voxels",
-    "voxel_compact", "qef", "quad_flags", "scan_quads", "quad_emit", "vertex_seam", "scan_seams", "seam_emit"};
+    "density", "csg", "edge_rows", "scan_edges", "edge_compact", "edge_cross",
+    "voxel_rows", "scan_voxels", "voxel_compact", "qef", "scan_quads_seams", "emit"};
 
 void launch_stage(const KArgs& a, int chunks, int i, hipStream_t s) {
     switch (i) {
     case 0: launch_density(a, chunks, s); break;
     case 1: launch_csg(a, chunks, s); break;
-    case 2: launch_edge_flags(a, chunks, s); break;
+    case 2: launch_edge_rows(a, chunks, s); break;
     case 3: launch_scan(a, chunks, kScanEdges, s); break;
     case 4: launch_edge_compact(a, chunks, s); break;
     case 5: launch_edge_cross(a, chunks, s); break;
-    case 6: launch_voxel_flags(a, chunks, s); break;
+    case 6: launch_voxel_rows(a, chunks, s); break;
     case 7: launch_scan(a, chunks, kScanVoxels, s); break;
     case 8: launch_voxel_compact(a, chunks, s); break;
     case 9: launch_qef(a, chunks, s); break;
-    case 10: launch_quad_flags(a, chunks, s); break;
-    case 11: launch_scan(a, chunks, kScanQuads, s); break;
-    case 12: launch_quad_emit(a, chunks, s); break;
-    case 13: launch_vertex_seam(a, chunks, s); break;
-    case 14: launch_scan(a, chunks, kScanSeams, s); break;
-    default: launch_seam_emit(a, chunks, s); break;
+    case 10: launch_scan(a, chunks, kScanQuadsSeams, s); break;
+    default: launch_emit(a, chunks, s); break;
     }
 }
 
 void launch_pipeline(const KArgs& a, int chunks, hipStream_t s) {
     launch_clear_counters(a, chunks, s);
-    launch_density(a, chunks, s);
-    launch_csg(a, chunks, s);
-    launch_edge_flags(a, chunks, s);
-    launch_scan(a, chunks, kScanEdges, s);
-    launch_edge_compact(a, chunks, s);
-    launch_edge_cross(a, chunks, s);
-    launch_voxel_flags(a, chunks, s);
-    launch_scan(a, chunks, kScanVoxels, s);
-    launch_voxel_compact(a, chunks, s);
-    launch_qef(a, chunks, s);
-    launch_quad_flags(a, chunks, s);
-    launch_scan(a, chunks, kScanQuads, s);
-    launch_quad_emit(a, chunks, s);
-    launch_vertex_seam(a, chunks, s);
-    launch_scan(a, chunks, kScanSeams, s);
-    launch_seam_emit(a, chunks, s);
+    for (int i = 0; i < kNumStages; ++i) launch_stage(a, chunks, i, s);
 }
 
 }  // namespace lvn

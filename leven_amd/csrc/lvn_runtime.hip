@@ -188,7 +188,8 @@ Geometry make_geometry(int n) {
     G.shift = G.depth + 1;
     G.field_samples = G.F * G.F * G.F;
     G.edge_slots = 3 * G.H * G.H * G.H;
-    G.edge_segs = (G.edge_slots + 63) / 64;
+    G.field_rows = G.F * G.F;
+    G.edge_rows = G.H * G.H;
     G.voxels = n * n * n;
     G.voxel_segs = std::max(1, G.voxels / 64);
     G.magic_h = (uint32_t)((((uint64_t)1 << 32) + G.H - 1) / G.H);
@@ -202,7 +203,6 @@ Caps make_caps(const Geometry& G, int edge_cap, int voxel_cap) {
     c.voxels = voxel_cap > 0 ? std::min(voxel_cap, G.voxels) : G.voxels;
     c.quads = 3 * c.voxels;
     c.seams = c.voxels;
-    c.edge_segs_q = (3 * c.voxels + 63) / 64;
     c.seam_segs = (c.voxels + 63) / 64;
     return c;
 }
@@ -219,9 +219,10 @@ SlotLayout make_layout(const Geometry& G, const Caps& c) {
     L.density = take(fs * 4);
     L.mat = take(fs);
     L.orig = take(fs);
-    L.e_mask = take((uint64_t)G.edge_segs * 8);
-    L.e_count = take((uint64_t)G.edge_segs * 4);
-    L.e_base = take((uint64_t)G.edge_segs * 4);
+    L.solid = take((uint64_t)G.field_rows * 16);
+    L.e_rows = take((uint64_t)G.edge_rows * 32);
+    L.e_count = take((uint64_t)G.edge_rows * 4);
+    L.e_base = take((uint64_t)G.edge_rows * 4);
     L.e_ids = take((uint64_t)c.edges * 4);
     L.e_data = take((uint64_t)c.edges * 16);
     L.v_mask = take((uint64_t)G.voxel_segs * 8);
@@ -230,11 +231,8 @@ SlotLayout make_layout(const Geometry& G, const Caps& c) {
     L.v_code = take((uint64_t)c.voxels * 4);
     L.v_emask = take((uint64_t)c.voxels * 4);
     L.v_mat = take((uint64_t)c.voxels * 4);
-    L.v_pos = take((uint64_t)c.voxels * 16);
-    L.v_norm = take((uint64_t)c.voxels * 16);
-    L.q_mask = take((uint64_t)c.edge_segs_q * 8);
-    L.q_count = take((uint64_t)c.edge_segs_q * 4);
-    L.q_base = take((uint64_t)c.edge_segs_q * 4);
+    L.q_count = take((uint64_t)c.seam_segs * 4);
+    L.q_base = take((uint64_t)c.seam_segs * 4);
     L.tris = take((uint64_t)c.quads * 24);
     L.s_mask = take((uint64_t)c.seam_segs * 8);
     L.s_count = take((uint64_t)c.seam_segs * 4);
@@ -400,7 +398,7 @@ static int run_field(Ctx* c, const int32_t* min, int32_t size, const std::vector
         launch_density(a, 1, c->stream);
     }
     if (!ops.empty()) launch_csg(a, 1, c->stream);
-    launch_edge_flags(a, 1, c->stream);
+    launch_edge_rows(a, 1, c->stream);
     launch_scan(a, 1, kScanEdges, c->stream);
     launch_edge_compact(a, 1, c->stream);
     launch_edge_cross(a, 1, c->stream);
@@ -410,16 +408,12 @@ static int run_field(Ctx* c, const int32_t* min, int32_t size, const std::vector
 
 static int run_mesh(Ctx* c) {
     KArgs a = c->kargs();
-    launch_voxel_flags(a, 1, c->stream);
+    launch_voxel_rows(a, 1, c->stream);
     launch_scan(a, 1, kScanVoxels, c->stream);
     launch_voxel_compact(a, 1, c->stream);
     launch_qef(a, 1, c->stream);
-    launch_quad_flags(a, 1, c->stream);
-    launch_scan(a, 1, kScanQuads, c->stream);
-    launch_quad_emit(a, 1, c->stream);
-    launch_vertex_seam(a, 1, c->stream);
-    launch_scan(a, 1, kScanSeams, c->stream);
-    launch_seam_emit(a, 1, c->stream);
+    launch_scan(a, 1, kScanQuadsSeams, c->stream);
+    launch_emit(a, 1, c->stream);
     LVN_HIP(hipGetLastError());
     return LVN_OK;
 }
@@ -908,10 +902,17 @@ int lvn_debug_chunk_stages(lvn_ctx* ctx, const int32_t min[3], int32_t size, con
     const int nvx = std::min(k.voxels, out->voxel_cap);
     if (nvx > 0) {
         if (out->voxel_codes) LVN_HIP(hipMemcpy(out->voxel_codes, slot + c->L.v_code, 4 * (size_t)nvx, hipMemcpyDeviceToHost));
-        if (out->voxel_edge_masks) LVN_HIP(hipMemcpy(out->voxel_edge_masks, slot + c->L.v_emask, 4 * (size_t)nvx, hipMemcpyDeviceToHost));
+        if (out->voxel_edge_masks) {
+            // bits 12-14 hold the voxel's quad flags
+            LVN_HIP(hipMemcpy(out->voxel_edge_masks, slot + c->L.v_emask, 4 * (size_t)nvx, hipMemcpyDeviceToHost));
+            for (int i = 0; i < nvx; ++i) out->voxel_edge_masks[i] &= 0xfff;
+        }
         if (out->voxel_materials) LVN_HIP(hipMemcpy(out->voxel_materials, slot + c->L.v_mat, 4 * (size_t)nvx, hipMemcpyDeviceToHost));
-        if (out->voxel_positions) LVN_HIP(hipMemcpy(out->voxel_positions, slot + c->L.v_pos, 16 * (size_t)nvx, hipMemcpyDeviceToHost));
-        if (out->voxel_normals) LVN_HIP(hipMemcpy(out->voxel_normals, slot + c->L.v_norm, 16 * (size_t)nvx, hipMemcpyDeviceToHost));
+        // position and normal are the first two float4 of each 48-byte vertex
+        if (out->voxel_positions && nvx)
+            LVN_HIP(hipMemcpy2D(out->voxel_positions, 16, slot + c->L.verts, 48, 16, (size_t)nvx, hipMemcpyDeviceToHost));
+        if (out->voxel_normals && nvx)
+            LVN_HIP(hipMemcpy2D(out->voxel_normals, 16, slot + c->L.verts + 16, 48, 16, (size_t)nvx, hipMemcpyDeviceToHost));
     }
     if (k.status) return LVN_ERR_OVERFLOW;
     if (k.edges > out->edge_cap || k.voxels > out->voxel_cap) return LVN_ERR_CAPACITY;

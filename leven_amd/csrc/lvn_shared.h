@@ -37,7 +37,8 @@ struct Geometry {
     int N, H, F, shift, depth;
     int field_samples;   // F^3
     int edge_slots;      // 3 H^3
-    int edge_segs;       // ceil(edge_slots / 64)
+    int field_rows;      // F^2 rows (y, z) of F samples: one solid bit per sample
+    int edge_rows;       // H^2 rows (y, z) of H hermite points: x, y, z edge bit masks
     int voxels;          // N^3
     int voxel_segs;      // N^3 / 64 (>= 1)
     uint32_t magic_h;    // ceil(2^32 / H): n / H == umulhi(n, magic_h) for n < 2^21
@@ -50,17 +51,17 @@ struct Caps {
     int voxels;    // V
     int quads;     // Q (<= 3 V)
     int seams;     // <= V
-    int edge_segs_q;   // ceil(3 V / 64)
-    int seam_segs;     // ceil(V / 64)
+    int seam_segs;     // ceil(V / 64): voxel segments of the quad and seam scans
 };
 
 // Byte offsets of every array inside one chunk slot of the arena.
 struct SlotLayout {
     uint64_t slot_bytes;
     uint64_t density, mat, orig;
-    uint64_t e_mask, e_count, e_base, e_ids, e_data;
-    uint64_t v_mask, v_count, v_base, v_code, v_emask, v_mat, v_pos, v_norm;
-    uint64_t q_mask, q_count, q_base, tris;
+    uint64_t solid;                       // field_rows x 2 words: bit x = sample (x, y, z) is not air
+    uint64_t e_rows, e_count, e_base, e_ids, e_data;  // e_rows: edge_rows x {x, y, z, x = 64} words
+    uint64_t v_mask, v_count, v_base, v_code, v_emask, v_mat;
+    uint64_t q_count, q_base, tris;
     uint64_t s_mask, s_count, s_base, seams, verts;
 };
 

@@ -224,7 +224,7 @@ class Context:
         )
 
 
-NUM_STAGES = 16
+NUM_STAGES = 12  # LVN_NUM_STAGES
 
 
 def chunk_reqs(mins, sizes):

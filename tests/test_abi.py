@@ -48,3 +48,22 @@ def test_calls_before_init_fail_cleanly():
     assert state in (abi.LVN_ERR_STATE, abi.LVN_OK)
     if state == abi.LVN_OK:
         L.lvn_ctx_destroy(h)
+
+
+def test_header_constants_match_binding():
+    text = open(os.path.join(ROOT, "include", "lvn_mesher.h")).read()
+    defines = {k: int(v) for k, v in re.findall(r"#define\s+(LVN_[A-Z0-9_]+)\s+\(?(-?\d+)\)?", text)}
+    assert defines["LVN_NUM_STAGES"] == mesher.NUM_STAGES
+    for name in ["LVN_OK", "LVN_ERR_INVALID", "LVN_ERR_NOMEM", "LVN_ERR_CAPACITY", "LVN_ERR_DEVICE",
+                 "LVN_ERR_STATE", "LVN_ERR_OVERFLOW", "LVN_MATERIAL_NONE", "LVN_MATERIAL_AIR",
+                 "LVN_CTX_FP16_DENSITY", "LVN_CTX_PRECISE_QEF", "LVN_REQ_MESH", "LVN_REQ_EDIT"]:
+        short = name[len("LVN_"):]
+        assert defines[name] == getattr(abi, short if hasattr(abi, short) else name), name
+
+
+def test_stage_names_cover_every_stage():
+    if not os.path.exists(mesher.LIB_PATH):
+        pytest.skip("library not built (python -m leven_amd.build)")
+    names = [mesher.lib().lvn_batch_stage_name(i) for i in range(mesher.NUM_STAGES + 1)]
+    assert all(n for n in names[:-1]) and names[-1] is None
+    assert len(set(names[:-1])) == mesher.NUM_STAGES

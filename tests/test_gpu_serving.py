@@ -112,15 +112,16 @@ def test_edit_forces_remesh(oracle):
 
 
 def test_eviction_under_small_budget():
-    """A budget of 40 slots for 512 distinct chunks: least-recently-used chunks are
+    """A budget of about 40 slots for 512 distinct chunks: least-recently-used chunks are
     evicted while others are in flight; every served count equals the batched form's
     and the chunks still resident read back equal to it."""
     w = _setup()
     probe = mesher.Server(64, hbm_budget=1 << 30, max_batch=1, lanes=1)
     slot_bytes = (1 << 30) // probe.stats()["slots"]
     probe.close()
-    s = mesher.Server(64, hbm_budget=40 * slot_bytes + slot_bytes // 2, max_batch=16, lanes=2)
-    assert s.stats()["slots"] == 40
+    s = mesher.Server(64, hbm_budget=40 * slot_bytes, max_batch=16, lanes=2)
+    slots = s.stats()["slots"]
+    assert 39 <= slots <= 41
     n = len(w["mins"])
     order = np.concatenate([np.arange(n), np.arange(0, n, 7)])
     tickets = [s.submit_mesh(w["mins"][i], int(w["sizes"][i])) for i in order]
@@ -128,7 +129,7 @@ def test_eviction_under_small_budget():
     res = s.poll(len(tickets) + 8)
     assert len(res) == len(tickets) and (res["status"] == 0).all()
     st = s.stats()
-    assert st["evictions"] > 0 and st["cached_chunks"] <= 40
+    assert st["evictions"] > 0 and st["cached_chunks"] <= slots
 
     b = mesher.Batch(64, n, edge_cap=262144, voxel_cap=131072)
     b.set_chunks(w["mins"], w["sizes"])
