@@ -22,7 +22,7 @@ sys.path.insert(0, ROOT)
 from leven_amd import mesher, workloads  # noqa: E402
 
 METRIC = "64^3 chunks meshed/sec (batched vs plain, same GPU) + p99 chunk latency"
-PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 (packed) vector peak
+PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak (fused multiply-adds)
 PEAK_HBM_GBS = 8000.0
 
 # FP32 operations per evaluation, counted from the kernel source (add/sub/mul/div/sqrt = 1)
@@ -44,6 +44,21 @@ def stage_flops(w, desc):
     density = w["density_height"] * fh + w["density_cave"] * fc + w["samples"] * 2
     cross = w["cross_height"] * fh + w["cross_cave"] * fc + w["edges"] * 200 + w["cross_brush"] * 20
     return density, cross
+
+
+def hbm_traffic(kernel, workload):
+    """HBM bytes per batched launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/hbm_traffic_<workload>.json, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    for name, entries in data["kernels"].items():
+        if name.split("::")[-1].split("<")[0] == kernel and "<true>" not in name:
+            e = max(entries, key=lambda x: x["grid"])  # the batched launch
+            return e["read_bytes"] + e["write_bytes"]
+    return None
 
 
 def cpu_baseline(wl, threads):
@@ -156,7 +171,10 @@ def main():
     counts = batch.counts()
     assert (counts[:, 4] == 0).all(), "a chunk overflowed its slot capacity"
 
-    stage_ms = batch.run_timed()
+    # per-stage launch durations: HIP events around each stage's launches on the batch
+    # stream, averaged over `steps` passes of the same pipeline
+    runs = [batch.run_timed() for _ in range(args.steps)]
+    stage_ms = {k: sum(r[k] for r in runs) / len(runs) for k in runs[0]}
     work = batch.work()
     d_flops, c_flops = stage_flops(work, desc)
     dom = max(("density", "edge_cross"), key=lambda k: stage_ms[k])
@@ -207,7 +225,7 @@ def main():
         "work": work,
         "roofline": {"bound": "valu", "kernel": "k_" + dom, "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": None, "flops_per_launch": int(dom_flops),
+                     "traffic": hbm_traffic("k_" + dom, wl["name"]), "flops_per_launch": int(dom_flops),
                      "launch_ms": round(stage_ms[dom], 4)},
     }
     if args.serve_seconds > 0:

@@ -15,8 +15,9 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["lvn_kernels.hip", "lvn_runtime.hip", "lvn_server.hip"]
 
-# -ffp-contract=off: the kernels follow the reference's operation order without
-# fused multiply-adds, so results are bit-identical to the IEEE FP32 restatement.
+# -ffp-contract=off: the kernels follow the reference's operation order with fused
+# multiply-adds only where written as fmaf(), as the IEEE FP32 restatement does,
+# so results are bit-identical to it.
 # Packed FP32 instructions are disabled on this platform.
 FLAGS = [
     "--offload-arch=gfx950",

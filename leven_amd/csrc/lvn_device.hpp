@@ -35,13 +35,15 @@ __device__ __forceinline__ int lds_texel(const NoiseLds& T, int tx, int ty) {
     return T.perm[((ty & 255) << 8) | (tx & 255)];
 }
 
-// One snoise2 corner (cl/simplex.cl:123-153).
+// One snoise2 corner (cl/simplex.cl:123-153).  The squared distance and the
+// gradient dot product are fused multiply-adds, as the restatement defines them
+// (the reference's -cl-fast-relaxed-math build, src/compute.cpp:213, allows it).
 __device__ __forceinline__ float corner2(const NoiseLds& T, int tx, int ty, float fx, float fy) {
     const int g = lds_texel(T, tx, ty) & 15;
-    const float r = 0.5f - (fx * fx + fy * fy);
+    const float r = 0.5f - fmaf(fy, fy, fx * fx);
     const float4 gr = T.grad[g];
     const float t = r * r;
-    const float n = (t * t) * (gr.x * fx + gr.y * fy);
+    const float n = (t * t) * fmaf(gr.y, fy, gr.x * fx);
     return r < 0.f ? 0.f : n;
 }
 
@@ -72,10 +74,10 @@ __device__ __forceinline__ float corner3(const NoiseLds& T, int tx, int ty, int 
     int p = lds_texel(T, tx, ty);
     p = p == 255 ? 0 : p;
     const int g = T.perm[((tz & 255) << 8) | p] & 15;
-    const float r = 0.6f - ((fx * fx + fy * fy) + fz * fz);
+    const float r = 0.6f - fmaf(fz, fz, fmaf(fy, fy, fx * fx));
     const float4 gr = T.grad[g];
     const float t = r * r;
-    const float n = (t * t) * ((gr.x * fx + gr.y * fy) + gr.z * fz);
+    const float n = (t * t) * fmaf(gr.z, fz, fmaf(gr.y, fy, gr.x * fx));
     return r < 0.f ? 0.f : n;
 }
 
@@ -116,7 +118,7 @@ __device__ __forceinline__ float fbm2(const NoiseLds& T, int octaves, float freq
                                       float y) {
     float px = x * freq, py = y * freq, amp = 1.f, n = 0.f;
     for (int i = 0; i < octaves; ++i) {
-        n = n + snoise2(T, px, py) * amp;
+        n = fmaf(snoise2(T, px, py), amp, n);
         px = px * lac;
         py = py * lac;
         amp = amp * gain;
@@ -129,7 +131,7 @@ __device__ __forceinline__ float fbm3(const NoiseLds& T, int octaves, float freq
                                       float y, float z) {
     float px = x * freq, py = y * freq, pz = z * freq, amp = 1.f, n = 0.f;
     for (int i = 0; i < octaves; ++i) {
-        n = n + snoise3(T, px, py, pz) * amp;
+        n = fmaf(snoise3(T, px, py, pz), amp, n);
         px = px * lac;
         py = py * lac;
         pz = pz * lac;

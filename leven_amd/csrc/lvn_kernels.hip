@@ -104,6 +104,7 @@ __constant__ int kEdgeMap[12][2] = {{0, 4}, {1, 5}, {2, 6}, {3, 7}, {0, 2}, {1, 
 // ---------------------------------------------------------------------------
 // One thread per (x, z) column segment: the height term depends on x and z
 // only, so it is evaluated once and reused down the column.
+template <bool kCount>
 __device__ __forceinline__ void density_column(const KArgs& a, const NoiseLds& T, int chunk, int t) {
     const Geometry& G = a.G;
     const int F = G.F;
@@ -126,7 +127,7 @@ __device__ __forceinline__ void density_column(const KArgs& a, const NoiseLds& T
     for (int y = y0; y < y1; ++y) {
         const float wy = (float)(y * cd.scale + cd.off[1]);
         const float d = combine(a.F, T, wy - H, wx, wy, wz);
-        if (a.work) caves += (a.F.kind == LVN_FIELD_TERRAIN_CAVES && !((wy - H) > a.F.cave_cut)) ? 1u : 0u;
+        if constexpr (kCount) caves += (a.F.kind == LVN_FIELD_TERRAIN_CAVES && !((wy - H) > a.F.cave_cut)) ? 1u : 0u;
         const int idx = field_index(G, x, y, z);
         if (a.fp16)
             dens16[idx] = to_half_keep_sign(d);
@@ -136,7 +137,7 @@ __device__ __forceinline__ void density_column(const KArgs& a, const NoiseLds& T
         mat[idx] = m;
         orig[idx] = m;
     }
-    if (a.work) {
+    if constexpr (kCount) {
         atomicAdd(&a.work[0], 1ull);
         atomicAdd(&a.work[1], (unsigned long long)caves);
         atomicAdd(&a.work[5], (unsigned long long)(y1 > y0 ? y1 - y0 : 0));
@@ -153,6 +154,9 @@ __device__ __forceinline__ int next_item(int* queue) {
 // Persistent: the grid fills the machine once (two 1024-thread workgroups per CU,
 // bounded by the 64 KiB LDS table), each workgroup loads the noise tables once,
 // and every wave pulls 64-column items of any chunk until the queue is empty.
+// kCount adds the work counters of lvn_batch_work (a separate instantiation, so
+// the timed kernels carry no counting code).
+template <bool kCount>
 __global__ __launch_bounds__(1024) void k_density(KArgs a, int chunks) {
     __shared__ NoiseLds T;
     load_noise_tables(T, a.perm, a.grads);
@@ -165,7 +169,7 @@ __global__ __launch_bounds__(1024) void k_density(KArgs a, int chunks) {
         if (item >= total) break;
         const int chunk = item / items_per_chunk;
         const int t = (item - chunk * items_per_chunk) * 64 + lane;
-        if (t < per_chunk) density_column(a, T, chunk, t);
+        if (t < per_chunk) density_column<kCount>(a, T, chunk, t);
     }
 }
 
@@ -485,6 +489,7 @@ __device__ __forceinline__ float4 brush_crossing(const KArgs& a, const ChunkDesc
 // One compacted edge.  An edge with an endpoint whose material a brush changed
 // takes brush data (src/compute_csg.cpp:97-217); otherwise the density crossing
 // (fresh field) or its data from the cached edge list.
+template <bool kCount>
 __device__ __forceinline__ void cross_edge(const KArgs& a, const NoiseLds& T, int chunk, int i) {
     const Geometry& G = a.G;
     const ChunkDesc& cd = a.chunks[chunk];
@@ -523,12 +528,12 @@ __device__ __forceinline__ void cross_edge(const KArgs& a, const NoiseLds& T, in
         const bool ends = !a.fp16;
         unsigned nh = 0, nc = 0;
         out = density_crossing(a, T, cd, x, y, z, axis, ends, ends ? dens[i0] : 0.f, ends ? dens[i1] : 0.f, nh, nc);
-        if (a.work) {
+        if constexpr (kCount) {
             atomicAdd(&a.work[2], (unsigned long long)nh);
             atomicAdd(&a.work[3], (unsigned long long)nc);
         }
     }
-    if (a.work) {
+    if constexpr (kCount) {
         atomicAdd(&a.work[6], 1ull);
         if (touched) atomicAdd(&a.work[4], (unsigned long long)(17 * cd.op_count));
     }
@@ -561,8 +566,15 @@ __global__ __launch_bounds__(1024) void k_edge_items(KArgs a, int chunks) {
     if (threadIdx.x == 0) a.item_prefix[chunks] = carry;
 }
 
-// Persistent like k_density: waves pull 64-edge items of any chunk.
-__global__ __launch_bounds__(1024) void k_edge_cross(KArgs a, int chunks) {
+// Persistent like k_density: waves pull 64-edge items of any chunk.  768-thread
+// workgroups: at the kernel's register count (6 waves per SIMD) two of them fit a
+// CU beside their two 64 KiB tables, where one 1024-thread workgroup would leave
+// a third of the wave slots empty.
+constexpr int kCrossThreads = 768;
+
+template <bool kCount>
+__global__ __attribute__((amdgpu_waves_per_eu(6, 8))) __launch_bounds__(kCrossThreads) void k_edge_cross(KArgs a,
+                                                                                                          int chunks) {
     __shared__ NoiseLds T;
     load_noise_tables(T, a.perm, a.grads);
     const int total = a.item_prefix[chunks];
@@ -576,7 +588,7 @@ __global__ __launch_bounds__(1024) void k_edge_cross(KArgs a, int chunks) {
             if (a.item_prefix[mid] <= item) lo = mid; else hi = mid;
         }
         const int i = (item - a.item_prefix[lo]) * 64 + lane;
-        if (i < a.counters[lo].edges) cross_edge(a, T, lo, i);
+        if (i < a.counters[lo].edges) cross_edge<kCount>(a, T, lo, i);
     }
 }
 
@@ -1223,7 +1235,10 @@ static unsigned persistent_blocks(const KArgs& a, long items) {
 
 void launch_density(const KArgs& a, int chunks, hipStream_t s) {
     const long items = (((long)a.G.F * a.G.F * a.ysplit + 63) / 64) * chunks;
-    hipLaunchKernelGGL(k_density, dim3(persistent_blocks(a, items)), dim3(1024), 0, s, a, chunks);
+    if (a.work)
+        hipLaunchKernelGGL(k_density<true>, dim3(persistent_blocks(a, items)), dim3(1024), 0, s, a, chunks);
+    else
+        hipLaunchKernelGGL(k_density<false>, dim3(persistent_blocks(a, items)), dim3(1024), 0, s, a, chunks);
 }
 void launch_csg(const KArgs& a, int chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_csg, dim3(1, chunks), dim3(1024), 0, s, a);
@@ -1256,7 +1271,11 @@ void launch_edge_compact(const KArgs& a, int chunks, hipStream_t s) {
 void launch_edge_cross(const KArgs& a, int chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_edge_items, dim3(1), dim3(1024), 0, s, a, chunks);
     // the item count is on the device; size the grid for 512 edge items per chunk
-    hipLaunchKernelGGL(k_edge_cross, dim3(persistent_blocks(a, 512L * chunks)), dim3(1024), 0, s, a, chunks);
+    const dim3 grid(persistent_blocks(a, 512L * chunks));
+    if (a.work)
+        hipLaunchKernelGGL(k_edge_cross<true>, grid, dim3(kCrossThreads), 0, s, a, chunks);
+    else
+        hipLaunchKernelGGL(k_edge_cross<false>, grid, dim3(kCrossThreads), 0, s, a, chunks);
 }
 void launch_voxel_rows(const KArgs& a, int chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_voxel_rows, dim3(chunk_blocks(chunks, a.G.voxel_segs), chunks), dim3(256), 0, s, a);

@@ -151,30 +151,30 @@ float snoise2(const uint8_t* perm, float px, float py) {
     float n0 = 0.f, n1 = 0.f, n2 = 0.f;
     {
         const int g = texel(perm, ix, iy);
-        float t0 = 0.5f - (f0x * f0x + f0y * f0y);
+        float t0 = 0.5f - std::fma(f0y, f0y, f0x * f0x);
         if (!(t0 < 0.f)) {
             t0 = t0 * t0;
-            n0 = (t0 * t0) * (kGrads.comp(g, 0) * f0x + kGrads.comp(g, 1) * f0y);
+            n0 = (t0 * t0) * std::fma(kGrads.comp(g, 1), f0y, kGrads.comp(g, 0) * f0x);
         }
     }
     {
         const float fx = (f0x - (float)o1x) + G2;
         const float fy = (f0y - (float)o1y) + G2;
         const int g = texel(perm, ix + o1x, iy + o1y);
-        float t1 = 0.5f - (fx * fx + fy * fy);
+        float t1 = 0.5f - std::fma(fy, fy, fx * fx);
         if (!(t1 < 0.f)) {
             t1 = t1 * t1;
-            n1 = (t1 * t1) * (kGrads.comp(g, 0) * fx + kGrads.comp(g, 1) * fy);
+            n1 = (t1 * t1) * std::fma(kGrads.comp(g, 1), fy, kGrads.comp(g, 0) * fx);
         }
     }
     {
         const float fx = f0x - C2;
         const float fy = f0y - C2;
         const int g = texel(perm, ix + 1, iy + 1);
-        float t2 = 0.5f - (fx * fx + fy * fy);
+        float t2 = 0.5f - std::fma(fy, fy, fx * fx);
         if (!(t2 < 0.f)) {
             t2 = t2 * t2;
-            n2 = (t2 * t2) * (kGrads.comp(g, 0) * fx + kGrads.comp(g, 1) * fy);
+            n2 = (t2 * t2) * std::fma(kGrads.comp(g, 1), fy, kGrads.comp(g, 0) * fx);
         }
     }
     return 70.f * ((n0 + n1) + n2);
@@ -184,10 +184,10 @@ float snoise2(const uint8_t* perm, float px, float py) {
 inline float snoise3_corner(const uint8_t* perm, int tx, int ty, int tz, float fx, float fy, float fz) {
     const int p = texel(perm, tx, ty);
     const int g = texel(perm, perm_as_texel(p), tz);
-    float t = 0.6f - ((fx * fx + fy * fy) + fz * fz);
+    float t = 0.6f - std::fma(fz, fz, std::fma(fy, fy, fx * fx));
     if (t < 0.f) return 0.f;
     t = t * t;
-    return (t * t) * ((kGrads.comp(g, 0) * fx + kGrads.comp(g, 1) * fy) + kGrads.comp(g, 2) * fz);
+    return (t * t) * std::fma(kGrads.comp(g, 2), fz, std::fma(kGrads.comp(g, 1), fy, kGrads.comp(g, 0) * fx));
 }
 
 // snoise3, cl/simplex.cl:159-230, with simplex() of :74-97.
@@ -235,7 +235,7 @@ float fbm2(const uint8_t* perm, int octaves, float frequency, float lacunarity, 
     float px = x * frequency, py = y * frequency;
     float amplitude = 1.f, noise = 0.f;
     for (int i = 0; i < octaves; ++i) {
-        noise = noise + snoise2(perm, px, py) * amplitude;
+        noise = std::fma(snoise2(perm, px, py), amplitude, noise);
         px = px * lacunarity;
         py = py * lacunarity;
         amplitude = amplitude * gain;
@@ -249,7 +249,7 @@ float fbm3(const uint8_t* perm, int octaves, float frequency, float lacunarity, 
     float px = x * frequency, py = y * frequency, pz = z * frequency;
     float amplitude = 1.f, noise = 0.f;
     for (int i = 0; i < octaves; ++i) {
-        noise = noise + snoise3(perm, px, py, pz) * amplitude;
+        noise = std::fma(snoise3(perm, px, py, pz), amplitude, noise);
         px = px * lacunarity;
         py = py * lacunarity;
         pz = pz * lacunarity;

@@ -8,9 +8,13 @@
  * links, loads or calls it.
  *
  * Arithmetic contract (shared with the HIP kernels, which are written
- * independently): IEEE FP32, round-to-nearest, no contraction
- * (-ffp-contract=off), correctly rounded sqrt and division, the operation
- * order given in each function's comment.  Unsuffixed double constants of the
+ * independently): IEEE FP32, round-to-nearest, no implicit contraction
+ * (-ffp-contract=off) and fused multiply-adds exactly where written as fma():
+ * the squared distance and the gradient dot product of each simplex corner and
+ * the fractal's octave sum (the reference builds with -cl-fast-relaxed-math,
+ * src/compute.cpp:213, which lets its compiler fuse these); correctly rounded
+ * sqrt and division, the operation order given in each function's comment.
+ * Unsuffixed double constants of the
  * reference (0.5, 0.6, 2.0 in cl/simplex.cl, cl/qef.cl) are restated as float
  * constants.  OpenCL builtins are restated as:
  *   dot(a,b)     = ((a.x*b.x + a.y*b.y) + a.z*b.z) [+ a.w*b.w]
@@ -20,7 +24,7 @@
  *   rsqrt(x)     = 1 / sqrt(x)
  *   pow(f, -1)   = 1 / f
  *
- * Parity pinning of each function is listed in oracle/README.md.
+ * Each function's pinning checks are in tests/test_oracle.py.
  */
 #ifndef LVN_ORACLE_H
 #define LVN_ORACLE_H
