@@ -61,6 +61,25 @@ def hbm_traffic(kernel, workload):
     return None
 
 
+# stage -> kernels whose HBM bytes it moves (batched launch)
+STAGE_KERNELS = {"density": ["k_density"], "edge_rows": ["k_solid_rows", "k_edge_rows"],
+                 "edge_compact": ["k_edge_compact"], "edge_cross": ["k_edge_cross"], "voxel_rows": ["k_voxel_rows"],
+                 "voxel_compact": ["k_voxel_compact"], "qef": ["k_qef"], "emit": ["k_emit"]}
+
+
+def hbm_rates(stage_ms, workload):
+    """HBM GB/s of each stage against the 8 TB/s peak: the committed rocprofv3 byte counts
+    per batched launch over the stage times measured in this run."""
+    out = {}
+    for stage, kernels in STAGE_KERNELS.items():
+        total = [hbm_traffic(k, workload) for k in kernels]
+        if any(t is None for t in total) or stage_ms.get(stage, 0) <= 0:
+            continue
+        gbs = sum(total) / (stage_ms[stage] * 1e-3) / 1e9
+        out[stage] = {"gbs": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4), "bytes": int(sum(total))}
+    return out
+
+
 def cpu_baseline(wl, threads):
     from oracle import py_oracle
     perm = py_oracle.perm_table(wl["seed"])
@@ -71,6 +90,40 @@ def cpu_baseline(wl, threads):
                 sample=f"all {n} chunks of {wl['name']} on {threads} threads, oracle (C++ restatement, "
                        f"FP32, one chunk per thread): {wall:.2f} s wall, p50 {np.percentile(secs, 50) * 1e3:.1f} ms, "
                        f"p99 {np.percentile(secs, 99) * 1e3:.1f} ms per chunk")
+
+
+def clipmap_leg(args):
+    """The 6-LOD clipmap (1,890 chunks at voxel sizes 1..32; coarse cave chunks have up to
+    ~237k of their 262,144 voxels active) as one batch with full per-slot capacities,
+    autotuned."""
+    wl = workloads.clipmap()
+    mesher.init(wl["desc"], wl["seed"], wl["default_material"])
+    mesher.clear_csg_ops()
+    for op, bb in zip(wl["ops"], wl["aabbs"]):
+        mesher.store_csg_op(op, bb[:3], bb[3:])
+    n = len(wl["sizes"])
+    b = mesher.Batch(64, n, edge_cap=0, voxel_cap=0)
+    b.set_chunks(wl["mins"], wl["sizes"])
+    tuning = b.autotune(reps=1)
+    steps = max(2, args.steps // 4)
+    for _ in range(max(1, args.warmup)):
+        b.run()
+    b.wait()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run()
+    b.wait()
+    elapsed = time.perf_counter() - t0
+    counts = b.counts()
+    assert (counts[:, 4] == 0).all(), "a clipmap chunk overflowed"
+    stage_ms = {k: round(v, 4) for k, v in b.run_timed().items()}
+    lods = np.log2(wl["sizes"] // 256).astype(int)
+    b.close()
+    return {"chunks": n, "chunks_per_lod": np.bincount(lods, minlength=6).tolist(),
+            "steps": steps, "chunks_per_s": round(n * steps / elapsed, 1), "ms_per_pass": round(elapsed / steps * 1e3, 3),
+            "nonempty_chunks": int((counts[:, 2] > 0).sum()), "max_active_voxels": int(counts[:, 1].max()),
+            "max_edges": int(counts[:, 0].max()), "triangles": int(counts[:, 2].sum()),
+            "stage_ms": stage_ms, "tuning": tuning}
 
 
 def serve(args):
@@ -137,6 +190,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--plain-chunks", type=int, default=0, help="chunks for the plain form (0 = all)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="length of the served request stream (0 = skip)")
+    ap.add_argument("--no-clipmap", action="store_true", help="skip the 6-LOD clipmap batch")
+    ap.add_argument("--no-autotune", action="store_true", help="time the default launch shapes")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -158,6 +213,8 @@ def main():
     # ---- batched form ---------------------------------------------------------
     batch = mesher.Batch(64, n, edge_cap=262144, voxel_cap=131072, flags=wl["flags"])
     batch.set_chunks(wl["mins"], wl["sizes"])
+    # launch shapes picked by the autotuner on these chunks (outside the timed region)
+    tuning = batch.autotune(reps=3) if not args.no_autotune else batch.tuning()
     for _ in range(args.warmup):
         batch.run()
     batch.wait()
@@ -198,6 +255,8 @@ def main():
         lat.append(time.perf_counter() - ts)
     plain_rate = m / (time.perf_counter() - t0)
     lat = np.array(lat) * 1e3
+    batch.close()
+    plain.close()
 
     out = {
         "metric": METRIC,
@@ -227,7 +286,13 @@ def main():
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": hbm_traffic("k_" + dom, wl["name"]), "flops_per_launch": int(dom_flops),
                      "launch_ms": round(stage_ms[dom], 4)},
+        "tuning": tuning,
     }
+    hbm = hbm_rates(stage_ms, wl["name"])
+    if hbm:
+        out["hbm_gbs"] = hbm
+    if not args.no_clipmap:
+        out["clipmap"] = clipmap_leg(args)
     if args.serve_seconds > 0:
         out["serving"] = serve(args)
         out["p99_chunk_latency_ms"] = out["serving"]["latency_ms"]["p99"]

@@ -168,6 +168,24 @@ int  lvn_batch_wait(lvn_batch* b);
  * on the batch stream: stage_ms[0 .. LVN_NUM_STAGES - 1] in pipeline order
  * (see lvn_batch_stage_name); n_stages >= LVN_NUM_STAGES. */
 #define LVN_NUM_STAGES 12
+
+/* Launch shapes of the batched pipeline (no counterpart in the reference, whose
+ * NDRanges are fixed).  Results never depend on them, only time does. */
+typedef struct lvn_tuning {
+    int32_t density_ysplit;     /* y segments per (x, z) column work item of the density stage, 1..F */
+    int32_t persistent_per_cu;  /* persistent workgroups per CU in the density and edge-crossing stages, 1..2 */
+    int32_t row_blocks;         /* workgroups over the batch for the edge / voxel bookkeeping kernels */
+    int32_t voxel_blocks;       /* workgroups over the batch for the per-voxel kernels (QEF, emit) */
+    int32_t reserved[4];
+} lvn_tuning;
+
+int lvn_batch_get_tuning(lvn_batch* b, lvn_tuning* out);
+/* Set the launch shapes (validated; the captured graph is rebuilt on the next run). */
+int lvn_batch_set_tuning(lvn_batch* b, const lvn_tuning* t);
+/* Autotuner: times candidate shapes on the staged chunks (one untimed and `reps`
+ * timed passes per candidate, per-stage HIP events), one parameter at a time, keeps
+ * the fastest for the stages each parameter drives, and writes the result to *out. */
+int lvn_batch_autotune(lvn_batch* b, int32_t reps, lvn_tuning* out);
 int  lvn_batch_run_timed(lvn_batch* b, float* stage_ms, int32_t n_stages);
 const char* lvn_batch_stage_name(int32_t stage);
 /* Per chunk: 6 ints = edges, active voxels, triangles, seam nodes, status, and the

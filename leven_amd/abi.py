@@ -130,6 +130,16 @@ class Result(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("density_ysplit", C.c_int32),
+        ("persistent_per_cu", C.c_int32),
+        ("row_blocks", C.c_int32),
+        ("voxel_blocks", C.c_int32),
+        ("reserved", C.c_int32 * 4),
+    ]
+
+
 assert C.sizeof(Request) == 96
 assert C.sizeof(Result) == 48
 

@@ -1228,9 +1228,10 @@ void launch_clear_counters(const KArgs& a, int n, hipStream_t s) {
                        a.counters, n, a.queue);
 }
 
-// Two 1024-thread workgroups per CU (the LDS table limit), fewer when the work is small.
+// persist_per_cu workgroups per CU (at most two: the LDS table limit), fewer when
+// the work is small.
 static unsigned persistent_blocks(const KArgs& a, long items) {
-    return (unsigned)std::max(1L, std::min((long)2 * a.cus, (items + 15) / 16));
+    return (unsigned)std::max(1L, std::min((long)a.persist_per_cu * a.cus, (items + 15) / 16));
 }
 
 void launch_density(const KArgs& a, int chunks, hipStream_t s) {
@@ -1244,17 +1245,17 @@ void launch_csg(const KArgs& a, int chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_csg, dim3(1, chunks), dim3(1024), 0, s, a);
 }
 // Workgroups per chunk for a kernel that loops over `threads` work items of 256-thread
-// workgroups: all of them for a few chunks, about 8192 workgroups over a large batch.
-static unsigned chunk_blocks(int chunks, long threads) {
+// workgroups: all of them for a few chunks, about `target` workgroups over a large batch.
+static unsigned chunk_blocks(int chunks, long threads, int target) {
     const long full = std::max(1L, (threads + 255) / 256);
-    const long share = std::max(1L, (8192L + chunks - 1) / chunks);
+    const long share = std::max(1L, ((long)target + chunks - 1) / chunks);
     return (unsigned)std::min(full, share);
 }
 
 void launch_edge_rows(const KArgs& a, int chunks, hipStream_t s) {
     const long solid_waves = (a.G.field_rows + kRowsPerBlock - 1) / kRowsPerBlock;
-    hipLaunchKernelGGL(k_solid_rows, dim3(chunk_blocks(chunks, 64L * solid_waves), chunks), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_edge_rows, dim3(chunk_blocks(chunks, a.G.edge_rows), chunks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_solid_rows, dim3(chunk_blocks(chunks, 64L * solid_waves, a.row_blocks), chunks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_edge_rows, dim3(chunk_blocks(chunks, a.G.edge_rows, a.row_blocks), chunks), dim3(256), 0, s, a);
 }
 void launch_scan(const KArgs& a, int chunks, ScanKind kind, hipStream_t s) {
     const dim3 grid(1, chunks), block(1024);
@@ -1266,7 +1267,7 @@ void launch_scan(const KArgs& a, int chunks, ScanKind kind, hipStream_t s) {
 }
 void launch_edge_compact(const KArgs& a, int chunks, hipStream_t s) {
     const long groups = (a.G.edge_rows + 63) / 64;
-    hipLaunchKernelGGL(k_edge_compact, dim3(chunk_blocks(chunks, 64L * groups), chunks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_edge_compact, dim3(chunk_blocks(chunks, 64L * groups, a.row_blocks), chunks), dim3(256), 0, s, a);
 }
 void launch_edge_cross(const KArgs& a, int chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_edge_items, dim3(1), dim3(1024), 0, s, a, chunks);
@@ -1278,21 +1279,21 @@ void launch_edge_cross(const KArgs& a, int chunks, hipStream_t s) {
         hipLaunchKernelGGL(k_edge_cross<false>, grid, dim3(kCrossThreads), 0, s, a, chunks);
 }
 void launch_voxel_rows(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_voxel_rows, dim3(chunk_blocks(chunks, a.G.voxel_segs), chunks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_voxel_rows, dim3(chunk_blocks(chunks, a.G.voxel_segs, a.row_blocks), chunks), dim3(256), 0, s, a);
 }
 void launch_voxel_compact(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_voxel_compact, dim3(chunk_blocks(chunks, a.G.voxel_segs), chunks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_voxel_compact, dim3(chunk_blocks(chunks, a.G.voxel_segs, a.row_blocks), chunks), dim3(256), 0, s, a);
 }
 // The kernels below loop up to the chunk's voxel count, known only on the device.
 void launch_qef(const KArgs& a, int chunks, hipStream_t s) {
-    const dim3 grid(chunk_blocks(chunks, a.caps.voxels), chunks);
+    const dim3 grid(chunk_blocks(chunks, a.caps.voxels, a.voxel_blocks), chunks);
     if (a.precise)
         hipLaunchKernelGGL(k_qef<true>, grid, dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL(k_qef<false>, grid, dim3(256), 0, s, a);
 }
 void launch_emit(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_emit, dim3(chunk_blocks(chunks, a.caps.voxels), chunks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_emit, dim3(chunk_blocks(chunks, a.caps.voxels, a.voxel_blocks), chunks), dim3(256), 0, s, a);
 }
 
 const char* const kStageNames[kNumStages] = {

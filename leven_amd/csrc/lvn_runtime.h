@@ -95,8 +95,14 @@ struct Batch {
     const void* graph_ops = nullptr;   // op arrays the graph was captured with
     const void* graph_index = nullptr;
     unsigned long long* d_work = nullptr;
+    lvn_tuning tuning = {0, 2, 8192, 8192, {0, 0, 0, 0}};  // density_ysplit 0 = by batch size
     ~Batch();
 };
+
+// Launch-shape defaults shared by every form (the batched form overrides them per batch).
+constexpr int kDefaultPersistPerCu = 2;
+constexpr int kDefaultRowBlocks = 8192;
+constexpr int kDefaultVoxelBlocks = 8192;
 
 DevOp make_dev_op(const lvn_csg_op& op);
 Geometry make_geometry(int n);

@@ -325,6 +325,9 @@ KArgs Ctx::kargs() const {
     a.queue = d_queue;
     a.item_prefix = d_item_prefix;
     a.cus = cus;
+    a.persist_per_cu = kDefaultPersistPerCu;
+    a.row_blocks = kDefaultRowBlocks;
+    a.voxel_blocks = kDefaultVoxelBlocks;
     return a;
 }
 
@@ -530,11 +533,50 @@ static int stage_batch(Batch* b, int n, const lvn_chunk_req* reqs) {
     return LVN_OK;
 }
 
-static KArgs batch_args(Batch* b) {
+// The batch's launch shapes with the automatic column split resolved: small batches
+// split the density columns to fill the GPU.
+static lvn_tuning effective_tuning(const Batch* b) {
+    lvn_tuning t = b->tuning;
+    if (t.density_ysplit == 0) t.density_ysplit = b->n >= 64 ? 1 : 6;
+    return t;
+}
+
+static KArgs batch_args(const Batch* b, const lvn_tuning& t) {
     KArgs a = b->ctx.kargs();
-    a.ysplit = b->n >= 64 ? 1 : 6;  // small batches split columns to fill the GPU
+    a.ysplit = t.density_ysplit;
+    a.persist_per_cu = t.persistent_per_cu;
+    a.row_blocks = t.row_blocks;
+    a.voxel_blocks = t.voxel_blocks;
     return a;
 }
+
+static KArgs batch_args(const Batch* b) { return batch_args(b, effective_tuning(b)); }
+
+// One pass of the pipeline without the graph, each stage bracketed by HIP events.
+static int time_stages(Batch* b, const KArgs& a, float* stage_ms) {
+    Ctx* c = &b->ctx;
+    hipEvent_t ev[kNumStages + 1];
+    for (auto& e : ev) LVN_HIP(hipEventCreate(&e));
+    launch_clear_counters(a, std::max(b->n, 1), c->stream);
+    for (int i = 0; i < kNumStages; ++i) {
+        LVN_HIP(hipEventRecord(ev[i], c->stream));
+        if (b->n) launch_stage(a, b->n, i, c->stream);
+    }
+    LVN_HIP(hipEventRecord(ev[kNumStages], c->stream));
+    LVN_HIP(hipEventSynchronize(ev[kNumStages]));
+    LVN_HIP(hipGetLastError());
+    for (int i = 0; i < kNumStages; ++i) LVN_HIP(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
+    for (auto& e : ev) LVN_HIP(hipEventDestroy(e));
+    return LVN_OK;
+}
+
+static bool valid_tuning(const Batch* b, const lvn_tuning& t) {
+    return t.density_ysplit >= 0 && t.density_ysplit <= b->ctx.G.F && t.persistent_per_cu >= 1 &&
+           t.persistent_per_cu <= 2 && t.row_blocks >= 1 && t.row_blocks <= (1 << 20) && t.voxel_blocks >= 1 &&
+           t.voxel_blocks <= (1 << 20);
+}
+
+static void drop_graph(Batch* b) { b->graph_n = -1; }
 
 }  // namespace lvn
 
@@ -794,20 +836,70 @@ const char* lvn_batch_stage_name(int32_t stage) {
 int lvn_batch_run_timed(lvn_batch* bh, float* stage_ms, int32_t n_stages) {
     Batch* b = reinterpret_cast<Batch*>(bh);
     if (!b || !stage_ms || n_stages < kNumStages) return LVN_ERR_INVALID;
-    Ctx* c = &b->ctx;
-    const KArgs a = batch_args(b);
-    hipEvent_t ev[kNumStages + 1];
-    for (auto& e : ev) LVN_HIP(hipEventCreate(&e));
-    launch_clear_counters(a, std::max(b->n, 1), c->stream);
-    for (int i = 0; i < kNumStages; ++i) {
-        LVN_HIP(hipEventRecord(ev[i], c->stream));
-        if (b->n) launch_stage(a, b->n, i, c->stream);
+    return time_stages(b, batch_args(b), stage_ms);
+}
+
+int lvn_batch_get_tuning(lvn_batch* bh, lvn_tuning* out) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (!b || !out) return LVN_ERR_INVALID;
+    *out = effective_tuning(b);
+    return LVN_OK;
+}
+
+int lvn_batch_set_tuning(lvn_batch* bh, const lvn_tuning* t) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (!b || !t || !valid_tuning(b, *t)) return LVN_ERR_INVALID;
+    b->tuning = *t;
+    drop_graph(b);
+    return LVN_OK;
+}
+
+int lvn_batch_autotune(lvn_batch* bh, int32_t reps, lvn_tuning* out) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (!b || reps < 1 || reps > 100) return LVN_ERR_INVALID;
+    lvn_tuning best = effective_tuning(b);
+    if (b->n > 0) {
+        struct Param {
+            int32_t lvn_tuning::*field;
+            std::vector<int32_t> values;
+            std::vector<int> stages;  // the stages whose time the parameter drives
+        };
+        const std::vector<Param> params = {
+            {&lvn_tuning::density_ysplit, {1, 2, 3, 6}, {0}},
+            {&lvn_tuning::persistent_per_cu, {1, 2}, {0, 5}},
+            {&lvn_tuning::row_blocks, {1024, 2048, 4096, 8192, 16384, 32768}, {2, 4, 6, 8}},
+            {&lvn_tuning::voxel_blocks, {1024, 2048, 4096, 8192, 16384, 32768}, {9, 11}},
+        };
+        float ms[kNumStages];
+        for (const Param& p : params) {
+            int32_t pick = best.*p.field;
+            float pick_ms = 0.f;
+            bool first = true;
+            for (int32_t v : p.values) {
+                lvn_tuning cand = best;
+                cand.*p.field = v;
+                if (!valid_tuning(b, cand)) continue;
+                const KArgs a = batch_args(b, cand);
+                int r = time_stages(b, a, ms);  // untimed warm-up of this shape
+                if (r) return r;
+                float total = 0.f;
+                for (int k = 0; k < reps; ++k) {
+                    r = time_stages(b, a, ms);
+                    if (r) return r;
+                    for (int st : p.stages) total += ms[st];
+                }
+                if (first || total < pick_ms) {
+                    pick = v;
+                    pick_ms = total;
+                    first = false;
+                }
+            }
+            best.*p.field = pick;
+        }
+        b->tuning = best;
+        drop_graph(b);
     }
-    LVN_HIP(hipEventRecord(ev[kNumStages], c->stream));
-    LVN_HIP(hipEventSynchronize(ev[kNumStages]));
-    LVN_HIP(hipGetLastError());
-    for (int i = 0; i < kNumStages; ++i) LVN_HIP(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
-    for (auto& e : ev) LVN_HIP(hipEventDestroy(e));
+    if (out) *out = best;
     return LVN_OK;
 }
 

@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 from .abi import (CSG_OP_DTYPE, ERROR_NAMES, LVN_ERR_CAPACITY, LVN_OK, REQ_EDIT, REQ_MESH, RESULT_DTYPE, SEAM_DTYPE,
-                  VERTEX_DTYPE, CsgOp, FieldDesc, Request, StageDump, ptr, vptr)
+                  VERTEX_DTYPE, CsgOp, FieldDesc, Request, StageDump, Tuning, ptr, vptr)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblvn_mesher.so")
@@ -22,6 +22,7 @@ EXPORTS = [
     "lvn_free_chunk", "lvn_chunk_has_surface", "lvn_generate_chunk_mesh", "lvn_debug_chunk_stages",
     "lvn_batch_create", "lvn_batch_destroy", "lvn_batch_set_chunks", "lvn_batch_run", "lvn_batch_wait",
     "lvn_batch_run_timed", "lvn_batch_stage_name", "lvn_batch_counts", "lvn_batch_get_mesh", "lvn_batch_work",
+    "lvn_batch_get_tuning", "lvn_batch_set_tuning", "lvn_batch_autotune",
     "lvn_server_create", "lvn_server_destroy", "lvn_server_submit", "lvn_server_poll", "lvn_server_drain",
     "lvn_server_get_mesh", "lvn_server_stats",
 ]
@@ -71,6 +72,9 @@ def lib():
             "lvn_batch_get_mesh": (i32, [vp, i32, vp, i32, C.POINTER(i32), vp, i32, C.POINTER(i32), vp, i32,
                                          C.POINTER(i32)]),
             "lvn_batch_work": (i32, [vp, vp]),
+            "lvn_batch_get_tuning": (i32, [vp, vp]),
+            "lvn_batch_set_tuning": (i32, [vp, vp]),
+            "lvn_batch_autotune": (i32, [vp, i32, vp]),
             "lvn_server_create": (i32, [i32, C.c_uint64, i32, i32, i32, i32, u32, C.POINTER(vp)]),
             "lvn_server_destroy": (None, [vp]),
             "lvn_server_submit": (i32, [vp, C.POINTER(Request), C.POINTER(C.c_int64)]),
@@ -271,6 +275,26 @@ class Batch:
         check(lib().lvn_batch_run_timed(self._h, vptr(ms), NUM_STAGES), "lvn_batch_run_timed")
         names = [lib().lvn_batch_stage_name(i).decode() for i in range(NUM_STAGES)]
         return dict(zip(names, ms.astype(float).tolist()))
+
+    TUNING = ["density_ysplit", "persistent_per_cu", "row_blocks", "voxel_blocks"]
+
+    def tuning(self):
+        t = Tuning()
+        check(lib().lvn_batch_get_tuning(self._h, C.byref(t)), "lvn_batch_get_tuning")
+        return {k: int(getattr(t, k)) for k in self.TUNING}
+
+    def set_tuning(self, **kw):
+        t = Tuning()
+        check(lib().lvn_batch_get_tuning(self._h, C.byref(t)), "lvn_batch_get_tuning")
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        check(lib().lvn_batch_set_tuning(self._h, C.byref(t)), "lvn_batch_set_tuning")
+
+    def autotune(self, reps=3):
+        """Time candidate launch shapes on the staged chunks and keep the fastest."""
+        t = Tuning()
+        check(lib().lvn_batch_autotune(self._h, int(reps), C.byref(t)), "lvn_batch_autotune")
+        return {k: int(getattr(t, k)) for k in self.TUNING}
 
     def work(self):
         w = np.zeros(8, dtype=np.uint64)

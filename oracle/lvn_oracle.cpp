@@ -450,6 +450,10 @@ struct Vec3 {
     float x, y, z;
 };
 
+struct Vec3d {
+    double x, y, z;
+};
+
 // normalize (see lvn_oracle.h): zero vectors are returned unchanged.
 Vec3 normalize3(Vec3 v) {
     if (v.x == 0.f && v.y == 0.f && v.z == 0.f) return v;
@@ -715,10 +719,11 @@ struct orc_chunk {
     std::vector<uint8_t> mat;     // F^3
     std::vector<int32_t> edgeIds; // ascending
     std::vector<float> edgeData;  // 4 per edge
+    std::vector<uint8_t> edgeBrush;  // 1 when the edge took brush data (CSG-touched)
 
     std::vector<int32_t> vcode, vmask, vmat;
     std::vector<float> vpos, vnorm;
-    std::vector<double> vpos64;
+    std::vector<double> vpos64, vnorm64;
     std::vector<int32_t> tris;
     int quads = 0;
     std::vector<lvn_vertex> verts;
@@ -818,6 +823,21 @@ struct orc_chunk {
         out[3] = crossing;
     }
 
+    // FP64 twin of density_edge's normal at the same crossing t: central differences
+    // with h = 0.001 of the FP64 density, normalised in FP64.
+    Vec3d density_normal_f64(int x, int y, int z, int axis, float t) const {
+        const double p0[3] = {(double)(x * scale + off[0]), (double)(y * scale + off[1]), (double)(z * scale + off[2])};
+        double p[3] = {p0[0], p0[1], p0[2]};
+        p[axis] += (double)scale * (double)t;
+        const double h = 0.001;
+        const double dx = density_d(desc, perm.data(), p[0] + h, p[1], p[2]) - density_d(desc, perm.data(), p[0] - h, p[1], p[2]);
+        const double dy = density_d(desc, perm.data(), p[0], p[1] + h, p[2]) - density_d(desc, perm.data(), p[0], p[1] - h, p[2]);
+        const double dz = density_d(desc, perm.data(), p[0], p[1], p[2] + h) - density_d(desc, perm.data(), p[0], p[1], p[2] - h);
+        const double len = std::sqrt(dx * dx + dy * dy + dz * dz);
+        if (len == 0.0) return {0.0, 0.0, 0.0};
+        return {dx / len, dy / len, dz / len};
+    }
+
     bool solid(int idx) const { return mat[idx] != kAir; }
 
     // Sign-changing edges of the hermite grid in ascending id order
@@ -826,6 +846,7 @@ struct orc_chunk {
     void rebuild_edges(DataFn data_for) {
         std::vector<int32_t> ids;
         std::vector<float> data;
+        std::vector<uint8_t> brush;
         for (int z = 0; z < g.H; ++z)
             for (int y = 0; y < g.H; ++y)
                 for (int x = 0; x < g.H; ++x) {
@@ -835,13 +856,14 @@ struct orc_chunk {
                         if (s0 == solid(nb[a])) continue;
                         const int32_t id = ((x | (y << g.shift) | (z << (2 * g.shift))) << 2) | a;
                         float d4[4];
-                        data_for(x, y, z, a, id, d4);
+                        brush.push_back(data_for(x, y, z, a, id, d4) ? 1 : 0);
                         ids.push_back(id);
                         data.insert(data.end(), d4, d4 + 4);
                     }
                 }
         edgeIds.swap(ids);
         edgeData.swap(data);
+        edgeBrush.swap(brush);
     }
 
     void default_materials() {
@@ -890,10 +912,12 @@ struct orc_chunk {
         default_materials();
         const std::vector<uint8_t> touched = apply_brush_materials(brushes);
         rebuild_edges([&](int x, int y, int z, int a, int32_t, float* d4) {
-            if (!brushes.empty() && edge_touched(touched, x, y, z, a))
+            if (!brushes.empty() && edge_touched(touched, x, y, z, a)) {
                 brush_edge(x, y, z, a, brushes, d4);
-            else
-                density_edge(x, y, z, a, d4);
+                return true;
+            }
+            density_edge(x, y, z, a, d4);
+            return false;
         });
     }
 
@@ -901,15 +925,17 @@ struct orc_chunk {
         if (brushes.empty()) return;
         const std::vector<int32_t> oldIds = edgeIds;
         const std::vector<float> oldData = edgeData;
+        const std::vector<uint8_t> oldBrush = edgeBrush;
         const std::vector<uint8_t> touched = apply_brush_materials(brushes);
         rebuild_edges([&](int x, int y, int z, int a, int32_t id, float* d4) {
             if (edge_touched(touched, x, y, z, a)) {
                 brush_edge(x, y, z, a, brushes, d4);
-                return;
+                return true;
             }
             const auto it = std::lower_bound(oldIds.begin(), oldIds.end(), id);
             const size_t k = (size_t)(it - oldIds.begin());
             for (int c = 0; c < 4; ++c) d4[c] = oldData[4 * k + c];
+            return oldBrush[k] != 0;
         });
     }
 
@@ -922,6 +948,7 @@ struct orc_chunk {
         vpos.clear();
         vnorm.clear();
         vpos64.clear();
+        vnorm64.clear();
         tris.clear();
         verts.clear();
         seams.clear();
@@ -963,6 +990,7 @@ struct orc_chunk {
         vpos.assign((size_t)V * 4, 0.f);
         vnorm.assign((size_t)V * 4, 0.f);
         vpos64.assign((size_t)V * 4, 0.0);
+        vnorm64.assign((size_t)V * 4, 0.0);
         const float fscale = (float)scale;
         for (int v = 0; v < V; ++v) {  // CreateLeafNodes + SolveQEFs, cl/octree.cl:236-331
             int p[3];
@@ -970,6 +998,7 @@ struct orc_chunk {
             Qef<float> q;
             Qef<double> q64;
             V4<float> normal;
+            double n64[3] = {0.0, 0.0, 0.0};
             float count = 0.f;
             for (int i = 0; i < 12; ++i) {
                 if (!((vmask[v] >> i) & 1)) continue;
@@ -1002,6 +1031,10 @@ struct orc_chunk {
                 normal.z = normal.z + nrm.z;
                 normal.w = (normal.w + nrm.w) + 1.f;
                 count += 1.f;
+                const Vec3d e64 = edgeBrush[k] ? Vec3d{ed[0], ed[1], ed[2]} : density_normal_f64(hx, hy, hz, axis, t);
+                n64[0] += e64.x;
+                n64[1] += e64.y;
+                n64[2] += e64.z;
             }
             qef_finish_mass(q);
             qef_finish_mass(q64);
@@ -1010,6 +1043,7 @@ struct orc_chunk {
             vnorm[4 * v + 1] = normal.y / nw;
             vnorm[4 * v + 2] = normal.z / nw;
             vnorm[4 * v + 3] = 0.f;
+            for (int c = 0; c < 3; ++c) vnorm64[4 * v + c] = n64[c] / (double)count;
             const V4<float> x = qef_solve(q);
             vpos[4 * v + 0] = (x.x * 4.f) + (float)min[0];
             vpos[4 * v + 1] = (x.y * 4.f) + (float)min[1];
@@ -1219,6 +1253,9 @@ void orc_chunk_get_positions(const orc_chunk* c, float* pos4, float* normal4) {
 }
 void orc_chunk_get_positions_f64(const orc_chunk* c, double* pos4) {
     std::copy(c->vpos64.begin(), c->vpos64.end(), pos4);
+}
+void orc_chunk_get_normals_f64(const orc_chunk* c, double* normal4) {
+    std::copy(c->vnorm64.begin(), c->vnorm64.end(), normal4);
 }
 void orc_chunk_get_triangles(const orc_chunk* c, int32_t* tri3) { std::copy(c->tris.begin(), c->tris.end(), tri3); }
 void orc_chunk_get_vertices(const orc_chunk* c, lvn_vertex* v) { std::copy(c->verts.begin(), c->verts.end(), v); }

@@ -95,6 +95,9 @@ void orc_chunk_get_edges(const orc_chunk* c, int32_t* ids, float* data4);
 void orc_chunk_get_voxels(const orc_chunk* c, int32_t* codes, int32_t* masks, int32_t* materials);
 void orc_chunk_get_positions(const orc_chunk* c, float* pos4, float* normal4);
 void orc_chunk_get_positions_f64(const orc_chunk* c, double* pos4);
+/* FP64 vertex normals: the mean of the edges' FP64 central-difference normals at the
+ * same crossings (brush edges keep their FP32 normal). */
+void orc_chunk_get_normals_f64(const orc_chunk* c, double* normal4);
 void orc_chunk_get_triangles(const orc_chunk* c, int32_t* tri3);
 void orc_chunk_get_vertices(const orc_chunk* c, lvn_vertex* v);
 void orc_chunk_get_seams(const orc_chunk* c, lvn_seam_node* s);

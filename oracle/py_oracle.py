@@ -65,6 +65,7 @@ def lib():
             "orc_chunk_get_voxels": (None, [vp, vp, vp, vp]),
             "orc_chunk_get_positions": (None, [vp, vp, vp]),
             "orc_chunk_get_positions_f64": (None, [vp, vp]),
+            "orc_chunk_get_normals_f64": (None, [vp, vp]),
             "orc_chunk_get_triangles": (None, [vp, vp]),
             "orc_chunk_get_vertices": (None, [vp, vp]),
             "orc_chunk_get_seams": (None, [vp, vp]),
@@ -226,6 +227,12 @@ class Chunk:
         pos = np.zeros((v, 4), dtype=np.float64)
         lib().orc_chunk_get_positions_f64(self._h, vptr(pos))
         return pos
+
+    def normals_f64(self):
+        v = self.counts()["voxels"]
+        nrm = np.zeros((v, 4), dtype=np.float64)
+        lib().orc_chunk_get_normals_f64(self._h, vptr(nrm))
+        return nrm
 
     def triangles(self):
         t = self.counts()["quads"] * 2

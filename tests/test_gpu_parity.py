@@ -251,3 +251,71 @@ def test_batched_matches_oracle_and_replays(oracle):
                 assert (s0.view(np.uint32) == s1.view(np.uint32)).all()
         other.debug_stages((0, -32, 0), 64)  # eager work between replays
     mesher.clear_csg_ops()
+
+
+def test_launch_shapes_do_not_change_results():
+    """Every launch shape (column split, persistent workgroups per CU, workgroups over the
+    batch) and the autotuner's pick give bit-identical meshes."""
+    w = workloads.terrain_512()
+    setup_field(w["desc"], w["seed"])
+    for op, bb in zip(w["ops"], w["aabbs"]):
+        mesher.store_csg_op(op, bb[:3], bb[3:])
+    pick = list(range(150, 330, 9))
+    b = mesher.Batch(64, len(pick), edge_cap=262144, voxel_cap=131072)
+    b.set_chunks(w["mins"][pick], w["sizes"][pick])
+    b.run()
+    b.wait()
+    ref = [b.get_mesh(i) for i in range(len(pick))]
+    assert sum(len(m[1]) > 0 for m in ref) >= 5
+    shapes = [dict(density_ysplit=3, persistent_per_cu=1, row_blocks=1, voxel_blocks=7),
+              dict(density_ysplit=11, persistent_per_cu=2, row_blocks=37, voxel_blocks=1),
+              dict(density_ysplit=66, persistent_per_cu=1, row_blocks=100000, voxel_blocks=100000)]
+    for shape in shapes:
+        b.set_tuning(**shape)
+        assert b.tuning() == shape
+        b.run()
+        b.wait()
+        for i in range(len(pick)):
+            assert all((x.view(np.uint32) == y.view(np.uint32)).all() for x, y in zip(b.get_mesh(i), ref[i])), \
+                f"{shape}: chunk {pick[i]}"
+    chosen = b.autotune(reps=1)
+    assert b.tuning() == chosen
+    b.run()
+    b.wait()
+    for i in range(len(pick)):
+        assert all((x.view(np.uint32) == y.view(np.uint32)).all() for x, y in zip(b.get_mesh(i), ref[i]))
+    with pytest.raises(RuntimeError):
+        b.set_tuning(persistent_per_cu=3)
+    mesher.clear_csg_ops()
+
+
+def test_batched_mixed_lods_match_oracle(oracle):
+    """A ragged batch: clipmap chunks of every LOD (voxel sizes 1..32) in one batch with
+    full per-slot capacities, from empty chunks to dense LOD-5 cave chunks, each
+    bit-identical to the oracle."""
+    w = workloads.clipmap()
+    setup_field(w["desc"], w["seed"])
+    for op, bb in zip(w["ops"], w["aabbs"]):
+        mesher.store_csg_op(op, bb[:3], bb[3:])
+    # per LOD 0..3 and 5: a surface or cave chunk and an empty (or sparse) one; chunk 0 is a
+    # LOD-5 cave chunk with ~237k of its 262,144 voxels active (the per-slot maximum case)
+    pick = [930, 1110, 923, 1117, 755, 1137, 748, 1140, 0, 1888]
+    mins, sizes = w["mins"][pick], w["sizes"][pick]
+    assert sorted(set((sizes // 256).tolist())) == [1, 2, 4, 8, 32]
+    b = mesher.Batch(64, len(pick), edge_cap=0, voxel_cap=0)
+    b.set_chunks(mins, sizes)
+    b.run()
+    b.wait()
+    counts = b.counts()
+    assert (counts[:, 4] == 0).all()
+    perm = oracle.perm_table(w["seed"])
+    for i, (mn, size) in enumerate(zip(mins, sizes)):
+        c = oracle.Chunk(64).generate(w["desc"], perm, mn, int(size), ops=_ops_for_chunk(w, mn, int(size))).build()
+        v, t, s = b.get_mesh(i)
+        assert (t == c.triangles()).all(), f"chunk {pick[i]} (size {size}) triangles"
+        assert (v.view(np.uint32) == c.vertices().view(np.uint32)).all(), f"chunk {pick[i]} vertices"
+        assert (s.view(np.uint32) == c.seams().view(np.uint32)).all(), f"chunk {pick[i]} seams"
+    print(f"\nmixed LODs: {len(pick)} chunks, active voxels {counts[:, 1].min()}..{counts[:, 1].max()}, "
+          f"empty {(counts[:, 2] == 0).sum()}")
+    assert counts[:, 1].max() > 200000 and (counts[:, 2] == 0).any()
+    mesher.clear_csg_ops()

@@ -275,6 +275,7 @@ def test_plane_closed_form(oracle, n):
         expect = (np.array([xyz[0] + 0.5, n // 2, xyz[2] + 0.5]) * 4 + np.array(mn))
         assert np.allclose(pos[i, :3], expect, atol=1e-4)
     assert (nrm[:, :3] == [0, 1, 0]).all()
+    assert np.allclose(c.normals_f64()[:, :3], [0, 1, 0], atol=1e-12)  # the FP64 twin of the normals
     tri = c.triangles()
     # winding: corner 5 (lower end of the host's edge 7) is solid -> flipped order {0,3,1,0,2,3}
     assert (tri[0::2, 0] == tri[1::2, 0]).all()
