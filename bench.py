@@ -126,6 +126,33 @@ def clipmap_leg(args):
             "stage_ms": stage_ms, "tuning": tuning}
 
 
+def dropin_latency(wl, count):
+    """The drop-in path a leven caller uses (Compute_MeshGenContext::generateChunkMesh =
+    lvn_generate_chunk_mesh): one fresh chunk per call, mesh copied into host buffers."""
+    import ctypes as C
+
+    from leven_amd.abi import SEAM_DTYPE, VERTEX_DTYPE, vptr
+    ctx = mesher.Context(64)
+    n3 = 64 ** 3
+    v = np.zeros(n3, dtype=VERTEX_DTYPE)
+    t = np.zeros((6 * n3, 3), dtype=np.int32)
+    s = np.zeros(n3, dtype=SEAM_DTYPE)
+    nv, nt, ns = C.c_int32(), C.c_int32(), C.c_int32()
+    lat = []
+    idx = np.linspace(0, len(wl["sizes"]) - 1, count).astype(int)
+    for i in idx:
+        mn = np.ascontiguousarray(wl["mins"][i], dtype=np.int32)
+        ts = time.perf_counter()
+        rc = mesher.lib().lvn_generate_chunk_mesh(ctx._h, vptr(mn), int(wl["sizes"][i]), vptr(v), n3, C.byref(nv),
+                                                  vptr(t), 6 * n3, C.byref(nt), vptr(s), n3, C.byref(ns))
+        lat.append(time.perf_counter() - ts)
+        assert rc == 0, f"lvn_generate_chunk_mesh failed: {rc}"
+    ctx.close()
+    lat = np.array(lat) * 1e3
+    return {"chunks": int(count), "chunks_per_s": round(count / (lat.sum() * 1e-3), 1),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)}}
+
+
 def serve_burst():
     """Capacity of the served form: the 512 terrain_512 chunks submitted at once to a
     fresh server (all cache misses), meshed by continuous batching on 3 lanes."""
@@ -276,6 +303,7 @@ def main():
     lat = np.array(lat) * 1e3
     batch.close()
     plain.close()
+    dropin = dropin_latency(wl, min(n, 128))
 
     out = {
         "metric": METRIC,
@@ -297,6 +325,7 @@ def main():
         "batched_over_plain": round(batched / plain_rate, 2),
         "plain_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                              "p99": round(float(np.percentile(lat, 99)), 3)},
+        "dropin_generate_chunk_mesh": dropin,
         "totals": {"edges": int(counts[:, 0].sum()), "voxels": int(counts[:, 1].sum()),
                    "triangles": int(counts[:, 2].sum()), "nonempty_chunks": int((counts[:, 2] > 0).sum())},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},

@@ -176,7 +176,8 @@ typedef struct lvn_tuning {
     int32_t persistent_per_cu;  /* persistent workgroups per CU in the density and edge-crossing stages, 1..2 */
     int32_t row_blocks;         /* workgroups over the batch for the edge / voxel bookkeeping kernels */
     int32_t voxel_blocks;       /* workgroups over the batch for the per-voxel kernels (QEF, emit) */
-    int32_t reserved[4];
+    int32_t cross_lanes;        /* lanes sharing one edge in the edge-crossing stage: 1, 2, 4 or 8 */
+    int32_t reserved[3];
 } lvn_tuning;
 
 int lvn_batch_get_tuning(lvn_batch* b, lvn_tuning* out);

@@ -136,7 +136,8 @@ class Tuning(C.Structure):
         ("persistent_per_cu", C.c_int32),
         ("row_blocks", C.c_int32),
         ("voxel_blocks", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("cross_lanes", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 

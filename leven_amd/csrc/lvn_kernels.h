@@ -32,6 +32,8 @@ struct KArgs {
     const int* slot_map;     // batch entry -> arena slot, or null for the identity
     int row_blocks;          // workgroups over the batch for the bookkeeping kernels
     int voxel_blocks;        // workgroups over the batch for the per-voxel kernels
+    int cross_lanes;         // lanes per edge in the crossing stage (1, 2, 4, 8), 0 = by batch size
+    int pad3;
 };
 
 constexpr int kNumQueues = 8;
@@ -56,6 +58,9 @@ void launch_voxel_rows(const KArgs& a, int chunks, hipStream_t s);
 void launch_voxel_compact(const KArgs& a, int chunks, hipStream_t s);
 void launch_qef(const KArgs& a, int chunks, hipStream_t s);
 void launch_emit(const KArgs& a, int chunks, hipStream_t s);
+
+// Lanes per edge in the crossing stage when the launch shapes leave it open.
+int default_cross_lanes(int chunks);
 
 // Whole pipeline for chunk slots [0, chunks), counters cleared first.
 void launch_pipeline(const KArgs& a, int chunks, hipStream_t s);

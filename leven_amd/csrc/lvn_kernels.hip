@@ -1,10 +1,11 @@
 // lvn_kernels.hip — the chunk-mesher stages as gfx950 kernels.
 //
-// Every kernel takes the chunk slot from blockIdx.y, so one launch serves one
-// chunk (plain form) or a whole batch (batched form).  Compaction is
-// deterministic: each stage writes a 64-bit ballot mask and a count per
-// 64-element segment, one workgroup per chunk scans the counts, and the
-// compacting kernel places element i at base[seg] + popcount(mask below i).
+// Every kernel takes the chunk from blockIdx.y (or, in the persistent kernels,
+// from a work item), so one launch serves one chunk (plain form) or a whole
+// batch (batched form).  Compaction is deterministic: a classification kernel
+// writes 64-bit masks and counts per row (edges: x/y/z masks of a hermite row;
+// voxels, quads, seams: one mask per 64 elements), one workgroup per chunk scans
+// the counts, and element i lands at base[row] + popcount(mask bits below i).
 // The element order is therefore the reference's linear order everywhere
 // (cl/density_field.cl:52-53, cl/octree.cl:153, cl/octree.cl:398-439).
 
@@ -114,15 +115,25 @@ __device__ __forceinline__ void density_column(const KArgs& a, const NoiseLds& T
     const ChunkDesc& cd = a.chunks[chunk];
     const int z = div_magic(col, G.magic_f);
     const int x = col - z * F;
+    const int y0 = seg * seg_len;
+    const int y1 = min(F, y0 + seg_len);
+    uint8_t* mat = slot_ptr<uint8_t>(a, chunk, a.L.mat);
+    uint8_t* orig = slot_ptr<uint8_t>(a, chunk, a.L.orig);
+    if (cd.mode == kModeCached) {  // a cached field: its materials, no density
+        const uint8_t* old = reinterpret_cast<const uint8_t*>(cd.old_mat);
+        for (int y = y0; y < y1; ++y) {
+            const int idx = field_index(G, x, y, z);
+            const uint8_t m = old[idx];
+            mat[idx] = m;
+            orig[idx] = m;
+        }
+        return;
+    }
     const float wx = (float)(x * cd.scale + cd.off[0]);
     const float wz = (float)(z * cd.scale + cd.off[2]);
     const float H = height_term(a.F, T, wx, wz);
     float* dens = slot_ptr<float>(a, chunk, a.L.density);
     __half* dens16 = slot_ptr<__half>(a, chunk, a.L.density);
-    uint8_t* mat = slot_ptr<uint8_t>(a, chunk, a.L.mat);
-    uint8_t* orig = slot_ptr<uint8_t>(a, chunk, a.L.orig);
-    const int y0 = seg * seg_len;
-    const int y1 = min(F, y0 + seg_len);
     unsigned caves = 0;
     for (int y = y0; y < y1; ++y) {
         const float wy = (float)(y * cd.scale + cd.off[1]);
@@ -540,16 +551,16 @@ __device__ __forceinline__ void cross_edge(const KArgs& a, const NoiseLds& T, in
     slot_ptr<float4>(a, chunk, a.L.e_data)[i] = out;
 }
 
-// Items of 64 edges over all chunks: item_prefix[c] = sum over earlier chunks of
-// ceil(E / 64) (k_edge_items), item_prefix[chunks] = total.
-__global__ __launch_bounds__(1024) void k_edge_items(KArgs a, int chunks) {
+// Items of `per_item` edges (64 / lanes per edge) over all chunks: item_prefix[c] =
+// sum over earlier chunks of ceil(E / per_item), item_prefix[chunks] = total.
+__global__ __launch_bounds__(1024) void k_edge_items(KArgs a, int chunks, int per_item) {
     __shared__ int buf[1024];
     __shared__ int carry;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     for (int base = 0; base < chunks; base += 1024) {
         const int c = base + threadIdx.x;
-        const int v = c < chunks ? (a.counters[c].edges + 63) >> 6 : 0;
+        const int v = c < chunks ? (a.counters[c].edges + per_item - 1) / per_item : 0;
         buf[threadIdx.x] = v;
         __syncthreads();
         for (int off = 1; off < 1024; off <<= 1) {
@@ -589,6 +600,146 @@ __global__ __attribute__((amdgpu_waves_per_eu(6, 8))) __launch_bounds__(kCrossTh
         }
         const int i = (item - a.item_prefix[lo]) * 64 + lane;
         if (i < a.counters[lo].edges) cross_edge<kCount>(a, T, lo, i);
+    }
+}
+
+// Small batches: kG lanes share one edge, so a chunk's few hundred edge items become
+// kG times as many waves with kG times shorter dependency chains.  Lane `sub`
+// evaluates samples k = sub, sub + kG, ... of the 17; the group reduces to the first
+// arg-min (smallest |d|, then smallest k, as the sequential scan decides), then the
+// six normal samples are spread over the group.  Every value is computed by the
+// same expression as in density_crossing, so results are identical.
+template <bool kCount, int kG>
+__device__ __forceinline__ void cross_edge_group(const KArgs& a, const NoiseLds& T, int chunk, int i, int sub) {
+    const Geometry& G = a.G;
+    const ChunkDesc& cd = a.chunks[chunk];
+    const int id = slot_ptr<int>(a, chunk, a.L.e_ids)[i];
+    const int axis = id & 3, hidx = id >> 2;
+    const int msk = (1 << G.shift) - 1;
+    const int x = hidx & msk, y = (hidx >> G.shift) & msk, z = (hidx >> (2 * G.shift)) & msk;
+    if (x >= G.H || y >= G.H || z >= G.H || axis > 2) {  // never produced by k_edge_compact
+        if (sub == 0) atomicOr(&a.counters[chunk].status, kStatusLookupMiss);
+        return;
+    }
+    const int i0 = field_index(G, x, y, z);
+    const int i1 = i0 + (axis == 0 ? 1 : axis == 1 ? G.F : G.F * G.F);
+    const uint8_t* mat = slot_ptr<uint8_t>(a, chunk, a.L.mat);
+    const uint8_t* orig = slot_ptr<uint8_t>(a, chunk, a.L.orig);
+    const bool touched = cd.op_count > 0 && (mat[i0] != orig[i0] || mat[i1] != orig[i1]);
+    if (touched || cd.mode == kModeCached) {  // brush data or the cached list: one lane
+        if (sub == 0) cross_edge<kCount>(a, T, chunk, i);
+        return;
+    }
+    const FieldParams& Fp = a.F;
+    const bool caves = Fp.kind == LVN_FIELD_TERRAIN_CAVES;
+    const bool ends = !a.fp16;
+    const float* dens = slot_ptr<float>(a, chunk, a.L.density);
+    const float d_start = ends ? dens[i0] : 0.f, d_end = ends ? dens[i1] : 0.f;
+    const int w0x = x * cd.scale + cd.off[0], w0y = y * cd.scale + cd.off[1], w0z = z * cd.scale + cd.off[2];
+    const float p0x = (float)w0x, p0y = (float)w0y, p0z = (float)w0z;
+    const float p1x = (float)(w0x + (axis == 0 ? cd.scale : 0));
+    const float p1y = (float)(w0y + (axis == 1 ? cd.scale : 0));
+    const float p1z = (float)(w0z + (axis == 2 ? cd.scale : 0));
+    unsigned nh = 0, nc = 0;
+    const float colH = axis == 1 ? height_term(Fp, T, p0x, p0z) : 0.f;
+    nh += (axis == 1 && sub == 0) ? 1u : 0u;
+    float minV = 3.402823466e+38f, Hbest = colH;
+    int kbest = 17;
+    for (int k = sub; k <= 16; k += kG) {
+        const float cur = (float)k * 0.0625f;  // = the sequential scan's running sum (exact)
+        const float px = p0x + (p1x - p0x) * cur;
+        const float py = p0y + (p1y - p0y) * cur;
+        const float pz = p0z + (p1z - p0z) * cur;
+        float d, H = colH;
+        if (ends && k == 0) {
+            d = fabsf(d_start);
+        } else if (ends && k == 16) {
+            d = fabsf(d_end);
+        } else {
+            if (axis != 1) {
+                H = height_term(Fp, T, px, pz);
+                ++nh;
+            }
+            nc += (caves && !((py - H) > Fp.cave_cut)) ? 1u : 0u;
+            d = fabsf(combine(Fp, T, py - H, px, py, pz));
+        }
+        if (d < minV) {
+            minV = d;
+            kbest = k;
+            Hbest = H;
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < kG; m <<= 1) {
+        const float ov = __shfl_xor(minV, m, kG);
+        const int ok = __shfl_xor(kbest, m, kG);
+        const float oh = __shfl_xor(Hbest, m, kG);
+        if (ov < minV || (ov == minV && ok < kbest)) {
+            minV = ov;
+            kbest = ok;
+            Hbest = oh;
+        }
+    }
+    const float tbest = (float)kbest * 0.0625f;
+    const float px = p0x + (p1x - p0x) * tbest;
+    const float py = p0y + (p1y - p0y) * tbest;
+    const float pz = p0z + (p1z - p0z) * tbest;
+    const float h = 0.001f;
+    const bool hb_known = axis == 1 || !(ends && (kbest == 0 || kbest == 16));
+    constexpr int kR = (6 + kG - 1) / kG;
+    float vals[kR];
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+        vals[r] = 0.f;
+        const int j = r * kG + sub;  // normal sample j: +x, -x, +y, -y, +z, -z
+        if (j >= 6) continue;
+        const float sx = j == 0 ? h : j == 1 ? -h : 0.f;
+        const float sz = j == 4 ? h : j == 5 ? -h : 0.f;
+        float H;
+        if (j == 2 || j == 3) {
+            H = hb_known ? Hbest : height_term(Fp, T, px, pz);
+            nh += (!hb_known && j == 2) ? 1u : 0u;
+        } else {
+            H = height_term(Fp, T, j < 2 ? px + sx : px, j < 2 ? pz : pz + sz);
+            ++nh;
+        }
+        const float qx = j < 2 ? px + sx : px;
+        const float qy = j == 2 ? py + h : j == 3 ? py - h : py;
+        const float qz = j >= 4 ? pz + sz : pz;
+        const float aa = qy - H;
+        nc += (caves && !(aa > Fp.cave_cut)) ? 1u : 0u;
+        vals[r] = combine(Fp, T, aa, qx, qy, qz);
+    }
+    float v[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) v[j] = __shfl(vals[j / kG], j % kG, kG);
+    const float3 n = normalize_or_zero(v[0] - v[1], v[2] - v[3], v[4] - v[5]);
+    if constexpr (kCount) {
+        atomicAdd(&a.work[2], (unsigned long long)nh);
+        atomicAdd(&a.work[3], (unsigned long long)nc);
+        if (sub == 0) atomicAdd(&a.work[6], 1ull);
+    }
+    if (sub == 0) slot_ptr<float4>(a, chunk, a.L.e_data)[i] = make_float4(n.x, n.y, n.z, tbest);
+}
+
+template <bool kCount, int kG>
+__global__ __attribute__((amdgpu_waves_per_eu(6, 8))) __launch_bounds__(kCrossThreads) void k_edge_cross_group(
+    KArgs a, int chunks) {
+    __shared__ NoiseLds T;
+    load_noise_tables(T, a.perm, a.grads);
+    constexpr int kPerItem = 64 / kG;
+    const int total = a.item_prefix[chunks];
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        const int item = next_item(&a.queue[1]);
+        if (item >= total) break;
+        int lo = 0, hi = chunks;  // last chunk with item_prefix[c] <= item
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (a.item_prefix[mid] <= item) lo = mid; else hi = mid;
+        }
+        const int i = (item - a.item_prefix[lo]) * kPerItem + lane / kG;
+        if (i < a.counters[lo].edges) cross_edge_group<kCount, kG>(a, T, lo, i, lane & (kG - 1));
     }
 }
 
@@ -1269,9 +1420,33 @@ void launch_edge_compact(const KArgs& a, int chunks, hipStream_t s) {
     const long groups = (a.G.edge_rows + 63) / 64;
     hipLaunchKernelGGL(k_edge_compact, dim3(chunk_blocks(chunks, 64L * groups, a.row_blocks), chunks), dim3(256), 0, s, a);
 }
+// Lanes per edge of the crossing stage: the tuned value, else by batch size (a
+// typical chunk has a few hundred 64-edge items; kG multiplies the waves).
+int default_cross_lanes(int chunks) { return chunks <= 1 ? 8 : chunks <= 3 ? 4 : chunks <= 9 ? 2 : 1; }
+
+static int cross_lanes(const KArgs& a, int chunks) {
+    return a.cross_lanes > 0 ? a.cross_lanes : default_cross_lanes(chunks);
+}
+
+template <int kG>
+static void launch_cross_group(const KArgs& a, int chunks, hipStream_t s) {
+    const dim3 grid(persistent_blocks(a, 512L * kG * chunks));
+    if (a.work)
+        hipLaunchKernelGGL((k_edge_cross_group<true, kG>), grid, dim3(kCrossThreads), 0, s, a, chunks);
+    else
+        hipLaunchKernelGGL((k_edge_cross_group<false, kG>), grid, dim3(kCrossThreads), 0, s, a, chunks);
+}
+
 void launch_edge_cross(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_edge_items, dim3(1), dim3(1024), 0, s, a, chunks);
+    const int g = cross_lanes(a, chunks);
+    hipLaunchKernelGGL(k_edge_items, dim3(1), dim3(1024), 0, s, a, chunks, 64 / g);
     // the item count is on the device; size the grid for 512 edge items per chunk
+    switch (g) {
+    case 2: launch_cross_group<2>(a, chunks, s); return;
+    case 4: launch_cross_group<4>(a, chunks, s); return;
+    case 8: launch_cross_group<8>(a, chunks, s); return;
+    default: break;
+    }
     const dim3 grid(persistent_blocks(a, 512L * chunks));
     if (a.work)
         hipLaunchKernelGGL(k_edge_cross<true>, grid, dim3(kCrossThreads), 0, s, a, chunks);

@@ -27,6 +27,7 @@ struct Global {
     float4* d_grads = nullptr;
     std::vector<lvn_csg_op> ops;
     std::vector<std::array<int32_t, 6>> aabbs;
+    uint64_t generation = 0;  // bumped when field, table pointers or default material change
     std::mutex mu;
 };
 extern Global g_state;
@@ -37,6 +38,7 @@ inline Key make_key(const int32_t* min, int32_t size) { return Key(min[0], min[1
 
 // Device-resident density-field state (GPUDensityField, src/compute_local.h:113-123):
 // final materials and the sorted edge list with its hermite data.
+// A cached density field: its materials and edge list, in the context's field pool.
 struct FieldEntry {
     uint8_t* d_mat = nullptr;
     int* d_ids = nullptr;
@@ -44,7 +46,17 @@ struct FieldEntry {
     int edges = 0;
     int cap = 0;
     int last_op = 0;
-    void release();
+};
+
+// Device memory for cached fields, handed out from large blocks: the reference keeps
+// a field until shutdown (src/compute_local.h:28-72), so entries are never freed one
+// by one and a bump allocator replaces a hipMalloc per cached chunk.
+struct DevicePool {
+    std::vector<void*> blocks;
+    char* cur = nullptr;
+    size_t left = 0;
+    void* take(size_t bytes);  // 256-byte aligned, or null when the device is out of memory
+    ~DevicePool();
 };
 
 // A finished chunk (the reference caches its octree; the mesh is a pure
@@ -78,6 +90,15 @@ struct Ctx {
     int cus = 0;
     std::map<Key, FieldEntry> fields;
     std::map<Key, MeshEntry> meshes;
+    DevicePool field_pool;
+    hipGraphExec_t exec = nullptr;       // the one-chunk pipeline, captured on first use
+    hipGraph_t graph = nullptr;
+    const void* graph_ops = nullptr;     // op arrays the graph was captured with
+    const void* graph_index = nullptr;
+    uint64_t graph_gen = 0;              // global generation the graph was captured with
+    float* h_verts = nullptr;   // pinned read-back staging for one chunk's mesh (caps sized)
+    int* h_tris = nullptr;
+    float* h_seams = nullptr;
 
     int allocate(int n, uint32_t flags, int nslots, int edge_cap, int voxel_cap);
     int ensure_ops(int nops, int nidx);
@@ -94,8 +115,9 @@ struct Batch {
     int graph_n = -1;
     const void* graph_ops = nullptr;   // op arrays the graph was captured with
     const void* graph_index = nullptr;
+    uint64_t graph_gen = 0;
     unsigned long long* d_work = nullptr;
-    lvn_tuning tuning = {0, 2, 8192, 8192, {0, 0, 0, 0}};  // density_ysplit 0 = by batch size
+    lvn_tuning tuning = {0, 2, 8192, 8192, 0, {0, 0, 0}};  // ysplit, cross_lanes 0 = by batch size
     ~Batch();
 };
 

@@ -333,7 +333,11 @@ KArgs Ctx::kargs() const {
 
 Ctx::~Ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
-    for (auto& kv : fields) kv.second.release();
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (h_verts) (void)hipHostFree(h_verts);
+    if (h_tris) (void)hipHostFree(h_tris);
+    if (h_seams) (void)hipHostFree(h_seams);
     if (d_arena) (void)hipFree(d_arena);
     if (d_chunks) (void)hipFree(d_chunks);
     if (d_counters) (void)hipFree(d_counters);
@@ -348,13 +352,24 @@ Ctx::~Ctx() {
     if (stream) (void)hipStreamDestroy(stream);
 }
 
-void FieldEntry::release() {
-    if (d_mat) (void)hipFree(d_mat);
-    if (d_ids) (void)hipFree(d_ids);
-    if (d_data) (void)hipFree(d_data);
-    d_mat = nullptr;
-    d_ids = nullptr;
-    d_data = nullptr;
+void* DevicePool::take(size_t bytes) {
+    bytes = align256(bytes);
+    if (bytes > left) {
+        const size_t block = std::max(bytes, (size_t)64 << 20);
+        void* p = nullptr;
+        if (hipMalloc(&p, block) != hipSuccess) return nullptr;
+        blocks.push_back(p);
+        cur = static_cast<char*>(p);
+        left = block;
+    }
+    void* out = cur;
+    cur += bytes;
+    left -= bytes;
+    return out;
+}
+
+DevicePool::~DevicePool() {
+    for (void* p : blocks) (void)hipFree(p);
 }
 
 // Upload one chunk descriptor + its op batch to slot 0.
@@ -378,28 +393,54 @@ static int upload_single(Ctx* c, const ChunkDesc& cd_in, const std::vector<lvn_c
     return LVN_OK;
 }
 
+// Cached mode of a chunk descriptor: materials and edge list of the cached field.
+static void set_cached(ChunkDesc& cd, const FieldEntry* cached) {
+    if (!cached) return;
+    cd.mode = kModeCached;
+    cd.old_edges = cached->edges;
+    cd.old_ids = (uint64_t)(uintptr_t)cached->d_ids;
+    cd.old_data = (uint64_t)(uintptr_t)cached->d_data;
+    cd.old_mat = (uint64_t)(uintptr_t)cached->d_mat;
+}
+
+// Launch shapes of the one-chunk forms: split density columns 6 ways.
+static KArgs single_args(const Ctx* c) {
+    KArgs a = c->kargs();
+    a.ysplit = 6;
+    return a;
+}
+
+// The whole pipeline for slot 0 as one graph launch (captured once, again when the op
+// arrays move).
+static int run_pipeline_graph(Ctx* c) {
+    if (!c->exec || c->graph_ops != c->d_ops || c->graph_index != c->d_op_index ||
+        c->graph_gen != g_state.generation) {
+        if (c->exec) LVN_HIP(hipGraphExecDestroy(c->exec));
+        if (c->graph) LVN_HIP(hipGraphDestroy(c->graph));
+        c->exec = nullptr;
+        c->graph = nullptr;
+        LVN_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        launch_pipeline(single_args(c), 1, c->stream);
+        LVN_HIP(hipStreamEndCapture(c->stream, &c->graph));
+        LVN_HIP(hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
+        c->graph_ops = c->d_ops;
+        c->graph_index = c->d_op_index;
+        c->graph_gen = g_state.generation;
+    }
+    LVN_HIP(hipGraphLaunch(c->exec, c->stream));
+    return LVN_OK;
+}
+
 // Field stages for slot 0: density (fresh) or cached materials, CSG batch, edges, crossings.
 static int run_field(Ctx* c, const int32_t* min, int32_t size, const std::vector<lvn_csg_op>& ops,
                      const FieldEntry* cached) {
     ChunkDesc cd;
     fill_chunk_desc(cd, min, size, c->N);
-    if (cached) {
-        cd.mode = kModeCached;
-        cd.old_edges = cached->edges;
-        cd.old_ids = (uint64_t)(uintptr_t)cached->d_ids;
-        cd.old_data = (uint64_t)(uintptr_t)cached->d_data;
-    }
+    set_cached(cd, cached);
     int r = upload_single(c, cd, ops);
     if (r) return r;
-    KArgs a = c->kargs();
-    a.ysplit = 6;
-    if (cached) {
-        char* slot = c->d_arena;
-        LVN_HIP(hipMemcpyAsync(slot + c->L.mat, cached->d_mat, c->G.field_samples, hipMemcpyDeviceToDevice, c->stream));
-        LVN_HIP(hipMemcpyAsync(slot + c->L.orig, cached->d_mat, c->G.field_samples, hipMemcpyDeviceToDevice, c->stream));
-    } else {
-        launch_density(a, 1, c->stream);
-    }
+    const KArgs a = single_args(c);
+    launch_density(a, 1, c->stream);  // or the cached field's materials
     if (!ops.empty()) launch_csg(a, 1, c->stream);
     launch_edge_rows(a, 1, c->stream);
     launch_scan(a, 1, kScanEdges, c->stream);
@@ -432,14 +473,13 @@ static int read_counters(Ctx* c, Counters* out) {
 // src/compute_density_field.cpp:303-308).
 static int store_field(Ctx* c, const Key& key, int edges, int last_op) {
     FieldEntry& f = c->fields[key];
-    if (!f.d_mat) LVN_HIP(hipMalloc(&f.d_mat, c->G.field_samples));
-    if (edges > f.cap) {
-        if (f.d_ids) LVN_HIP(hipFree(f.d_ids));
-        if (f.d_data) LVN_HIP(hipFree(f.d_data));
-        f.cap = std::max(edges, 1);
-        LVN_HIP(hipMalloc(&f.d_ids, sizeof(int) * f.cap));
-        LVN_HIP(hipMalloc(&f.d_data, 16 * (size_t)f.cap));
+    if (!f.d_mat) f.d_mat = static_cast<uint8_t*>(c->field_pool.take(c->G.field_samples));
+    if (!f.d_ids || edges > f.cap) {  // a grown edge list moves to new pool space (the old stays with the pool)
+        f.cap = std::max(edges + edges / 4, 64);
+        f.d_ids = static_cast<int*>(c->field_pool.take(sizeof(int) * (size_t)f.cap));
+        f.d_data = static_cast<float4*>(c->field_pool.take(16 * (size_t)f.cap));
     }
+    if (!f.d_mat || !f.d_ids || !f.d_data) return LVN_ERR_NOMEM;
     char* slot = c->d_arena;
     LVN_HIP(hipMemcpyAsync(f.d_mat, slot + c->L.mat, c->G.field_samples, hipMemcpyDeviceToDevice, c->stream));
     if (edges) {
@@ -448,8 +488,7 @@ static int store_field(Ctx* c, const Key& key, int edges, int last_op) {
     }
     f.edges = edges;
     f.last_op = last_op;
-    LVN_HIP(hipStreamSynchronize(c->stream));
-    return LVN_OK;
+    return LVN_OK;  // the copies run on the context's stream before any later use of the slot
 }
 
 static std::vector<lvn_csg_op> stored_ops_for(const int32_t* min, int32_t size, int from) {
@@ -480,19 +519,23 @@ static int load_field(Ctx* c, const int32_t* min, int32_t size, Counters* cnt) {
     return LVN_OK;
 }
 
+// Read the mesh of slot 0 back through pinned staging (one copy per array, one sync).
 static int copy_out(Ctx* c, const Counters& k, MeshEntry* m) {
     const int V = k.voxels, Q = k.quads, S = k.seams;
-    m->verts.resize(Q > 0 ? (size_t)V * 12 : 0);
-    m->tris.resize((size_t)Q * 6);
-    m->seams.resize((size_t)S * 12);
+    if (!c->h_verts) {
+        LVN_HIP(hipHostMalloc(&c->h_verts, 48 * (size_t)c->caps.voxels, hipHostMallocDefault));
+        LVN_HIP(hipHostMalloc(&c->h_tris, 24 * (size_t)c->caps.quads, hipHostMallocDefault));
+        LVN_HIP(hipHostMalloc(&c->h_seams, 48 * (size_t)c->caps.seams, hipHostMallocDefault));
+    }
+    const size_t nv = Q > 0 ? (size_t)V * 12 : 0, nt = (size_t)Q * 6, ns = (size_t)S * 12;
     char* slot = c->d_arena;
-    if (!m->verts.empty())
-        LVN_HIP(hipMemcpyAsync(m->verts.data(), slot + c->L.verts, m->verts.size() * 4, hipMemcpyDeviceToHost, c->stream));
-    if (!m->tris.empty())
-        LVN_HIP(hipMemcpyAsync(m->tris.data(), slot + c->L.tris, m->tris.size() * 4, hipMemcpyDeviceToHost, c->stream));
-    if (!m->seams.empty())
-        LVN_HIP(hipMemcpyAsync(m->seams.data(), slot + c->L.seams, m->seams.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    if (nv) LVN_HIP(hipMemcpyAsync(c->h_verts, slot + c->L.verts, nv * 4, hipMemcpyDeviceToHost, c->stream));
+    if (nt) LVN_HIP(hipMemcpyAsync(c->h_tris, slot + c->L.tris, nt * 4, hipMemcpyDeviceToHost, c->stream));
+    if (ns) LVN_HIP(hipMemcpyAsync(c->h_seams, slot + c->L.seams, ns * 4, hipMemcpyDeviceToHost, c->stream));
     LVN_HIP(hipStreamSynchronize(c->stream));
+    m->verts.assign(c->h_verts, c->h_verts + nv);
+    m->tris.assign(c->h_tris, c->h_tris + nt);
+    m->seams.assign(c->h_seams, c->h_seams + ns);
     return LVN_OK;
 }
 
@@ -538,6 +581,7 @@ static int stage_batch(Batch* b, int n, const lvn_chunk_req* reqs) {
 static lvn_tuning effective_tuning(const Batch* b) {
     lvn_tuning t = b->tuning;
     if (t.density_ysplit == 0) t.density_ysplit = b->n >= 64 ? 1 : 6;
+    if (t.cross_lanes == 0) t.cross_lanes = default_cross_lanes(b->n);
     return t;
 }
 
@@ -547,6 +591,7 @@ static KArgs batch_args(const Batch* b, const lvn_tuning& t) {
     a.persist_per_cu = t.persistent_per_cu;
     a.row_blocks = t.row_blocks;
     a.voxel_blocks = t.voxel_blocks;
+    a.cross_lanes = t.cross_lanes;
     return a;
 }
 
@@ -573,7 +618,9 @@ static int time_stages(Batch* b, const KArgs& a, float* stage_ms) {
 static bool valid_tuning(const Batch* b, const lvn_tuning& t) {
     return t.density_ysplit >= 0 && t.density_ysplit <= b->ctx.G.F && t.persistent_per_cu >= 1 &&
            t.persistent_per_cu <= 2 && t.row_blocks >= 1 && t.row_blocks <= (1 << 20) && t.voxel_blocks >= 1 &&
-           t.voxel_blocks <= (1 << 20);
+           t.voxel_blocks <= (1 << 20) &&
+           (t.cross_lanes == 0 || t.cross_lanes == 1 || t.cross_lanes == 2 || t.cross_lanes == 4 ||
+            t.cross_lanes == 8);
 }
 
 static void drop_graph(Batch* b) { b->graph_n = -1; }
@@ -609,6 +656,7 @@ int lvn_init(const lvn_field_desc* field, int32_t noise_seed, uint32_t default_m
     LVN_HIP(hipMemcpy(g_state.d_grads, grads, sizeof(grads), hipMemcpyHostToDevice));
     g_state.seed = noise_seed;
     g_state.init = true;
+    ++g_state.generation;
     return LVN_OK;
 }
 
@@ -746,15 +794,28 @@ int lvn_generate_chunk_mesh(lvn_ctx* ctx, const int32_t min[3], int32_t size, lv
     if (it != c->meshes.end()) {
         m = &it->second;  // LoadOctree cache hit (src/compute_octree.cpp:157-163)
     } else {
+        // LoadDensityField + ConstructOctreeFromField + GenerateMeshFromOctree as one
+        // graph launch over slot 0, then the field is cached (when ops were applied)
+        // and the mesh read back.
+        auto fit = c->fields.find(key);
+        const FieldEntry* cached = fit == c->fields.end() ? nullptr : &fit->second;
+        const std::vector<lvn_csg_op> ops = stored_ops_for(min, size, cached ? cached->last_op : 0);
+        ChunkDesc cd;
+        fill_chunk_desc(cd, min, size, c->N);
+        set_cached(cd, cached);
+        int r = upload_single(c, cd, ops);
+        if (r) return r;
+        r = run_pipeline_graph(c);
+        if (r) return r;
         Counters k;
-        int r = load_field(c, min, size, &k);
-        if (r) return r;
-        if (k.edges == 0) return LVN_OK;  // no octree, nothing cached (src/compute_octree.cpp:167-171)
-        r = run_mesh(c);
-        if (r) return r;
         r = read_counters(c, &k);
         if (r) return r;
         if (k.status) return LVN_ERR_OVERFLOW;
+        if (!ops.empty()) {
+            r = store_field(c, key, k.edges, (int)g_state.ops.size());
+            if (r) return r;
+        }
+        if (k.edges == 0) return LVN_OK;  // no octree, nothing cached (src/compute_octree.cpp:167-171)
         r = copy_out(c, k, &fresh);
         if (r) return r;
         m = &(c->meshes[key] = std::move(fresh));
@@ -804,7 +865,8 @@ int lvn_batch_run(lvn_batch* bh) {
     if (!b) return LVN_ERR_INVALID;
     if (b->n == 0) return LVN_OK;
     Ctx* c = &b->ctx;
-    if (b->graph_n != b->n || b->graph_ops != c->d_ops || b->graph_index != c->d_op_index) {
+    if (b->graph_n != b->n || b->graph_ops != c->d_ops || b->graph_index != c->d_op_index ||
+        b->graph_gen != g_state.generation) {
         if (b->exec) LVN_HIP(hipGraphExecDestroy(b->exec));
         if (b->graph) LVN_HIP(hipGraphDestroy(b->graph));
         b->exec = nullptr;
@@ -817,6 +879,7 @@ int lvn_batch_run(lvn_batch* bh) {
         b->graph_n = b->n;
         b->graph_ops = c->d_ops;
         b->graph_index = c->d_op_index;
+        b->graph_gen = g_state.generation;
     }
     LVN_HIP(hipGraphLaunch(b->exec, c->stream));
     return LVN_OK;
@@ -867,6 +930,7 @@ int lvn_batch_autotune(lvn_batch* bh, int32_t reps, lvn_tuning* out) {
         const std::vector<Param> params = {
             {&lvn_tuning::density_ysplit, {1, 2, 3, 6}, {0}},
             {&lvn_tuning::persistent_per_cu, {1, 2}, {0, 5}},
+            {&lvn_tuning::cross_lanes, {1, 2, 4, 8}, {5}},
             {&lvn_tuning::row_blocks, {1024, 2048, 4096, 8192, 16384, 32768}, {2, 4, 6, 8}},
             {&lvn_tuning::voxel_blocks, {1024, 2048, 4096, 8192, 16384, 32768}, {9, 11}},
         };

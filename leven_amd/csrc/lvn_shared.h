@@ -78,10 +78,11 @@ struct ChunkDesc {
     float colour[4];
     uint64_t old_ids;   // cached mode: device pointer to the old sorted edge ids
     uint64_t old_data;  // cached mode: device pointer to the old edge data (float4)
+    uint64_t old_mat;   // cached mode: device pointer to the cached materials (F^3 bytes)
 };
 
-constexpr int kModeFresh = 0;   // untouched edges take density crossings
-constexpr int kModeCached = 1;  // untouched edges keep their data from the old list
+constexpr int kModeFresh = 0;   // density from the field; untouched edges take density crossings
+constexpr int kModeCached = 1;  // materials from old_mat; untouched edges keep their data from the old list
 
 // Per-chunk counters written by the kernels.
 struct Counters {

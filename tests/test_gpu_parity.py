@@ -255,7 +255,8 @@ def test_batched_matches_oracle_and_replays(oracle):
 
 def test_launch_shapes_do_not_change_results():
     """Every launch shape (column split, persistent workgroups per CU, workgroups over the
-    batch) and the autotuner's pick give bit-identical meshes."""
+    batch, lanes per edge in the crossing stage) and the autotuner's pick give
+    bit-identical meshes."""
     w = workloads.terrain_512()
     setup_field(w["desc"], w["seed"])
     for op, bb in zip(w["ops"], w["aabbs"]):
@@ -267,9 +268,10 @@ def test_launch_shapes_do_not_change_results():
     b.wait()
     ref = [b.get_mesh(i) for i in range(len(pick))]
     assert sum(len(m[1]) > 0 for m in ref) >= 5
-    shapes = [dict(density_ysplit=3, persistent_per_cu=1, row_blocks=1, voxel_blocks=7),
-              dict(density_ysplit=11, persistent_per_cu=2, row_blocks=37, voxel_blocks=1),
-              dict(density_ysplit=66, persistent_per_cu=1, row_blocks=100000, voxel_blocks=100000)]
+    shapes = [dict(density_ysplit=3, persistent_per_cu=1, row_blocks=1, voxel_blocks=7, cross_lanes=2),
+              dict(density_ysplit=11, persistent_per_cu=2, row_blocks=37, voxel_blocks=1, cross_lanes=4),
+              dict(density_ysplit=66, persistent_per_cu=1, row_blocks=100000, voxel_blocks=100000, cross_lanes=8),
+              dict(density_ysplit=2, persistent_per_cu=2, row_blocks=4096, voxel_blocks=4096, cross_lanes=1)]
     for shape in shapes:
         b.set_tuning(**shape)
         assert b.tuning() == shape
@@ -286,6 +288,8 @@ def test_launch_shapes_do_not_change_results():
         assert all((x.view(np.uint32) == y.view(np.uint32)).all() for x, y in zip(b.get_mesh(i), ref[i]))
     with pytest.raises(RuntimeError):
         b.set_tuning(persistent_per_cu=3)
+    with pytest.raises(RuntimeError):
+        b.set_tuning(cross_lanes=3)
     mesher.clear_csg_ops()
 
 
@@ -319,3 +323,24 @@ def test_batched_mixed_lods_match_oracle(oracle):
           f"empty {(counts[:, 2] == 0).sum()}")
     assert counts[:, 1].max() > 200000 and (counts[:, 2] == 0).any()
     mesher.clear_csg_ops()
+
+
+def test_context_follows_reinitialisation(oracle):
+    """A context (and its captured pipeline) used across lvn_init calls with different
+    fields and seeds meshes each chunk with the current field."""
+    ctx = mesher.Context(16)
+    b = mesher.Batch(16, 1)
+    cases = [(workloads.heightfield_desc(), 1, (0, -32, 0)), (workloads.terrain_desc(), 2, (64, -64, 0)),
+             (workloads.heightfield_desc(), 7, (128, -32, 64))]
+    for desc, seed, mn in cases:
+        setup_field(desc, seed)
+        v, t, s = ctx.generate_chunk_mesh(mn, 64)
+        c = oracle_chunk(oracle, desc, seed, 16, mn, 64)
+        assert len(t) > 0 and (t == c.triangles()).all() and (v.view(np.uint32) == c.vertices().view(np.uint32)).all()
+        b.set_chunks(np.array([mn]), np.array([64]))
+        b.run()
+        b.wait()
+        bv, bt, _ = b.get_mesh(0)
+        assert (bt == t).all() and (bv.view(np.uint32) == v.view(np.uint32)).all()
+    ctx.close()
+    b.close()
