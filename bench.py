@@ -92,6 +92,36 @@ def cpu_baseline(wl, threads):
                        f"p99 {np.percentile(secs, 99) * 1e3:.1f} ms per chunk")
 
 
+def precision_leg(args):
+    """The precision form on the 512-chunk field: FP16 density storage, compensated FP32
+    QEF with FP64 re-solves, 1024 Y-rotated boxes; same batched pipeline."""
+    wl = workloads.precision()
+    mesher.init(wl["desc"], wl["seed"], wl["default_material"])
+    mesher.clear_csg_ops()
+    for op, bb in zip(wl["ops"], wl["aabbs"]):
+        mesher.store_csg_op(op, bb[:3], bb[3:])
+    n = len(wl["sizes"])
+    b = mesher.Batch(64, n, edge_cap=262144, voxel_cap=131072, flags=wl["flags"])
+    b.set_chunks(wl["mins"], wl["sizes"])
+    tuning = b.autotune(reps=1)
+    for _ in range(max(1, args.warmup)):
+        b.run()
+    b.wait()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.run()
+    b.wait()
+    elapsed = time.perf_counter() - t0
+    counts = b.counts()
+    assert (counts[:, 4] == 0).all(), "a precision-form chunk overflowed"
+    stage_ms = {k: round(v, 4) for k, v in b.run_timed().items()}
+    b.close()
+    return {"chunks": n, "csg_ops": int(len(wl["ops"])), "chunks_per_s": round(n * args.steps / elapsed, 1),
+            "ms_per_step": round(elapsed / args.steps * 1e3, 3), "vertices": int(counts[:, 1].sum()),
+            "fp64_resolved": int(counts[:, 5].sum()), "stage_ms": stage_ms, "tuning": tuning,
+            "accuracy": "vertex error <= 1.9e-5 voxel vs FP64 (tests/test_gpu_precision.py)"}
+
+
 def clipmap_leg(args):
     """The 6-LOD clipmap (1,890 chunks at voxel sizes 1..32; coarse cave chunks have up to
     ~237k of their 262,144 voxels active) as one batch with full per-slot capacities,
@@ -237,6 +267,7 @@ def main():
     ap.add_argument("--plain-chunks", type=int, default=0, help="chunks for the plain form (0 = all)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="length of the served request stream (0 = skip)")
     ap.add_argument("--no-clipmap", action="store_true", help="skip the 6-LOD clipmap batch")
+    ap.add_argument("--no-precision", action="store_true", help="skip the precision-form batch")
     ap.add_argument("--no-autotune", action="store_true", help="time the default launch shapes")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (calibrates HBM byte counters) and exit")
@@ -339,6 +370,8 @@ def main():
     hbm = hbm_rates(stage_ms, wl["name"])
     if hbm:
         out["hbm_gbs"] = hbm
+    if not args.no_precision:
+        out["precision"] = precision_leg(args)
     if not args.no_clipmap:
         out["clipmap"] = clipmap_leg(args)
     if args.serve_seconds > 0:

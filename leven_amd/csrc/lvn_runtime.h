@@ -59,12 +59,22 @@ struct DevicePool {
     ~DevicePool();
 };
 
-// A finished chunk (the reference caches its octree; the mesh is a pure
-// function of it, so the mesh is what is kept).
+// Power-of-two size classes over a DevicePool with per-class free lists: cached meshes
+// come and go (freeChunkOctree), so their blocks are recycled.
+struct ClassPool {
+    DevicePool base;
+    std::map<size_t, std::vector<char*>> free_blocks;
+    char* take(size_t bytes, size_t* cap);
+    void give(char* p, size_t cap) { free_blocks[cap].push_back(p); }
+};
+
+// A finished chunk (the reference caches its octree on the device; the mesh is a
+// pure function of it, so the mesh is what is kept, on the device): vertices,
+// triangles and seam nodes back to back in one block of the context's mesh pool.
 struct MeshEntry {
-    std::vector<float> verts;   // 12 floats per vertex
-    std::vector<int> tris;      // 3 ints per triangle
-    std::vector<float> seams;   // 12 words per seam node
+    char* d = nullptr;
+    size_t cap = 0;             // block size class
+    int V = 0, T = 0, S = 0;    // exported vertices (0 without triangles), triangles, seam nodes
 };
 
 struct Ctx {
@@ -91,14 +101,13 @@ struct Ctx {
     std::map<Key, FieldEntry> fields;
     std::map<Key, MeshEntry> meshes;
     DevicePool field_pool;
+    ClassPool mesh_pool;
     hipGraphExec_t exec = nullptr;       // the one-chunk pipeline, captured on first use
     hipGraph_t graph = nullptr;
     const void* graph_ops = nullptr;     // op arrays the graph was captured with
     const void* graph_index = nullptr;
     uint64_t graph_gen = 0;              // global generation the graph was captured with
     float* h_verts = nullptr;   // pinned read-back staging for one chunk's mesh (caps sized)
-    int* h_tris = nullptr;
-    float* h_seams = nullptr;
 
     int allocate(int n, uint32_t flags, int nslots, int edge_cap, int voxel_cap);
     int ensure_ops(int nops, int nidx);

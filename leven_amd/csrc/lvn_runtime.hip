@@ -336,8 +336,6 @@ Ctx::~Ctx() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (h_verts) (void)hipHostFree(h_verts);
-    if (h_tris) (void)hipHostFree(h_tris);
-    if (h_seams) (void)hipHostFree(h_seams);
     if (d_arena) (void)hipFree(d_arena);
     if (d_chunks) (void)hipFree(d_chunks);
     if (d_counters) (void)hipFree(d_counters);
@@ -370,6 +368,19 @@ void* DevicePool::take(size_t bytes) {
 
 DevicePool::~DevicePool() {
     for (void* p : blocks) (void)hipFree(p);
+}
+
+char* ClassPool::take(size_t bytes, size_t* cap) {
+    size_t c = (size_t)64 << 10;
+    while (c < bytes) c <<= 1;
+    *cap = c;
+    auto& list = free_blocks[c];
+    if (!list.empty()) {
+        char* p = list.back();
+        list.pop_back();
+        return p;
+    }
+    return static_cast<char*>(base.take(c));
 }
 
 // Upload one chunk descriptor + its op batch to slot 0.
@@ -519,23 +530,36 @@ static int load_field(Ctx* c, const int32_t* min, int32_t size, Counters* cnt) {
     return LVN_OK;
 }
 
-// Read the mesh of slot 0 back through pinned staging (one copy per array, one sync).
-static int copy_out(Ctx* c, const Counters& k, MeshEntry* m) {
-    const int V = k.voxels, Q = k.quads, S = k.seams;
-    if (!c->h_verts) {
-        LVN_HIP(hipHostMalloc(&c->h_verts, 48 * (size_t)c->caps.voxels, hipHostMallocDefault));
-        LVN_HIP(hipHostMalloc(&c->h_tris, 24 * (size_t)c->caps.quads, hipHostMallocDefault));
-        LVN_HIP(hipHostMalloc(&c->h_seams, 48 * (size_t)c->caps.seams, hipHostMallocDefault));
+// Keep the mesh of slot 0 in the context's mesh pool (device to device, asynchronous).
+static int cache_mesh(Ctx* c, const Counters& k, MeshEntry* m) {
+    m->V = k.quads > 0 ? k.voxels : 0;  // the export rule, src/compute_octree.cpp:227-232
+    m->T = 2 * k.quads;
+    m->S = k.seams;
+    const size_t bv = 48 * (size_t)m->V, bt = 12 * (size_t)m->T, bs = 48 * (size_t)m->S;
+    m->d = c->mesh_pool.take(std::max<size_t>(bv + bt + bs, 1), &m->cap);
+    if (!m->d) return LVN_ERR_NOMEM;
+    const char* slot = c->d_arena;
+    if (bv) LVN_HIP(hipMemcpyAsync(m->d, slot + c->L.verts, bv, hipMemcpyDeviceToDevice, c->stream));
+    if (bt) LVN_HIP(hipMemcpyAsync(m->d + bv, slot + c->L.tris, bt, hipMemcpyDeviceToDevice, c->stream));
+    if (bs) LVN_HIP(hipMemcpyAsync(m->d + bv + bt, slot + c->L.seams, bs, hipMemcpyDeviceToDevice, c->stream));
+    return LVN_OK;
+}
+
+// ExportMeshBuffer (src/compute_octree.cpp:326-347): a cached mesh into the caller's
+// buffers through pinned staging (one read-back, one host copy per array).
+static int export_mesh(Ctx* c, const MeshEntry& m, lvn_vertex* v, int32_t* tri, lvn_seam_node* s) {
+    const size_t bv = 48 * (size_t)m.V, bt = 12 * (size_t)m.T, bs = 48 * (size_t)m.S;
+    if (bv + bt + bs == 0) return LVN_OK;
+    if (!c->h_verts) {  // staging sized for the largest mesh a slot can hold
+        LVN_HIP(hipHostMalloc(&c->h_verts, 48 * (size_t)c->caps.voxels + 24 * (size_t)c->caps.quads +
+                                               48 * (size_t)c->caps.seams, hipHostMallocDefault));
     }
-    const size_t nv = Q > 0 ? (size_t)V * 12 : 0, nt = (size_t)Q * 6, ns = (size_t)S * 12;
-    char* slot = c->d_arena;
-    if (nv) LVN_HIP(hipMemcpyAsync(c->h_verts, slot + c->L.verts, nv * 4, hipMemcpyDeviceToHost, c->stream));
-    if (nt) LVN_HIP(hipMemcpyAsync(c->h_tris, slot + c->L.tris, nt * 4, hipMemcpyDeviceToHost, c->stream));
-    if (ns) LVN_HIP(hipMemcpyAsync(c->h_seams, slot + c->L.seams, ns * 4, hipMemcpyDeviceToHost, c->stream));
+    char* h = reinterpret_cast<char*>(c->h_verts);
+    LVN_HIP(hipMemcpyAsync(h, m.d, bv + bt + bs, hipMemcpyDeviceToHost, c->stream));
     LVN_HIP(hipStreamSynchronize(c->stream));
-    m->verts.assign(c->h_verts, c->h_verts + nv);
-    m->tris.assign(c->h_tris, c->h_tris + nt);
-    m->seams.assign(c->h_seams, c->h_seams + ns);
+    if (bv && v) std::memcpy(v, h, bv);
+    if (bt && tri) std::memcpy(tri, h + bv, bt);
+    if (bs && s) std::memcpy(s, h + bv + bt, bs);
     return LVN_OK;
 }
 
@@ -727,7 +751,11 @@ int lvn_ctx_voxels_per_chunk(const lvn_ctx* ctx) {
 int lvn_free_chunk(lvn_ctx* ctx, const int32_t min[3], int32_t size) {
     if (!ctx || !min) return LVN_ERR_INVALID;
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
-    c->meshes.erase(make_key(min, size));  // Compute_FreeChunkOctree erases the octree only
+    auto it = c->meshes.find(make_key(min, size));  // Compute_FreeChunkOctree erases the octree only
+    if (it != c->meshes.end()) {
+        c->mesh_pool.give(it->second.d, it->second.cap);
+        c->meshes.erase(it);
+    }
     return LVN_OK;
 }
 
@@ -789,7 +817,6 @@ int lvn_generate_chunk_mesh(lvn_ctx* ctx, const int32_t min[3], int32_t size, lv
     *nv = *nt = *ns = 0;
     const Key key = make_key(min, size);
     auto it = c->meshes.find(key);
-    MeshEntry fresh;
     const MeshEntry* m = nullptr;
     if (it != c->meshes.end()) {
         m = &it->second;  // LoadOctree cache hit (src/compute_octree.cpp:157-163)
@@ -816,19 +843,16 @@ int lvn_generate_chunk_mesh(lvn_ctx* ctx, const int32_t min[3], int32_t size, lv
             if (r) return r;
         }
         if (k.edges == 0) return LVN_OK;  // no octree, nothing cached (src/compute_octree.cpp:167-171)
-        r = copy_out(c, k, &fresh);
+        MeshEntry fresh;
+        r = cache_mesh(c, k, &fresh);
         if (r) return r;
-        m = &(c->meshes[key] = std::move(fresh));
+        m = &(c->meshes[key] = fresh);
     }
-    const int V = (int)(m->verts.size() / 12), T = (int)(m->tris.size() / 3), S = (int)(m->seams.size() / 12);
-    *nv = V;
-    *nt = T;
-    *ns = S;
-    if (V > v_cap || T > tri_cap || S > s_cap) return LVN_ERR_CAPACITY;
-    if (V && v) std::memcpy(v, m->verts.data(), m->verts.size() * 4);
-    if (T && tri) std::memcpy(tri, m->tris.data(), m->tris.size() * 4);
-    if (S && s) std::memcpy(s, m->seams.data(), m->seams.size() * 4);
-    return LVN_OK;
+    *nv = m->V;
+    *nt = m->T;
+    *ns = m->S;
+    if (m->V > v_cap || m->T > tri_cap || m->S > s_cap) return LVN_ERR_CAPACITY;
+    return export_mesh(c, *m, v, tri, s);
 }
 
 int lvn_batch_create(int32_t voxels_per_chunk, int32_t max_chunks, int32_t edge_cap, int32_t voxel_cap,
