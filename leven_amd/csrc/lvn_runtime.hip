@@ -662,10 +662,10 @@ extern "C" {
 int lvn_init(const lvn_field_desc* field, int32_t noise_seed, uint32_t default_material) {
     if (default_material > 255) return LVN_ERR_INVALID;
     std::lock_guard<std::mutex> lock(g_state.mu);
-    g_state.desc = field ? *field : default_field();
-    if (g_state.desc.kind < 0 || g_state.desc.kind > 2 || g_state.desc.height_octaves < 0 ||
-        g_state.desc.cave_octaves < 0)
-        return LVN_ERR_INVALID;
+    const lvn_field_desc d = field ? *field : default_field();
+    if (d.kind < 0 || d.kind > 2 || d.height_octaves < 0 || d.cave_octaves < 0)
+        return LVN_ERR_INVALID;  // nothing changed
+    g_state.desc = d;
     g_state.fp = make_field_params(g_state.desc);
     g_state.defmat = default_material;
     if (!g_state.d_perm) {

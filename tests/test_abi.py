@@ -67,3 +67,16 @@ def test_stage_names_cover_every_stage():
     names = [mesher.lib().lvn_batch_stage_name(i) for i in range(mesher.NUM_STAGES + 1)]
     assert all(n for n in names[:-1]) and names[-1] is None
     assert len(set(names[:-1])) == mesher.NUM_STAGES
+
+
+def test_init_rejects_unsupported_fields():
+    """Invalid field descriptors fail before any device work (no GPU needed)."""
+    if not os.path.exists(mesher.LIB_PATH):
+        pytest.skip("library not built (python -m leven_amd.build)")
+    L = mesher.lib()
+    for kw in [dict(height_octaves=-1), dict(cave_octaves=-1), dict(kind=3)]:
+        d = abi.field_desc()
+        for k, v in kw.items():
+            setattr(d, k, v)
+        assert L.lvn_init(C.byref(d), 1, 0) == abi.LVN_ERR_INVALID, kw
+    assert L.lvn_init(C.byref(abi.field_desc()), 1, 256) == abi.LVN_ERR_INVALID  # material > 255
