@@ -274,6 +274,7 @@ __global__ __launch_bounds__(256) void k_edge_rows(KArgs a) {
     const uint64_t* S = slot_ptr<const uint64_t>(a, chunk, a.L.solid);
     uint64_t* E = slot_ptr<uint64_t>(a, chunk, a.L.e_rows);
     int* count = slot_ptr<int>(a, chunk, a.L.e_count);
+    int* count_y = slot_ptr<int>(a, chunk, a.L.e_count_y);
     const uint64_t mlo = H >= 64 ? ~0ull : ((1ull << H) - 1ull);  // x in [0, H)
     const uint64_t mhi = H > 64 ? 1ull : 0ull;                     // x = 64
     for (int r = thread_in_chunk(); r < G.edge_rows; r += threads_per_chunk()) {
@@ -292,6 +293,7 @@ __global__ __launch_bounds__(256) void k_edge_rows(KArgs a) {
         E[4 * r + 2] = zlo;
         E[4 * r + 3] = hi;
         count[r] = __popcll(xlo) + __popcll(ylo) + __popcll(zlo) + __popcll(hi);
+        count_y[r] = __popcll(ylo) + (int)((hi >> 1) & 1ull);
     }
 }
 
@@ -335,8 +337,11 @@ __global__ __launch_bounds__(1024) void k_scan(KArgs a) {
     if constexpr (KIND == kScanEdges) {
         const int t = block_scan(slot_ptr<const int>(a, chunk, a.L.e_count), slot_ptr<int>(a, chunk, a.L.e_base),
                                  a.G.edge_rows, buf);
+        const int ty = block_scan(slot_ptr<const int>(a, chunk, a.L.e_count_y), slot_ptr<int>(a, chunk, a.L.e_base_y),
+                                  a.G.edge_rows, buf);
         if (threadIdx.x == 0) {
             cnt->edges = min(t, a.caps.edges);
+            cnt->edges_y = min(ty, a.caps.edges);
             if (t > a.caps.edges) atomicOr(&cnt->status, kStatusEdgeOverflow);
         }
     } else {
@@ -362,30 +367,39 @@ __global__ __launch_bounds__(256) void k_edge_compact(KArgs a) {
     const uint64_t* E = slot_ptr<const uint64_t>(a, chunk, a.L.e_rows);
     const int* count = slot_ptr<const int>(a, chunk, a.L.e_count);
     const int* base = slot_ptr<const int>(a, chunk, a.L.e_base);
+    const int* base_y = slot_ptr<const int>(a, chunk, a.L.e_base_y);
     int* ids = slot_ptr<int>(a, chunk, a.L.e_ids);
-    const int S = G.shift, cap = a.caps.edges;
+    int* order = slot_ptr<int>(a, chunk, a.L.e_order);
+    const int S = G.shift, cap = a.caps.edges, ey_total = a.counters[chunk].edges_y;
     for (int t = thread_in_chunk(); t < G.edge_rows; t += threads_per_chunk()) {
         const int y = div_magic(t, G.magic_h), z = t - y * G.H;
         const int r = y + G.H * z;
         if (count[r] == 0) continue;
         const uint64_t ex = E[4 * r], ey = E[4 * r + 1], ez = E[4 * r + 2], hi = E[4 * r + 3];
         const int row_id = (y << S) | (z << (2 * S));
-        int k = base[r];
+        int k = base[r], ky = base_y[r];  // edges and y-edges before this one
+        // rank k in id order; crossing position: y-edges first, then x and z edges
+        auto put = [&](int id, bool is_y) {
+            if (k < cap) {
+                ids[k] = id;
+                const int pos = is_y ? ky : ey_total + (k - ky);
+                if (pos < cap) order[pos] = k;
+            }
+            ++k;
+            ky += is_y ? 1 : 0;
+        };
         uint64_t any = ex | ey | ez;
         while (any) {
             const int x = (int)__builtin_ctzll(any);
             any &= any - 1ull;
             const int id = (x | row_id) << 2;
-            if ((ex >> x) & 1ull) { if (k < cap) ids[k] = id; ++k; }
-            if ((ey >> x) & 1ull) { if (k < cap) ids[k] = id | 1; ++k; }
-            if ((ez >> x) & 1ull) { if (k < cap) ids[k] = id | 2; ++k; }
+            if ((ex >> x) & 1ull) put(id, false);
+            if ((ey >> x) & 1ull) put(id | 1, true);
+            if ((ez >> x) & 1ull) put(id | 2, false);
         }
         const int id64 = (64 | row_id) << 2;  // x = 64 (N = 64 only)
         for (int axis = 0; axis < 3; ++axis)
-            if ((hi >> axis) & 1ull) {
-                if (k < cap) ids[k] = id64 | axis;
-                ++k;
-            }
+            if ((hi >> axis) & 1ull) put(id64 | axis, axis == 1);
     }
 }
 
@@ -551,8 +565,27 @@ __device__ __forceinline__ void cross_edge(const KArgs& a, const NoiseLds& T, in
     slot_ptr<float4>(a, chunk, a.L.e_data)[i] = out;
 }
 
+// A chunk's edges are crossed in two runs, its y-edges (one height evaluation per
+// edge) and its x and z edges (one per sample), so that a wave holds one kind and
+// skips the per-sample height work when all its edges are y-edges.
+__device__ __forceinline__ int edge_items_y(const Counters& c, int per_item) {
+    return (min(c.edges_y, c.edges) + per_item - 1) / per_item;
+}
+__device__ __forceinline__ int edge_items_xz(const Counters& c, int per_item) {
+    return (c.edges - min(c.edges_y, c.edges) + per_item - 1) / per_item;
+}
+// Edge rank (index into e_ids / e_data) of item j's slot `e`, or -1 past the end.
+__device__ __forceinline__ int edge_of_item(const KArgs& a, int chunk, int j, int e, int per_item) {
+    const Counters& c = a.counters[chunk];
+    const int ey = min(c.edges_y, c.edges), ny = (ey + per_item - 1) / per_item;
+    const int pos = j < ny ? j * per_item + e : ey + (j - ny) * per_item + e;
+    if (j < ny ? pos >= ey : pos >= c.edges) return -1;
+    const int i = slot_ptr<const int>(a, chunk, a.L.e_order)[pos];
+    return (unsigned)i < (unsigned)c.edges ? i : -1;  // entries past a capped list are not written
+}
+
 // Items of `per_item` edges (64 / lanes per edge) over all chunks: item_prefix[c] =
-// sum over earlier chunks of ceil(E / per_item), item_prefix[chunks] = total.
+// sum over earlier chunks of the chunk's y-edge and x/z-edge items, item_prefix[chunks] = total.
 __global__ __launch_bounds__(1024) void k_edge_items(KArgs a, int chunks, int per_item) {
     __shared__ int buf[1024];
     __shared__ int carry;
@@ -560,7 +593,7 @@ __global__ __launch_bounds__(1024) void k_edge_items(KArgs a, int chunks, int pe
     __syncthreads();
     for (int base = 0; base < chunks; base += 1024) {
         const int c = base + threadIdx.x;
-        const int v = c < chunks ? (a.counters[c].edges + per_item - 1) / per_item : 0;
+        const int v = c < chunks ? edge_items_y(a.counters[c], per_item) + edge_items_xz(a.counters[c], per_item) : 0;
         buf[threadIdx.x] = v;
         __syncthreads();
         for (int off = 1; off < 1024; off <<= 1) {
@@ -598,8 +631,8 @@ __global__ __attribute__((amdgpu_waves_per_eu(6, 8))) __launch_bounds__(kCrossTh
             const int mid = (lo + hi) >> 1;
             if (a.item_prefix[mid] <= item) lo = mid; else hi = mid;
         }
-        const int i = (item - a.item_prefix[lo]) * 64 + lane;
-        if (i < a.counters[lo].edges) cross_edge<kCount>(a, T, lo, i);
+        const int i = edge_of_item(a, lo, item - a.item_prefix[lo], lane, 64);
+        if (i >= 0) cross_edge<kCount>(a, T, lo, i);
     }
 }
 
@@ -738,8 +771,8 @@ __global__ __attribute__((amdgpu_waves_per_eu(6, 8))) __launch_bounds__(kCrossTh
             const int mid = (lo + hi) >> 1;
             if (a.item_prefix[mid] <= item) lo = mid; else hi = mid;
         }
-        const int i = (item - a.item_prefix[lo]) * kPerItem + lane / kG;
-        if (i < a.counters[lo].edges) cross_edge_group<kCount, kG>(a, T, lo, i, lane & (kG - 1));
+        const int i = edge_of_item(a, lo, item - a.item_prefix[lo], lane / kG, kPerItem);
+        if (i >= 0) cross_edge_group<kCount, kG>(a, T, lo, i, lane & (kG - 1));
     }
 }
 
@@ -1366,7 +1399,7 @@ static inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) 
 // captured graph clears them correctly on every replay).
 __global__ __launch_bounds__(256) void k_clear_counters(Counters* c, int n, int* queue) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) c[i] = Counters{0, 0, 0, 0, 0, 0, {0, 0}};
+    if (i < n) c[i] = Counters{0, 0, 0, 0, 0, 0, 0, 0};
     if (i < kNumQueues) queue[i] = 0;
 }
 

@@ -223,8 +223,11 @@ SlotLayout make_layout(const Geometry& G, const Caps& c) {
     L.e_rows = take((uint64_t)G.edge_rows * 32);
     L.e_count = take((uint64_t)G.edge_rows * 4);
     L.e_base = take((uint64_t)G.edge_rows * 4);
+    L.e_count_y = take((uint64_t)G.edge_rows * 4);
+    L.e_base_y = take((uint64_t)G.edge_rows * 4);
     L.e_ids = take((uint64_t)c.edges * 4);
     L.e_data = take((uint64_t)c.edges * 16);
+    L.e_order = take((uint64_t)c.edges * 4);
     L.v_mask = take((uint64_t)G.voxel_segs * 8);
     L.v_count = take((uint64_t)G.voxel_segs * 4);
     L.v_base = take((uint64_t)G.voxel_segs * 4);

@@ -60,6 +60,7 @@ struct SlotLayout {
     uint64_t density, mat, orig;
     uint64_t solid;                       // field_rows x 2 words: bit x = sample (x, y, z) is not air
     uint64_t e_rows, e_count, e_base, e_ids, e_data;  // e_rows: edge_rows x {x, y, z, x = 64} words
+    uint64_t e_count_y, e_base_y, e_order;  // y-edges per row and their scan; crossing order (y-edges first)
     uint64_t v_mask, v_count, v_base, v_code, v_emask, v_mat;
     uint64_t q_count, q_base, tris;
     uint64_t s_mask, s_count, s_base, seams, verts;
@@ -89,7 +90,8 @@ struct Counters {
     int edges, voxels, quads, seams;
     int status;      // 0 ok, else LVN_ERR_OVERFLOW bits
     int resolved;    // precise QEF: vertices re-solved in FP64
-    int pad[2];
+    int edges_y;     // y-edges among `edges` (capped like edges)
+    int pad;
 };
 
 constexpr int kStatusEdgeOverflow = 1;
