@@ -138,6 +138,64 @@ int lvn_generate_chunk_mesh(lvn_ctx* ctx, const int32_t min[3], int32_t size,
                             int32_t* tri, int32_t tri_cap, int32_t* nt,
                             lvn_seam_node* s, int32_t s_cap, int32_t* ns);
 
+/* ---- seam stitching (GenerateClipmapSeamMesh, src/clipmap.cpp:573-611) ---------- */
+
+/* One clipmap node's seam nodes offered to a host node's seam mesh: the node's min
+ * and size in world units, its list as lvn_generate_chunk_mesh returned it, and the
+ * candidate neighbour it was found under: neighbour i of the host lies at
+ * host_min + host_size * CHILD_MIN_OFFSETS[i] (src/volume_constants.h:24-35), i.e.
+ * x = bit 2, y = bit 1, z = bit 0 of i; 0 is the host node itself. */
+typedef struct lvn_seam_source {
+    int32_t min[3];
+    int32_t size;
+    int32_t neighbour;           /* 0..7 */
+    int32_t count;               /* -1 with nodes == NULL: the seam nodes of the context's cached
+                                    chunk at (min, size), read where they lie in device memory */
+    const lvn_seam_node* nodes;  /* host memory */
+} lvn_seam_source;
+
+/* Replaces the body of GenerateClipmapSeamMesh (src/clipmap.cpp:573-611): SelectSeamNodes
+ * (:546-569, FilterSeamNode :508-542), Octree_ConstructUpwards (src/octree.cpp:90-148)
+ * and Octree_GenerateMesh (src/octree.cpp:520-550: GenerateVertexIndices :196-231,
+ * ContourCellProc / FaceProc / EdgeProc / ProcessEdge :235-474) over the selected nodes,
+ * in a root of size 2 * host_size at host_min.  Vertices: one per selected seam node in
+ * the octree's depth-first order, pos.w = 0, normal.w = 0, colour = (colour.rgb,
+ * material >> 8).  Triangles: 3 indices each, in the reference's contouring order.
+ * The reference discards a seam mesh without triangles; here *nv still counts its
+ * vertices.  LVN_ERR_INVALID when two selected nodes overlap or a node is not aligned
+ * to its size in the root (the reference asserts); LVN_ERR_CAPACITY with the needed
+ * counts when an output does not fit. */
+int lvn_generate_seam_mesh(lvn_ctx* ctx, const int32_t host_min[3], int32_t host_size,
+                           const lvn_seam_source* sources, int32_t n_sources, const float colour[3],
+                           lvn_vertex* v, int32_t v_cap, int32_t* nv,
+                           int32_t* tri, int32_t tri_cap, int32_t* nt);
+
+/* Many seams in one launch sequence, one workgroup per seam.  reqs[i] takes sources
+ * [first_source, first_source + n_sources) of `sources`; a source list shared by
+ * several seams (the same nodes pointer) is uploaded once.  Per-seam results go to
+ * outs[i] (status LVN_OK, LVN_ERR_INVALID or LVN_ERR_CAPACITY as above); the call
+ * returns the first failing status.  No reference counterpart: the reference stitches
+ * one seam per call. */
+typedef struct lvn_seam_req {
+    int32_t host_min[3];
+    int32_t host_size;
+    float   colour[4];           /* rgb used */
+    int32_t first_source;
+    int32_t n_sources;
+} lvn_seam_req;
+
+typedef struct lvn_seam_out {
+    lvn_vertex* v;
+    int32_t*    tri;
+    int32_t     v_cap, tri_cap;  /* tri_cap in triangles */
+    int32_t     nv, nt;
+    int32_t     status;
+    int32_t     reserved;
+} lvn_seam_out;
+
+int lvn_generate_seam_meshes(lvn_ctx* ctx, int32_t n, const lvn_seam_req* reqs,
+                             const lvn_seam_source* sources, lvn_seam_out* outs);
+
 /* ---- batched form: many chunks per launch, device-resident outputs ---------- */
 
 /* One chunk request: min in world units, size = 256 * 2^LOD (src/clipmap.cpp:378). */

@@ -141,8 +141,60 @@ class Tuning(C.Structure):
     ]
 
 
+class SeamSource(C.Structure):
+    _fields_ = [
+        ("min", C.c_int32 * 3),
+        ("size", C.c_int32),
+        ("neighbour", C.c_int32),
+        ("count", C.c_int32),
+        ("nodes", C.c_void_p),
+    ]
+
+
+class SeamReq(C.Structure):
+    _fields_ = [
+        ("host_min", C.c_int32 * 3),
+        ("host_size", C.c_int32),
+        ("colour", C.c_float * 4),
+        ("first_source", C.c_int32),
+        ("n_sources", C.c_int32),
+    ]
+
+
+class SeamOut(C.Structure):
+    _fields_ = [
+        ("v", C.c_void_p),
+        ("tri", C.c_void_p),
+        ("v_cap", C.c_int32),
+        ("tri_cap", C.c_int32),
+        ("nv", C.c_int32),
+        ("nt", C.c_int32),
+        ("status", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
 assert C.sizeof(Request) == 96
 assert C.sizeof(Result) == 48
+assert C.sizeof(SeamSource) == 32 and C.sizeof(SeamReq) == 40 and C.sizeof(SeamOut) == 40
+
+
+def seam_sources(sources):
+    """ctypes array of SeamSource from dicts {min, size, neighbour, nodes (SEAM_DTYPE array or None)}.
+    The caller keeps the node arrays alive while the array is in use."""
+    arr = (SeamSource * max(1, len(sources)))()
+    for i, s in enumerate(sources):
+        nodes = s["nodes"]
+        arr[i].min[:] = [int(v) for v in s["min"]]
+        arr[i].size = int(s["size"])
+        arr[i].neighbour = int(s["neighbour"])
+        if nodes is None:  # the context's cached chunk supplies them
+            arr[i].count, arr[i].nodes = -1, None
+            continue
+        assert nodes.dtype == SEAM_DTYPE and nodes.flags["C_CONTIGUOUS"]
+        arr[i].count = len(nodes)
+        arr[i].nodes = nodes.ctypes.data if len(nodes) else None
+    return arr
 
 RESULT_DTYPE = np.dtype([("ticket", "<i8"), ("kind", "<i4"), ("status", "<i4"), ("nv", "<i4"), ("nt", "<i4"),
                          ("ns", "<i4"), ("cache_hit", "<i4"), ("submit_s", "<f8"), ("complete_s", "<f8")])

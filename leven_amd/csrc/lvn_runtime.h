@@ -13,6 +13,10 @@
 
 namespace lvn {
 
+// Seam-stitching buffers of a context (lvn_seams.hip).
+struct SeamState;
+void destroy_seam_state(SeamState* s);
+
 // Global state set by lvn_init / lvn_set_noise_seed / lvn_store_csg_op
 // (the reference keeps the same as process globals: src/compute.cpp:168-191,
 // src/compute_density_field.cpp:23-24).
@@ -108,6 +112,7 @@ struct Ctx {
     const void* graph_index = nullptr;
     uint64_t graph_gen = 0;              // global generation the graph was captured with
     float* h_verts = nullptr;   // pinned read-back staging for one chunk's mesh (caps sized)
+    SeamState* seams = nullptr; // seam stitching buffers, made on first use
 
     int allocate(int n, uint32_t flags, int nslots, int edge_cap, int voxel_cap);
     int ensure_ops(int nops, int nidx);

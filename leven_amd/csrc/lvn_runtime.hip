@@ -339,6 +339,7 @@ Ctx::~Ctx() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (h_verts) (void)hipHostFree(h_verts);
+    if (seams) destroy_seam_state(seams);
     if (d_arena) (void)hipFree(d_arena);
     if (d_chunks) (void)hipFree(d_chunks);
     if (d_counters) (void)hipFree(d_counters);

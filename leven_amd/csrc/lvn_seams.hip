@@ -1,0 +1,969 @@
+// lvn_seams.hip — seam stitching on the GPU.
+//
+// Replaces leven's CPU seam path for a clipmap node (GenerateClipmapSeamMesh,
+// src/clipmap.cpp:573-611): SelectSeamNodes / FilterSeamNode (:508-569), the
+// bottom-up seam octree (Octree_ConstructUpwards, src/octree.cpp:22-148) and its
+// mesh (Octree_GenerateMesh, src/octree.cpp:520-550: GenerateVertexIndices
+// :196-231, ContourCellProc / FaceProc / EdgeProc / ProcessEdge :235-474).
+//
+// One workgroup stitches one seam; a launch covers every seam of a request.
+// The reference builds the octree with a hash map per level and contours it by
+// recursion; here each phase is data-parallel over the workgroup:
+//   1. select: every offered seam node is filtered and its leaf key
+//      (level << 48 | Morton code in the root) inserted into an open-addressing
+//      table in HBM (a repeated node keeps the last offer, as the reference's
+//      child-slot assignment does);
+//   2. ancestors: each leaf inserts its chain of parents, stopping at the first
+//      one already present, so every internal node is created exactly once;
+//   3. links, subtree leaf counts (bottom-up atomics) and depth-first leaf ranks
+//      (top-down prefix over the eight children, level by level): the rank is the
+//      vertex index GenerateVertexIndices assigns;
+//   4. contouring as a breadth-first expansion of the recursion's calls: each
+//      round replaces every pending cell / face / edge call by the calls it makes,
+//      in call order, at positions given by a workgroup prefix sum; a finished
+//      quad is carried forward in place.  The list therefore stays in depth-first
+//      call order and its final content is the reference's triangle order.
+// Everything is integer bookkeeping; vertex data is copied from the seam nodes.
+
+#include "lvn_runtime.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#define LVN_HIP(call)                                                                                 \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess) {                                                                       \
+            fprintf(stderr, "lvn: HIP error %s at %s:%d (%s)\n", hipGetErrorString(e_), __FILE__,     \
+                    __LINE__, #call);                                                                 \
+            return LVN_ERR_DEVICE;                                                                    \
+        }                                                                                             \
+    } while (0)
+
+namespace lvn {
+
+// ---------------------------------------------------------------------------
+// Device records
+// ---------------------------------------------------------------------------
+
+struct SeamSrcDev {  // one offered seam node list
+    const lvn_seam_node* nodes;
+    int count;
+    int neighbour;
+    int size;
+    int min[3];
+};
+
+struct SeamJobDev {  // one seam
+    int host_min[3];
+    int host_size;
+    float colour[4];
+    int src_begin, src_count;
+    int vpc;
+    int leaf_cap, node_cap, list_cap, hash_cap;
+    int pad;
+    uint64_t scratch;  // byte offset of the seam's scratch
+    uint64_t v_off;    // vertex offset in the unpacked output (leaf_cap vertices)
+    uint64_t t_off;    // triangle offset in the unpacked output (2 * list_cap triangles)
+};
+
+struct SeamResDev {
+    int nv, nt, status, rounds;
+};
+
+enum : int { kNodeInternal = 1, kNodeLeaf = 2 };
+enum : int { kCallCell = 0, kCallFace = 1, kCallEdge = 2, kCallQuad = 3 };
+
+struct SNode {
+    int min[3];
+    int size;
+    int child[8];
+    int parent, level, mat, count;
+    int offset, type, pad0, pad1;
+    uint64_t code;
+    uint64_t payload;  // leaves: (source + 1) << 32 | node index
+};
+
+// A pending call of the recursion: node indices and meta = kind | dir << 2 | flip << 4.
+struct Call {
+    int n[4];
+    int meta;
+};
+
+constexpr int kSeamThreads = 256;
+constexpr uint64_t kEmptyKey = ~0ull;
+constexpr int kMaxRounds = 64;
+constexpr int kChunkMask = ~(256 - 1);  // ChunkMinForPosition, src/volume.cpp:30-34 (CLIPMAP_LEAF_SIZE)
+
+// Dual contouring tables (src/contour_constants.h); child i of a node sits at
+// (bit 2, bit 1, bit 0) of i (CHILD_MIN_OFFSETS, src/volume_constants.h:24-35).
+__constant__ int cEdgeCorners[12][2] = {{0, 4}, {1, 5}, {2, 6}, {3, 7}, {0, 2}, {1, 3},
+                                        {4, 6}, {5, 7}, {0, 1}, {2, 3}, {4, 5}, {6, 7}};
+__constant__ int cCellFaces[12][3] = {{0, 4, 0}, {1, 5, 0}, {2, 6, 0}, {3, 7, 0}, {0, 2, 1}, {4, 6, 1},
+                                      {1, 3, 1}, {5, 7, 1}, {0, 1, 2}, {2, 3, 2}, {4, 5, 2}, {6, 7, 2}};
+__constant__ int cCellEdges[6][5] = {{0, 1, 2, 3, 0}, {4, 5, 6, 7, 0}, {0, 4, 1, 5, 1},
+                                     {2, 6, 3, 7, 1}, {0, 2, 4, 6, 2}, {1, 3, 5, 7, 2}};
+__constant__ int cFaceFaces[3][4][3] = {{{4, 0, 0}, {5, 1, 0}, {6, 2, 0}, {7, 3, 0}},
+                                        {{2, 0, 1}, {6, 4, 1}, {3, 1, 1}, {7, 5, 1}},
+                                        {{1, 0, 2}, {3, 2, 2}, {5, 4, 2}, {7, 6, 2}}};
+__constant__ int cFaceEdges[3][4][6] = {
+    {{1, 4, 0, 5, 1, 1}, {1, 6, 2, 7, 3, 1}, {0, 4, 6, 0, 2, 2}, {0, 5, 7, 1, 3, 2}},
+    {{0, 2, 3, 0, 1, 0}, {0, 6, 7, 4, 5, 0}, {1, 2, 0, 6, 4, 2}, {1, 3, 1, 7, 5, 2}},
+    {{1, 1, 0, 3, 2, 0}, {1, 5, 4, 7, 6, 0}, {0, 1, 5, 0, 4, 1}, {0, 3, 7, 2, 6, 1}}};
+__constant__ int cEdgeEdges[3][2][5] = {{{3, 2, 1, 0, 0}, {7, 6, 5, 4, 0}},
+                                        {{5, 1, 4, 0, 1}, {7, 3, 6, 2, 1}},
+                                        {{6, 4, 2, 0, 2}, {7, 5, 3, 1, 2}}};
+__constant__ int cProcessEdge[3][4] = {{3, 2, 1, 0}, {7, 5, 6, 4}, {11, 10, 9, 8}};
+__constant__ int cFaceEdgeOrder[2][4] = {{0, 0, 1, 1}, {0, 1, 0, 1}};
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+
+// Values that other waves changed with atomics (which act in L2) are read past the
+// CU's L1 with an agent-scope atomic load.
+template <class T>
+__device__ inline T load_l2(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline uint64_t morton(int x, int y, int z, int levels) {
+    uint64_t c = 0;
+    for (int k = 0; k < levels; ++k)
+        c |= (uint64_t)(((x >> k) & 1) << 2 | ((y >> k) & 1) << 1 | ((z >> k) & 1)) << (3 * k);
+    return c;
+}
+__device__ inline void unmorton(uint64_t c, int levels, int* xyz) {
+    xyz[0] = xyz[1] = xyz[2] = 0;
+    for (int k = 0; k < levels; ++k) {
+        const int b = (int)((c >> (3 * k)) & 7);
+        xyz[0] |= ((b >> 2) & 1) << k;
+        xyz[1] |= ((b >> 1) & 1) << k;
+        xyz[2] |= (b & 1) << k;
+    }
+}
+__device__ inline int hash_home(uint64_t key, int mask) {
+    return (int)((key * 0x9E3779B97F4A7C15ull) >> 40) & mask;
+}
+// Insert `key`; returns its slot (-1 when the table is full), *fresh = inserted now.
+__device__ inline int hash_insert(uint64_t* keys, int mask, uint64_t key, bool* fresh) {
+    int s = hash_home(key, mask);
+    for (int probe = 0; probe <= mask; ++probe) {
+        const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(keys + s), kEmptyKey, key);
+        if (prev == kEmptyKey || prev == key) {
+            *fresh = prev == kEmptyKey;
+            return s;
+        }
+        s = (s + 1) & mask;
+    }
+    return -1;
+}
+__device__ inline int hash_find(const uint64_t* keys, int mask, uint64_t key) {
+    int s = hash_home(key, mask);
+    for (int probe = 0; probe <= mask; ++probe) {
+        const uint64_t k = load_l2(keys + s);
+        if (k == key) return s;
+        if (k == kEmptyKey) return -1;
+        s = (s + 1) & mask;
+    }
+    return -1;
+}
+__device__ inline bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+
+// FilterSeamNode, src/clipmap.cpp:508-542.
+__device__ inline bool filter_seam_node(int neighbour, const int* b, const int* mn, const int* mx) {
+    switch (neighbour) {
+    case 0: return mx[0] == b[0] || mx[1] == b[1] || mx[2] == b[2];
+    case 1: return mn[2] == b[2];
+    case 2: return mn[1] == b[1];
+    case 3: return mn[1] == b[1] || mn[2] == b[2];
+    case 4: return mn[0] == b[0];
+    case 5: return mn[0] == b[0] || mn[2] == b[2];
+    case 6: return mn[0] == b[0] || mn[1] == b[1];
+    case 7: return mn[0] == b[0] && mn[1] == b[1] && mn[2] == b[2];
+    }
+    return false;
+}
+
+__device__ inline bool same_chunk(const SNode& a, const SNode& b) {
+    return (a.min[0] & kChunkMask) == (b.min[0] & kChunkMask) && (a.min[1] & kChunkMask) == (b.min[1] & kChunkMask) &&
+           (a.min[2] & kChunkMask) == (b.min[2] & kChunkMask);
+}
+
+__device__ inline void set_status(int* s, int code) { atomicCAS(s, 0, code); }
+
+// Exclusive workgroup scan of one value per thread; *total = sum.
+__device__ inline int block_exclusive(int v, int* buf, int* total) {
+    const int t = threadIdx.x;
+    buf[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kSeamThreads; off <<= 1) {
+        const int u = t >= off ? buf[t - off] : 0;
+        __syncthreads();
+        buf[t] += u;
+        __syncthreads();
+    }
+    const int incl = buf[t];
+    *total = buf[kSeamThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// The calls one pending call makes, in the reference's call order.  kWrite = false
+// only counts them.  Calls that would return at once are not made: a face with a
+// missing node, both nodes in one chunk or no branch; an edge with a missing node or
+// all four nodes in one chunk.  An edge of four leaves is resolved on the spot
+// (ContourProcessEdge) into a quad or nothing.
+struct Expander {
+    const SNode* N;
+    Call* dst;
+    int at;
+    int k;
+    int more;
+    bool write;
+
+    __device__ void push(int a, int b, int c, int d, int meta) {
+        if (write) {
+            dst[at + k] = Call{{a, b, c, d}, meta};
+            if ((meta & 3) != kCallQuad) more = 1;
+        }
+        ++k;
+    }
+    __device__ bool internal(int i) const { return N[i].type == kNodeInternal; }
+
+    __device__ void face(int a, int b, int dir) {
+        if (a < 0 || b < 0) return;
+        if (same_chunk(N[a], N[b])) return;
+        if (!internal(a) && !internal(b)) return;
+        push(a, b, -1, -1, kCallFace | dir << 2);
+    }
+    __device__ void edge(const int* nd, int dir) {
+        for (int i = 0; i < 4; ++i)
+            if (nd[i] < 0) return;
+        bool spans = false, any = false;
+        for (int i = 1; i < 4; ++i) spans |= !same_chunk(N[nd[0]], N[nd[i]]);
+        if (!spans) return;
+        for (int i = 0; i < 4; ++i) any |= internal(nd[i]);
+        if (any) {
+            push(nd[0], nd[1], nd[2], nd[3], kCallEdge | dir << 2);
+            return;
+        }
+        // ContourProcessEdge, src/octree.cpp:235-282
+        int min_size = 0x7fffffff, min_index = 0;
+        bool flip = false, change[4];
+        for (int i = 0; i < 4; ++i) {
+            const SNode& n = N[nd[i]];
+            const int e = cProcessEdge[dir][i];
+            const int corners = n.mat & 0xff;
+            const int m0 = (corners >> cEdgeCorners[e][0]) & 1;
+            const int m1 = (corners >> cEdgeCorners[e][1]) & 1;
+            if (n.size < min_size) {
+                min_size = n.size;
+                min_index = i;
+                flip = m1 != 1;
+            }
+            change[i] = m0 != m1;
+        }
+        if (change[min_index]) push(nd[0], nd[1], nd[2], nd[3], kCallQuad | (flip ? 16 : 0));
+    }
+    __device__ void expand(const Call& c) {
+        const int kind = c.meta & 3, dir = (c.meta >> 2) & 3;
+        if (kind == kCallQuad) {
+            push(c.n[0], c.n[1], c.n[2], c.n[3], c.meta);
+        } else if (kind == kCallCell) {  // ContourCellProc, src/octree.cpp:422-474
+            const SNode& p = N[c.n[0]];
+            for (int i = 0; i < 8; ++i)
+                if (p.child[i] >= 0 && internal(p.child[i])) push(p.child[i], -1, -1, -1, kCallCell);
+            for (int i = 0; i < 12; ++i) face(p.child[cCellFaces[i][0]], p.child[cCellFaces[i][1]], cCellFaces[i][2]);
+            for (int i = 0; i < 6; ++i) {
+                const int nd[4] = {p.child[cCellEdges[i][0]], p.child[cCellEdges[i][1]], p.child[cCellEdges[i][2]],
+                                   p.child[cCellEdges[i][3]]};
+                edge(nd, cCellEdges[i][4]);
+            }
+        } else if (kind == kCallFace) {  // ContourFaceProc, src/octree.cpp:342-418
+            const bool br[2] = {internal(c.n[0]), internal(c.n[1])};
+            for (int i = 0; i < 4; ++i) {
+                int sub[2];
+                for (int j = 0; j < 2; ++j) sub[j] = br[j] ? N[c.n[j]].child[cFaceFaces[dir][i][j]] : c.n[j];
+                face(sub[0], sub[1], cFaceFaces[dir][i][2]);
+            }
+            for (int i = 0; i < 4; ++i) {
+                const int* order = cFaceEdgeOrder[cFaceEdges[dir][i][0]];
+                int sub[4];
+                for (int j = 0; j < 4; ++j) {
+                    const int s = c.n[order[j]];
+                    sub[j] = br[order[j]] ? N[s].child[cFaceEdges[dir][i][1 + j]] : s;
+                }
+                edge(sub, cFaceEdges[dir][i][5]);
+            }
+        } else {  // ContourEdgeProc, src/octree.cpp:286-338
+            bool br[4];
+            for (int j = 0; j < 4; ++j) br[j] = internal(c.n[j]);
+            for (int i = 0; i < 2; ++i) {
+                int sub[4];
+                for (int j = 0; j < 4; ++j) sub[j] = br[j] ? N[c.n[j]].child[cEdgeEdges[dir][i][j]] : c.n[j];
+                edge(sub, cEdgeEdges[dir][i][4]);
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// The seam kernel: one workgroup per seam
+// ---------------------------------------------------------------------------
+
+struct SeamScratch {
+    uint64_t* keys;
+    uint64_t* pick;
+    int* vals;
+    int* leaf;
+    SNode* nodes;
+    Call* list[2];
+};
+
+__host__ __device__ inline uint64_t align_up(uint64_t v) { return (v + 255) & ~255ull; }
+__host__ __device__ inline uint64_t seam_scratch_bytes(int hash_cap, int node_cap, int list_cap) {
+    return align_up(8ull * hash_cap) * 2 + align_up(4ull * hash_cap) * 2 + align_up(sizeof(SNode) * (uint64_t)node_cap) +
+           2 * align_up(sizeof(Call) * (uint64_t)list_cap);
+}
+__device__ inline SeamScratch seam_scratch(char* base, const SeamJobDev& J) {
+    SeamScratch s;
+    char* p = base;
+    s.keys = reinterpret_cast<uint64_t*>(p);
+    p += align_up(8ull * J.hash_cap);
+    s.pick = reinterpret_cast<uint64_t*>(p);
+    p += align_up(8ull * J.hash_cap);
+    s.vals = reinterpret_cast<int*>(p);
+    p += align_up(4ull * J.hash_cap);
+    s.leaf = reinterpret_cast<int*>(p);
+    p += align_up(4ull * J.hash_cap);
+    s.nodes = reinterpret_cast<SNode*>(p);
+    p += align_up(sizeof(SNode) * (uint64_t)J.node_cap);
+    s.list[0] = reinterpret_cast<Call*>(p);
+    p += align_up(sizeof(Call) * (uint64_t)J.list_cap);
+    s.list[1] = reinterpret_cast<Call*>(p);
+    return s;
+}
+
+__global__ __launch_bounds__(kSeamThreads) void k_seam(const SeamJobDev* __restrict__ jobs,
+                                                       const SeamSrcDev* __restrict__ srcs, char* scratch,
+                                                       lvn_vertex* out_v, int* out_t, SeamResDev* res) {
+    const SeamJobDev J = jobs[blockIdx.x];
+    const int tid = threadIdx.x;
+    SeamScratch S = seam_scratch(scratch + J.scratch, J);
+    SNode* N = S.nodes;
+    const int mask = J.hash_cap - 1;
+    const int root_size = 2 * J.host_size;
+    const int root_levels = __builtin_ctz((unsigned)root_size);
+
+    __shared__ int s_nodes, s_status, s_maxlev, s_n, s_more, s_carry;
+    __shared__ int s_buf[kSeamThreads];
+    if (tid == 0) {
+        s_nodes = 0;
+        s_status = 0;
+        s_maxlev = 0;
+    }
+    for (int i = tid; i < J.hash_cap; i += kSeamThreads) {
+        S.keys[i] = kEmptyKey;
+        S.pick[i] = 0;
+        S.vals[i] = -1;
+        S.leaf[i] = 0;
+    }
+    __syncthreads();
+
+    // 1. SelectSeamNodes (src/clipmap.cpp:546-569) over every offered list; the seam
+    //    node -> leaf conversion of src/clipmap.cpp:398-412.
+    const int bounds[3] = {J.host_min[0] + J.host_size, J.host_min[1] + J.host_size, J.host_min[2] + J.host_size};
+    for (int si = 0; si < J.src_count; ++si) {
+        const SeamSrcDev src = srcs[J.src_begin + si];
+        const int leaf_size = src.size / J.vpc;
+        const int filter_size = src.size / (J.vpc * 4);
+        for (int j = tid; j < src.count; j += kSeamThreads) {
+            const lvn_seam_node& sn = src.nodes[j];
+            int mn[3], mx[3];
+            bool inside = true;
+            for (int a = 0; a < 3; ++a) {
+                mn[a] = sn.local_min_and_material[a] * leaf_size + src.min[a];
+                mx[a] = mn[a] + filter_size * 4;
+                inside &= mn[a] >= J.host_min[a] && mn[a] < J.host_min[a] + root_size;
+            }
+            if (!filter_seam_node(src.neighbour, bounds, mn, mx) || !inside) continue;
+            bool aligned = pow2(leaf_size) && leaf_size < root_size;
+            for (int a = 0; a < 3; ++a) aligned &= ((mn[a] - J.host_min[a]) & (leaf_size - 1)) == 0;
+            if (!aligned) {
+                set_status(&s_status, LVN_ERR_INVALID);
+                continue;
+            }
+            const int level = root_levels - __builtin_ctz((unsigned)leaf_size);
+            const uint64_t key = (uint64_t)level << 48 |
+                                 morton((mn[0] - J.host_min[0]) / leaf_size, (mn[1] - J.host_min[1]) / leaf_size,
+                                        (mn[2] - J.host_min[2]) / leaf_size, level);
+            bool fresh;
+            const int slot = hash_insert(S.keys, mask, key, &fresh);
+            if (slot < 0) {
+                set_status(&s_status, LVN_ERR_OVERFLOW);
+                continue;
+            }
+            atomicMax(reinterpret_cast<unsigned long long*>(S.pick + slot),
+                      (unsigned long long)((uint64_t)(si + 1) << 32 | (uint32_t)j));
+            atomicMax(&s_maxlev, level);
+        }
+    }
+    __syncthreads();
+
+    // leaves: one node per distinct key, carrying the last offer
+    for (int i = tid; i < J.hash_cap; i += kSeamThreads) {
+        const uint64_t key = load_l2(S.keys + i);
+        if (key == kEmptyKey) continue;
+        const int idx = atomicAdd(&s_nodes, 1);
+        if (idx >= J.node_cap) {
+            set_status(&s_status, LVN_ERR_OVERFLOW);
+            continue;
+        }
+        const uint64_t p = load_l2(S.pick + i);
+        const SeamSrcDev src = srcs[J.src_begin + (int)(p >> 32) - 1];
+        const lvn_seam_node& sn = src.nodes[(uint32_t)p];
+        SNode n;
+        n.size = src.size / J.vpc;
+        for (int a = 0; a < 3; ++a) n.min[a] = sn.local_min_and_material[a] * n.size + src.min[a];
+        for (int c = 0; c < 8; ++c) n.child[c] = -1;
+        n.parent = -1;
+        n.level = (int)(key >> 48);
+        n.mat = sn.local_min_and_material[3];
+        n.count = 1;
+        n.offset = 0;
+        n.type = kNodeLeaf;
+        n.pad0 = n.pad1 = 0;
+        n.code = key & ((1ull << 48) - 1);
+        n.payload = p;
+        N[idx] = n;
+        S.vals[i] = idx;
+        S.leaf[i] = 1;
+    }
+    __syncthreads();
+    const int nleaves = min(s_nodes, J.node_cap);
+
+    // 2. ancestors (ConstructParents, src/octree.cpp:29-88): a leaf inserts its parent
+    //    chain up to the first parent already present.  A parent key held by a leaf
+    //    means two selected nodes overlap.
+    if (s_status == 0) {
+        for (int i = tid; i < nleaves; i += kSeamThreads) {
+            const int level = N[i].level;
+            const uint64_t code = N[i].code;
+            for (int l = level - 1; l >= 0; --l) {
+                const uint64_t c = code >> (3 * (level - l));
+                bool fresh;
+                const int slot = hash_insert(S.keys, mask, (uint64_t)l << 48 | c, &fresh);
+                if (slot < 0) {
+                    set_status(&s_status, LVN_ERR_OVERFLOW);
+                    break;
+                }
+                if (!fresh) {
+                    if (S.leaf[slot]) set_status(&s_status, LVN_ERR_INVALID);
+                    break;
+                }
+                const int idx = atomicAdd(&s_nodes, 1);
+                if (idx >= J.node_cap) {
+                    set_status(&s_status, LVN_ERR_OVERFLOW);
+                    break;
+                }
+                SNode n;
+                n.size = root_size >> l;
+                int xyz[3];
+                unmorton(c, l, xyz);
+                for (int a = 0; a < 3; ++a) n.min[a] = J.host_min[a] + xyz[a] * n.size;
+                for (int k = 0; k < 8; ++k) n.child[k] = -1;
+                n.parent = -1;
+                n.level = l;
+                n.mat = 0;
+                n.count = 0;
+                n.offset = 0;
+                n.type = kNodeInternal;
+                n.pad0 = n.pad1 = 0;
+                n.code = c;
+                n.payload = 0;
+                N[idx] = n;
+                S.vals[slot] = idx;
+            }
+        }
+    }
+    __syncthreads();
+    const int nnodes = min(s_nodes, J.node_cap);
+    const bool ok = s_status == 0 && nleaves > 0;
+
+    // 3. links, leaf counts, depth-first ranks
+    if (ok) {
+        for (int i = tid; i < nnodes; i += kSeamThreads) {
+            if (N[i].level == 0) continue;
+            const uint64_t c = N[i].code;
+            const int slot = hash_find(S.keys, mask, (uint64_t)(N[i].level - 1) << 48 | (c >> 3));
+            const int p = slot < 0 ? -1 : S.vals[slot];
+            if (p < 0) {  // every parent was inserted: not reached
+                set_status(&s_status, LVN_ERR_OVERFLOW);
+                continue;
+            }
+            N[i].parent = p;
+            N[p].child[c & 7] = i;
+        }
+    }
+    __syncthreads();
+    if (ok && s_status == 0) {
+        for (int i = tid; i < nleaves; i += kSeamThreads)
+            for (int p = N[i].parent; p >= 0; p = N[p].parent) atomicAdd(&N[p].count, 1);
+    }
+    __syncthreads();
+    int root = -1;
+    if (ok && s_status == 0) {
+        const int slot = hash_find(S.keys, mask, 0ull);
+        root = slot < 0 ? -1 : S.vals[slot];
+    }
+    const bool linked = ok && s_status == 0 && root >= 0;
+    if (linked) {
+        for (int l = 0; l < s_maxlev; ++l) {
+            for (int i = tid; i < nnodes; i += kSeamThreads) {
+                if (N[i].level != l || N[i].type != kNodeInternal) continue;
+                int off = N[i].offset;
+                for (int c = 0; c < 8; ++c) {
+                    const int ch = N[i].child[c];
+                    if (ch < 0) continue;
+                    N[ch].offset = off;
+                    off += load_l2(&N[ch].count);
+                }
+            }
+            __syncthreads();
+        }
+        // GenerateVertexIndices (src/octree.cpp:196-231): vertex = rank of the leaf
+        for (int i = tid; i < nleaves; i += kSeamThreads) {
+            const uint64_t p = N[i].payload;
+            const lvn_seam_node& sn = srcs[J.src_begin + (int)(p >> 32) - 1].nodes[(uint32_t)p];
+            lvn_vertex v;
+            v.pos[0] = sn.pos[0];
+            v.pos[1] = sn.pos[1];
+            v.pos[2] = sn.pos[2];
+            v.pos[3] = 0.f;
+            v.normal[0] = sn.normal[0];
+            v.normal[1] = sn.normal[1];
+            v.normal[2] = sn.normal[2];
+            v.normal[3] = 0.f;
+            v.colour[0] = J.colour[0];
+            v.colour[1] = J.colour[1];
+            v.colour[2] = J.colour[2];
+            v.colour[3] = (float)(N[i].mat >> 8);
+            out_v[J.v_off + N[i].offset] = v;
+        }
+    }
+
+    // 4. contouring rounds
+    if (tid == 0) {
+        s_n = 0;
+        if (linked && N[root].type == kNodeInternal) {
+            S.list[0][0] = Call{{root, -1, -1, -1}, kCallCell};
+            s_n = 1;
+        }
+    }
+    __syncthreads();
+    int cur = 0, rounds = 0;
+    bool more = true;
+    for (;;) {
+        __syncthreads();
+        if (s_status != 0 || s_n == 0 || !more) break;
+        const int n = s_n;
+        const Call* A = S.list[cur];
+        Call* B = S.list[cur ^ 1];
+        __syncthreads();  // every thread has read s_n before it changes
+        if (tid == 0) {
+            s_carry = 0;
+            s_more = 0;
+        }
+        __syncthreads();
+        for (int t0 = 0; t0 < n; t0 += kSeamThreads) {
+            const int i = t0 + tid;
+            Call c{{-1, -1, -1, -1}, kCallQuad};
+            Expander e{N, B, 0, 0, 0, false};
+            if (i < n) {
+                c = A[i];
+                e.expand(c);
+            }
+            int total;
+            const int excl = block_exclusive(e.k, s_buf, &total);
+            const int base = s_carry;
+            if (i < n && e.k > 0 && base + excl + e.k <= J.list_cap) {
+                Expander w{N, B, base + excl, 0, 0, true};
+                w.expand(c);
+                if (w.more) s_more = 1;
+            }
+            __syncthreads();
+            if (tid == 0) s_carry = base + total;
+            __syncthreads();
+        }
+        if (tid == 0) {
+            if (s_carry > J.list_cap || rounds + 1 >= kMaxRounds) set_status(&s_status, LVN_ERR_OVERFLOW);
+            s_n = s_carry;
+        }
+        __syncthreads();
+        more = s_more != 0;
+        cur ^= 1;
+        ++rounds;
+    }
+
+    // triangles of the quads, in list order (ContourProcessEdge's two triangles)
+    const int nq = s_status == 0 && linked ? s_n : 0;
+    const Call* Q = S.list[cur];
+    for (int i = tid; i < nq; i += kSeamThreads) {
+        const Call c = Q[i];
+        int v[4];
+        for (int j = 0; j < 4; ++j) v[j] = N[c.n[j]].offset;
+        int* t = out_t + 3 * (J.t_off + 2 * (uint64_t)i);
+        if (c.meta & 16) {
+            t[0] = v[0]; t[1] = v[3]; t[2] = v[1];
+            t[3] = v[0]; t[4] = v[2]; t[5] = v[3];
+        } else {
+            t[0] = v[0]; t[1] = v[1]; t[2] = v[3];
+            t[3] = v[0]; t[4] = v[3]; t[5] = v[2];
+        }
+    }
+    if (tid == 0) {
+        SeamResDev r;
+        r.status = s_status;
+        r.nv = s_status == 0 && linked ? nleaves : 0;
+        r.nt = 2 * nq;
+        r.rounds = rounds;
+        res[blockIdx.x] = r;
+    }
+}
+
+// Exclusive offsets of every seam's vertices and triangles in the packed output
+// (one workgroup); offs[2n], offs[2n+1] = totals.
+__global__ __launch_bounds__(1024) void k_seam_offsets(const SeamResDev* res, int n, uint64_t* offs) {
+    __shared__ unsigned long long bv[1024], bt[1024];
+    __shared__ unsigned long long cv, ct;
+    if (threadIdx.x == 0) cv = ct = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < n; b0 += 1024) {
+        const int i = b0 + threadIdx.x;
+        const unsigned long long v = i < n ? res[i].nv : 0, t = i < n ? res[i].nt : 0;
+        bv[threadIdx.x] = v;
+        bt[threadIdx.x] = t;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const unsigned long long uv = threadIdx.x >= off ? bv[threadIdx.x - off] : 0;
+            const unsigned long long ut = threadIdx.x >= off ? bt[threadIdx.x - off] : 0;
+            __syncthreads();
+            bv[threadIdx.x] += uv;
+            bt[threadIdx.x] += ut;
+            __syncthreads();
+        }
+        if (i < n) {
+            offs[2 * i] = cv + bv[threadIdx.x] - v;
+            offs[2 * i + 1] = ct + bt[threadIdx.x] - t;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) {
+            cv += bv[1023];
+            ct += bt[1023];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        offs[2 * n] = cv;
+        offs[2 * n + 1] = ct;
+    }
+}
+
+// Copies each seam's vertices and triangles into the packed output.
+__global__ __launch_bounds__(256) void k_seam_pack(const SeamJobDev* jobs, const SeamResDev* res,
+                                                   const uint64_t* offs, const lvn_vertex* out_v, const int* out_t,
+                                                   lvn_vertex* pv, int* pt) {
+    const SeamJobDev& J = jobs[blockIdx.x];
+    const SeamResDev r = res[blockIdx.x];
+    const uint64_t ov = offs[2 * blockIdx.x], ot = offs[2 * blockIdx.x + 1];
+    for (int i = threadIdx.x; i < r.nv; i += blockDim.x) pv[ov + i] = out_v[J.v_off + i];
+    for (int i = threadIdx.x; i < 3 * r.nt; i += blockDim.x) pt[3 * ot + i] = out_t[3 * J.t_off + i];
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need) {
+        if (need <= bytes) return LVN_OK;
+        if (p) LVN_HIP(hipFree(p));
+        p = nullptr;
+        bytes = 0;
+        const size_t want = std::max(need, bytes + bytes / 2);
+        if (hipMalloc(&p, want) != hipSuccess) {
+            p = nullptr;
+            return LVN_ERR_NOMEM;
+        }
+        bytes = want;
+        return LVN_OK;
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+struct HostBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need) {
+        if (need <= bytes) return LVN_OK;
+        if (p) LVN_HIP(hipHostFree(p));
+        p = nullptr;
+        bytes = 0;
+        const size_t want = std::max(need, bytes + bytes / 2);
+        if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return LVN_ERR_NOMEM;
+        }
+        bytes = want;
+        return LVN_OK;
+    }
+    ~HostBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+}  // namespace
+
+// Seam-stitching buffers of a context, grown on demand and kept for reuse.
+struct SeamState {
+    DevBuf in, jobs, srcs, scratch, out_v, out_t, res, offs, pv, pt;
+    HostBuf h_in, h_jobs, h_srcs, h_res, h_pv, h_pt;
+};
+
+void destroy_seam_state(SeamState* s) { delete s; }
+
+namespace {
+
+bool host_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// Caps of one seam from the number of offered nodes; `grow` scales them after an overflow.
+void seam_caps(SeamJobDev& J, int offered, int grow) {
+    J.leaf_cap = std::max(offered, 1);
+    J.node_cap = (2 * J.leaf_cap + 64) * grow;
+    J.list_cap = (6 * J.leaf_cap + 1024) * grow;
+    int h = 1;
+    while (h < 2 * J.node_cap) h <<= 1;
+    J.hash_cap = h;
+}
+
+int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* sources, lvn_seam_out* outs) {
+    if (!c->seams) c->seams = new SeamState();
+    SeamState& st = *c->seams;
+    // upload every distinct host list once; cached chunks are read where they lie
+    std::map<const lvn_seam_node*, std::pair<uint64_t, size_t>> uploaded;  // offset, nodes
+    std::vector<SeamSrcDev> srcs;
+    std::vector<int> src_offered;  // offered nodes per source entry
+    size_t in_nodes = 0;
+    int total_sources = 0;
+    for (int i = 0; i < n; ++i) total_sources = std::max(total_sources, reqs[i].first_source + reqs[i].n_sources);
+    for (int s = 0; s < total_sources; ++s) {
+        const lvn_seam_source& q = sources[s];
+        if (q.nodes && q.count > 0) {
+            auto it = uploaded.find(q.nodes);
+            if (it == uploaded.end()) {
+                uploaded[q.nodes] = {in_nodes, (size_t)q.count};
+                in_nodes += (size_t)q.count;
+            } else if ((size_t)q.count > it->second.second) {  // the same list offered with a longer count
+                it->second = {in_nodes, (size_t)q.count};
+                in_nodes += (size_t)q.count;
+            }
+        }
+    }
+    int r = st.in.ensure(std::max<size_t>(in_nodes, 1) * sizeof(lvn_seam_node));
+    if (!r) r = st.h_in.ensure(std::max<size_t>(in_nodes, 1) * sizeof(lvn_seam_node));
+    if (r) return r;
+    auto* h_in = static_cast<lvn_seam_node*>(st.h_in.p);
+    auto* d_in = static_cast<lvn_seam_node*>(st.in.p);
+    for (auto& kv : uploaded)
+        std::memcpy(h_in + kv.second.first, kv.first, kv.second.second * sizeof(lvn_seam_node));
+    srcs.resize(std::max(total_sources, 1));
+    src_offered.assign(std::max(total_sources, 1), 0);
+    for (int s = 0; s < total_sources; ++s) {
+        const lvn_seam_source& q = sources[s];
+        SeamSrcDev& d = srcs[s];
+        d.neighbour = q.neighbour;
+        d.size = q.size;
+        for (int a = 0; a < 3; ++a) d.min[a] = q.min[a];
+        if (q.nodes || q.count >= 0) {
+            d.nodes = q.nodes && q.count > 0 ? d_in + uploaded[q.nodes].first : nullptr;
+            d.count = q.nodes ? q.count : 0;
+        } else {  // count < 0: the context's cached chunk (lvn_generate_chunk_mesh) supplies its seam nodes
+            auto it = c->meshes.find(make_key(q.min, q.size));
+            if (it == c->meshes.end()) return LVN_ERR_STATE;
+            const MeshEntry& m = it->second;
+            d.nodes = reinterpret_cast<const lvn_seam_node*>(m.d + 48 * (size_t)m.V + 12 * (size_t)m.T);
+            d.count = m.S;
+        }
+        src_offered[s] = d.count;
+    }
+    if (in_nodes)
+        LVN_HIP(hipMemcpyAsync(d_in, h_in, in_nodes * sizeof(lvn_seam_node), hipMemcpyHostToDevice, c->stream));
+    if ((r = st.srcs.ensure(srcs.size() * sizeof(SeamSrcDev))) || (r = st.h_srcs.ensure(srcs.size() * sizeof(SeamSrcDev))))
+        return r;
+    std::memcpy(st.h_srcs.p, srcs.data(), srcs.size() * sizeof(SeamSrcDev));
+    LVN_HIP(hipMemcpyAsync(st.srcs.p, st.h_srcs.p, srcs.size() * sizeof(SeamSrcDev), hipMemcpyHostToDevice, c->stream));
+
+    std::vector<int> grow(n, 1), todo(n);
+    for (int i = 0; i < n; ++i) {
+        todo[i] = i;
+        outs[i].status = LVN_ERR_OVERFLOW;
+        outs[i].nv = outs[i].nt = 0;
+    }
+    for (int attempt = 0; attempt < 4 && !todo.empty(); ++attempt) {
+        const int m = (int)todo.size();
+        std::vector<SeamJobDev> jobs(m);
+        uint64_t scratch = 0, v_off = 0, t_off = 0;
+        for (int k = 0; k < m; ++k) {
+            const lvn_seam_req& q = reqs[todo[k]];
+            SeamJobDev& J = jobs[k];
+            std::memset(&J, 0, sizeof J);
+            for (int a = 0; a < 3; ++a) J.host_min[a] = q.host_min[a];
+            J.host_size = q.host_size;
+            for (int a = 0; a < 4; ++a) J.colour[a] = q.colour[a];
+            J.src_begin = q.first_source;
+            J.src_count = q.n_sources;
+            J.vpc = c->N;
+            int offered = 0;
+            for (int s = 0; s < q.n_sources; ++s) offered += src_offered[q.first_source + s];
+            seam_caps(J, offered, grow[todo[k]]);
+            J.scratch = scratch;
+            J.v_off = v_off;
+            J.t_off = t_off;
+            scratch += seam_scratch_bytes(J.hash_cap, J.node_cap, J.list_cap);
+            v_off += (uint64_t)J.leaf_cap;
+            t_off += 2 * (uint64_t)J.list_cap;
+        }
+        if ((r = st.jobs.ensure(m * sizeof(SeamJobDev))) || (r = st.h_jobs.ensure(m * sizeof(SeamJobDev))) ||
+            (r = st.scratch.ensure(scratch)) || (r = st.out_v.ensure(v_off * sizeof(lvn_vertex))) ||
+            (r = st.out_t.ensure(t_off * 12)) || (r = st.res.ensure(m * sizeof(SeamResDev))) ||
+            (r = st.h_res.ensure(m * sizeof(SeamResDev) + (2 * (size_t)m + 2) * 8)) ||
+            (r = st.offs.ensure((2 * (size_t)m + 2) * 8)))
+            return r;
+        std::memcpy(st.h_jobs.p, jobs.data(), m * sizeof(SeamJobDev));
+        LVN_HIP(hipMemcpyAsync(st.jobs.p, st.h_jobs.p, m * sizeof(SeamJobDev), hipMemcpyHostToDevice, c->stream));
+        const auto* d_jobs = static_cast<const SeamJobDev*>(st.jobs.p);
+        auto* d_res = static_cast<SeamResDev*>(st.res.p);
+        auto* d_offs = static_cast<uint64_t*>(st.offs.p);
+        hipLaunchKernelGGL(k_seam, dim3(m), dim3(kSeamThreads), 0, c->stream, d_jobs,
+                           static_cast<const SeamSrcDev*>(st.srcs.p), static_cast<char*>(st.scratch.p),
+                           static_cast<lvn_vertex*>(st.out_v.p), static_cast<int*>(st.out_t.p), d_res);
+        LVN_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_seam_offsets, dim3(1), dim3(1024), 0, c->stream, d_res, m, d_offs);
+        LVN_HIP(hipGetLastError());
+        char* h_res = static_cast<char*>(st.h_res.p);
+        LVN_HIP(hipMemcpyAsync(h_res, d_res, m * sizeof(SeamResDev), hipMemcpyDeviceToHost, c->stream));
+        LVN_HIP(hipMemcpyAsync(h_res + m * sizeof(SeamResDev), d_offs, (2 * (size_t)m + 2) * 8, hipMemcpyDeviceToHost,
+                               c->stream));
+        LVN_HIP(hipStreamSynchronize(c->stream));
+        const auto* rs = reinterpret_cast<const SeamResDev*>(h_res);
+        const auto* offs = reinterpret_cast<const uint64_t*>(h_res + m * sizeof(SeamResDev));
+        const uint64_t tv = offs[2 * m], tt = offs[2 * m + 1];
+        if ((r = st.pv.ensure(std::max<uint64_t>(tv, 1) * sizeof(lvn_vertex))) ||
+            (r = st.pt.ensure(std::max<uint64_t>(tt, 1) * 12)) ||
+            (r = st.h_pv.ensure(std::max<uint64_t>(tv, 1) * sizeof(lvn_vertex))) ||
+            (r = st.h_pt.ensure(std::max<uint64_t>(tt, 1) * 12)))
+            return r;
+        hipLaunchKernelGGL(k_seam_pack, dim3(m), dim3(256), 0, c->stream, d_jobs, d_res, d_offs,
+                           static_cast<const lvn_vertex*>(st.out_v.p), static_cast<const int*>(st.out_t.p),
+                           static_cast<lvn_vertex*>(st.pv.p), static_cast<int*>(st.pt.p));
+        LVN_HIP(hipGetLastError());
+        if (tv) LVN_HIP(hipMemcpyAsync(st.h_pv.p, st.pv.p, tv * sizeof(lvn_vertex), hipMemcpyDeviceToHost, c->stream));
+        if (tt) LVN_HIP(hipMemcpyAsync(st.h_pt.p, st.pt.p, tt * 12, hipMemcpyDeviceToHost, c->stream));
+        LVN_HIP(hipStreamSynchronize(c->stream));
+        std::vector<int> again;
+        for (int k = 0; k < m; ++k) {
+            const int i = todo[k];
+            lvn_seam_out& o = outs[i];
+            if (rs[k].status == LVN_ERR_OVERFLOW) {
+                grow[i] *= 4;
+                again.push_back(i);
+                continue;
+            }
+            o.status = rs[k].status;
+            o.nv = rs[k].nv;
+            o.nt = rs[k].nt;
+            if (o.status) continue;
+            if (o.nv > o.v_cap || o.nt > o.tri_cap) {
+                o.status = LVN_ERR_CAPACITY;
+                continue;
+            }
+            if (o.nv && o.v)
+                std::memcpy(o.v, static_cast<const lvn_vertex*>(st.h_pv.p) + offs[2 * k], o.nv * sizeof(lvn_vertex));
+            if (o.nt && o.tri)
+                std::memcpy(o.tri, static_cast<const int*>(st.h_pt.p) + 3 * offs[2 * k + 1], (size_t)o.nt * 12);
+        }
+        todo.swap(again);
+    }
+    int first = LVN_OK;
+    for (int i = 0; i < n && !first; ++i) first = outs[i].status;
+    return first;
+}
+
+int check_seam_args(int n, const lvn_seam_req* reqs, const lvn_seam_source* sources) {
+    if (n < 0 || (n > 0 && !reqs)) return LVN_ERR_INVALID;
+    for (int i = 0; i < n; ++i) {
+        const lvn_seam_req& q = reqs[i];
+        if (!host_pow2(q.host_size) || q.host_size > (1 << 20) || q.first_source < 0 || q.n_sources < 0)
+            return LVN_ERR_INVALID;
+        if (q.n_sources > 0 && !sources) return LVN_ERR_INVALID;
+        for (int s = 0; s < q.n_sources; ++s) {
+            const lvn_seam_source& src = sources[q.first_source + s];
+            if (src.neighbour < 0 || src.neighbour > 7 || src.size <= 0 || (src.nodes && src.count < 0) ||
+                (!src.nodes && src.count > 0))
+                return LVN_ERR_INVALID;
+        }
+    }
+    return LVN_OK;
+}
+
+}  // namespace
+}  // namespace lvn
+
+using namespace lvn;
+
+extern "C" {
+
+int lvn_generate_seam_meshes(lvn_ctx* ctx, int32_t n, const lvn_seam_req* reqs, const lvn_seam_source* sources,
+                             lvn_seam_out* outs) {
+    if (!ctx || (n > 0 && !outs)) return LVN_ERR_INVALID;
+    if (!g_state.init) return LVN_ERR_STATE;
+    const int r = check_seam_args(n, reqs, sources);
+    if (r) return r;
+    if (n == 0) return LVN_OK;
+    return run_seams(reinterpret_cast<Ctx*>(ctx), n, reqs, sources, outs);
+}
+
+int lvn_generate_seam_mesh(lvn_ctx* ctx, const int32_t host_min[3], int32_t host_size,
+                           const lvn_seam_source* sources, int32_t n_sources, const float colour[3], lvn_vertex* v,
+                           int32_t v_cap, int32_t* nv, int32_t* tri, int32_t tri_cap, int32_t* nt) {
+    if (!host_min || !nv || !nt || !colour || n_sources < 0 || v_cap < 0 || tri_cap < 0) return LVN_ERR_INVALID;
+    lvn_seam_req q;
+    std::memset(&q, 0, sizeof q);
+    for (int a = 0; a < 3; ++a) q.host_min[a] = host_min[a];
+    q.host_size = host_size;
+    for (int a = 0; a < 3; ++a) q.colour[a] = colour[a];
+    q.first_source = 0;
+    q.n_sources = n_sources;
+    lvn_seam_out o;
+    std::memset(&o, 0, sizeof o);
+    o.v = v;
+    o.tri = tri;
+    o.v_cap = v ? v_cap : 0;
+    o.tri_cap = tri ? tri_cap : 0;
+    const int r = lvn_generate_seam_meshes(ctx, 1, &q, sources, &o);
+    *nv = o.nv;
+    *nt = o.nt;
+    return r;
+}
+
+}  // extern "C"

@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 from .abi import (CSG_OP_DTYPE, ERROR_NAMES, LVN_ERR_CAPACITY, LVN_OK, REQ_EDIT, REQ_MESH, RESULT_DTYPE, SEAM_DTYPE,
-                  VERTEX_DTYPE, CsgOp, FieldDesc, Request, StageDump, Tuning, ptr, vptr)
+                  VERTEX_DTYPE, CsgOp, FieldDesc, Request, SeamOut, SeamReq, StageDump, Tuning, ptr, seam_sources,
+                  vptr)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblvn_mesher.so")
@@ -24,7 +25,7 @@ EXPORTS = [
     "lvn_batch_run_timed", "lvn_batch_stage_name", "lvn_batch_counts", "lvn_batch_get_mesh", "lvn_batch_work",
     "lvn_batch_get_tuning", "lvn_batch_set_tuning", "lvn_batch_autotune",
     "lvn_server_create", "lvn_server_destroy", "lvn_server_submit", "lvn_server_poll", "lvn_server_drain",
-    "lvn_server_get_mesh", "lvn_server_stats",
+    "lvn_server_get_mesh", "lvn_server_stats", "lvn_generate_seam_mesh", "lvn_generate_seam_meshes",
 ]
 
 _lib = None
@@ -83,6 +84,9 @@ def lib():
             "lvn_server_get_mesh": (i32, [vp, C.c_int64, vp, i32, C.POINTER(i32), vp, i32, C.POINTER(i32), vp, i32,
                                           C.POINTER(i32)]),
             "lvn_server_stats": (i32, [vp, vp]),
+            "lvn_generate_seam_mesh": (i32, [vp, vp, i32, vp, i32, vp, vp, i32, C.POINTER(i32), vp, i32,
+                                             C.POINTER(i32)]),
+            "lvn_generate_seam_meshes": (i32, [vp, i32, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -185,6 +189,46 @@ class Context:
                                f"{ns.value} seam nodes)")
         check(rc, "lvn_generate_chunk_mesh")
         return v[:nv.value].copy(), t[:nt.value].copy(), s[:ns.value].copy()
+
+    def seam_mesh(self, host_min, host_size, sources, colour=(1.0, 1.0, 1.0)):
+        """The seam mesh of one clipmap node (GenerateClipmapSeamMesh, src/clipmap.cpp:573-611) from
+        source dicts {min, size, neighbour, nodes}; nodes=None takes the seam nodes of this
+        context's cached chunk at (min, size).  Returns (vertices, triangles int32[T, 3])."""
+        return self.seam_meshes([(host_min, host_size, colour)], [sources])[0]
+
+    def seam_meshes(self, hosts, sources_per_host, v_caps=None, tri_caps=None):
+        """Many seams in one call: hosts = [(min, size, colour)], sources_per_host = [[source dicts]].
+        Returns [(vertices, triangles)]; raises LvnError on the first failing seam."""
+        flat, reqs = [], (SeamReq * max(1, len(hosts)))()
+        for i, ((mn, size, colour), srcs) in enumerate(zip(hosts, sources_per_host)):
+            reqs[i].host_min[:] = [int(v) for v in mn]
+            reqs[i].host_size = int(size)
+            reqs[i].colour[:] = [float(colour[0]), float(colour[1]), float(colour[2]), 0.0]
+            reqs[i].first_source = len(flat)
+            reqs[i].n_sources = len(srcs)
+            flat.extend(srcs)
+        src = seam_sources(flat)
+        n = len(hosts)
+        offered = [sum(len(s["nodes"]) if s["nodes"] is not None else 32768 for s in srcs)
+                   for srcs in sources_per_host]
+        vc = list(v_caps) if v_caps is not None else offered
+        tc = list(tri_caps) if tri_caps is not None else [4 * o + 64 for o in offered]
+        for _ in range(2):
+            bufs, outs = [], (SeamOut * max(1, n))()
+            for i in range(n):
+                v = np.zeros(max(1, vc[i]), dtype=VERTEX_DTYPE)
+                t = np.zeros((max(1, tc[i]), 3), dtype=np.int32)
+                bufs.append((v, t))
+                outs[i].v, outs[i].tri = v.ctypes.data, t.ctypes.data
+                outs[i].v_cap, outs[i].tri_cap = vc[i], tc[i]
+            rc = lib().lvn_generate_seam_meshes(self._h, n, C.cast(reqs, C.c_void_p), C.cast(src, C.c_void_p),
+                                                C.cast(outs, C.c_void_p))
+            if rc != LVN_ERR_CAPACITY:
+                break
+            vc = [max(vc[i], outs[i].nv) for i in range(n)]
+            tc = [max(tc[i], outs[i].nt) for i in range(n)]
+        check(rc, "lvn_generate_seam_meshes")
+        return [(bufs[i][0][:outs[i].nv].copy(), bufs[i][1][:outs[i].nt].copy()) for i in range(n)]
 
     def debug_stages(self, min_xyz, size, ops=None, edge_cap=None, voxel_cap=None):
         """Every stage of one fresh chunk (density ... vertex positions)."""

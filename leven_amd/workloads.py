@@ -208,5 +208,44 @@ def serving(seed=5, duration=2.0, rate=4000.0, edit_fraction=0.05, speed_voxels=
                 sizes=np.array([e[2][1] for e in events if e[1] == "mesh"], np.int32))
 
 
+# CHILD_MIN_OFFSETS, src/volume_constants.h:24-35: child / neighbour i at (bit 2, bit 1, bit 0) of i
+CHILD_MIN_OFFSETS = [((i >> 2) & 1, (i >> 1) & 1, i & 1) for i in range(8)]
+
+
+def seam_sources(nodes):
+    """The seam node lists GenerateClipmapSeamMesh gathers for each active clipmap node
+    (src/clipmap.cpp:579-597): for neighbour i at host.min + host.size * CHILD_MIN_OFFSETS[i],
+    every active node that contains that point or whose min lies in the neighbour's region
+    (FindActiveNodes, src/clipmap.cpp:1449-1481; the active nodes partition the meshed world).
+    nodes: list of (min, size, seams) with seams a SEAM_DTYPE array.  Returns, per node, the
+    list of source dicts {min, size, neighbour, nodes} for lvn_generate_seam_mesh."""
+    index = {(tuple(int(v) for v in mn), int(size)): i for i, (mn, size, _) in enumerate(nodes)}
+    sizes = sorted({int(size) for _, size, _ in nodes})
+    inside = {}
+    for i, (mn, size, _) in enumerate(nodes):
+        for S in sizes:
+            if S >= size:
+                key = (tuple((int(v) // S) * S for v in mn), S)
+                inside.setdefault(key, []).append(i)
+    out = []
+    for hmin, hsize, _ in nodes:
+        hsize = int(hsize)
+        srcs = []
+        for i, off in enumerate(CHILD_MIN_OFFSETS):
+            nmin = tuple(int(hmin[a]) + hsize * off[a] for a in range(3))
+            found = set(inside.get((nmin, hsize), []))
+            for S in sizes:
+                if S > hsize:
+                    j = index.get((tuple((v // S) * S for v in nmin), S))
+                    if j is not None:
+                        found.add(j)
+            for j in sorted(found):
+                mn, size, seams = nodes[j]
+                if len(seams):
+                    srcs.append(dict(min=tuple(int(v) for v in mn), size=int(size), neighbour=i, nodes=seams))
+        out.append(srcs)
+    return out
+
+
 WORKLOADS = {"plain_cpu": plain_cpu, "terrain_512": terrain_512, "precision": precision, "clipmap": clipmap,
              "serving": serving}

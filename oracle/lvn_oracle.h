@@ -102,6 +102,24 @@ void orc_chunk_get_triangles(const orc_chunk* c, int32_t* tri3);
 void orc_chunk_get_vertices(const orc_chunk* c, lvn_vertex* v);
 void orc_chunk_get_seams(const orc_chunk* c, lvn_seam_node* s);
 
+/* ---- seam stitching (lvn_oracle_seams.cpp) --------------------------------- */
+/* The seam mesh of one host clipmap node, with the contract of lvn_generate_seam_mesh:
+ * SelectSeamNodes + Octree_ConstructUpwards + Octree_GenerateMesh (src/clipmap.cpp:508-611,
+ * src/octree.cpp:22-148,196-474,520-550).  Returns LVN_OK, LVN_ERR_INVALID or
+ * LVN_ERR_CAPACITY (needed counts in *nv / *nt). */
+int orc_seam_mesh(const int32_t host_min[3], int32_t host_size, const lvn_seam_source* sources,
+                  int32_t n_sources, const float colour[3], int32_t voxels_per_chunk, lvn_vertex* v,
+                  int32_t v_cap, int32_t* nv, int32_t* tri, int32_t tri_cap, int32_t* nt);
+/* Test hook: the octree leaf (min xyz, size; world units) behind each vertex of the same
+ * seam mesh, in vertex order (leaf_min_size holds 4 * cap ints). */
+int orc_seam_mesh_leaves(const int32_t host_min[3], int32_t host_size, const lvn_seam_source* sources,
+                         int32_t n_sources, int32_t voxels_per_chunk, int32_t cap, int32_t* n,
+                         int32_t* leaf_min_size);
+/* n seams, one seam per thread on `threads` threads; totals[0..2] += vertices, triangles,
+ * failed seams.  Returns wall seconds. */
+double orc_seam_meshes(int32_t n, const lvn_seam_req* reqs, const lvn_seam_source* sources,
+                       int32_t voxels_per_chunk, int32_t threads, int64_t* totals);
+
 /* ---- many chunks, timed (CPU baseline) ------------------------------------ */
 /* Meshes n chunks, one chunk per thread on `threads` threads.  Stored ops are filtered per
  * chunk by AABB overlap like LoadDensityField (src/compute_density_field.cpp:255-263);

@@ -10,7 +10,8 @@ import subprocess
 
 import numpy as np
 
-from leven_amd.abi import (CSG_OP_DTYPE, SEAM_DTYPE, VERTEX_DTYPE, FieldDesc, ptr, vptr)  # noqa: F401
+from leven_amd.abi import (CSG_OP_DTYPE, SEAM_DTYPE, VERTEX_DTYPE, FieldDesc, ptr, seam_sources,  # noqa: F401
+                           vptr)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblvn_oracle.so")
@@ -70,6 +71,9 @@ def lib():
             "orc_chunk_get_vertices": (None, [vp, vp]),
             "orc_chunk_get_seams": (None, [vp, vp]),
             "orc_mesh_chunks": (f64, [P(FieldDesc), vp, i32, i32, vp, vp, u32, vp, i32, vp, i32, vp, vp]),
+            "orc_seam_mesh": (i32, [vp, i32, vp, i32, vp, i32, vp, i32, vp, vp, i32, vp]),
+            "orc_seam_meshes": (f64, [i32, vp, vp, i32, i32, vp]),
+            "orc_seam_mesh_leaves": (i32, [vp, i32, vp, i32, i32, i32, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -267,3 +271,43 @@ def mesh_chunks(desc, perm, voxels_per_chunk, mins, sizes, default_material=0, o
                                  vptr(sizes), int(default_material), vptr(o), 0 if o is None else len(o),
                                  vptr(bb), int(threads), vptr(secs), vptr(totals))
     return wall, secs, totals
+
+
+def seam_mesh(host_min, host_size, sources, colour=(1.0, 1.0, 1.0), voxels_per_chunk=64):
+    """Seam mesh of one host node from source dicts {min, size, neighbour, nodes}.
+    Returns (status, vertices VERTEX_DTYPE, triangles int32 (T, 3))."""
+    src = seam_sources(sources)
+    mn = np.ascontiguousarray(host_min, dtype=np.int32)
+    col = np.ascontiguousarray(colour, dtype=np.float32)
+    nv, nt = C.c_int32(0), C.c_int32(0)
+    rc = lib().orc_seam_mesh(vptr(mn), int(host_size), C.cast(src, C.c_void_p), len(sources), vptr(col),
+                             int(voxels_per_chunk), None, 0, C.byref(nv), None, 0, C.byref(nt))
+    if rc not in (0, -3):
+        return rc, np.zeros(0, dtype=VERTEX_DTYPE), np.zeros((0, 3), dtype=np.int32)
+    v = np.zeros(nv.value, dtype=VERTEX_DTYPE)
+    t = np.zeros((nt.value, 3), dtype=np.int32)
+    rc = lib().orc_seam_mesh(vptr(mn), int(host_size), C.cast(src, C.c_void_p), len(sources), vptr(col),
+                             int(voxels_per_chunk), vptr(v), len(v), C.byref(nv), vptr(t), len(t), C.byref(nt))
+    return rc, v, t
+
+
+def seam_mesh_leaves(host_min, host_size, sources, voxels_per_chunk=64):
+    """The leaf (min xyz, size) behind each vertex of the seam mesh, int32 (nv, 4)."""
+    src = seam_sources(sources)
+    mn = np.ascontiguousarray(host_min, dtype=np.int32)
+    cap = max(1, sum(len(s["nodes"]) for s in sources))
+    out = np.zeros((cap, 4), dtype=np.int32)
+    n = C.c_int32(0)
+    rc = lib().orc_seam_mesh_leaves(vptr(mn), int(host_size), C.cast(src, C.c_void_p), len(sources),
+                                    int(voxels_per_chunk), cap, C.byref(n), vptr(out))
+    assert rc == 0, rc
+    return out[:n.value].copy()
+
+
+def seam_meshes(reqs, sources, voxels_per_chunk=64, threads=1):
+    """Time the oracle over many seams (CPU baseline): reqs is a ctypes SeamReq array, sources
+    a ctypes SeamSource array.  Returns (wall_s, totals [vertices, triangles, failed])."""
+    totals = np.zeros(3, dtype=np.int64)
+    wall = lib().orc_seam_meshes(len(reqs), C.cast(reqs, C.c_void_p), C.cast(sources, C.c_void_p),
+                                 int(voxels_per_chunk), int(threads), vptr(totals))
+    return wall, totals

@@ -1,0 +1,151 @@
+"""Seam stitching on the GPU against the oracle: vertices and triangles bit for bit, in order.
+
+Inputs are seam node lists of oracle-meshed chunks (equal-LOD block and a mixed LOD0/LOD1
+layout), offered to each host node as GenerateClipmapSeamMesh gathers them
+(src/clipmap.cpp:579-597).  The cached-source form reads the lists of chunks that the
+context meshed itself.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from leven_amd import mesher, workloads
+from leven_amd.abi import LVN_ERR_CAPACITY, LVN_ERR_INVALID, LVN_OK, SEAM_DTYPE, VERTEX_DTYPE
+from oracle import py_oracle
+
+from seam_scene import Scene, overlapping_offer
+
+pytestmark = pytest.mark.gpu
+
+SEED = 2
+
+
+def _equal_lod_nodes():
+    return [((x * 256, -256 + y * 256, z * 256), 256) for x, y, z in itertools.product(range(2), repeat=3)]
+
+
+def _mixed_lod_nodes():
+    """Two 512 regions split into LOD0 chunks among LOD1 chunks (sizes 256 and 512): LOD0
+    hosts with coarser neighbours, LOD1 hosts with finer ones (at -x of the split regions)."""
+    nodes = []
+    for base in ((0, -512, 0), (0, 0, 0)):
+        for x, y, z in itertools.product(range(2), repeat=3):
+            nodes.append(((base[0] + 256 * x, base[1] + 256 * y, base[2] + 256 * z), 256))
+    for x, y, z in ((512, -512, 0), (512, 0, 0), (0, -512, 512), (0, 0, 512), (512, -512, 512), (512, 0, 512),
+                    (-512, -512, 0), (-512, 0, 0)):
+        nodes.append(((x, y, z), 512))
+    return nodes
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    mesher.init(workloads.terrain_desc(), SEED, 0)
+    mesher.clear_csg_ops()
+    c = mesher.Context(64)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def equal_lod():
+    return Scene(workloads.terrain_desc(), SEED, _equal_lod_nodes())
+
+
+@pytest.fixture(scope="module")
+def mixed_lod():
+    return Scene(workloads.terrain_desc(), SEED, _mixed_lod_nodes())
+
+
+def _colour(i):
+    return (0.25 * (i % 4), 0.5, 1.0 - 0.125 * (i % 8))
+
+
+def _check_scene(ctx, scene):
+    srcs = scene.sources()
+    hosts = [(mn, size, _colour(i)) for i, (mn, size, _) in enumerate(scene.nodes)]
+    got = ctx.seam_meshes(hosts, srcs)
+    tris = 0
+    for i, ((mn, size, colour), s) in enumerate(zip(hosts, srcs)):
+        rc, v, t = py_oracle.seam_mesh(mn, size, s, colour=colour)
+        assert rc == LVN_OK
+        gv, gt = got[i]
+        print(f"seam {i} host {mn} size {size}: {len(v)} vertices, {len(t)} triangles", flush=True)
+        assert len(gv) == len(v) and len(gt) == len(t), f"seam {i}"
+        assert gv.tobytes() == v.tobytes(), f"seam {i} vertices"
+        assert (gt == t).all(), f"seam {i} triangles"
+        tris += len(t)
+    assert tris > 0
+    return got
+
+
+def test_equal_lod_seams_match_oracle(ctx, equal_lod):
+    got = _check_scene(ctx, equal_lod)
+    # one seam per call gives the same as the batch
+    mn, size, _ = equal_lod.nodes[0]
+    v, t = ctx.seam_mesh(mn, size, equal_lod.sources()[0], colour=_colour(0))
+    assert v.tobytes() == got[0][0].tobytes() and (t == got[0][1]).all()
+
+
+def test_mixed_lod_seams_match_oracle(ctx, mixed_lod):
+    sizes = {s for _, s, _ in mixed_lod.nodes}
+    assert sizes == {256, 512}
+    _check_scene(ctx, mixed_lod)
+
+
+def test_cached_sources(ctx, equal_lod):
+    """Seam lists of chunks the context meshed itself, read where they lie in device memory,
+    give the seams the same lists give when passed from the host, and the oracle's."""
+    nodes = []
+    for mn, size, seams in equal_lod.nodes:
+        v, t, s = ctx.generate_chunk_mesh(mn, size)
+        assert s.tobytes() == seams.tobytes(), f"chunk {mn} seam nodes"
+        nodes.append((mn, size, s))
+    host_srcs = workloads.seam_sources(nodes)
+    cached = [[dict(d, nodes=None) for d in srcs] for srcs in host_srcs]
+    hosts = [(mn, size, _colour(i)) for i, (mn, size, _) in enumerate(nodes)]
+    a = ctx.seam_meshes(hosts, host_srcs)
+    b = ctx.seam_meshes(hosts, cached)
+    for i in range(len(hosts)):
+        assert a[i][0].tobytes() == b[i][0].tobytes() and (a[i][1] == b[i][1]).all(), f"seam {i}"
+        rc, v, t = py_oracle.seam_mesh(hosts[i][0], hosts[i][1], host_srcs[i], colour=hosts[i][2])
+        assert v.tobytes() == b[i][0].tobytes() and (t == b[i][1]).all(), f"seam {i} vs oracle"
+    for mn, size, _ in nodes:
+        ctx.free_chunk(mn, size)
+
+
+def test_seam_errors(ctx, equal_lod):
+    mn, size, _ = equal_lod.nodes[0]
+    srcs = equal_lod.sources()[0]
+    # empty
+    v, t = ctx.seam_mesh(mn, size, [])
+    assert len(v) == 0 and len(t) == 0
+    # a list offered twice: each node once
+    v0, t0 = ctx.seam_mesh(mn, size, srcs)
+    v1, t1 = ctx.seam_mesh(mn, size, srcs + srcs)
+    assert v0.tobytes() == v1.tobytes() and (t0 == t1).all()
+    # overlapping offers (a coarser node over one of the host's leaves): rejected by both
+    bad = srcs + [overlapping_offer(mn, size, srcs)]
+    rc, _, _ = py_oracle.seam_mesh(mn, size, bad)
+    with pytest.raises(mesher.LvnError) as e:
+        ctx.seam_mesh(mn, size, bad)
+    assert e.value.code == rc == LVN_ERR_INVALID
+    # caller buffers too small: LVN_ERR_CAPACITY with the needed counts
+    with pytest.raises(mesher.LvnError) as e:
+        _seam_capacity(ctx, mn, size, srcs, len(v0), len(t0))
+    assert e.value.code == LVN_ERR_CAPACITY
+
+
+def _seam_capacity(ctx, mn, size, srcs, nv, nt):
+    import ctypes as C
+    from leven_amd.abi import seam_sources
+    src = seam_sources(srcs)
+    v = np.zeros(max(1, nv - 1), dtype=VERTEX_DTYPE)
+    t = np.zeros((max(1, nt), 3), dtype=np.int32)
+    gnv, gnt = C.c_int32(), C.c_int32()
+    col = np.ones(3, dtype=np.float32)
+    rc = mesher.lib().lvn_generate_seam_mesh(ctx._h, mesher.vptr(np.asarray(mn, dtype=np.int32)), int(size),
+                                             C.cast(src, C.c_void_p), len(srcs), mesher.vptr(col), mesher.vptr(v),
+                                             len(v), C.byref(gnv), mesher.vptr(t), len(t), C.byref(gnt))
+    assert gnv.value == nv and gnt.value == nt
+    mesher.check(rc, "lvn_generate_seam_mesh")
