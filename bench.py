@@ -167,12 +167,112 @@ def clipmap_leg(args):
     assert (counts[:, 4] == 0).all(), "a clipmap chunk overflowed"
     stage_ms = {k: round(v, 4) for k, v in b.run_timed().items()}
     lods = np.log2(wl["sizes"] // 256).astype(int)
+    # active clipmap nodes (a mesh or seam nodes, src/clipmap.cpp:468) with their seam node lists
+    nodes = [(tuple(int(v) for v in wl["mins"][i]), int(wl["sizes"][i]), b.get_mesh(i)[2])
+             for i in range(n) if counts[i, 2] > 0 or counts[i, 3] > 0] if not args.no_seams else None
     b.close()
-    return {"chunks": n, "chunks_per_lod": np.bincount(lods, minlength=6).tolist(),
+    return nodes, {"chunks": n, "chunks_per_lod": np.bincount(lods, minlength=6).tolist(),
             "steps": steps, "chunks_per_s": round(n * steps / elapsed, 1), "ms_per_pass": round(elapsed / steps * 1e3, 3),
             "nonempty_chunks": int((counts[:, 2] > 0).sum()), "max_active_voxels": int(counts[:, 1].max()),
             "max_edges": int(counts[:, 0].max()), "triangles": int(counts[:, 2].sum()),
             "stage_ms": stage_ms, "tuning": tuning}
+
+
+def seam_leg(nodes, args):
+    """Seam meshes of every active clipmap node (GenerateClipmapSeamMesh, src/clipmap.cpp:573-611):
+    neighbour seam node lists gathered as the reference does, all seams in one
+    lvn_generate_seam_meshes call (host lists in, host meshes out), against the oracle on
+    the host's cores."""
+    import ctypes as C
+
+    from leven_amd.abi import SEAM_DTYPE, VERTEX_DTYPE, SeamOut, SeamReq, seam_sources, vptr
+    per_host = workloads.seam_sources(nodes)
+    flat, reqs = [], (SeamReq * len(nodes))()
+    for i, ((mn, size, _), srcs) in enumerate(zip(nodes, per_host)):
+        reqs[i].host_min[:] = list(mn)
+        reqs[i].host_size = size
+        reqs[i].colour[:] = [1.0, 1.0, 1.0, 0.0]
+        reqs[i].first_source = len(flat)
+        reqs[i].n_sources = len(srcs)
+        flat.extend(srcs)
+    src = seam_sources(flat)
+    offered = [sum(len(d["nodes"]) for d in srcs) for srcs in per_host]
+    outs = (SeamOut * len(nodes))()
+    bufs = []
+    for i, o in enumerate(offered):
+        v = np.zeros(max(1, o), dtype=VERTEX_DTYPE)
+        t = np.zeros((max(1, 6 * o + 64), 3), dtype=np.int32)
+        bufs.append((v, t))
+        outs[i].v, outs[i].tri, outs[i].v_cap, outs[i].tri_cap = v.ctypes.data, t.ctypes.data, len(v), len(t)
+    ctx = mesher.Context(64)
+    L = mesher.lib()
+    # the drop-in form: host seam node lists in, host meshes out
+
+    def run():
+        mesher.check(L.lvn_generate_seam_meshes(ctx._h, len(nodes), C.cast(reqs, C.c_void_p), C.cast(src, C.c_void_p),
+                                                C.cast(outs, C.c_void_p)), "lvn_generate_seam_meshes")
+
+    for _ in range(max(1, args.warmup)):
+        run()
+    steps = max(3, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    elapsed = time.perf_counter() - t0
+    nv = sum(o.nv for o in outs)
+    nt = sum(o.nt for o in outs)
+    # the device-resident form: the context meshes (and caches) every node, then stitches
+    # from the cached lists in device memory and leaves the seams there
+    n3 = 64 ** 3
+    cv = np.zeros(n3, dtype=VERTEX_DTYPE)
+    ct = np.zeros((6 * n3, 3), dtype=np.int32)
+    cs = np.zeros(n3, dtype=SEAM_DTYPE)
+    a, b_, c_ = C.c_int32(), C.c_int32(), C.c_int32()
+    for mn, size, _ in nodes:
+        mesher.check(L.lvn_generate_chunk_mesh(ctx._h, vptr(np.asarray(mn, dtype=np.int32)), size, vptr(cv), n3,
+                                               C.byref(a), vptr(ct), 6 * n3, C.byref(b_), vptr(cs), n3, C.byref(c_)),
+                     "lvn_generate_chunk_mesh")
+    dev_src = seam_sources([dict(d, nodes=None) for d in flat])
+    dev_outs = (SeamOut * len(nodes))()
+
+    def run_dev():
+        mesher.check(L.lvn_generate_seam_meshes(ctx._h, len(nodes), C.cast(reqs, C.c_void_p),
+                                                C.cast(dev_src, C.c_void_p), C.cast(dev_outs, C.c_void_p)),
+                     "lvn_generate_seam_meshes")
+
+    run_dev()
+    assert sum(o.nv for o in dev_outs) == nv and sum(o.nt for o in dev_outs) == nt
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run_dev()
+    dev_elapsed = time.perf_counter() - t0
+    ctx.close()
+    out = {"seams": len(nodes), "sources": len(flat), "seam_nodes_offered": int(sum(offered)),
+           "vertices": int(nv), "triangles": int(nt), "steps": steps,
+           "seams_per_s": round(len(nodes) * steps / elapsed, 1), "ms_per_pass": round(elapsed / steps * 1e3, 3),
+           "note": "drop-in form: host seam node lists in, host meshes out (one upload, one read-back per pass)",
+           "device_resident": {"seams_per_s": round(len(nodes) * steps / dev_elapsed, 1),
+                               "ms_per_pass": round(dev_elapsed / steps * 1e3, 3),
+                               "note": "seam lists read from the context's cached chunks, meshes left in HBM"}}
+    if not args.no_cpu:
+        from oracle import py_oracle
+        threads = max(1, min(16, os.cpu_count() or 1))
+        wall, totals = py_oracle.seam_meshes(reqs, src, threads=threads)
+        assert totals[2] == 0 and totals[0] == nv and totals[1] == nt, "oracle and GPU seam totals differ"
+        one = (SeamReq * 64)()
+        pick = np.linspace(0, len(nodes) - 1, 64).astype(int)
+        for k, i in enumerate(pick):
+            one[k] = reqs[i]
+        wall1, _ = py_oracle.seam_meshes(one, src, threads=1)
+        out["cpu_baseline"] = {"value": round(len(nodes) / wall, 1), "unit": "seams/s", "cores": threads,
+                               "kind": "port", "single_thread_seams_per_s": round(64 / wall1, 1),
+                               "sample": f"all {len(nodes)} seams on {threads} threads, oracle restatement of the "
+                                         f"reference's CPU seam path ({wall:.3f} s); single thread 64 seams in "
+                                         f"{wall1:.3f} s"}
+        out["gpu_over_cpu"] = round(out["seams_per_s"] / out["cpu_baseline"]["value"], 1)
+        out["device_resident"]["gpu_over_cpu"] = round(out["device_resident"]["seams_per_s"] /
+                                                       out["cpu_baseline"]["value"], 1)
+    return out
 
 
 def dropin_latency(wl, count):
@@ -286,6 +386,7 @@ def main():
     ap.add_argument("--plain-chunks", type=int, default=0, help="chunks for the plain form (0 = all)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="length of the served request stream (0 = skip)")
     ap.add_argument("--no-clipmap", action="store_true", help="skip the 6-LOD clipmap batch")
+    ap.add_argument("--no-seams", action="store_true", help="skip seam stitching of the clipmap")
     ap.add_argument("--no-precision", action="store_true", help="skip the precision-form batch")
     ap.add_argument("--no-autotune", action="store_true", help="time the default launch shapes")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -395,7 +496,9 @@ def main():
     if not args.no_precision:
         out["precision"] = precision_leg(args)
     if not args.no_clipmap:
-        out["clipmap"] = clipmap_leg(args)
+        nodes, out["clipmap"] = clipmap_leg(args)
+        if nodes:
+            out["seams"] = seam_leg(nodes, args)
     if args.serve_seconds > 0:
         out["serving"] = serve(args)
         out["p99_chunk_latency_ms"] = out["serving"]["latency_ms"]["p99"]

@@ -196,6 +196,15 @@ typedef struct lvn_seam_out {
 int lvn_generate_seam_meshes(lvn_ctx* ctx, int32_t n, const lvn_seam_req* reqs,
                              const lvn_seam_source* sources, lvn_seam_out* outs);
 
+/* Device-resident results: a seam passed with v == NULL and tri == NULL is not copied
+ * out (its counts still are); this returns device pointers to seam i's vertices and
+ * triangles from the last lvn_generate_seam_meshes / lvn_generate_seam_mesh call on ctx,
+ * valid until the next such call or lvn_ctx_destroy.  With cached sources (count -1) a
+ * clipmap's seams are stitched without crossing PCIe.  No reference counterpart: for
+ * callers that render or post-process on the GPU. */
+int lvn_seam_device_mesh(lvn_ctx* ctx, int32_t i, const lvn_vertex** v, const int32_t** tri,
+                         int32_t* nv, int32_t* nt);
+
 /* ---- batched form: many chunks per launch, device-resident outputs ---------- */
 
 /* One chunk request: min in world units, size = 256 * 2^LOD (src/clipmap.cpp:378). */

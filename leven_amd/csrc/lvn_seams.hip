@@ -30,6 +30,8 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <thread>
 #include <vector>
 
 #define LVN_HIP(call)                                                                                 \
@@ -79,12 +81,28 @@ enum : int { kCallCell = 0, kCallFace = 1, kCallEdge = 2, kCallQuad = 3 };
 struct SNode {
     int min[3];
     int size;
-    int child[8];
     int parent, level, mat, count;
-    int offset, type, pad0, pad1;
+    int offset, type;
     uint64_t code;
     uint64_t payload;  // leaves: (source + 1) << 32 | node index
 };
+
+// What contouring reads of a node, in one word beside its eight child links: the LOD0
+// chunk of its min relative to the root's (3 x 14 bits; ChunkMinForPosition,
+// src/volume.cpp:30-34, masks with CLIPMAP_LEAF_SIZE = 256), internal flag, log2 size,
+// corner signs, and whether its whole region lies in that one chunk.
+constexpr uint64_t kShapeChunkMask = (1ull << 42) - 1;
+__device__ inline uint64_t node_shape(const int* world_min, const int* host_min, int size, bool internal,
+                                      int corners) {
+    uint64_t key = 0;
+    bool one_chunk = true;
+    for (int a = 0; a < 3; ++a) {
+        key |= (uint64_t)(((world_min[a] >> 8) - (host_min[a] >> 8)) & 0x3fff) << (14 * a);
+        one_chunk &= (world_min[a] >> 8) == ((world_min[a] + size - 1) >> 8);
+    }
+    return key | (uint64_t)internal << 42 | (uint64_t)(__builtin_ctz((unsigned)size) & 31) << 43 |
+           (uint64_t)(corners & 0xff) << 48 | (uint64_t)one_chunk << 56;
+}
 
 // A pending call of the recursion: node indices and meta = kind | dir << 2 | flip << 4.
 struct Call {
@@ -92,31 +110,32 @@ struct Call {
     int meta;
 };
 
-constexpr int kSeamThreads = 256;
+constexpr int kSeamThreads = 1024;
+constexpr int kSeamWaves = kSeamThreads / 64;
+constexpr int kCallsPerThread = 4;  // pending calls one thread expands per round tile
 constexpr uint64_t kEmptyKey = ~0ull;
 constexpr int kMaxRounds = 64;
-constexpr int kChunkMask = ~(256 - 1);  // ChunkMinForPosition, src/volume.cpp:30-34 (CLIPMAP_LEAF_SIZE)
 
 // Dual contouring tables (src/contour_constants.h); child i of a node sits at
 // (bit 2, bit 1, bit 0) of i (CHILD_MIN_OFFSETS, src/volume_constants.h:24-35).
-__constant__ int cEdgeCorners[12][2] = {{0, 4}, {1, 5}, {2, 6}, {3, 7}, {0, 2}, {1, 3},
-                                        {4, 6}, {5, 7}, {0, 1}, {2, 3}, {4, 5}, {6, 7}};
-__constant__ int cCellFaces[12][3] = {{0, 4, 0}, {1, 5, 0}, {2, 6, 0}, {3, 7, 0}, {0, 2, 1}, {4, 6, 1},
-                                      {1, 3, 1}, {5, 7, 1}, {0, 1, 2}, {2, 3, 2}, {4, 5, 2}, {6, 7, 2}};
-__constant__ int cCellEdges[6][5] = {{0, 1, 2, 3, 0}, {4, 5, 6, 7, 0}, {0, 4, 1, 5, 1},
-                                     {2, 6, 3, 7, 1}, {0, 2, 4, 6, 2}, {1, 3, 5, 7, 2}};
-__constant__ int cFaceFaces[3][4][3] = {{{4, 0, 0}, {5, 1, 0}, {6, 2, 0}, {7, 3, 0}},
-                                        {{2, 0, 1}, {6, 4, 1}, {3, 1, 1}, {7, 5, 1}},
-                                        {{1, 0, 2}, {3, 2, 2}, {5, 4, 2}, {7, 6, 2}}};
-__constant__ int cFaceEdges[3][4][6] = {
+constexpr int kEdgeCorners[12][2] = {{0, 4}, {1, 5}, {2, 6}, {3, 7}, {0, 2}, {1, 3},
+                                     {4, 6}, {5, 7}, {0, 1}, {2, 3}, {4, 5}, {6, 7}};
+constexpr int kCellFaces[12][3] = {{0, 4, 0}, {1, 5, 0}, {2, 6, 0}, {3, 7, 0}, {0, 2, 1}, {4, 6, 1},
+                                   {1, 3, 1}, {5, 7, 1}, {0, 1, 2}, {2, 3, 2}, {4, 5, 2}, {6, 7, 2}};
+constexpr int kCellEdges[6][5] = {{0, 1, 2, 3, 0}, {4, 5, 6, 7, 0}, {0, 4, 1, 5, 1},
+                                  {2, 6, 3, 7, 1}, {0, 2, 4, 6, 2}, {1, 3, 5, 7, 2}};
+constexpr int kFaceFaces[3][4][3] = {{{4, 0, 0}, {5, 1, 0}, {6, 2, 0}, {7, 3, 0}},
+                                     {{2, 0, 1}, {6, 4, 1}, {3, 1, 1}, {7, 5, 1}},
+                                     {{1, 0, 2}, {3, 2, 2}, {5, 4, 2}, {7, 6, 2}}};
+constexpr int kFaceEdges[3][4][6] = {
     {{1, 4, 0, 5, 1, 1}, {1, 6, 2, 7, 3, 1}, {0, 4, 6, 0, 2, 2}, {0, 5, 7, 1, 3, 2}},
     {{0, 2, 3, 0, 1, 0}, {0, 6, 7, 4, 5, 0}, {1, 2, 0, 6, 4, 2}, {1, 3, 1, 7, 5, 2}},
     {{1, 1, 0, 3, 2, 0}, {1, 5, 4, 7, 6, 0}, {0, 1, 5, 0, 4, 1}, {0, 3, 7, 2, 6, 1}}};
-__constant__ int cEdgeEdges[3][2][5] = {{{3, 2, 1, 0, 0}, {7, 6, 5, 4, 0}},
-                                        {{5, 1, 4, 0, 1}, {7, 3, 6, 2, 1}},
-                                        {{6, 4, 2, 0, 2}, {7, 5, 3, 1, 2}}};
-__constant__ int cProcessEdge[3][4] = {{3, 2, 1, 0}, {7, 5, 6, 4}, {11, 10, 9, 8}};
-__constant__ int cFaceEdgeOrder[2][4] = {{0, 0, 1, 1}, {0, 1, 0, 1}};
+constexpr int kEdgeEdges[3][2][5] = {{{3, 2, 1, 0, 0}, {7, 6, 5, 4, 0}},
+                                     {{5, 1, 4, 0, 1}, {7, 3, 6, 2, 1}},
+                                     {{6, 4, 2, 0, 2}, {7, 5, 3, 1, 2}}};
+constexpr int kProcessEdge[3][4] = {{3, 2, 1, 0}, {7, 5, 6, 4}, {11, 10, 9, 8}};
+constexpr int kFaceEdgeOrder[2][4] = {{0, 0, 1, 1}, {0, 1, 0, 1}};
 
 // ---------------------------------------------------------------------------
 // Device helpers
@@ -188,125 +207,181 @@ __device__ inline bool filter_seam_node(int neighbour, const int* b, const int* 
     return false;
 }
 
-__device__ inline bool same_chunk(const SNode& a, const SNode& b) {
-    return (a.min[0] & kChunkMask) == (b.min[0] & kChunkMask) && (a.min[1] & kChunkMask) == (b.min[1] & kChunkMask) &&
-           (a.min[2] & kChunkMask) == (b.min[2] & kChunkMask);
-}
+__device__ inline bool same_chunk(uint64_t a, uint64_t b) { return ((a ^ b) & kShapeChunkMask) == 0; }
 
 __device__ inline void set_status(int* s, int code) { atomicCAS(s, 0, code); }
 
-// Exclusive workgroup scan of one value per thread; *total = sum.
-__device__ inline int block_exclusive(int v, int* buf, int* total) {
-    const int t = threadIdx.x;
-    buf[t] = v;
-    __syncthreads();
-    for (int off = 1; off < kSeamThreads; off <<= 1) {
-        const int u = t >= off ? buf[t - off] : 0;
-        __syncthreads();
-        buf[t] += u;
-        __syncthreads();
+// SelectSeamNodes' test for one offered node (src/clipmap.cpp:552-567) with the node's
+// leaf min (src/clipmap.cpp:402): FilterSeamNode and the host's seam AABB (min, 2 * size).
+__device__ inline bool select_leaf(const SeamJobDev& J, const SeamSrcDev& src, const lvn_seam_node& sn, int* mn) {
+    const int leaf_size = src.size / J.vpc;
+    const int filter_size = src.size / (J.vpc * 4);
+    const int root_size = 2 * J.host_size;
+    const int bounds[3] = {J.host_min[0] + J.host_size, J.host_min[1] + J.host_size, J.host_min[2] + J.host_size};
+    int mx[3];
+    bool inside = true;
+    for (int a = 0; a < 3; ++a) {
+        mn[a] = sn.local_min_and_material[a] * leaf_size + src.min[a];
+        mx[a] = mn[a] + filter_size * 4;
+        inside &= mn[a] >= J.host_min[a] && mn[a] < J.host_min[a] + root_size;
     }
-    const int incl = buf[t];
-    *total = buf[kSeamThreads - 1];
-    __syncthreads();
-    return incl - v;
+    return inside && filter_seam_node(src.neighbour, bounds, mn, mx);
 }
 
-// The calls one pending call makes, in the reference's call order.  kWrite = false
-// only counts them.  Calls that would return at once are not made: a face with a
-// missing node, both nodes in one chunk or no branch; an edge with a missing node or
-// all four nodes in one chunk.  An edge of four leaves is resolved on the spot
-// (ContourProcessEdge) into a quad or nothing.
+__device__ inline int wave_inclusive(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+
+// Exclusive workgroup scan of one value per thread (wave scans, then a scan of the
+// wave totals in `sw`); *total = sum.
+__device__ inline int block_exclusive(int v, int* sw, int* total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int incl = wave_inclusive(v);
+    if (lane == 63) sw[w] = incl;
+    __syncthreads();
+    if (w == 0) {
+        int x = lane < kSeamWaves ? sw[lane] : 0;
+        x = wave_inclusive(x);
+        if (lane < kSeamWaves) sw[lane] = x;
+    }
+    __syncthreads();
+    *total = sw[kSeamWaves - 1];
+    return (w ? sw[w - 1] : 0) + incl - v;
+}
+
+// The calls one pending call makes, in the reference's call order (write = false only
+// counts them).  Calls that would return at once are not made: a face with a missing
+// node, both nodes in one chunk or no branch; an edge with a missing node or all four
+// nodes in one chunk.  An edge of four leaves is resolved on the spot
+// (ContourProcessEdge) into a quad or nothing.  Directions are template arguments so
+// that every table index is a constant.
 struct Expander {
-    const SNode* N;
+    const int* C;        // child links, 8 per node
+    const uint64_t* H;   // node shapes
     Call* dst;
     int at;
     int k;
     int more;
     bool write;
 
-    __device__ void push(int a, int b, int c, int d, int meta) {
+    __device__ __forceinline__ void push(int a, int b, int c, int d, int meta) {
         if (write) {
             dst[at + k] = Call{{a, b, c, d}, meta};
             if ((meta & 3) != kCallQuad) more = 1;
         }
         ++k;
     }
-    __device__ bool internal(int i) const { return N[i].type == kNodeInternal; }
+    __device__ bool internal(int i) const { return (H[i] >> 42) & 1; }
+    // A cell call on a node inside one chunk makes only calls on nodes of that chunk, which
+    // the chunk test returns from: it emits nothing and is not made.
+    __device__ bool cell_emits(int i) const { return ((H[i] >> 42) & 1) && !((H[i] >> 56) & 1); }
+    __device__ int child(int i, int k) const { return C[8 * i + k]; }
 
-    __device__ void face(int a, int b, int dir) {
+    __device__ __forceinline__ void face(int a, int b, int dir) {
         if (a < 0 || b < 0) return;
-        if (same_chunk(N[a], N[b])) return;
+        if (same_chunk(H[a], H[b])) return;
         if (!internal(a) && !internal(b)) return;
         push(a, b, -1, -1, kCallFace | dir << 2);
     }
-    __device__ void edge(const int* nd, int dir) {
-        for (int i = 0; i < 4; ++i)
-            if (nd[i] < 0) return;
-        bool spans = false, any = false;
-        for (int i = 1; i < 4; ++i) spans |= !same_chunk(N[nd[0]], N[nd[i]]);
-        if (!spans) return;
-        for (int i = 0; i < 4; ++i) any |= internal(nd[i]);
-        if (any) {
-            push(nd[0], nd[1], nd[2], nd[3], kCallEdge | dir << 2);
+    template <int D>
+    __device__ __forceinline__ void edge_d(int n0, int n1, int n2, int n3) {
+        if (n0 < 0 || n1 < 0 || n2 < 0 || n3 < 0) return;
+        const uint64_t h[4] = {H[n0], H[n1], H[n2], H[n3]};
+        if (same_chunk(h[0], h[1]) && same_chunk(h[0], h[2]) && same_chunk(h[0], h[3])) return;
+        if (((h[0] | h[1] | h[2] | h[3]) >> 42) & 1) {
+            push(n0, n1, n2, n3, kCallEdge | D << 2);
             return;
         }
         // ContourProcessEdge, src/octree.cpp:235-282
-        int min_size = 0x7fffffff, min_index = 0;
-        bool flip = false, change[4];
+        int min_size = 0x7fffffff;
+        bool flip = false, change = false;
+#pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const SNode& n = N[nd[i]];
-            const int e = cProcessEdge[dir][i];
-            const int corners = n.mat & 0xff;
-            const int m0 = (corners >> cEdgeCorners[e][0]) & 1;
-            const int m1 = (corners >> cEdgeCorners[e][1]) & 1;
-            if (n.size < min_size) {
-                min_size = n.size;
-                min_index = i;
+            const int e = kProcessEdge[D][i];
+            const int corners = (int)(h[i] >> 48) & 0xff;
+            const int size = (int)(h[i] >> 43) & 31;  // log2: the order of sizes is kept
+            const int m0 = (corners >> kEdgeCorners[e][0]) & 1;
+            const int m1 = (corners >> kEdgeCorners[e][1]) & 1;
+            if (size < min_size) {
+                min_size = size;
                 flip = m1 != 1;
+                change = m0 != m1;
             }
-            change[i] = m0 != m1;
         }
-        if (change[min_index]) push(nd[0], nd[1], nd[2], nd[3], kCallQuad | (flip ? 16 : 0));
+        if (change) push(n0, n1, n2, n3, kCallQuad | (flip ? 16 : 0));
     }
-    __device__ void expand(const Call& c) {
+    __device__ __forceinline__ void edge(int n0, int n1, int n2, int n3, int dir) {
+        if (dir == 0) edge_d<0>(n0, n1, n2, n3);
+        else if (dir == 1) edge_d<1>(n0, n1, n2, n3);
+        else edge_d<2>(n0, n1, n2, n3);
+    }
+    // ContourFaceProc, src/octree.cpp:342-418
+    template <int D>
+    __device__ __forceinline__ void face_call(const Call& c) {
+        const bool b0 = internal(c.n[0]), b1 = internal(c.n[1]);
+        int ch0[8], ch1[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            ch0[k] = b0 ? child(c.n[0], k) : c.n[0];
+            ch1[k] = b1 ? child(c.n[1], k) : c.n[1];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) face(ch0[kFaceFaces[D][i][0]], ch1[kFaceFaces[D][i][1]], kFaceFaces[D][i][2]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int* order = kFaceEdgeOrder[kFaceEdges[D][i][0]];
+            const int s0 = order[0] ? ch1[kFaceEdges[D][i][1]] : ch0[kFaceEdges[D][i][1]];
+            const int s1 = order[1] ? ch1[kFaceEdges[D][i][2]] : ch0[kFaceEdges[D][i][2]];
+            const int s2 = order[2] ? ch1[kFaceEdges[D][i][3]] : ch0[kFaceEdges[D][i][3]];
+            const int s3 = order[3] ? ch1[kFaceEdges[D][i][4]] : ch0[kFaceEdges[D][i][4]];
+            edge(s0, s1, s2, s3, kFaceEdges[D][i][5]);
+        }
+    }
+    // ContourEdgeProc, src/octree.cpp:286-338
+    template <int D>
+    __device__ __forceinline__ void edge_call(const Call& c) {
+        bool br[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) br[j] = internal(c.n[j]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            int sub[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) sub[j] = br[j] ? child(c.n[j], kEdgeEdges[D][i][j]) : c.n[j];
+            edge(sub[0], sub[1], sub[2], sub[3], kEdgeEdges[D][i][4]);
+        }
+    }
+    __device__ __forceinline__ void expand(const Call& c) {
         const int kind = c.meta & 3, dir = (c.meta >> 2) & 3;
         if (kind == kCallQuad) {
             push(c.n[0], c.n[1], c.n[2], c.n[3], c.meta);
         } else if (kind == kCallCell) {  // ContourCellProc, src/octree.cpp:422-474
-            const SNode& p = N[c.n[0]];
+            int ch[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) ch[i] = child(c.n[0], i);
+#pragma unroll
             for (int i = 0; i < 8; ++i)
-                if (p.child[i] >= 0 && internal(p.child[i])) push(p.child[i], -1, -1, -1, kCallCell);
-            for (int i = 0; i < 12; ++i) face(p.child[cCellFaces[i][0]], p.child[cCellFaces[i][1]], cCellFaces[i][2]);
-            for (int i = 0; i < 6; ++i) {
-                const int nd[4] = {p.child[cCellEdges[i][0]], p.child[cCellEdges[i][1]], p.child[cCellEdges[i][2]],
-                                   p.child[cCellEdges[i][3]]};
-                edge(nd, cCellEdges[i][4]);
-            }
-        } else if (kind == kCallFace) {  // ContourFaceProc, src/octree.cpp:342-418
-            const bool br[2] = {internal(c.n[0]), internal(c.n[1])};
-            for (int i = 0; i < 4; ++i) {
-                int sub[2];
-                for (int j = 0; j < 2; ++j) sub[j] = br[j] ? N[c.n[j]].child[cFaceFaces[dir][i][j]] : c.n[j];
-                face(sub[0], sub[1], cFaceFaces[dir][i][2]);
-            }
-            for (int i = 0; i < 4; ++i) {
-                const int* order = cFaceEdgeOrder[cFaceEdges[dir][i][0]];
-                int sub[4];
-                for (int j = 0; j < 4; ++j) {
-                    const int s = c.n[order[j]];
-                    sub[j] = br[order[j]] ? N[s].child[cFaceEdges[dir][i][1 + j]] : s;
-                }
-                edge(sub, cFaceEdges[dir][i][5]);
-            }
-        } else {  // ContourEdgeProc, src/octree.cpp:286-338
-            bool br[4];
-            for (int j = 0; j < 4; ++j) br[j] = internal(c.n[j]);
-            for (int i = 0; i < 2; ++i) {
-                int sub[4];
-                for (int j = 0; j < 4; ++j) sub[j] = br[j] ? N[c.n[j]].child[cEdgeEdges[dir][i][j]] : c.n[j];
-                edge(sub, cEdgeEdges[dir][i][4]);
-            }
+                if (ch[i] >= 0 && cell_emits(ch[i])) push(ch[i], -1, -1, -1, kCallCell);
+#pragma unroll
+            for (int i = 0; i < 12; ++i) face(ch[kCellFaces[i][0]], ch[kCellFaces[i][1]], kCellFaces[i][2]);
+#pragma unroll
+            for (int i = 0; i < 6; ++i)
+                edge(ch[kCellEdges[i][0]], ch[kCellEdges[i][1]], ch[kCellEdges[i][2]], ch[kCellEdges[i][3]],
+                     kCellEdges[i][4]);
+        } else if (kind == kCallFace) {
+            if (dir == 0) face_call<0>(c);
+            else if (dir == 1) face_call<1>(c);
+            else face_call<2>(c);
+        } else {
+            if (dir == 0) edge_call<0>(c);
+            else if (dir == 1) edge_call<1>(c);
+            else edge_call<2>(c);
         }
     }
 };
@@ -321,12 +396,17 @@ struct SeamScratch {
     int* vals;
     int* leaf;
     SNode* nodes;
-    Call* list[2];
+    int* ch;           // 8 child links per node
+    uint64_t* shape;   // node shapes
+    int* order;        // node indices bucketed by level
+    Call* list0;       // pending calls, two rounds alternating
+    Call* list1;
 };
 
 __host__ __device__ inline uint64_t align_up(uint64_t v) { return (v + 255) & ~255ull; }
 __host__ __device__ inline uint64_t seam_scratch_bytes(int hash_cap, int node_cap, int list_cap) {
     return align_up(8ull * hash_cap) * 2 + align_up(4ull * hash_cap) * 2 + align_up(sizeof(SNode) * (uint64_t)node_cap) +
+           align_up(32ull * node_cap) + align_up(8ull * node_cap) + align_up(4ull * node_cap) +
            2 * align_up(sizeof(Call) * (uint64_t)list_cap);
 }
 __device__ inline SeamScratch seam_scratch(char* base, const SeamJobDev& J) {
@@ -342,13 +422,19 @@ __device__ inline SeamScratch seam_scratch(char* base, const SeamJobDev& J) {
     p += align_up(4ull * J.hash_cap);
     s.nodes = reinterpret_cast<SNode*>(p);
     p += align_up(sizeof(SNode) * (uint64_t)J.node_cap);
-    s.list[0] = reinterpret_cast<Call*>(p);
+    s.ch = reinterpret_cast<int*>(p);
+    p += align_up(32ull * J.node_cap);
+    s.shape = reinterpret_cast<uint64_t*>(p);
+    p += align_up(8ull * J.node_cap);
+    s.order = reinterpret_cast<int*>(p);
+    p += align_up(4ull * J.node_cap);
+    s.list0 = reinterpret_cast<Call*>(p);
     p += align_up(sizeof(Call) * (uint64_t)J.list_cap);
-    s.list[1] = reinterpret_cast<Call*>(p);
+    s.list1 = reinterpret_cast<Call*>(p);
     return s;
 }
 
-__global__ __launch_bounds__(kSeamThreads) void k_seam(const SeamJobDev* __restrict__ jobs,
+__global__ __launch_bounds__(kSeamThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_seam(const SeamJobDev* __restrict__ jobs,
                                                        const SeamSrcDev* __restrict__ srcs, char* scratch,
                                                        lvn_vertex* out_v, int* out_t, SeamResDev* res) {
     const SeamJobDev J = jobs[blockIdx.x];
@@ -360,7 +446,8 @@ __global__ __launch_bounds__(kSeamThreads) void k_seam(const SeamJobDev* __restr
     const int root_levels = __builtin_ctz((unsigned)root_size);
 
     __shared__ int s_nodes, s_status, s_maxlev, s_n, s_more, s_carry;
-    __shared__ int s_buf[kSeamThreads];
+    __shared__ int s_wave[kSeamWaves];
+    __shared__ int s_lev[33], s_fill[32];
     if (tid == 0) {
         s_nodes = 0;
         s_status = 0;
@@ -376,21 +463,12 @@ __global__ __launch_bounds__(kSeamThreads) void k_seam(const SeamJobDev* __restr
 
     // 1. SelectSeamNodes (src/clipmap.cpp:546-569) over every offered list; the seam
     //    node -> leaf conversion of src/clipmap.cpp:398-412.
-    const int bounds[3] = {J.host_min[0] + J.host_size, J.host_min[1] + J.host_size, J.host_min[2] + J.host_size};
     for (int si = 0; si < J.src_count; ++si) {
         const SeamSrcDev src = srcs[J.src_begin + si];
         const int leaf_size = src.size / J.vpc;
-        const int filter_size = src.size / (J.vpc * 4);
         for (int j = tid; j < src.count; j += kSeamThreads) {
-            const lvn_seam_node& sn = src.nodes[j];
-            int mn[3], mx[3];
-            bool inside = true;
-            for (int a = 0; a < 3; ++a) {
-                mn[a] = sn.local_min_and_material[a] * leaf_size + src.min[a];
-                mx[a] = mn[a] + filter_size * 4;
-                inside &= mn[a] >= J.host_min[a] && mn[a] < J.host_min[a] + root_size;
-            }
-            if (!filter_seam_node(src.neighbour, bounds, mn, mx) || !inside) continue;
+            int mn[3];
+            if (!select_leaf(J, src, src.nodes[j], mn)) continue;
             bool aligned = pow2(leaf_size) && leaf_size < root_size;
             for (int a = 0; a < 3; ++a) aligned &= ((mn[a] - J.host_min[a]) & (leaf_size - 1)) == 0;
             if (!aligned) {
@@ -429,17 +507,17 @@ __global__ __launch_bounds__(kSeamThreads) void k_seam(const SeamJobDev* __restr
         SNode n;
         n.size = src.size / J.vpc;
         for (int a = 0; a < 3; ++a) n.min[a] = sn.local_min_and_material[a] * n.size + src.min[a];
-        for (int c = 0; c < 8; ++c) n.child[c] = -1;
         n.parent = -1;
         n.level = (int)(key >> 48);
         n.mat = sn.local_min_and_material[3];
         n.count = 1;
         n.offset = 0;
         n.type = kNodeLeaf;
-        n.pad0 = n.pad1 = 0;
         n.code = key & ((1ull << 48) - 1);
         n.payload = p;
         N[idx] = n;
+        for (int c = 0; c < 8; ++c) S.ch[8 * idx + c] = -1;
+        S.shape[idx] = node_shape(n.min, J.host_min, n.size, false, n.mat);
         S.vals[i] = idx;
         S.leaf[i] = 1;
     }
@@ -475,17 +553,17 @@ __global__ __launch_bounds__(kSeamThreads) void k_seam(const SeamJobDev* __restr
                 int xyz[3];
                 unmorton(c, l, xyz);
                 for (int a = 0; a < 3; ++a) n.min[a] = J.host_min[a] + xyz[a] * n.size;
-                for (int k = 0; k < 8; ++k) n.child[k] = -1;
                 n.parent = -1;
                 n.level = l;
                 n.mat = 0;
                 n.count = 0;
                 n.offset = 0;
                 n.type = kNodeInternal;
-                n.pad0 = n.pad1 = 0;
                 n.code = c;
                 n.payload = 0;
                 N[idx] = n;
+                for (int k = 0; k < 8; ++k) S.ch[8 * idx + k] = -1;
+                S.shape[idx] = node_shape(n.min, J.host_min, n.size, true, 0);
                 S.vals[slot] = idx;
             }
         }
@@ -506,13 +584,34 @@ __global__ __launch_bounds__(kSeamThreads) void k_seam(const SeamJobDev* __restr
                 continue;
             }
             N[i].parent = p;
-            N[p].child[c & 7] = i;
+            S.ch[8 * p + (c & 7)] = i;
         }
+        // bucket the nodes by level
+        for (int i = tid; i < 33; i += kSeamThreads) s_lev[i] = 0;
     }
     __syncthreads();
+    if (ok) {
+        for (int i = tid; i < nnodes; i += kSeamThreads) atomicAdd(&s_lev[N[i].level + 1], 1);
+    }
+    __syncthreads();
+    if (ok && tid == 0) {
+        for (int l = 1; l < 33; ++l) s_lev[l] += s_lev[l - 1];  // s_lev[l] = first node of level l
+        for (int l = 0; l < 32; ++l) s_fill[l] = s_lev[l];
+    }
+    __syncthreads();
+    if (ok) {
+        for (int i = tid; i < nnodes; i += kSeamThreads) S.order[atomicAdd(&s_fill[N[i].level], 1)] = i;
+    }
+    __syncthreads();
+    // leaf counts, deepest level first: each node adds its count to its parent once
     if (ok && s_status == 0) {
-        for (int i = tid; i < nleaves; i += kSeamThreads)
-            for (int p = N[i].parent; p >= 0; p = N[p].parent) atomicAdd(&N[p].count, 1);
+        for (int l = s_maxlev; l >= 1; --l) {
+            for (int k = s_lev[l] + tid; k < s_lev[l + 1]; k += kSeamThreads) {
+                const int i = S.order[k];
+                atomicAdd(&N[N[i].parent].count, load_l2(&N[i].count));
+            }
+            __syncthreads();
+        }
     }
     __syncthreads();
     int root = -1;
@@ -522,12 +621,14 @@ __global__ __launch_bounds__(kSeamThreads) void k_seam(const SeamJobDev* __restr
     }
     const bool linked = ok && s_status == 0 && root >= 0;
     if (linked) {
+        // depth-first ranks, top level first: children in order 0..7 after their parent's offset
         for (int l = 0; l < s_maxlev; ++l) {
-            for (int i = tid; i < nnodes; i += kSeamThreads) {
-                if (N[i].level != l || N[i].type != kNodeInternal) continue;
+            for (int k = s_lev[l] + tid; k < s_lev[l + 1]; k += kSeamThreads) {
+                const int i = S.order[k];
+                if (N[i].type != kNodeInternal) continue;
                 int off = N[i].offset;
                 for (int c = 0; c < 8; ++c) {
-                    const int ch = N[i].child[c];
+                    const int ch = S.ch[8 * i + c];
                     if (ch < 0) continue;
                     N[ch].offset = off;
                     off += load_l2(&N[ch].count);
@@ -559,8 +660,8 @@ __global__ __launch_bounds__(kSeamThreads) void k_seam(const SeamJobDev* __restr
     // 4. contouring rounds
     if (tid == 0) {
         s_n = 0;
-        if (linked && N[root].type == kNodeInternal) {
-            S.list[0][0] = Call{{root, -1, -1, -1}, kCallCell};
+        if (linked && N[root].type == kNodeInternal && ((S.shape[root] >> 56) & 1) == 0) {
+            S.list0[0] = Call{{root, -1, -1, -1}, kCallCell};
             s_n = 1;
         }
     }
@@ -571,29 +672,36 @@ __global__ __launch_bounds__(kSeamThreads) void k_seam(const SeamJobDev* __restr
         __syncthreads();
         if (s_status != 0 || s_n == 0 || !more) break;
         const int n = s_n;
-        const Call* A = S.list[cur];
-        Call* B = S.list[cur ^ 1];
+        const Call* A = cur ? S.list1 : S.list0;  // (no dynamic index into a private array)
+        Call* B = cur ? S.list0 : S.list1;
         __syncthreads();  // every thread has read s_n before it changes
         if (tid == 0) {
             s_carry = 0;
             s_more = 0;
         }
         __syncthreads();
-        for (int t0 = 0; t0 < n; t0 += kSeamThreads) {
-            const int i = t0 + tid;
-            Call c{{-1, -1, -1, -1}, kCallQuad};
-            Expander e{N, B, 0, 0, 0, false};
-            if (i < n) {
-                c = A[i];
-                e.expand(c);
+        for (int t0 = 0; t0 < n; t0 += kSeamThreads * kCallsPerThread) {
+            // thread t expands calls i0 .. i0 + kCallsPerThread - 1, in order
+            const int i0 = t0 + tid * kCallsPerThread;
+            int cnt = 0;
+            for (int k = 0; k < kCallsPerThread && i0 + k < n; ++k) {
+                Expander e{S.ch, S.shape, B, 0, 0, 0, false};
+                e.expand(A[i0 + k]);
+                cnt += e.k;
             }
             int total;
-            const int excl = block_exclusive(e.k, s_buf, &total);
+            const int excl = block_exclusive(cnt, s_wave, &total);
             const int base = s_carry;
-            if (i < n && e.k > 0 && base + excl + e.k <= J.list_cap) {
-                Expander w{N, B, base + excl, 0, 0, true};
-                w.expand(c);
-                if (w.more) s_more = 1;
+            if (cnt > 0 && base + excl + cnt <= J.list_cap) {
+                int at = base + excl;
+                bool m = false;
+                for (int k = 0; k < kCallsPerThread && i0 + k < n; ++k) {
+                    Expander w{S.ch, S.shape, B, at, 0, 0, true};
+                    w.expand(A[i0 + k]);
+                    at += w.k;
+                    m |= w.more != 0;
+                }
+                if (m) s_more = 1;
             }
             __syncthreads();
             if (tid == 0) s_carry = base + total;
@@ -611,7 +719,7 @@ __global__ __launch_bounds__(kSeamThreads) void k_seam(const SeamJobDev* __restr
 
     // triangles of the quads, in list order (ContourProcessEdge's two triangles)
     const int nq = s_status == 0 && linked ? s_n : 0;
-    const Call* Q = S.list[cur];
+    const Call* Q = cur ? S.list1 : S.list0;
     for (int i = tid; i < nq; i += kSeamThreads) {
         const Call c = Q[i];
         int v[4];
@@ -633,6 +741,26 @@ __global__ __launch_bounds__(kSeamThreads) void k_seam(const SeamJobDev* __restr
         r.rounds = rounds;
         res[blockIdx.x] = r;
     }
+}
+
+// Selected seam nodes of each seam, to size its scratch.
+__global__ __launch_bounds__(256) void k_seam_count(const SeamJobDev* __restrict__ jobs,
+                                                    const SeamSrcDev* __restrict__ srcs, int* counts) {
+    const SeamJobDev J = jobs[blockIdx.x];
+    __shared__ int s_count;
+    if (threadIdx.x == 0) s_count = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int si = 0; si < J.src_count; ++si) {
+        const SeamSrcDev src = srcs[J.src_begin + si];
+        for (int j = threadIdx.x; j < src.count; j += blockDim.x) {
+            int mn[3];
+            mine += select_leaf(J, src, src.nodes[j], mn) ? 1 : 0;
+        }
+    }
+    atomicAdd(&s_count, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = s_count;
 }
 
 // Exclusive offsets of every seam's vertices and triangles in the packed output
@@ -734,10 +862,21 @@ struct HostBuf {
 
 }  // namespace
 
-// Seam-stitching buffers of a context, grown on demand and kept for reuse.
+// Where a seam's mesh of the last call lies in device memory: the unpacked output of
+// the attempt that finished it.
+struct SeamLoc {
+    int attempt = -1;
+    int nv = 0, nt = 0;
+    uint64_t v_off = 0, t_off = 0;
+};
+
+// Seam-stitching buffers of a context, grown on demand and kept for reuse.  Each retry
+// attempt writes its own output buffers, so every finished seam stays addressable.
 struct SeamState {
-    DevBuf in, jobs, srcs, scratch, out_v, out_t, res, offs, pv, pt;
-    HostBuf h_in, h_jobs, h_srcs, h_res, h_pv, h_pt;
+    DevBuf in, jobs, srcs, scratch, res, offs, pv, pt, counts;
+    HostBuf h_in, h_jobs, h_srcs, h_res, h_pv, h_pt, h_counts;
+    std::vector<std::unique_ptr<DevBuf>> out_v, out_t;  // per attempt
+    std::vector<SeamLoc> loc;                            // per seam of the last call
 };
 
 void destroy_seam_state(SeamState* s) { delete s; }
@@ -746,13 +885,37 @@ namespace {
 
 bool host_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
+// Runs f(lo, hi) over [0, n) split across host threads when `bytes` of copying justify
+// it: the seam lists and meshes of a clipmap are gigabytes, more than one core copies
+// at PCIe speed.
+template <class F>
+void parallel_ranges(int n, size_t bytes, F&& f) {
+    const int hw = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    const int parts = (int)std::min<size_t>((size_t)std::min(hw, std::max(n, 1)), std::max<size_t>(1, bytes >> 24));
+    if (parts <= 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const int step = (n + parts - 1) / parts;
+    for (int p = 1; p < parts; ++p) {
+        const int lo = p * step, hi = std::min(n, lo + step);
+        if (lo < hi) pool.emplace_back([&f, lo, hi] { f(lo, hi); });
+    }
+    f(0, std::min(n, step));
+    for (auto& t : pool) t.join();
+}
+
 // Caps of one seam from the number of offered nodes; `grow` scales them after an overflow.
-void seam_caps(SeamJobDev& J, int offered, int grow) {
-    J.leaf_cap = std::max(offered, 1);
+// Caps of one seam from its selected nodes; `grow` scales them after an overflow.  A
+// seam's octree has about 1.4 nodes per leaf (the leaves lie on faces) and at most ~3
+// quads per leaf.
+void seam_caps(SeamJobDev& J, int selected, int grow) {
+    J.leaf_cap = std::max(selected, 1);
     J.node_cap = (2 * J.leaf_cap + 64) * grow;
     J.list_cap = (6 * J.leaf_cap + 1024) * grow;
     int h = 1;
-    while (h < 2 * J.node_cap) h <<= 1;
+    while (h < J.node_cap + J.node_cap / 2) h <<= 1;
     J.hash_cap = h;
 }
 
@@ -762,7 +925,6 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
     // upload every distinct host list once; cached chunks are read where they lie
     std::map<const lvn_seam_node*, std::pair<uint64_t, size_t>> uploaded;  // offset, nodes
     std::vector<SeamSrcDev> srcs;
-    std::vector<int> src_offered;  // offered nodes per source entry
     size_t in_nodes = 0;
     int total_sources = 0;
     for (int i = 0; i < n; ++i) total_sources = std::max(total_sources, reqs[i].first_source + reqs[i].n_sources);
@@ -784,10 +946,16 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
     if (r) return r;
     auto* h_in = static_cast<lvn_seam_node*>(st.h_in.p);
     auto* d_in = static_cast<lvn_seam_node*>(st.in.p);
-    for (auto& kv : uploaded)
-        std::memcpy(h_in + kv.second.first, kv.first, kv.second.second * sizeof(lvn_seam_node));
+    {
+        std::vector<std::pair<const lvn_seam_node*, std::pair<uint64_t, size_t>>> lists(uploaded.begin(),
+                                                                                         uploaded.end());
+        parallel_ranges((int)lists.size(), in_nodes * sizeof(lvn_seam_node), [&](int lo, int hi) {
+            for (int k = lo; k < hi; ++k)
+                std::memcpy(h_in + lists[k].second.first, lists[k].first,
+                            lists[k].second.second * sizeof(lvn_seam_node));
+        });
+    }
     srcs.resize(std::max(total_sources, 1));
-    src_offered.assign(std::max(total_sources, 1), 0);
     for (int s = 0; s < total_sources; ++s) {
         const lvn_seam_source& q = sources[s];
         SeamSrcDev& d = srcs[s];
@@ -804,7 +972,6 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
             d.nodes = reinterpret_cast<const lvn_seam_node*>(m.d + 48 * (size_t)m.V + 12 * (size_t)m.T);
             d.count = m.S;
         }
-        src_offered[s] = d.count;
     }
     if (in_nodes)
         LVN_HIP(hipMemcpyAsync(d_in, h_in, in_nodes * sizeof(lvn_seam_node), hipMemcpyHostToDevice, c->stream));
@@ -813,29 +980,56 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
     std::memcpy(st.h_srcs.p, srcs.data(), srcs.size() * sizeof(SeamSrcDev));
     LVN_HIP(hipMemcpyAsync(st.srcs.p, st.h_srcs.p, srcs.size() * sizeof(SeamSrcDev), hipMemcpyHostToDevice, c->stream));
 
+    // the selected nodes of each seam (one counting pass) size its scratch
+    std::vector<SeamJobDev> base(n);
+    for (int i = 0; i < n; ++i) {
+        const lvn_seam_req& q = reqs[i];
+        SeamJobDev& J = base[i];
+        std::memset(&J, 0, sizeof J);
+        for (int a = 0; a < 3; ++a) J.host_min[a] = q.host_min[a];
+        J.host_size = q.host_size;
+        for (int a = 0; a < 4; ++a) J.colour[a] = q.colour[a];
+        J.src_begin = q.first_source;
+        J.src_count = q.n_sources;
+        J.vpc = c->N;
+    }
+    if ((r = st.jobs.ensure(n * sizeof(SeamJobDev))) || (r = st.h_jobs.ensure(n * sizeof(SeamJobDev))) ||
+        (r = st.counts.ensure(n * sizeof(int))) || (r = st.h_counts.ensure(n * sizeof(int))))
+        return r;
+    std::memcpy(st.h_jobs.p, base.data(), n * sizeof(SeamJobDev));
+    LVN_HIP(hipMemcpyAsync(st.jobs.p, st.h_jobs.p, n * sizeof(SeamJobDev), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_seam_count, dim3(n), dim3(256), 0, c->stream, static_cast<const SeamJobDev*>(st.jobs.p),
+                       static_cast<const SeamSrcDev*>(st.srcs.p), static_cast<int*>(st.counts.p));
+    LVN_HIP(hipGetLastError());
+    LVN_HIP(hipMemcpyAsync(st.h_counts.p, st.counts.p, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    LVN_HIP(hipStreamSynchronize(c->stream));
+    std::vector<int> selected(static_cast<const int*>(st.h_counts.p), static_cast<const int*>(st.h_counts.p) + n);
+
+    // largest seams first: a seam is one workgroup, the biggest set the pass's length
     std::vector<int> grow(n, 1), todo(n);
     for (int i = 0; i < n; ++i) {
         todo[i] = i;
         outs[i].status = LVN_ERR_OVERFLOW;
         outs[i].nv = outs[i].nt = 0;
     }
+    std::stable_sort(todo.begin(), todo.end(), [&](int a, int b) { return selected[a] > selected[b]; });
+    st.loc.assign(n, SeamLoc{});
+    bool host_out = false;
+    for (int i = 0; i < n; ++i) host_out |= outs[i].v != nullptr || outs[i].tri != nullptr;
     for (int attempt = 0; attempt < 4 && !todo.empty(); ++attempt) {
+        if ((int)st.out_v.size() <= attempt) {
+            st.out_v.emplace_back(new DevBuf());
+            st.out_t.emplace_back(new DevBuf());
+        }
+        DevBuf& ov = *st.out_v[attempt];
+        DevBuf& ot = *st.out_t[attempt];
         const int m = (int)todo.size();
         std::vector<SeamJobDev> jobs(m);
         uint64_t scratch = 0, v_off = 0, t_off = 0;
         for (int k = 0; k < m; ++k) {
-            const lvn_seam_req& q = reqs[todo[k]];
             SeamJobDev& J = jobs[k];
-            std::memset(&J, 0, sizeof J);
-            for (int a = 0; a < 3; ++a) J.host_min[a] = q.host_min[a];
-            J.host_size = q.host_size;
-            for (int a = 0; a < 4; ++a) J.colour[a] = q.colour[a];
-            J.src_begin = q.first_source;
-            J.src_count = q.n_sources;
-            J.vpc = c->N;
-            int offered = 0;
-            for (int s = 0; s < q.n_sources; ++s) offered += src_offered[q.first_source + s];
-            seam_caps(J, offered, grow[todo[k]]);
+            J = base[todo[k]];
+            seam_caps(J, selected[todo[k]], grow[todo[k]]);
             J.scratch = scratch;
             J.v_off = v_off;
             J.t_off = t_off;
@@ -844,8 +1038,8 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
             t_off += 2 * (uint64_t)J.list_cap;
         }
         if ((r = st.jobs.ensure(m * sizeof(SeamJobDev))) || (r = st.h_jobs.ensure(m * sizeof(SeamJobDev))) ||
-            (r = st.scratch.ensure(scratch)) || (r = st.out_v.ensure(v_off * sizeof(lvn_vertex))) ||
-            (r = st.out_t.ensure(t_off * 12)) || (r = st.res.ensure(m * sizeof(SeamResDev))) ||
+            (r = st.scratch.ensure(scratch)) || (r = ov.ensure(std::max<uint64_t>(v_off, 1) * sizeof(lvn_vertex))) ||
+            (r = ot.ensure(std::max<uint64_t>(t_off, 1) * 12)) || (r = st.res.ensure(m * sizeof(SeamResDev))) ||
             (r = st.h_res.ensure(m * sizeof(SeamResDev) + (2 * (size_t)m + 2) * 8)) ||
             (r = st.offs.ensure((2 * (size_t)m + 2) * 8)))
             return r;
@@ -856,7 +1050,7 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
         auto* d_offs = static_cast<uint64_t*>(st.offs.p);
         hipLaunchKernelGGL(k_seam, dim3(m), dim3(kSeamThreads), 0, c->stream, d_jobs,
                            static_cast<const SeamSrcDev*>(st.srcs.p), static_cast<char*>(st.scratch.p),
-                           static_cast<lvn_vertex*>(st.out_v.p), static_cast<int*>(st.out_t.p), d_res);
+                           static_cast<lvn_vertex*>(ov.p), static_cast<int*>(ot.p), d_res);
         LVN_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_seam_offsets, dim3(1), dim3(1024), 0, c->stream, d_res, m, d_offs);
         LVN_HIP(hipGetLastError());
@@ -868,18 +1062,21 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
         const auto* rs = reinterpret_cast<const SeamResDev*>(h_res);
         const auto* offs = reinterpret_cast<const uint64_t*>(h_res + m * sizeof(SeamResDev));
         const uint64_t tv = offs[2 * m], tt = offs[2 * m + 1];
-        if ((r = st.pv.ensure(std::max<uint64_t>(tv, 1) * sizeof(lvn_vertex))) ||
-            (r = st.pt.ensure(std::max<uint64_t>(tt, 1) * 12)) ||
-            (r = st.h_pv.ensure(std::max<uint64_t>(tv, 1) * sizeof(lvn_vertex))) ||
-            (r = st.h_pt.ensure(std::max<uint64_t>(tt, 1) * 12)))
-            return r;
-        hipLaunchKernelGGL(k_seam_pack, dim3(m), dim3(256), 0, c->stream, d_jobs, d_res, d_offs,
-                           static_cast<const lvn_vertex*>(st.out_v.p), static_cast<const int*>(st.out_t.p),
-                           static_cast<lvn_vertex*>(st.pv.p), static_cast<int*>(st.pt.p));
-        LVN_HIP(hipGetLastError());
-        if (tv) LVN_HIP(hipMemcpyAsync(st.h_pv.p, st.pv.p, tv * sizeof(lvn_vertex), hipMemcpyDeviceToHost, c->stream));
-        if (tt) LVN_HIP(hipMemcpyAsync(st.h_pt.p, st.pt.p, tt * 12, hipMemcpyDeviceToHost, c->stream));
-        LVN_HIP(hipStreamSynchronize(c->stream));
+        if (host_out) {  // pack every seam's mesh back to back and read it once
+            if ((r = st.pv.ensure(std::max<uint64_t>(tv, 1) * sizeof(lvn_vertex))) ||
+                (r = st.pt.ensure(std::max<uint64_t>(tt, 1) * 12)) ||
+                (r = st.h_pv.ensure(std::max<uint64_t>(tv, 1) * sizeof(lvn_vertex))) ||
+                (r = st.h_pt.ensure(std::max<uint64_t>(tt, 1) * 12)))
+                return r;
+            hipLaunchKernelGGL(k_seam_pack, dim3(m), dim3(256), 0, c->stream, d_jobs, d_res, d_offs,
+                               static_cast<const lvn_vertex*>(ov.p), static_cast<const int*>(ot.p),
+                               static_cast<lvn_vertex*>(st.pv.p), static_cast<int*>(st.pt.p));
+            LVN_HIP(hipGetLastError());
+            if (tv)
+                LVN_HIP(hipMemcpyAsync(st.h_pv.p, st.pv.p, tv * sizeof(lvn_vertex), hipMemcpyDeviceToHost, c->stream));
+            if (tt) LVN_HIP(hipMemcpyAsync(st.h_pt.p, st.pt.p, tt * 12, hipMemcpyDeviceToHost, c->stream));
+            LVN_HIP(hipStreamSynchronize(c->stream));
+        }
         std::vector<int> again;
         for (int k = 0; k < m; ++k) {
             const int i = todo[k];
@@ -893,15 +1090,26 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
             o.nv = rs[k].nv;
             o.nt = rs[k].nt;
             if (o.status) continue;
-            if (o.nv > o.v_cap || o.nt > o.tri_cap) {
-                o.status = LVN_ERR_CAPACITY;
-                continue;
-            }
-            if (o.nv && o.v)
-                std::memcpy(o.v, static_cast<const lvn_vertex*>(st.h_pv.p) + offs[2 * k], o.nv * sizeof(lvn_vertex));
-            if (o.nt && o.tri)
-                std::memcpy(o.tri, static_cast<const int*>(st.h_pt.p) + 3 * offs[2 * k + 1], (size_t)o.nt * 12);
+            SeamLoc& l = st.loc[i];
+            l.attempt = attempt;
+            l.nv = o.nv;
+            l.nt = o.nt;
+            l.v_off = jobs[k].v_off;
+            l.t_off = jobs[k].t_off;
+            if ((o.v && o.nv > o.v_cap) || (o.tri && o.nt > o.tri_cap)) o.status = LVN_ERR_CAPACITY;
         }
+        if (host_out)
+            parallel_ranges(m, (tv * sizeof(lvn_vertex) + tt * 12), [&](int lo, int hi) {
+                for (int k = lo; k < hi; ++k) {
+                    const lvn_seam_out& o = outs[todo[k]];
+                    if (rs[k].status != LVN_OK || o.status != LVN_OK) continue;
+                    if (o.nv && o.v)
+                        std::memcpy(o.v, static_cast<const lvn_vertex*>(st.h_pv.p) + offs[2 * k],
+                                    o.nv * sizeof(lvn_vertex));
+                    if (o.nt && o.tri)
+                        std::memcpy(o.tri, static_cast<const int*>(st.h_pt.p) + 3 * offs[2 * k + 1], (size_t)o.nt * 12);
+                }
+            });
         todo.swap(again);
     }
     int first = LVN_OK;
@@ -941,6 +1149,20 @@ int lvn_generate_seam_meshes(lvn_ctx* ctx, int32_t n, const lvn_seam_req* reqs, 
     if (r) return r;
     if (n == 0) return LVN_OK;
     return run_seams(reinterpret_cast<Ctx*>(ctx), n, reqs, sources, outs);
+}
+
+int lvn_seam_device_mesh(lvn_ctx* ctx, int32_t i, const lvn_vertex** v, const int32_t** tri, int32_t* nv,
+                         int32_t* nt) {
+    if (!ctx || !v || !tri || !nv || !nt) return LVN_ERR_INVALID;
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c->seams || i < 0 || i >= (int)c->seams->loc.size()) return LVN_ERR_INVALID;
+    const SeamLoc& l = c->seams->loc[i];
+    if (l.attempt < 0) return LVN_ERR_STATE;
+    *v = static_cast<const lvn_vertex*>(c->seams->out_v[l.attempt]->p) + l.v_off;
+    *tri = static_cast<const int32_t*>(c->seams->out_t[l.attempt]->p) + 3 * l.t_off;
+    *nv = l.nv;
+    *nt = l.nt;
+    return LVN_OK;
 }
 
 int lvn_generate_seam_mesh(lvn_ctx* ctx, const int32_t host_min[3], int32_t host_size,

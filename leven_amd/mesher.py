@@ -26,6 +26,7 @@ EXPORTS = [
     "lvn_batch_get_tuning", "lvn_batch_set_tuning", "lvn_batch_autotune",
     "lvn_server_create", "lvn_server_destroy", "lvn_server_submit", "lvn_server_poll", "lvn_server_drain",
     "lvn_server_get_mesh", "lvn_server_stats", "lvn_generate_seam_mesh", "lvn_generate_seam_meshes",
+    "lvn_seam_device_mesh",
 ]
 
 _lib = None
@@ -87,6 +88,7 @@ def lib():
             "lvn_generate_seam_mesh": (i32, [vp, vp, i32, vp, i32, vp, vp, i32, C.POINTER(i32), vp, i32,
                                              C.POINTER(i32)]),
             "lvn_generate_seam_meshes": (i32, [vp, i32, vp, vp, vp]),
+            "lvn_seam_device_mesh": (i32, [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(i32), C.POINTER(i32)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -133,6 +135,20 @@ def perm_table():
     out = np.zeros(65536, dtype=np.uint8)
     check(lib().lvn_get_perm_table(vptr(out)), "lvn_get_perm_table")
     return out
+
+
+def _seam_request(hosts, sources_per_host):
+    """ctypes SeamReq and SeamSource arrays (plus the flat source list that keeps the node
+    arrays alive) for hosts = [(min, size, colour)] and their source dicts."""
+    flat, reqs = [], (SeamReq * max(1, len(hosts)))()
+    for i, ((mn, size, colour), srcs) in enumerate(zip(hosts, sources_per_host)):
+        reqs[i].host_min[:] = [int(v) for v in mn]
+        reqs[i].host_size = int(size)
+        reqs[i].colour[:] = [float(colour[0]), float(colour[1]), float(colour[2]), 0.0]
+        reqs[i].first_source = len(flat)
+        reqs[i].n_sources = len(srcs)
+        flat.extend(srcs)
+    return reqs, seam_sources(flat), flat
 
 
 class Context:
@@ -199,15 +215,7 @@ class Context:
     def seam_meshes(self, hosts, sources_per_host, v_caps=None, tri_caps=None):
         """Many seams in one call: hosts = [(min, size, colour)], sources_per_host = [[source dicts]].
         Returns [(vertices, triangles)]; raises LvnError on the first failing seam."""
-        flat, reqs = [], (SeamReq * max(1, len(hosts)))()
-        for i, ((mn, size, colour), srcs) in enumerate(zip(hosts, sources_per_host)):
-            reqs[i].host_min[:] = [int(v) for v in mn]
-            reqs[i].host_size = int(size)
-            reqs[i].colour[:] = [float(colour[0]), float(colour[1]), float(colour[2]), 0.0]
-            reqs[i].first_source = len(flat)
-            reqs[i].n_sources = len(srcs)
-            flat.extend(srcs)
-        src = seam_sources(flat)
+        reqs, src, flat = _seam_request(hosts, sources_per_host)
         n = len(hosts)
         offered = [sum(len(s["nodes"]) if s["nodes"] is not None else 32768 for s in srcs)
                    for srcs in sources_per_host]
@@ -229,6 +237,23 @@ class Context:
             tc = [max(tc[i], outs[i].nt) for i in range(n)]
         check(rc, "lvn_generate_seam_meshes")
         return [(bufs[i][0][:outs[i].nv].copy(), bufs[i][1][:outs[i].nt].copy()) for i in range(n)]
+
+    def seam_meshes_on_device(self, hosts, sources_per_host):
+        """Stitch many seams and leave the meshes in device memory.  Returns [(nv, nt)];
+        seam_device_mesh(i) gives their device pointers until the next seam call."""
+        reqs, src, flat = _seam_request(hosts, sources_per_host)
+        n = len(hosts)
+        outs = (SeamOut * max(1, n))()
+        check(lib().lvn_generate_seam_meshes(self._h, n, C.cast(reqs, C.c_void_p), C.cast(src, C.c_void_p),
+                                             C.cast(outs, C.c_void_p)), "lvn_generate_seam_meshes")
+        return [(outs[i].nv, outs[i].nt) for i in range(n)]
+
+    def seam_device_mesh(self, i):
+        """(vertex device pointer, triangle device pointer, nv, nt) of seam i of the last call."""
+        v, t, nv, nt = C.c_void_p(), C.c_void_p(), C.c_int32(), C.c_int32()
+        check(lib().lvn_seam_device_mesh(self._h, int(i), C.byref(v), C.byref(t), C.byref(nv), C.byref(nt)),
+              "lvn_seam_device_mesh")
+        return v.value, t.value, nv.value, nt.value
 
     def debug_stages(self, min_xyz, size, ops=None, edge_cap=None, voxel_cap=None):
         """Every stage of one fresh chunk (density ... vertex positions)."""

@@ -110,8 +110,26 @@ def test_cached_sources(ctx, equal_lod):
         assert a[i][0].tobytes() == b[i][0].tobytes() and (a[i][1] == b[i][1]).all(), f"seam {i}"
         rc, v, t = py_oracle.seam_mesh(hosts[i][0], hosts[i][1], host_srcs[i], colour=hosts[i][2])
         assert v.tobytes() == b[i][0].tobytes() and (t == b[i][1]).all(), f"seam {i} vs oracle"
+    # device-resident results: nothing copied out, the meshes read back from device memory
+    counts = ctx.seam_meshes_on_device(hosts, cached)
+    for i in range(len(hosts)):
+        pv, pt, nv, nt = ctx.seam_device_mesh(i)
+        assert (nv, nt) == counts[i] == (len(b[i][0]), len(b[i][1])), f"seam {i} counts"
+        assert _d2h(pv, 48 * nv) == b[i][0].tobytes(), f"seam {i} device vertices"
+        assert _d2h(pt, 12 * nt) == b[i][1].tobytes(), f"seam {i} device triangles"
     for mn, size, _ in nodes:
         ctx.free_chunk(mn, size)
+
+
+def _d2h(ptr, nbytes):
+    """Copy device memory to the host with the HIP runtime (hipMemcpyDeviceToHost)."""
+    import ctypes as C
+    out = np.zeros(max(1, nbytes), dtype=np.uint8)
+    if nbytes:
+        hip = C.CDLL("libamdhip64.so")
+        hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        assert hip.hipMemcpy(out.ctypes.data, ptr, nbytes, 2) == 0
+    return out[:nbytes].tobytes()
 
 
 def test_seam_errors(ctx, equal_lod):
