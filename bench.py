@@ -170,22 +170,22 @@ def clipmap_leg(args):
     # active clipmap nodes (a mesh or seam nodes, src/clipmap.cpp:468) with their seam node lists
     nodes = [(tuple(int(v) for v in wl["mins"][i]), int(wl["sizes"][i]), b.get_mesh(i)[2])
              for i in range(n) if counts[i, 2] > 0 or counts[i, 3] > 0] if not args.no_seams else None
-    b.close()
-    return nodes, {"chunks": n, "chunks_per_lod": np.bincount(lods, minlength=6).tolist(),
+    return b, nodes, {"chunks": n, "chunks_per_lod": np.bincount(lods, minlength=6).tolist(),
             "steps": steps, "chunks_per_s": round(n * steps / elapsed, 1), "ms_per_pass": round(elapsed / steps * 1e3, 3),
             "nonempty_chunks": int((counts[:, 2] > 0).sum()), "max_active_voxels": int(counts[:, 1].max()),
             "max_edges": int(counts[:, 0].max()), "triangles": int(counts[:, 2].sum()),
             "stage_ms": stage_ms, "tuning": tuning}
 
 
-def seam_leg(nodes, args):
+def seam_leg(batch, nodes, args):
     """Seam meshes of every active clipmap node (GenerateClipmapSeamMesh, src/clipmap.cpp:573-611):
-    neighbour seam node lists gathered as the reference does, all seams in one
-    lvn_generate_seam_meshes call (host lists in, host meshes out), against the oracle on
-    the host's cores."""
+    neighbour seam node lists gathered as the reference does, all seams in one call, in two
+    forms: drop-in (host lists in, host meshes out, lvn_generate_seam_meshes) and
+    device-resident (lists read from the clipmap batch's slots, meshes left in HBM,
+    lvn_batch_seam_meshes); against the oracle on the host's cores."""
     import ctypes as C
 
-    from leven_amd.abi import SEAM_DTYPE, VERTEX_DTYPE, SeamOut, SeamReq, seam_sources, vptr
+    from leven_amd.abi import VERTEX_DTYPE, SeamOut, SeamReq, seam_sources
     per_host = workloads.seam_sources(nodes)
     flat, reqs = [], (SeamReq * len(nodes))()
     for i, ((mn, size, _), srcs) in enumerate(zip(nodes, per_host)):
@@ -221,24 +221,14 @@ def seam_leg(nodes, args):
     elapsed = time.perf_counter() - t0
     nv = sum(o.nv for o in outs)
     nt = sum(o.nt for o in outs)
-    # the device-resident form: the context meshes (and caches) every node, then stitches
-    # from the cached lists in device memory and leaves the seams there
-    n3 = 64 ** 3
-    cv = np.zeros(n3, dtype=VERTEX_DTYPE)
-    ct = np.zeros((6 * n3, 3), dtype=np.int32)
-    cs = np.zeros(n3, dtype=SEAM_DTYPE)
-    a, b_, c_ = C.c_int32(), C.c_int32(), C.c_int32()
-    for mn, size, _ in nodes:
-        mesher.check(L.lvn_generate_chunk_mesh(ctx._h, vptr(np.asarray(mn, dtype=np.int32)), size, vptr(cv), n3,
-                                               C.byref(a), vptr(ct), 6 * n3, C.byref(b_), vptr(cs), n3, C.byref(c_)),
-                     "lvn_generate_chunk_mesh")
+    # the device-resident form: the seam lists of the clipmap batch's slots, meshes left in HBM
     dev_src = seam_sources([dict(d, nodes=None) for d in flat])
     dev_outs = (SeamOut * len(nodes))()
 
     def run_dev():
-        mesher.check(L.lvn_generate_seam_meshes(ctx._h, len(nodes), C.cast(reqs, C.c_void_p),
-                                                C.cast(dev_src, C.c_void_p), C.cast(dev_outs, C.c_void_p)),
-                     "lvn_generate_seam_meshes")
+        mesher.check(L.lvn_batch_seam_meshes(batch._h, len(nodes), C.cast(reqs, C.c_void_p),
+                                             C.cast(dev_src, C.c_void_p), C.cast(dev_outs, C.c_void_p)),
+                     "lvn_batch_seam_meshes")
 
     run_dev()
     assert sum(o.nv for o in dev_outs) == nv and sum(o.nt for o in dev_outs) == nt
@@ -253,7 +243,7 @@ def seam_leg(nodes, args):
            "note": "drop-in form: host seam node lists in, host meshes out (one upload, one read-back per pass)",
            "device_resident": {"seams_per_s": round(len(nodes) * steps / dev_elapsed, 1),
                                "ms_per_pass": round(dev_elapsed / steps * 1e3, 3),
-                               "note": "seam lists read from the context's cached chunks, meshes left in HBM"}}
+                               "note": "seam lists read from the clipmap batch's slots, meshes left in HBM"}}
     if not args.no_cpu:
         from oracle import py_oracle
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -496,9 +486,10 @@ def main():
     if not args.no_precision:
         out["precision"] = precision_leg(args)
     if not args.no_clipmap:
-        nodes, out["clipmap"] = clipmap_leg(args)
+        cb, nodes, out["clipmap"] = clipmap_leg(args)
         if nodes:
-            out["seams"] = seam_leg(nodes, args)
+            out["seams"] = seam_leg(cb, nodes, args)
+        cb.close()
     if args.serve_seconds > 0:
         out["serving"] = serve(args)
         out["p99_chunk_latency_ms"] = out["serving"]["latency_ms"]["p99"]

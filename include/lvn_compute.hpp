@@ -110,6 +110,42 @@ public:
         return LVN_OK;
     }
 
+    // The body of GenerateClipmapSeamMesh (src/clipmap.cpp:573-611) after its neighbour
+    // gathering: the seam mesh of the node at (min, size) from the gathered lists (one
+    // seamSource per list SelectSeamNodes would have filtered, src/clipmap.cpp:594-595).
+    // Writes into the caller's MeshBuffer; LVN_ERR_CAPACITY when the seam does not fit.
+    template <class IVec3, class Vec3>
+    int generateSeamMesh(const IVec3& min, int size, const std::vector<lvn_seam_source>& sources, const Vec3& colour,
+                         MeshBufferT* meshBuffer) {
+        const int32_t m[3] = {min.x, min.y, min.z};
+        const float c[3] = {colour.x, colour.y, colour.z};
+        int32_t nv = 0, nt = 0;
+        const int r = lvn_generate_seam_mesh(ctx_, m, size, sources.data(), (int32_t)sources.size(), c,
+                                             reinterpret_cast<lvn_vertex*>(&meshBuffer->vertices[0]), kMaxVertices,
+                                             &nv, reinterpret_cast<int32_t*>(&meshBuffer->triangles[0]), kMaxTriangles,
+                                             &nt);
+        if (r != LVN_OK) return r;
+        meshBuffer->numVertices = nv;
+        meshBuffer->numTriangles = nt;
+        return LVN_OK;
+    }
+
+    // A clipmap node's seam node list (generateChunkMesh's output) found under candidate
+    // neighbour `neighbour` (CHILD_MIN_OFFSETS index) of the host.
+    template <class IVec3>
+    static lvn_seam_source seamSource(const IVec3& min, int size, int neighbour, const std::vector<SeamNodeT>& nodes) {
+        static_assert(sizeof(SeamNodeT) == sizeof(lvn_seam_node), "SeamNodeInfo must be 48 bytes");
+        lvn_seam_source s;
+        s.min[0] = min.x;
+        s.min[1] = min.y;
+        s.min[2] = min.z;
+        s.size = size;
+        s.neighbour = neighbour;
+        s.count = (int32_t)nodes.size();
+        s.nodes = reinterpret_cast<const lvn_seam_node*>(nodes.data());
+        return s;
+    }
+
     lvn_ctx* handle() const { return ctx_; }
 
 private:

@@ -269,6 +269,17 @@ int  lvn_batch_get_mesh(lvn_batch* b, int32_t i, lvn_vertex* v, int32_t v_cap, i
  * evaluations in the edge-crossing stage, work[5] density samples, work[6] edges. */
 int  lvn_batch_work(lvn_batch* b, uint64_t* work);
 
+/* The seams of a batch's chunks, stitched where they lie: as lvn_generate_seam_meshes,
+ * but a source with count = -1 and no list is the batch's staged chunk at (min, size),
+ * read from its slot after the last lvn_batch_run (LVN_ERR_STATE when no staged chunk
+ * matches, LVN_ERR_OVERFLOW when that chunk overflowed).  Meshing a clipmap selection and
+ * its seams is then two launches over device memory.  lvn_batch_seam_device_mesh gives
+ * the device results of seams passed without host buffers.  No reference counterpart. */
+int lvn_batch_seam_meshes(lvn_batch* b, int32_t n, const lvn_seam_req* reqs,
+                          const lvn_seam_source* sources, lvn_seam_out* outs);
+int lvn_batch_seam_device_mesh(lvn_batch* b, int32_t i, const lvn_vertex** v, const int32_t** tri,
+                               int32_t* nv, int32_t* nt);
+
 /* ---- serving form: continuous batching with device-resident chunk state ---- */
 
 #define LVN_REQ_MESH  0   /* mesh the chunk (min, size) */

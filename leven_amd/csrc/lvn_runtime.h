@@ -132,6 +132,7 @@ struct Batch {
     uint64_t graph_gen = 0;
     unsigned long long* d_work = nullptr;
     lvn_tuning tuning = {0, 2, 8192, 8192, 0, {0, 0, 0}};  // ysplit, cross_lanes 0 = by batch size
+    std::map<Key, int> slot_of;      // staged chunk (min, size) -> slot
     ~Batch();
 };
 

@@ -601,6 +601,8 @@ static int stage_batch(Batch* b, int n, const lvn_chunk_req* reqs) {
     LVN_HIP(hipMemcpyAsync(c->d_chunks, c->h_chunks, sizeof(ChunkDesc) * n, hipMemcpyHostToDevice, c->stream));
     LVN_HIP(hipStreamSynchronize(c->stream));
     b->n = n;
+    b->slot_of.clear();
+    for (int i = 0; i < n; ++i) b->slot_of[make_key(reqs[i].min, reqs[i].size)] = i;
     return LVN_OK;
 }
 

@@ -919,7 +919,34 @@ void seam_caps(SeamJobDev& J, int selected, int grow) {
     J.hash_cap = h;
 }
 
-int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* sources, lvn_seam_out* outs) {
+// Device location and length of a source given as count = -1 without a list: a cached
+// chunk of the context, or a staged chunk of a batch.
+using SourceResolver = int (*)(void* owner, const lvn_seam_source& q, SeamSrcDev& d);
+
+int resolve_cached(void* owner, const lvn_seam_source& q, SeamSrcDev& d) {
+    Ctx* c = static_cast<Ctx*>(owner);
+    auto it = c->meshes.find(make_key(q.min, q.size));
+    if (it == c->meshes.end()) return LVN_ERR_STATE;
+    const MeshEntry& m = it->second;
+    d.nodes = reinterpret_cast<const lvn_seam_node*>(m.d + 48 * (size_t)m.V + 12 * (size_t)m.T);
+    d.count = m.S;
+    return LVN_OK;
+}
+
+int resolve_batch(void* owner, const lvn_seam_source& q, SeamSrcDev& d) {
+    Batch* b = static_cast<Batch*>(owner);
+    auto it = b->slot_of.find(make_key(q.min, q.size));
+    if (it == b->slot_of.end()) return LVN_ERR_STATE;
+    const Ctx& c = b->ctx;
+    const Counters& k = c.h_counters[it->second];  // read back by the caller of run_seams
+    if (k.status) return LVN_ERR_OVERFLOW;
+    d.nodes = reinterpret_cast<const lvn_seam_node*>(c.d_arena + (uint64_t)it->second * c.L.slot_bytes + c.L.seams);
+    d.count = k.seams;
+    return LVN_OK;
+}
+
+int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* sources, lvn_seam_out* outs,
+              SourceResolver resolve, void* owner) {
     if (!c->seams) c->seams = new SeamState();
     SeamState& st = *c->seams;
     // upload every distinct host list once; cached chunks are read where they lie
@@ -965,12 +992,9 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
         if (q.nodes || q.count >= 0) {
             d.nodes = q.nodes && q.count > 0 ? d_in + uploaded[q.nodes].first : nullptr;
             d.count = q.nodes ? q.count : 0;
-        } else {  // count < 0: the context's cached chunk (lvn_generate_chunk_mesh) supplies its seam nodes
-            auto it = c->meshes.find(make_key(q.min, q.size));
-            if (it == c->meshes.end()) return LVN_ERR_STATE;
-            const MeshEntry& m = it->second;
-            d.nodes = reinterpret_cast<const lvn_seam_node*>(m.d + 48 * (size_t)m.V + 12 * (size_t)m.T);
-            d.count = m.S;
+        } else {  // count < 0: seam nodes already in device memory
+            const int rr = resolve(owner, q, d);
+            if (rr) return rr;
         }
     }
     if (in_nodes)
@@ -1148,7 +1172,31 @@ int lvn_generate_seam_meshes(lvn_ctx* ctx, int32_t n, const lvn_seam_req* reqs, 
     const int r = check_seam_args(n, reqs, sources);
     if (r) return r;
     if (n == 0) return LVN_OK;
-    return run_seams(reinterpret_cast<Ctx*>(ctx), n, reqs, sources, outs);
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    return run_seams(c, n, reqs, sources, outs, resolve_cached, c);
+}
+
+int lvn_batch_seam_meshes(lvn_batch* bh, int32_t n, const lvn_seam_req* reqs, const lvn_seam_source* sources,
+                          lvn_seam_out* outs) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (!b || (n > 0 && !outs)) return LVN_ERR_INVALID;
+    if (!g_state.init) return LVN_ERR_STATE;
+    const int r = check_seam_args(n, reqs, sources);
+    if (r) return r;
+    if (n == 0) return LVN_OK;
+    Ctx* c = &b->ctx;
+    if (b->n > 0) {  // seam counts of the staged chunks, after the batch's last run
+        LVN_HIP(hipMemcpyAsync(c->h_counters, c->d_counters, sizeof(Counters) * b->n, hipMemcpyDeviceToHost, c->stream));
+        LVN_HIP(hipStreamSynchronize(c->stream));
+    }
+    return run_seams(c, n, reqs, sources, outs, resolve_batch, b);
+}
+
+int lvn_batch_seam_device_mesh(lvn_batch* bh, int32_t i, const lvn_vertex** v, const int32_t** tri, int32_t* nv,
+                               int32_t* nt) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (!b) return LVN_ERR_INVALID;
+    return lvn_seam_device_mesh(reinterpret_cast<lvn_ctx*>(&b->ctx), i, v, tri, nv, nt);
 }
 
 int lvn_seam_device_mesh(lvn_ctx* ctx, int32_t i, const lvn_vertex** v, const int32_t** tri, int32_t* nv,

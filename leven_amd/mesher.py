@@ -26,7 +26,7 @@ EXPORTS = [
     "lvn_batch_get_tuning", "lvn_batch_set_tuning", "lvn_batch_autotune",
     "lvn_server_create", "lvn_server_destroy", "lvn_server_submit", "lvn_server_poll", "lvn_server_drain",
     "lvn_server_get_mesh", "lvn_server_stats", "lvn_generate_seam_mesh", "lvn_generate_seam_meshes",
-    "lvn_seam_device_mesh",
+    "lvn_seam_device_mesh", "lvn_batch_seam_meshes", "lvn_batch_seam_device_mesh",
 ]
 
 _lib = None
@@ -89,6 +89,9 @@ def lib():
                                              C.POINTER(i32)]),
             "lvn_generate_seam_meshes": (i32, [vp, i32, vp, vp, vp]),
             "lvn_seam_device_mesh": (i32, [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(i32), C.POINTER(i32)]),
+            "lvn_batch_seam_meshes": (i32, [vp, i32, vp, vp, vp]),
+            "lvn_batch_seam_device_mesh": (i32, [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(i32),
+                                                 C.POINTER(i32)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -149,6 +152,43 @@ def _seam_request(hosts, sources_per_host):
         reqs[i].n_sources = len(srcs)
         flat.extend(srcs)
     return reqs, seam_sources(flat), flat
+
+
+def _seam_meshes(fn, h, hosts, sources_per_host, v_caps=None, tri_caps=None):
+    reqs, src, flat = _seam_request(hosts, sources_per_host)
+    n = len(hosts)
+    offered = [sum(len(s["nodes"]) if s["nodes"] is not None else 32768 for s in srcs) for srcs in sources_per_host]
+    vc = list(v_caps) if v_caps is not None else offered
+    tc = list(tri_caps) if tri_caps is not None else [4 * o + 64 for o in offered]
+    for _ in range(2):
+        bufs, outs = [], (SeamOut * max(1, n))()
+        for i in range(n):
+            v = np.zeros(max(1, vc[i]), dtype=VERTEX_DTYPE)
+            t = np.zeros((max(1, tc[i]), 3), dtype=np.int32)
+            bufs.append((v, t))
+            outs[i].v, outs[i].tri = v.ctypes.data, t.ctypes.data
+            outs[i].v_cap, outs[i].tri_cap = vc[i], tc[i]
+        rc = fn(h, n, C.cast(reqs, C.c_void_p), C.cast(src, C.c_void_p), C.cast(outs, C.c_void_p))
+        if rc != LVN_ERR_CAPACITY:
+            break
+        vc = [max(vc[i], outs[i].nv) for i in range(n)]
+        tc = [max(tc[i], outs[i].nt) for i in range(n)]
+    check(rc, "seam meshes")
+    return [(bufs[i][0][:outs[i].nv].copy(), bufs[i][1][:outs[i].nt].copy()) for i in range(n)]
+
+
+def _seam_meshes_on_device(fn, h, hosts, sources_per_host):
+    reqs, src, flat = _seam_request(hosts, sources_per_host)
+    n = len(hosts)
+    outs = (SeamOut * max(1, n))()
+    check(fn(h, n, C.cast(reqs, C.c_void_p), C.cast(src, C.c_void_p), C.cast(outs, C.c_void_p)), "seam meshes")
+    return [(outs[i].nv, outs[i].nt) for i in range(n)]
+
+
+def _seam_device_mesh(fn, h, i):
+    v, t, nv, nt = C.c_void_p(), C.c_void_p(), C.c_int32(), C.c_int32()
+    check(fn(h, int(i), C.byref(v), C.byref(t), C.byref(nv), C.byref(nt)), "seam device mesh")
+    return v.value, t.value, nv.value, nt.value
 
 
 class Context:
@@ -215,45 +255,16 @@ class Context:
     def seam_meshes(self, hosts, sources_per_host, v_caps=None, tri_caps=None):
         """Many seams in one call: hosts = [(min, size, colour)], sources_per_host = [[source dicts]].
         Returns [(vertices, triangles)]; raises LvnError on the first failing seam."""
-        reqs, src, flat = _seam_request(hosts, sources_per_host)
-        n = len(hosts)
-        offered = [sum(len(s["nodes"]) if s["nodes"] is not None else 32768 for s in srcs)
-                   for srcs in sources_per_host]
-        vc = list(v_caps) if v_caps is not None else offered
-        tc = list(tri_caps) if tri_caps is not None else [4 * o + 64 for o in offered]
-        for _ in range(2):
-            bufs, outs = [], (SeamOut * max(1, n))()
-            for i in range(n):
-                v = np.zeros(max(1, vc[i]), dtype=VERTEX_DTYPE)
-                t = np.zeros((max(1, tc[i]), 3), dtype=np.int32)
-                bufs.append((v, t))
-                outs[i].v, outs[i].tri = v.ctypes.data, t.ctypes.data
-                outs[i].v_cap, outs[i].tri_cap = vc[i], tc[i]
-            rc = lib().lvn_generate_seam_meshes(self._h, n, C.cast(reqs, C.c_void_p), C.cast(src, C.c_void_p),
-                                                C.cast(outs, C.c_void_p))
-            if rc != LVN_ERR_CAPACITY:
-                break
-            vc = [max(vc[i], outs[i].nv) for i in range(n)]
-            tc = [max(tc[i], outs[i].nt) for i in range(n)]
-        check(rc, "lvn_generate_seam_meshes")
-        return [(bufs[i][0][:outs[i].nv].copy(), bufs[i][1][:outs[i].nt].copy()) for i in range(n)]
+        return _seam_meshes(lib().lvn_generate_seam_meshes, self._h, hosts, sources_per_host, v_caps, tri_caps)
 
     def seam_meshes_on_device(self, hosts, sources_per_host):
         """Stitch many seams and leave the meshes in device memory.  Returns [(nv, nt)];
         seam_device_mesh(i) gives their device pointers until the next seam call."""
-        reqs, src, flat = _seam_request(hosts, sources_per_host)
-        n = len(hosts)
-        outs = (SeamOut * max(1, n))()
-        check(lib().lvn_generate_seam_meshes(self._h, n, C.cast(reqs, C.c_void_p), C.cast(src, C.c_void_p),
-                                             C.cast(outs, C.c_void_p)), "lvn_generate_seam_meshes")
-        return [(outs[i].nv, outs[i].nt) for i in range(n)]
+        return _seam_meshes_on_device(lib().lvn_generate_seam_meshes, self._h, hosts, sources_per_host)
 
     def seam_device_mesh(self, i):
         """(vertex device pointer, triangle device pointer, nv, nt) of seam i of the last call."""
-        v, t, nv, nt = C.c_void_p(), C.c_void_p(), C.c_int32(), C.c_int32()
-        check(lib().lvn_seam_device_mesh(self._h, int(i), C.byref(v), C.byref(t), C.byref(nv), C.byref(nt)),
-              "lvn_seam_device_mesh")
-        return v.value, t.value, nv.value, nt.value
+        return _seam_device_mesh(lib().lvn_seam_device_mesh, self._h, i)
 
     def debug_stages(self, min_xyz, size, ops=None, edge_cap=None, voxel_cap=None):
         """Every stage of one fresh chunk (density ... vertex positions)."""
@@ -386,6 +397,18 @@ class Batch:
         check(lib().lvn_batch_get_mesh(self._h, int(i), vptr(v), len(v), C.byref(nv), vptr(t), len(t), C.byref(nt),
                                        vptr(s), len(s), C.byref(ns)), "lvn_batch_get_mesh")
         return v[:nv.value].copy(), t[:nt.value].copy(), s[:ns.value].copy()
+
+    def seam_meshes(self, hosts, sources_per_host):
+        """Seams of this batch's chunks (sources with nodes=None are staged chunks, read from
+        their slots).  Returns [(vertices, triangles)]."""
+        return _seam_meshes(lib().lvn_batch_seam_meshes, self._h, hosts, sources_per_host)
+
+    def seam_meshes_on_device(self, hosts, sources_per_host):
+        """As seam_meshes, leaving the meshes in device memory; returns [(nv, nt)]."""
+        return _seam_meshes_on_device(lib().lvn_batch_seam_meshes, self._h, hosts, sources_per_host)
+
+    def seam_device_mesh(self, i):
+        return _seam_device_mesh(lib().lvn_batch_seam_device_mesh, self._h, i)
 
 
 class Server:

@@ -108,6 +108,17 @@ int main() {
     ctx->freeChunkOctree(min, size);
     if (report("applied", ctx->generateChunkMesh(min, size, &mb, seams), mb, seams)) return 1;
 
+    // the seam between this chunk and its +x neighbour (GenerateClipmapSeamMesh,
+    // src/clipmap.cpp:573-611, with the two lists its gathering finds)
+    std::vector<app::SeamNodeInfo> nbSeams;
+    const app::ivec3 nbMin = {min.x + size, min.y, min.z};
+    if (report("neighbour", ctx->generateChunkMesh(nbMin, size, &mb, nbSeams), mb, nbSeams)) return 1;
+    std::vector<lvn_seam_source> sources = {app::MeshGenContext::seamSource(min, size, 0, seams),
+                                            app::MeshGenContext::seamSource(nbMin, size, 4, nbSeams)};
+    const app::vec4 colour = {1.f, 0.5f, 0.25f, 0.f};
+    const std::vector<app::SeamNodeInfo> none;
+    if (report("seam", ctx->generateSeamMesh(min, size, sources, colour, &mb), mb, none)) return 1;
+
     delete ctx;
     lvn::Compute_ClearCSGOperations();                              // src/volume.cpp:149
     lvn::Compute_Shutdown();                                        // src/main.cpp:599

@@ -54,9 +54,18 @@ def test_cpp_dropin_matches_c_abi():
     box["dims"] = [5, 6, 4, 0]
     ctx.apply_csg(box, mn, size)
     ctx.free_chunk(mn, size)
-    want.append(_line("applied", ctx.generate_chunk_mesh(mn, size)))
+    applied = ctx.generate_chunk_mesh(mn, size)
+    want.append(_line("applied", applied))
+    nb_min = (mn[0] + size, mn[1], mn[2])
+    neighbour = ctx.generate_chunk_mesh(nb_min, size)
+    want.append(_line("neighbour", neighbour))
+    v, t = ctx.seam_mesh(mn, size, [dict(min=mn, size=size, neighbour=0, nodes=applied[2]),
+                                    dict(min=nb_min, size=size, neighbour=4, nodes=neighbour[2])],
+                         colour=(1.0, 0.5, 0.25))
+    assert len(t) > 0
+    want.append(_line("seam", (v, t, np.zeros(0, np.uint8))))
     ctx.close()
     mesher.clear_csg_ops()
     assert got == want
     # the edits changed the mesh each time
-    assert len({w.split()[-1] for w in want[1:] if not w.startswith("isChunk")}) == 3
+    assert len({w.split()[-1] for w in want[1:5] if not w.startswith("isChunk")}) == 3

@@ -132,6 +132,33 @@ def _d2h(ptr, nbytes):
     return out[:nbytes].tobytes()
 
 
+def test_batch_sources(ctx, equal_lod):
+    """Seams stitched straight from a batch's slots equal the oracle's, on the host and in
+    device memory."""
+    mins = np.array([mn for mn, _, _ in equal_lod.nodes], dtype=np.int32)
+    sizes = np.array([size for _, size, _ in equal_lod.nodes], dtype=np.int32)
+    b = mesher.Batch(64, len(sizes))
+    b.set_chunks(mins, sizes)
+    b.run()
+    b.wait()
+    srcs = equal_lod.sources()
+    slot_srcs = [[dict(d, nodes=None) for d in s] for s in srcs]
+    hosts = [(mn, size, _colour(i)) for i, (mn, size, _) in enumerate(equal_lod.nodes)]
+    got = b.seam_meshes(hosts, slot_srcs)
+    counts = b.seam_meshes_on_device(hosts, slot_srcs)
+    for i, ((mn, size, colour), s) in enumerate(zip(hosts, srcs)):
+        rc, v, t = py_oracle.seam_mesh(mn, size, s, colour=colour)
+        assert got[i][0].tobytes() == v.tobytes() and (got[i][1] == t).all(), f"seam {i}"
+        pv, pt, nv, nt = b.seam_device_mesh(i)
+        assert (nv, nt) == counts[i] == (len(v), len(t))
+        assert _d2h(pv, 48 * nv) == v.tobytes() and _d2h(pt, 12 * nt) == t.tobytes(), f"seam {i} on device"
+    # a source that is not a staged chunk
+    with pytest.raises(mesher.LvnError) as e:
+        b.seam_meshes([hosts[0]], [[dict(min=(4096, 0, 0), size=256, neighbour=4, nodes=None)]])
+    assert e.value.code == -5
+    b.close()
+
+
 def test_seam_errors(ctx, equal_lod):
     mn, size, _ = equal_lod.nodes[0]
     srcs = equal_lod.sources()[0]
