@@ -185,7 +185,7 @@ def seam_leg(batch, nodes, args):
     lvn_batch_seam_meshes); against the oracle on the host's cores."""
     import ctypes as C
 
-    from leven_amd.abi import VERTEX_DTYPE, SeamOut, SeamReq, seam_sources
+    from leven_amd.abi import VERTEX_DTYPE, SeamOut, SeamReq, seam_sources, vptr
     per_host = workloads.seam_sources(nodes)
     flat, reqs = [], (SeamReq * len(nodes))()
     for i, ((mn, size, _), srcs) in enumerate(zip(nodes, per_host)):
@@ -236,6 +236,21 @@ def seam_leg(batch, nodes, args):
     for _ in range(steps):
         run_dev()
     dev_elapsed = time.perf_counter() - t0
+    # one seam per call (lvn_generate_seam_mesh, the drop-in for one GenerateClipmapSeamMesh)
+    pick = np.linspace(0, len(nodes) - 1, 64).astype(int)
+    lat = []
+    snv, snt = C.c_int32(), C.c_int32()
+    col = np.ones(3, dtype=np.float32)
+    for i in pick:
+        q = reqs[i]
+        hmin = np.array(q.host_min[:], dtype=np.int32)
+        first = C.cast(src, C.c_void_p).value + q.first_source * C.sizeof(src._type_)
+        o = outs[i]
+        t1 = time.perf_counter()
+        mesher.check(L.lvn_generate_seam_mesh(ctx._h, vptr(hmin), q.host_size, C.c_void_p(first), q.n_sources,
+                                              vptr(col), C.c_void_p(o.v), o.v_cap, C.byref(snv), C.c_void_p(o.tri),
+                                              o.tri_cap, C.byref(snt)), "lvn_generate_seam_mesh")
+        lat.append((time.perf_counter() - t1) * 1e3)
     ctx.close()
     out = {"seams": len(nodes), "sources": len(flat), "seam_nodes_offered": int(sum(offered)),
            "vertices": int(nv), "triangles": int(nt), "steps": steps,
@@ -243,7 +258,11 @@ def seam_leg(batch, nodes, args):
            "note": "drop-in form: host seam node lists in, host meshes out (one upload, one read-back per pass)",
            "device_resident": {"seams_per_s": round(len(nodes) * steps / dev_elapsed, 1),
                                "ms_per_pass": round(dev_elapsed / steps * 1e3, 3),
-                               "note": "seam lists read from the clipmap batch's slots, meshes left in HBM"}}
+                               "note": "seam lists read from the clipmap batch's slots, meshes left in HBM"},
+           "single_seam_ms": {"mean": round(float(np.mean(lat)), 3), "p50": round(float(np.percentile(lat, 50)), 3),
+                              "p99": round(float(np.percentile(lat, 99)), 3),
+                              "note": "64 seams spread over the clipmap, one lvn_generate_seam_mesh call each "
+                                      "(host lists in, host mesh out)"}}
     if not args.no_cpu:
         from oracle import py_oracle
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -254,6 +273,7 @@ def seam_leg(batch, nodes, args):
         for k, i in enumerate(pick):
             one[k] = reqs[i]
         wall1, _ = py_oracle.seam_meshes(one, src, threads=1)
+        out["single_seam_ms"]["oracle_one_thread_mean"] = round(wall1 / 64 * 1e3, 3)
         out["cpu_baseline"] = {"value": round(len(nodes) / wall, 1), "unit": "seams/s", "cores": threads,
                                "kind": "port", "single_thread_seams_per_s": round(64 / wall1, 1),
                                "sample": f"all {len(nodes)} seams on {threads} threads, oracle restatement of the "

@@ -112,7 +112,7 @@ struct Call {
 
 constexpr int kSeamThreads = 1024;
 constexpr int kSeamWaves = kSeamThreads / 64;
-constexpr int kCallsPerThread = 4;  // pending calls one thread expands per round tile
+constexpr int kCallsPerThread = 2;  // pending calls one thread expands per round tile (1024 x 2 timed best)
 constexpr uint64_t kEmptyKey = ~0ull;
 constexpr int kMaxRounds = 64;
 
@@ -322,14 +322,22 @@ struct Expander {
         else edge_d<2>(n0, n1, n2, n3);
     }
     // ContourFaceProc, src/octree.cpp:342-418
+    // Child rows are read whether or not the node is internal (a leaf's row is all -1),
+    // so they are in flight together with the node shapes.
     template <int D>
     __device__ __forceinline__ void face_call(const Call& c) {
+        int r0[8], r1[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            r0[k] = child(c.n[0], k);
+            r1[k] = child(c.n[1], k);
+        }
         const bool b0 = internal(c.n[0]), b1 = internal(c.n[1]);
         int ch0[8], ch1[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            ch0[k] = b0 ? child(c.n[0], k) : c.n[0];
-            ch1[k] = b1 ? child(c.n[1], k) : c.n[1];
+            ch0[k] = b0 ? r0[k] : c.n[0];
+            ch1[k] = b1 ? r1[k] : c.n[1];
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) face(ch0[kFaceFaces[D][i][0]], ch1[kFaceFaces[D][i][1]], kFaceFaces[D][i][2]);
@@ -346,6 +354,11 @@ struct Expander {
     // ContourEdgeProc, src/octree.cpp:286-338
     template <int D>
     __device__ __forceinline__ void edge_call(const Call& c) {
+        int r[2][4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) r[i][j] = child(c.n[j], kEdgeEdges[D][i][j]);
         bool br[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) br[j] = internal(c.n[j]);
@@ -353,7 +366,7 @@ struct Expander {
         for (int i = 0; i < 2; ++i) {
             int sub[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) sub[j] = br[j] ? child(c.n[j], kEdgeEdges[D][i][j]) : c.n[j];
+            for (int j = 0; j < 4; ++j) sub[j] = br[j] ? r[i][j] : c.n[j];
             edge(sub[0], sub[1], sub[2], sub[3], kEdgeEdges[D][i][4]);
         }
     }
