@@ -495,8 +495,8 @@ def main():
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": hbm_traffic("k_" + dom, wl["name"]), "flops_per_launch": int(dom_flops),
                      "launch_ms": round(stage_ms[dom], 4),
-                     "note": "FP32 vector-ALU bound (neither HBM nor MFMA): rocprofv3 counts 0.43 VALU "
-                             "instructions per SIMD-cycle against a 0.5 ceiling, 56% of them FP32 arithmetic, "
+                     "note": "FP32 vector-ALU bound (neither HBM nor MFMA): rocprofv3 counts 0.42 VALU "
+                             "instructions per SIMD-cycle against a 0.5 ceiling, 55% of them FP32 arithmetic, "
                              "0.2 TB/s of HBM traffic (profiles/kernels_terrain512.md)"},
         "tuning": tuning,
     }
