@@ -159,6 +159,42 @@ def test_batch_sources(ctx, equal_lod):
     b.close()
 
 
+def test_clipmap_region_with_brushes():
+    """A region of the 6-LOD clipmap selection (mixed LODs, caves, 256 brushes) meshed in one
+    batch; its seams stitched from the slots equal the oracle's on the same lists."""
+    wl = workloads.clipmap()
+    mesher.init(wl["desc"], wl["seed"], wl["default_material"])
+    mesher.clear_csg_ops()
+    for op, bb in zip(wl["ops"], wl["aabbs"]):
+        mesher.store_csg_op(op, bb[:3], bb[3:])
+    cam = wl["camera"]
+    d = np.abs(wl["mins"] + wl["sizes"][:, None] // 2 - cam).max(axis=1)
+    keep = np.argsort(d)[:160]  # the chunks nearest the camera: LOD 0-2
+    mins, sizes = wl["mins"][keep], wl["sizes"][keep]
+    assert len(set(sizes.tolist())) >= 3
+    b = mesher.Batch(64, len(sizes))
+    b.set_chunks(mins, sizes)
+    b.run()
+    b.wait()
+    counts = b.counts()
+    assert (counts[:, 4] == 0).all()
+    nodes = [(tuple(int(v) for v in mins[i]), int(sizes[i]), b.get_mesh(i)[2]) for i in range(len(sizes))
+             if counts[i, 2] > 0 or counts[i, 3] > 0]
+    srcs = workloads.seam_sources(nodes)
+    hosts = [(mn, size, _colour(i)) for i, (mn, size, _) in enumerate(nodes)]
+    got = b.seam_meshes(hosts, [[dict(x, nodes=None) for x in s] for s in srcs])
+    tris = 0
+    for i, ((mn, size, colour), s) in enumerate(zip(hosts, srcs)):
+        rc, v, t = py_oracle.seam_mesh(mn, size, s, colour=colour)
+        assert rc == LVN_OK
+        assert got[i][0].tobytes() == v.tobytes() and (got[i][1] == t).all(), f"seam {i} host {mn} size {size}"
+        tris += len(t)
+    print(f"{len(nodes)} seams, {tris} triangles", flush=True)
+    assert tris > 1000
+    b.close()
+    mesher.clear_csg_ops()
+
+
 def test_seam_errors(ctx, equal_lod):
     mn, size, _ = equal_lod.nodes[0]
     srcs = equal_lod.sources()[0]
