@@ -15,9 +15,14 @@
 
 namespace lvn {
 
-// LDS-resident noise tables: 65,536 + 256 bytes per workgroup.
+// LDS-resident noise tables: the 256 x 256 permutation table stored with a pitch of
+// 257 and one extra row and column (texel (x, y) at y * 257 + x for x, y in 0..256,
+// row and column 256 repeating 0), so that a corner's texel, whose coordinates are
+// the cell's wrapped ones plus 0 or 1, is one add away from the cell's base index,
+// with no wrap masks (REPEAT addressing, cl/simplex.cl:59) per corner.  66,320 bytes
+// per workgroup with the 16 gradient vectors; the padded table is built by the host.
 struct NoiseLds {
-    uint8_t perm[65536];
+    uint8_t perm[kPermPaddedBytes];
     float4 grad[16];
 };
 
@@ -26,20 +31,16 @@ __device__ __forceinline__ void load_noise_tables(NoiseLds& lds, const uint8_t* 
                                                   const float4* __restrict__ grads) {
     const uint4* src = reinterpret_cast<const uint4*>(perm);
     uint4* dst = reinterpret_cast<uint4*>(lds.perm);
-    for (int i = threadIdx.x; i < 65536 / 16; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < kPermPaddedBytes / 16; i += blockDim.x) dst[i] = src[i];
     if (threadIdx.x < 16) lds.grad[threadIdx.x] = grads[threadIdx.x];
     __syncthreads();
-}
-
-__device__ __forceinline__ int lds_texel(const NoiseLds& T, int tx, int ty) {
-    return T.perm[((ty & 255) << 8) | (tx & 255)];
 }
 
 // One snoise2 corner (cl/simplex.cl:123-153).  The squared distance and the
 // gradient dot product are fused multiply-adds, as the restatement defines them
 // (the reference's -cl-fast-relaxed-math build, src/compute.cpp:213, allows it).
-__device__ __forceinline__ float corner2(const NoiseLds& T, int tx, int ty, float fx, float fy) {
-    const int g = lds_texel(T, tx, ty) & 15;
+__device__ __forceinline__ float corner2(const NoiseLds& T, int texel, float fx, float fy) {
+    const int g = T.perm[texel] & 15;
     const float r = 0.5f - fmaf(fy, fy, fx * fx);
     const float4 gr = T.grad[g];
     const float t = r * r;
@@ -58,22 +59,22 @@ __device__ __forceinline__ float snoise2(const NoiseLds& T, float px, float py) 
     const float t = (pix + piy) * G2;
     const float f0x = px - (pix - t);
     const float f0y = py - (piy - t);
-    const int ix = (int)pix;
-    const int iy = (int)piy;
+    const int base = ((int)piy & 255) * kPermPitch + ((int)pix & 255);  // texel of the cell corner
     const int o1x = f0x > f0y ? 1 : 0;
     const int o1y = 1 - o1x;
-    const float n0 = corner2(T, ix, iy, f0x, f0y);
-    const float n1 = corner2(T, ix + o1x, iy + o1y, (f0x - (float)o1x) + G2, (f0y - (float)o1y) + G2);
-    const float n2 = corner2(T, ix + 1, iy + 1, f0x - C2, f0y - C2);
+    const float n0 = corner2(T, base, f0x, f0y);
+    const float n1 = corner2(T, base + o1y * kPermPitch + o1x, (f0x - (float)o1x) + G2, (f0y - (float)o1y) + G2);
+    const float n2 = corner2(T, base + kPermPitch + 1, f0x - C2, f0y - C2);
     return 70.f * ((n0 + n1) + n2);
 }
 
 // One snoise3 corner (cl/simplex.cl:181-226).  The second table read uses the
 // first read's value as x texel; value 255 wraps to texel 0 (normalised REPEAT).
-__device__ __forceinline__ float corner3(const NoiseLds& T, int tx, int ty, int tz, float fx, float fy, float fz) {
-    int p = lds_texel(T, tx, ty);
+// `texel` is the corner's (x, y) texel, `zrow` its z texel's row offset.
+__device__ __forceinline__ float corner3(const NoiseLds& T, int texel, int zrow, float fx, float fy, float fz) {
+    int p = T.perm[texel];
     p = p == 255 ? 0 : p;
-    const int g = T.perm[((tz & 255) << 8) | p] & 15;
+    const int g = T.perm[zrow + p] & 15;
     const float r = 0.6f - fmaf(fz, fz, fmaf(fy, fy, fx * fx));
     const float4 gr = T.grad[g];
     const float t = r * r;
@@ -95,7 +96,8 @@ __device__ __forceinline__ float snoise3(const NoiseLds& T, float px, float py, 
     const float f0x = px - (pix - t);
     const float f0y = py - (piy - t);
     const float f0z = pz - (piz - t);
-    const int ix = (int)pix, iy = (int)piy, iz = (int)piz;
+    const int base = ((int)piy & 255) * kPermPitch + ((int)pix & 255);  // (x, y) texel of the cell corner
+    const int zrow = ((int)piz & 255) * kPermPitch;
     const int xy = f0x >= f0y ? 1 : 0;
     const int xz = f0x >= f0z ? 1 : 0;
     const int yz = f0y >= f0z ? 1 : 0;
@@ -104,12 +106,12 @@ __device__ __forceinline__ float snoise3(const NoiseLds& T, float px, float py, 
     const int rz = (1 - xz) + (1 - yz);
     const int a1x = rx >> 1, a1y = ry >> 1, a1z = rz >> 1;            // rank 2 -> 1
     const int a2x = rx > 0 ? 1 : 0, a2y = ry > 0 ? 1 : 0, a2z = rz > 0 ? 1 : 0;  // rank >= 1 -> 1
-    const float n0 = corner3(T, ix, iy, iz, f0x, f0y, f0z);
-    const float n1 = corner3(T, ix + a1x, iy + a1y, iz + a1z, (f0x - (float)a1x) + G3, (f0y - (float)a1y) + G3,
-                             (f0z - (float)a1z) + G3);
-    const float n2 = corner3(T, ix + a2x, iy + a2y, iz + a2z, (f0x - (float)a2x) + C2, (f0y - (float)a2y) + C2,
-                             (f0z - (float)a2z) + C2);
-    const float n3 = corner3(T, ix + 1, iy + 1, iz + 1, f0x - C3, f0y - C3, f0z - C3);
+    const float n0 = corner3(T, base, zrow, f0x, f0y, f0z);
+    const float n1 = corner3(T, base + a1y * kPermPitch + a1x, zrow + a1z * kPermPitch, (f0x - (float)a1x) + G3,
+                             (f0y - (float)a1y) + G3, (f0z - (float)a1z) + G3);
+    const float n2 = corner3(T, base + a2y * kPermPitch + a2x, zrow + a2z * kPermPitch, (f0x - (float)a2x) + C2,
+                             (f0y - (float)a2y) + C2, (f0z - (float)a2z) + C2);
+    const float n3 = corner3(T, base + kPermPitch + 1, zrow + kPermPitch, f0x - C3, f0y - C3, f0z - C3);
     return 32.f * (((n0 + n1) + n2) + n3);
 }
 

@@ -663,6 +663,17 @@ using namespace lvn;
 // C ABI
 // ---------------------------------------------------------------------------
 
+namespace lvn {
+// The kernels' copy of the permutation table: pitch 257 with a wrap row and column.
+static int upload_perm_table() {
+    std::vector<uint8_t> pad(kPermPaddedBytes, 0);
+    for (int y = 0; y < kPermPitch; ++y)
+        for (int x = 0; x < kPermPitch; ++x) pad[y * kPermPitch + x] = g_state.perm[((y & 255) << 8) | (x & 255)];
+    LVN_HIP(hipMemcpy(g_state.d_perm, pad.data(), kPermPaddedBytes, hipMemcpyHostToDevice));
+    return LVN_OK;
+}
+}  // namespace lvn
+
 extern "C" {
 
 int lvn_init(const lvn_field_desc* field, int32_t noise_seed, uint32_t default_material) {
@@ -675,14 +686,15 @@ int lvn_init(const lvn_field_desc* field, int32_t noise_seed, uint32_t default_m
     g_state.fp = make_field_params(g_state.desc);
     g_state.defmat = default_material;
     if (!g_state.d_perm) {
-        LVN_HIP(hipMalloc(&g_state.d_perm, 65536));
+        LVN_HIP(hipMalloc(&g_state.d_perm, kPermPaddedBytes));
         LVN_HIP(hipMalloc(&g_state.d_grads, sizeof(float4) * 16));
     }
     g_state.perm.resize(65536);
     build_perm_table(noise_seed, g_state.perm.data());
     float4 grads[16];
     build_grads(grads);
-    LVN_HIP(hipMemcpy(g_state.d_perm, g_state.perm.data(), 65536, hipMemcpyHostToDevice));
+    int r = upload_perm_table();
+    if (r) return r;
     LVN_HIP(hipMemcpy(g_state.d_grads, grads, sizeof(grads), hipMemcpyHostToDevice));
     g_state.seed = noise_seed;
     g_state.init = true;
@@ -705,7 +717,8 @@ int lvn_shutdown(void) {
 int lvn_set_noise_seed(int32_t seed) {
     if (!g_state.init) return LVN_ERR_STATE;
     build_perm_table(seed, g_state.perm.data());
-    LVN_HIP(hipMemcpy(g_state.d_perm, g_state.perm.data(), 65536, hipMemcpyHostToDevice));
+    const int r = upload_perm_table();
+    if (r) return r;
     g_state.seed = seed;
     return LVN_OK;
 }

@@ -7,6 +7,12 @@
 
 namespace lvn {
 
+// The permutation table in device memory and LDS: pitch 257 with a wrap row and column
+// (texel (x, y) at y * 257 + x, x and y in 0..256), padded to 16 bytes.
+constexpr int kPermPitch = 257;
+constexpr int kPermPaddedBytes = ((kPermPitch * kPermPitch + 15) / 16) * 16;
+
+
 constexpr int kAir = LVN_MATERIAL_AIR;    // src/volume_materials.h:9
 constexpr int kNone = LVN_MATERIAL_NONE;  // src/volume_materials.h:8
 constexpr int kLeafSizeScale = 4;         // src/volume_constants.h:7-8
