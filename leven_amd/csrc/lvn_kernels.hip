@@ -119,13 +119,16 @@ __device__ __forceinline__ void density_column(const KArgs& a, const NoiseLds& T
     const int y1 = min(F, y0 + seg_len);
     uint8_t* mat = slot_ptr<uint8_t>(a, chunk, a.L.mat);
     uint8_t* orig = slot_ptr<uint8_t>(a, chunk, a.L.orig);
+    // the materials before CSG are read only to find edges a brush touched: chunks
+    // without ops skip them (uniform over a 64-column item, which lies in one chunk)
+    const bool keep_orig = cd.op_count > 0;
     if (cd.mode == kModeCached) {  // a cached field: its materials, no density
         const uint8_t* old = reinterpret_cast<const uint8_t*>(cd.old_mat);
         for (int y = y0; y < y1; ++y) {
             const int idx = field_index(G, x, y, z);
             const uint8_t m = old[idx];
             mat[idx] = m;
-            orig[idx] = m;
+            if (keep_orig) orig[idx] = m;
         }
         return;
     }
@@ -146,7 +149,7 @@ __device__ __forceinline__ void density_column(const KArgs& a, const NoiseLds& T
             dens[idx] = d;
         const uint8_t m = d < 0.f ? (uint8_t)a.defmat : (uint8_t)kAir;
         mat[idx] = m;
-        orig[idx] = m;
+        if (keep_orig) orig[idx] = m;
     }
     if constexpr (kCount) {
         atomicAdd(&a.work[0], 1ull);
