@@ -496,8 +496,8 @@ def main():
                      "traffic": hbm_traffic("k_" + dom, wl["name"]), "flops_per_launch": int(dom_flops),
                      "launch_ms": round(stage_ms[dom], 4),
                      "note": "FP32 vector-ALU bound (neither HBM nor MFMA): rocprofv3 counts 0.42 VALU "
-                             "instructions per SIMD-cycle against a 0.5 ceiling, 55% of them FP32 arithmetic, "
-                             "0.2 TB/s of HBM traffic (profiles/kernels_terrain512.md)"},
+                             "instructions per SIMD-cycle against a 0.5 ceiling, 58% of them FP32 arithmetic, "
+                             "0.35 TB/s of HBM traffic (profiles/kernels_terrain512.md)"},
         "tuning": tuning,
     }
     hbm = hbm_rates(stage_ms, wl["name"])
