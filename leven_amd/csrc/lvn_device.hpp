@@ -189,7 +189,7 @@ __device__ __forceinline__ float cave_term(const FieldParams& F, const NoiseLds&
 }
 
 // density = max(a, cave) with a = y - H.  The cave term is bounded by F.cave_cut
-// (|snoise3| <= 3.82 times the sum of octave amplitudes, minus the threshold);
+// (|snoise3| <= 1.19 times the sum of octave amplitudes, minus the threshold);
 // when a exceeds it, max(a, cave) == a exactly and the cave term is skipped.
 __device__ __forceinline__ float combine(const FieldParams& F, const NoiseLds& T, float a, float x, float y, float z) {
     if (F.kind != LVN_FIELD_TERRAIN_CAVES || a > F.cave_cut) return a;

@@ -117,13 +117,17 @@ FieldParams make_field_params(const lvn_field_desc& d) {
     f.clac = d.cave_lacunarity;
     f.cgain = d.cave_gain;
     f.cthr = d.cave_threshold;
-    // |snoise3| <= 32 * 4 * max_r (0.6 - r^2)^4 * r * |g| with |g| <= 1.4253: 3.811.
+    // |snoise3| <= 1.1863: its value is 32 times the sum of the four corner terms of one
+    // simplex cell, each at most (0.6 - r^2)^4 * r * |g| at its corner distance r
+    // (|g| <= 1.4253 for the decoded gradients); the largest sum over a cell, found on a
+    // 480-step grid of the cell plus a Lipschitz margin (tests/test_oracle.py recomputes
+    // it), gives the bound.  1.19 leaves room for FP32 rounding.
     double amp = 1.0, sum = 0.0;
     for (int i = 0; i < d.cave_octaves; ++i) {
         sum += std::fabs(amp);
         amp *= (double)d.cave_gain;
     }
-    f.cave_cut = (float)((3.82 * sum + 1e-3) * 1.001 - (double)d.cave_threshold);
+    f.cave_cut = (float)((1.19 * sum + 1e-3) * 1.001 - (double)d.cave_threshold);
     return f;
 }
 

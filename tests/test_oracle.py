@@ -450,3 +450,48 @@ def test_topology_brute_force(oracle, seed):
     expect = brute_force_dc(c.materials(), n)
     assert len(expect) > 0
     assert (tri == expect).all()
+
+
+def test_snoise3_bound(oracle):
+    """The cave early-out of the kernels (skip the cave term where y - H exceeds
+    1.19 x the sum of octave amplitudes minus the threshold) relies on |snoise3| <= 1.1863.
+    Recomputed here: 32 x |g|max x max over one simplex cell of the sum of the four corner
+    terms (0.6 - r^2)^4 r, on a 480-step grid of the cell's skewed coordinates (ordered
+    region x >= y >= z; the other five orderings are axis permutations with the same
+    distances) plus the grid's Lipschitz margin; then checked against random samples."""
+    G3 = 1.0 / 6.0
+    n = 480
+    h = 1.0 / n
+    g = np.arange(n + 1) * h
+
+    def phi(r2):
+        return np.maximum(0.6 - r2, 0.0) ** 4 * np.sqrt(r2)
+
+    best = 0.0
+    for ix, sx in enumerate(g):
+        sy, sz = np.meshgrid(g[:ix + 1], g[:ix + 1], indexing="ij")
+        keep = sz <= sy
+        sy, sz = sy[keep], sz[keep]
+        t = (sx + sy + sz) * G3
+        f = np.stack([sx - t, sy - t, sz - t], axis=-1)
+        total = 0.0
+        for off in ([0, 0, 0], [1 - G3, -G3, -G3], [1 - 2 * G3, 1 - 2 * G3, -2 * G3], [1 - 3 * G3] * 3):
+            d = f - np.array(off)
+            total = total + phi((d * d).sum(-1))
+        best = max(best, float(total.max()))
+    r = np.linspace(0.0, np.sqrt(0.6), 200001)
+    lip = 4 * np.abs((0.6 - r * r) ** 3 * (0.6 - 9 * r * r)).max()  # |d/ds sum phi|, unskew norm 1
+    one, zero = ((1 * 64 + 64) / 255.0) * 4 - 1, ((0 * 64 + 64) / 255.0) * 4 - 1
+    gmax = np.sqrt(2 * one * one + zero * zero)
+    bound = 32 * gmax * (best + lip * h * np.sqrt(3) / 2)
+    assert bound <= 1.1863 + 1e-4
+    # the decoded gradient components are the ones bounded here: Perlin's 16 directions
+    # have two components of magnitude 1 and one 0, decoded as byte/255*4-1
+    decoded = np.asarray(oracle.grad_values(), dtype=np.float64)  # values of -1, 0, +1
+    one_o, zero_o = np.abs(decoded[[0, 2]]).max(), abs(decoded[1])
+    assert np.sqrt(2 * one_o ** 2 + zero_o ** 2) <= gmax + 1e-6
+    rng = np.random.default_rng(3)
+    perm = oracle.perm_table(7)
+    pts = rng.uniform(-300, 300, (20000, 3)).astype(np.float32)
+    m = max(abs(oracle.snoise3(perm, float(x), float(y), float(z))) for x, y, z in pts)
+    assert m < 1.0
