@@ -290,7 +290,7 @@ def dropin_latency(wl, count):
     lvn_generate_chunk_mesh): one fresh chunk per call, mesh copied into host buffers."""
     import ctypes as C
 
-    from leven_amd.abi import SEAM_DTYPE, VERTEX_DTYPE, vptr
+    from leven_amd.abi import CSG_OP_DTYPE, MATERIAL_AIR, SEAM_DTYPE, VERTEX_DTYPE, vptr
     ctx = mesher.Context(64)
     n3 = 64 ** 3
     v = np.zeros(n3, dtype=VERTEX_DTYPE)
@@ -306,10 +306,39 @@ def dropin_latency(wl, count):
                                                   vptr(t), 6 * n3, C.byref(nt), vptr(s), n3, C.byref(ns))
         lat.append(time.perf_counter() - ts)
         assert rc == 0, f"lvn_generate_chunk_mesh failed: {rc}"
+    # an edit the way the clipmap applies one (src/clipmap.cpp:1689-1710): applyCSGOperations
+    # on the chunk's field, freeChunkOctree, generateChunkMesh.  The first edit of a chunk
+    # loads its field into the context's cache; the second is timed: the incremental path
+    # (cached materials, brush-touched edges re-crossed, the rest from the cached list).
+    edit = []
+    rng = np.random.default_rng(11)
+    for i in idx[: min(len(idx), 64)]:
+        mn = np.ascontiguousarray(wl["mins"][i], dtype=np.int32)
+        size = int(wl["sizes"][i])
+        for k in range(2):
+            op = np.zeros(1, CSG_OP_DTYPE)
+            op["type"], op["brush"] = k % 2, 1
+            op["material"] = 2 if k % 2 == 0 else MATERIAL_AIR
+            c = mn / 4 + rng.uniform(16, 48, 3)
+            op["origin"] = [c[0] + 0.5, c[1] + 0.5, c[2] + 0.5, 0.0]
+            op["dims"] = [6.0, 6.0, 6.0, 0.0]
+            ts = time.perf_counter()
+            mesher.check(mesher.lib().lvn_apply_csg(ctx._h, vptr(op), 1, vptr(mn), size), "lvn_apply_csg")
+            mesher.check(mesher.lib().lvn_free_chunk(ctx._h, vptr(mn), size), "lvn_free_chunk")
+            rc = mesher.lib().lvn_generate_chunk_mesh(ctx._h, vptr(mn), size, vptr(v), n3, C.byref(nv), vptr(t),
+                                                      6 * n3, C.byref(nt), vptr(s), n3, C.byref(ns))
+            assert rc == 0, f"lvn_generate_chunk_mesh after an edit failed: {rc}"
+            if k == 1:
+                edit.append(time.perf_counter() - ts)
     ctx.close()
     lat = np.array(lat) * 1e3
+    edit = np.array(edit) * 1e3
     return {"chunks": int(count), "chunks_per_s": round(count / (lat.sum() * 1e-3), 1),
-            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)}}
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
+            "edit_latency_ms": {"p50": round(float(np.percentile(edit, 50)), 3),
+                                "p99": round(float(np.percentile(edit, 99)), 3),
+                                "note": "applyCSGOperations + freeChunkOctree + generateChunkMesh on a chunk "
+                                        "whose field is cached (lvn_apply_csg, incremental), mesh copied out"}}
 
 
 def serve_burst():

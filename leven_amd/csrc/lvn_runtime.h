@@ -11,6 +11,20 @@
 
 #include "lvn_kernels.h"
 
+#include <cstdio>
+
+// Returns LVN_ERR_DEVICE from the calling function when a HIP call fails, after
+// printing the call and the runtime's message.
+#define LVN_HIP(call)                                                                                 \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess) {                                                                       \
+            fprintf(stderr, "lvn: HIP error %s at %s:%d (%s)\n", hipGetErrorString(e_), __FILE__,     \
+                    __LINE__, #call);                                                                 \
+            return LVN_ERR_DEVICE;                                                                    \
+        }                                                                                             \
+    } while (0)
+
 namespace lvn {
 
 // Seam-stitching buffers of a context (lvn_seams.hip).

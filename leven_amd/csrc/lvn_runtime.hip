@@ -20,16 +20,6 @@
 #include <mutex>
 #include <vector>
 
-#define LVN_HIP(call)                                                                                 \
-    do {                                                                                              \
-        hipError_t e_ = (call);                                                                       \
-        if (e_ != hipSuccess) {                                                                       \
-            fprintf(stderr, "lvn: HIP error %s at %s:%d (%s)\n", hipGetErrorString(e_), __FILE__,     \
-                    __LINE__, #call);                                                                 \
-            return LVN_ERR_DEVICE;                                                                    \
-        }                                                                                             \
-    } while (0)
-
 namespace lvn {
 
 Global g_state;

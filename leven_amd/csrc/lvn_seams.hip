@@ -34,16 +34,6 @@
 #include <thread>
 #include <vector>
 
-#define LVN_HIP(call)                                                                                 \
-    do {                                                                                              \
-        hipError_t e_ = (call);                                                                       \
-        if (e_ != hipSuccess) {                                                                       \
-            fprintf(stderr, "lvn: HIP error %s at %s:%d (%s)\n", hipGetErrorString(e_), __FILE__,     \
-                    __LINE__, #call);                                                                 \
-            return LVN_ERR_DEVICE;                                                                    \
-        }                                                                                             \
-    } while (0)
-
 namespace lvn {
 
 // ---------------------------------------------------------------------------
