@@ -806,11 +806,20 @@ int lvn_apply_csg(lvn_ctx* ctx, const lvn_csg_op* ops, int32_t n, const int32_t 
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     // Compute_ApplyCSGOperations (src/compute_csg.cpp:224-242): LoadDensityField, apply, store.
     Counters k;
-    int r = load_field(c, min, size, &k);
-    if (r) return r;
+    int r;
     const Key key = make_key(min, size);
-    r = store_field(c, key, k.edges, (int)g_state.ops.size());  // the field of the load step (slot 0)
-    if (r) return r;
+    auto fit = c->fields.find(key);
+    if (fit != c->fields.end() && stored_ops_for(min, size, fit->second.last_op).empty()) {
+        // the cached field already is what LoadDensityField would produce (no stored op
+        // newer than it overlaps the chunk): loading it again would rebuild the same
+        // materials and edge list
+        fit->second.last_op = (int)g_state.ops.size();
+    } else {
+        r = load_field(c, min, size, &k);
+        if (r) return r;
+        r = store_field(c, key, k.edges, (int)g_state.ops.size());  // the field of the load step (slot 0)
+        if (r) return r;
+    }
     if (n == 0) return LVN_OK;
     std::vector<lvn_csg_op> batch(ops, ops + n);
     r = run_field(c, min, size, batch, &c->fields[key]);
