@@ -5,7 +5,7 @@
 // correctly rounded division and sqrt (HIP's default for FP32), and float
 // constants in place of the reference's unsuffixed double literals.  The
 // permutation table lives in LDS (64 KiB of bytes, row = texel y) together
-// with the 16 decoded gradient vectors.
+// with the decoded gradient vectors.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -15,15 +15,18 @@
 
 namespace lvn {
 
-// LDS-resident noise tables: the 256 x 256 permutation table stored with a pitch of
-// 257 and one extra row and column (texel (x, y) at y * 257 + x for x, y in 0..256,
-// row and column 256 repeating 0), so that a corner's texel, whose coordinates are
-// the cell's wrapped ones plus 0 or 1, is one add away from the cell's base index,
-// with no wrap masks (REPEAT addressing, cl/simplex.cl:59) per corner.  66,320 bytes
-// per workgroup with the 16 gradient vectors; the padded table is built by the host.
+// LDS-resident noise tables, built by the host.  The 256 x 256 permutation table is
+// stored in rows of 258 bytes with a wrap row: texel (x, y) at y * 258 + 1 + x for x, y
+// in 0..256 (256 repeating 0), so a corner's texel, whose coordinates are the cell's
+// wrapped ones plus 0 or 1, is one add away from the cell's base index with no wrap
+// masks per corner (REPEAT addressing, cl/simplex.cl:59).  Each byte holds the texel's
+// value plus one, mod 256, and byte y * 258 holds texel (0, y): snoise3's second read,
+// at x = value with 255 wrapping to 0, is then row + stored byte with no compare, and
+// the gradient table, indexed by the stored byte, replaces the "& 15".  70,416 bytes
+// per workgroup.
 struct NoiseLds {
     uint8_t perm[kPermPaddedBytes];
-    float4 grad[16];
+    float4 grad[kGradEntries];  // entry w: gradient (w - 1) & 15
 };
 
 // Cooperative copy of the tables into LDS (all threads of the block call it).
@@ -32,7 +35,7 @@ __device__ __forceinline__ void load_noise_tables(NoiseLds& lds, const uint8_t* 
     const uint4* src = reinterpret_cast<const uint4*>(perm);
     uint4* dst = reinterpret_cast<uint4*>(lds.perm);
     for (int i = threadIdx.x; i < kPermPaddedBytes / 16; i += blockDim.x) dst[i] = src[i];
-    if (threadIdx.x < 16) lds.grad[threadIdx.x] = grads[threadIdx.x];
+    for (int i = threadIdx.x; i < kGradEntries; i += blockDim.x) lds.grad[i] = grads[i];
     __syncthreads();
 }
 
@@ -40,9 +43,8 @@ __device__ __forceinline__ void load_noise_tables(NoiseLds& lds, const uint8_t* 
 // gradient dot product are fused multiply-adds, as the restatement defines them
 // (the reference's -cl-fast-relaxed-math build, src/compute.cpp:213, allows it).
 __device__ __forceinline__ float corner2(const NoiseLds& T, int texel, float fx, float fy) {
-    const int g = T.perm[texel] & 15;
     const float r = 0.5f - fmaf(fy, fy, fx * fx);
-    const float4 gr = T.grad[g];
+    const float4 gr = T.grad[T.perm[texel]];
     const float t = r * r;
     const float n = (t * t) * fmaf(gr.y, fy, gr.x * fx);
     return r < 0.f ? 0.f : n;
@@ -59,7 +61,7 @@ __device__ __forceinline__ float snoise2(const NoiseLds& T, float px, float py) 
     const float t = (pix + piy) * G2;
     const float f0x = px - (pix - t);
     const float f0y = py - (piy - t);
-    const int base = ((int)piy & 255) * kPermPitch + ((int)pix & 255);  // texel of the cell corner
+    const int base = ((int)piy & 255) * kPermPitch + ((int)pix & 255) + 1;  // texel of the cell corner
     const int o1x = f0x > f0y ? 1 : 0;
     const int o1y = 1 - o1x;
     const float n0 = corner2(T, base, f0x, f0y);
@@ -70,13 +72,12 @@ __device__ __forceinline__ float snoise2(const NoiseLds& T, float px, float py) 
 
 // One snoise3 corner (cl/simplex.cl:181-226).  The second table read uses the
 // first read's value as x texel; value 255 wraps to texel 0 (normalised REPEAT).
-// `texel` is the corner's (x, y) texel, `zrow` its z texel's row offset.
+// `texel` is the corner's (x, y) texel, `zrow` the start of its z texel's row; the
+// stored byte (value + 1, or 0 for 255) is the second read's offset in that row.
 __device__ __forceinline__ float corner3(const NoiseLds& T, int texel, int zrow, float fx, float fy, float fz) {
-    int p = T.perm[texel];
-    p = p == 255 ? 0 : p;
-    const int g = T.perm[zrow + p] & 15;
+    const int w = T.perm[texel];
     const float r = 0.6f - fmaf(fz, fz, fmaf(fy, fy, fx * fx));
-    const float4 gr = T.grad[g];
+    const float4 gr = T.grad[T.perm[zrow + w]];
     const float t = r * r;
     const float n = (t * t) * fmaf(gr.z, fz, fmaf(gr.y, fy, gr.x * fx));
     return r < 0.f ? 0.f : n;
@@ -96,7 +97,7 @@ __device__ __forceinline__ float snoise3(const NoiseLds& T, float px, float py, 
     const float f0x = px - (pix - t);
     const float f0y = py - (piy - t);
     const float f0z = pz - (piz - t);
-    const int base = ((int)piy & 255) * kPermPitch + ((int)pix & 255);  // (x, y) texel of the cell corner
+    const int base = ((int)piy & 255) * kPermPitch + ((int)pix & 255) + 1;  // (x, y) texel of the cell corner
     const int zrow = ((int)piz & 255) * kPermPitch;
     const int xy = f0x >= f0y ? 1 : 0;
     const int xz = f0x >= f0z ? 1 : 0;

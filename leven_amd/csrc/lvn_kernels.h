@@ -19,7 +19,7 @@ struct KArgs {
     const int* op_index;
     FieldParams F;
     const uint8_t* perm;     // 65,536 bytes
-    const float4* grads;     // 16 decoded gradients
+    const float4* grads;     // decoded gradients, one per stored table byte
     int defmat;
     int fp16;                // density stored as sign-preserving FP16
     int ysplit;              // density kernel: y segments per column

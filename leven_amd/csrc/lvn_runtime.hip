@@ -85,12 +85,14 @@ void build_perm_table(int32_t seed, uint8_t* out) {
         }
 }
 
-// UNORM8 decode of the gradient bytes g*64+64, then *4-1 (cl/simplex.cl:124).
+// UNORM8 decode of the gradient bytes g*64+64, then *4-1 (cl/simplex.cl:124), one entry
+// per stored table byte w = value + 1: the gradient of value w - 1, i.e. (w + 15) & 15.
 void build_grads(float4* out) {
-    for (int i = 0; i < 16; ++i) {
+    for (int w = 0; w < kGradEntries; ++w) {
+        const int i = (w + 15) & 15;
         float c[3];
         for (int k = 0; k < 3; ++k) c[k] = ((float)(kGrad3[i][k] * 64 + 64) / 255.0f) * 4.0f - 1.0f;
-        out[i] = make_float4(c[0], c[1], c[2], 0.f);
+        out[w] = make_float4(c[0], c[1], c[2], 0.f);
     }
 }
 
@@ -658,11 +660,15 @@ using namespace lvn;
 // ---------------------------------------------------------------------------
 
 namespace lvn {
-// The kernels' copy of the permutation table: pitch 257 with a wrap row and column.
+// The kernels' copy of the permutation table: values plus one, rows of a wrap column,
+// texels 0..255 and a second wrap column, and a wrap row.
 static int upload_perm_table() {
     std::vector<uint8_t> pad(kPermPaddedBytes, 0);
-    for (int y = 0; y < kPermPitch; ++y)
-        for (int x = 0; x < kPermPitch; ++x) pad[y * kPermPitch + x] = g_state.perm[((y & 255) << 8) | (x & 255)];
+    auto stored = [](int x, int y) { return (uint8_t)((g_state.perm[((y & 255) << 8) | (x & 255)] + 1) & 255); };
+    for (int y = 0; y < kPermRows; ++y) {
+        pad[y * kPermPitch] = stored(0, y);
+        for (int x = 0; x <= 256; ++x) pad[y * kPermPitch + 1 + x] = stored(x, y);
+    }
     LVN_HIP(hipMemcpy(g_state.d_perm, pad.data(), kPermPaddedBytes, hipMemcpyHostToDevice));
     return LVN_OK;
 }
@@ -681,11 +687,11 @@ int lvn_init(const lvn_field_desc* field, int32_t noise_seed, uint32_t default_m
     g_state.defmat = default_material;
     if (!g_state.d_perm) {
         LVN_HIP(hipMalloc(&g_state.d_perm, kPermPaddedBytes));
-        LVN_HIP(hipMalloc(&g_state.d_grads, sizeof(float4) * 16));
+        LVN_HIP(hipMalloc(&g_state.d_grads, sizeof(float4) * kGradEntries));
     }
     g_state.perm.resize(65536);
     build_perm_table(noise_seed, g_state.perm.data());
-    float4 grads[16];
+    float4 grads[kGradEntries];
     build_grads(grads);
     int r = upload_perm_table();
     if (r) return r;

@@ -7,10 +7,14 @@
 
 namespace lvn {
 
-// The permutation table in device memory and LDS: pitch 257 with a wrap row and column
-// (texel (x, y) at y * 257 + x, x and y in 0..256), padded to 16 bytes.
-constexpr int kPermPitch = 257;
-constexpr int kPermPaddedBytes = ((kPermPitch * kPermPitch + 15) / 16) * 16;
+// The permutation table in device memory and LDS.  Each byte holds its texel's value
+// plus one (mod 256).  Rows of 258 bytes: texel (x, y) at y * 258 + 1 + x for x, y in
+// 0..256 (x or y = 256 repeating 0), and byte y * 258 holds texel (0, y) once more.
+// 257 rows, padded to 16 bytes.  The gradient table has one entry per stored byte.
+constexpr int kPermPitch = 258;
+constexpr int kPermRows = 257;
+constexpr int kPermPaddedBytes = ((kPermPitch * kPermRows + 15) / 16) * 16;
+constexpr int kGradEntries = 256;
 
 
 constexpr int kAir = LVN_MATERIAL_AIR;    // src/volume_materials.h:9
