@@ -39,15 +39,43 @@ __device__ __forceinline__ void load_noise_tables(NoiseLds& lds, const uint8_t* 
     __syncthreads();
 }
 
+// Correctly rounded square root of x >= 1 (and of +inf): the hardware square root,
+// within 1 ulp, replaced by its lower or upper neighbour when the FMA residual
+// x - neighbour * root says that one is nearer.  These are the steps of the
+// compiler's IEEE sqrtf without its range scaling (taken below 2^-96 only) and its
+// zero / infinity select (+inf already comes out as +inf: both residuals are NaN).
+__device__ __forceinline__ float sqrt_ge1(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float lo = __int_as_float(__float_as_int(s) - 1);
+    const float hi = __int_as_float(__float_as_int(s) + 1);
+    const float r = fmaf(-lo, s, x) <= 0.f ? lo : s;
+    return fmaf(-hi, s, x) > 0.f ? hi : r;
+}
+
+// Correctly rounded 1 / d for d in [1, 2): the hardware reciprocal refined by one
+// Newton step, then the quotient corrected twice by its FMA residual.  These are the
+// steps of the compiler's IEEE division 1.f / d with its operand scaling and its
+// special-case fixup left out: for this range neither changes a value.
+__device__ __forceinline__ float recip_1_2(float d) {
+    const float r0 = __builtin_amdgcn_rcpf(d);
+    const float r1 = fmaf(fmaf(-d, r0, 1.f), r0, r0);
+    const float q1 = fmaf(fmaf(-d, r1, 1.f), r1, r1);
+    return fmaf(fmaf(-d, q1, 1.f), r1, q1);
+}
+
 // One snoise2 corner (cl/simplex.cl:123-153).  The squared distance and the
 // gradient dot product are fused multiply-adds, as the restatement defines them
 // (the reference's -cl-fast-relaxed-math build, src/compute.cpp:213, allows it).
+// A corner beyond the radius (r < 0) contributes zero: r is clamped at +0, which
+// gives that zero up to its sign (+0 times the dot product).  No result can see the
+// sign: the corner sum of a point with a non-zero corner is unchanged by it, and
+// every noise value feeds either fBm's fma chain, which starts at +0 (a zero noise
+// value then adds +0 to a sum that is never -0), or the ridged form's fabsf.
 __device__ __forceinline__ float corner2(const NoiseLds& T, int texel, float fx, float fy) {
-    const float r = 0.5f - fmaf(fy, fy, fx * fx);
+    const float r = fmaxf(0.5f - fmaf(fy, fy, fx * fx), 0.f);
     const float4 gr = T.grad[T.perm[texel]];
     const float t = r * r;
-    const float n = (t * t) * fmaf(gr.y, fy, gr.x * fx);
-    return r < 0.f ? 0.f : n;
+    return (t * t) * fmaf(gr.y, fy, gr.x * fx);
 }
 
 // snoise2, cl/simplex.cl:99-157.
@@ -74,13 +102,13 @@ __device__ __forceinline__ float snoise2(const NoiseLds& T, float px, float py) 
 // first read's value as x texel; value 255 wraps to texel 0 (normalised REPEAT).
 // `texel` is the corner's (x, y) texel, `zrow` the start of its z texel's row; the
 // stored byte (value + 1, or 0 for 255) is the second read's offset in that row.
+// r is clamped at +0 as in corner2.
 __device__ __forceinline__ float corner3(const NoiseLds& T, int texel, int zrow, float fx, float fy, float fz) {
     const int w = T.perm[texel];
-    const float r = 0.6f - fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+    const float r = fmaxf(0.6f - fmaf(fz, fz, fmaf(fy, fy, fx * fx)), 0.f);
     const float4 gr = T.grad[T.perm[zrow + w]];
     const float t = r * r;
-    const float n = (t * t) * fmaf(gr.z, fz, fmaf(gr.y, fy, gr.x * fx));
-    return r < 0.f ? 0.f : n;
+    return (t * t) * fmaf(gr.z, fz, fmaf(gr.y, fy, gr.x * fx));
 }
 
 // snoise3, cl/simplex.cl:159-230 (simplex ranking of :74-97).
@@ -107,11 +135,13 @@ __device__ __forceinline__ float snoise3(const NoiseLds& T, float px, float py, 
     const int rz = (1 - xz) + (1 - yz);
     const int a1x = rx >> 1, a1y = ry >> 1, a1z = rz >> 1;            // rank 2 -> 1
     const int a2x = rx > 0 ? 1 : 0, a2y = ry > 0 ? 1 : 0, a2z = rz > 0 ? 1 : 0;  // rank >= 1 -> 1
+    // f0 - offset for offsets 0 and 1: f0 - 0.f is f0 itself (also for -0)
+    const float g0x = f0x - 1.f, g0y = f0y - 1.f, g0z = f0z - 1.f;
     const float n0 = corner3(T, base, zrow, f0x, f0y, f0z);
-    const float n1 = corner3(T, base + a1y * kPermPitch + a1x, zrow + a1z * kPermPitch, (f0x - (float)a1x) + G3,
-                             (f0y - (float)a1y) + G3, (f0z - (float)a1z) + G3);
-    const float n2 = corner3(T, base + a2y * kPermPitch + a2x, zrow + a2z * kPermPitch, (f0x - (float)a2x) + C2,
-                             (f0y - (float)a2y) + C2, (f0z - (float)a2z) + C2);
+    const float n1 = corner3(T, base + a1y * kPermPitch + a1x, zrow + a1z * kPermPitch, (a1x ? g0x : f0x) + G3,
+                             (a1y ? g0y : f0y) + G3, (a1z ? g0z : f0z) + G3);
+    const float n2 = corner3(T, base + a2y * kPermPitch + a2x, zrow + a2z * kPermPitch, (a2x ? g0x : f0x) + C2,
+                             (a2y ? g0y : f0y) + C2, (a2z ? g0z : f0z) + C2);
     const float n3 = corner3(T, base + kPermPitch + 1, zrow + kPermPitch, f0x - C3, f0y - C3, f0z - C3);
     return 32.f * (((n0 + n1) + n2) + n3);
 }

@@ -901,11 +901,14 @@ __device__ __forceinline__ float dot4(const Q4& a, const Q4& b) {
     return ((a.x * b.x + a.y * b.y) + a.z * b.z) + a.w * b.w;
 }
 
+// Both square roots have arguments >= 1 and the cosine's divisor lies in [1, sqrt 2]
+// (|tn| <= 1), so they take the short correctly rounded forms; the other two
+// divisions see any magnitude (tau overflows to inf for a tiny apq).
 __device__ __forceinline__ void givens(float app, float apq, float aqq, float& c, float& s) {  // cl/qef.cl:31-42
     const float tau = (aqq - app) / (2.f * apq);
-    const float stt = sqrtf(1.f + tau * tau);
+    const float stt = sqrt_ge1(1.f + tau * tau);
     const float tn = 1.f / ((tau >= 0.f) ? (tau + stt) : (tau - stt));
-    c = 1.f / sqrtf(1.f + tn * tn);
+    c = recip_1_2(sqrt_ge1(1.f + tn * tn));
     s = tn * c;
 }
 
