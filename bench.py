@@ -25,9 +25,10 @@ METRIC = "64^3 chunks meshed/sec (batched vs plain, same GPU) + p99 chunk latenc
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak (fused multiply-adds)
 PEAK_HBM_GBS = 8000.0
 
-# FP32 operations per evaluation, counted from the kernel source (add/sub/mul/div/sqrt = 1)
+# FP32 operations per evaluation, counted from the kernel source (add/sub/mul/div/sqrt = 1,
+# fused multiply-add = 2; selects, max and floor not counted)
 FLOPS_SNOISE2 = 49
-FLOPS_SNOISE3 = 90
+FLOPS_SNOISE3 = 87
 
 
 def flops_height(desc):
@@ -524,9 +525,9 @@ def main():
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": hbm_traffic("k_" + dom, wl["name"]), "flops_per_launch": int(dom_flops),
                      "launch_ms": round(stage_ms[dom], 4),
-                     "note": "FP32 vector-ALU bound (neither HBM nor MFMA): rocprofv3 counts 0.42 VALU "
-                             "instructions per SIMD-cycle against a 0.5 ceiling, 58% of them FP32 arithmetic, "
-                             "0.35 TB/s of HBM traffic (profiles/kernels_terrain512.md)"},
+                     "note": "FP32 vector-ALU bound (neither HBM nor MFMA): rocprofv3 counts 0.40 VALU "
+                             "instructions per SIMD-cycle against a 0.5 ceiling, 65% of them FP32 arithmetic, "
+                             "0.37 TB/s of HBM traffic (profiles/kernels_terrain512.md)"},
         "tuning": tuning,
     }
     hbm = hbm_rates(stage_ms, wl["name"])

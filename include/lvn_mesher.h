@@ -240,11 +240,12 @@ int  lvn_batch_wait(lvn_batch* b);
  * NDRanges are fixed).  Results never depend on them, only time does. */
 typedef struct lvn_tuning {
     int32_t density_ysplit;     /* y segments per (x, z) column work item of the density stage, 1..F */
-    int32_t persistent_per_cu;  /* persistent workgroups per CU in the density and edge-crossing stages, 1..2 */
+    int32_t persistent_per_cu;  /* persistent workgroups per CU in the edge-crossing stage, 1..2 */
     int32_t row_blocks;         /* workgroups over the batch for the edge / voxel bookkeeping kernels */
     int32_t voxel_blocks;       /* workgroups over the batch for the per-voxel kernels (QEF, emit) */
     int32_t cross_lanes;        /* lanes sharing one edge in the edge-crossing stage: 1, 2, 4 or 8 */
-    int32_t reserved[3];
+    int32_t density_per_cu;     /* persistent workgroups per CU in the density stage, 1..2 (0: 1) */
+    int32_t reserved[2];
 } lvn_tuning;
 
 int lvn_batch_get_tuning(lvn_batch* b, lvn_tuning* out);

@@ -137,7 +137,8 @@ class Tuning(C.Structure):
         ("row_blocks", C.c_int32),
         ("voxel_blocks", C.c_int32),
         ("cross_lanes", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("density_per_cu", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 

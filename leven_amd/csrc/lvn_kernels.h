@@ -28,7 +28,8 @@ struct KArgs {
     int* queue;              // kNumQueues work-queue heads, zeroed by launch_clear_counters
     int* item_prefix;        // chunks + 1 edge-item offsets (k_edge_items)
     int cus;                 // compute units of the device
-    int persist_per_cu;      // persistent workgroups per CU (density, edge crossing)
+    int persist_per_cu;      // persistent workgroups per CU (edge crossing)
+    int density_per_cu;      // persistent workgroups per CU (density)
     const int* slot_map;     // batch entry -> arena slot, or null for the identity
     int row_blocks;          // workgroups over the batch for the bookkeeping kernels
     int voxel_blocks;        // workgroups over the batch for the per-voxel kernels

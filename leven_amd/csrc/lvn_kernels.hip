@@ -1418,18 +1418,23 @@ void launch_clear_counters(const KArgs& a, int n, hipStream_t s) {
                        a.counters, n, a.queue);
 }
 
-// persist_per_cu workgroups per CU (at most two: the LDS table limit), fewer when
-// the work is small.
-static unsigned persistent_blocks(const KArgs& a, long items) {
-    return (unsigned)std::max(1L, std::min((long)a.persist_per_cu * a.cus, (items + 15) / 16));
+// per_cu workgroups per CU (at most two: the LDS table limit), fewer when the work is
+// small.
+static unsigned persistent_blocks(const KArgs& a, int per_cu, long items) {
+    return (unsigned)std::max(1L, std::min((long)per_cu * a.cus, (items + 15) / 16));
 }
 
 void launch_density(const KArgs& a, int chunks, hipStream_t s) {
     const long items = (((long)a.G.F * a.G.F * a.ysplit + 63) / 64) * chunks;
+    // One workgroup per CU by default: at k_density's 93 SGPRs a SIMD holds the 4 waves
+    // of one 1024-thread workgroup but not the 8 of two, so a second workgroup would
+    // only start once the first has emptied the queue (it then loads its tables and
+    // exits, 0.1 ms of tail on a 512-chunk batch).
+    const unsigned blocks = persistent_blocks(a, a.density_per_cu, items);
     if (a.work)
-        hipLaunchKernelGGL(k_density<true>, dim3(persistent_blocks(a, items)), dim3(1024), 0, s, a, chunks);
+        hipLaunchKernelGGL(k_density<true>, dim3(blocks), dim3(1024), 0, s, a, chunks);
     else
-        hipLaunchKernelGGL(k_density<false>, dim3(persistent_blocks(a, items)), dim3(1024), 0, s, a, chunks);
+        hipLaunchKernelGGL(k_density<false>, dim3(blocks), dim3(1024), 0, s, a, chunks);
 }
 void launch_csg(const KArgs& a, int chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_csg, dim3(1, chunks), dim3(1024), 0, s, a);
@@ -1469,7 +1474,7 @@ static int cross_lanes(const KArgs& a, int chunks) {
 
 template <int kG>
 static void launch_cross_group(const KArgs& a, int chunks, hipStream_t s) {
-    const dim3 grid(persistent_blocks(a, 512L * kG * chunks));
+    const dim3 grid(persistent_blocks(a, a.persist_per_cu, 512L * kG * chunks));
     if (a.work)
         hipLaunchKernelGGL((k_edge_cross_group<true, kG>), grid, dim3(kCrossThreads), 0, s, a, chunks);
     else
@@ -1486,7 +1491,7 @@ void launch_edge_cross(const KArgs& a, int chunks, hipStream_t s) {
     case 8: launch_cross_group<8>(a, chunks, s); return;
     default: break;
     }
-    const dim3 grid(persistent_blocks(a, 512L * chunks));
+    const dim3 grid(persistent_blocks(a, a.persist_per_cu, 512L * chunks));
     if (a.work)
         hipLaunchKernelGGL(k_edge_cross<true>, grid, dim3(kCrossThreads), 0, s, a, chunks);
     else

@@ -325,6 +325,7 @@ KArgs Ctx::kargs() const {
     a.item_prefix = d_item_prefix;
     a.cus = cus;
     a.persist_per_cu = kDefaultPersistPerCu;
+    a.density_per_cu = kDefaultDensityPerCu;
     a.row_blocks = kDefaultRowBlocks;
     a.voxel_blocks = kDefaultVoxelBlocks;
     return a;
@@ -608,6 +609,7 @@ static lvn_tuning effective_tuning(const Batch* b) {
     lvn_tuning t = b->tuning;
     if (t.density_ysplit == 0) t.density_ysplit = b->n >= 64 ? 1 : 6;
     if (t.cross_lanes == 0) t.cross_lanes = default_cross_lanes(b->n);
+    if (t.density_per_cu == 0) t.density_per_cu = kDefaultDensityPerCu;
     return t;
 }
 
@@ -615,6 +617,7 @@ static KArgs batch_args(const Batch* b, const lvn_tuning& t) {
     KArgs a = b->ctx.kargs();
     a.ysplit = t.density_ysplit;
     a.persist_per_cu = t.persistent_per_cu;
+    a.density_per_cu = t.density_per_cu;
     a.row_blocks = t.row_blocks;
     a.voxel_blocks = t.voxel_blocks;
     a.cross_lanes = t.cross_lanes;
@@ -646,7 +649,8 @@ static bool valid_tuning(const Batch* b, const lvn_tuning& t) {
            t.persistent_per_cu <= 2 && t.row_blocks >= 1 && t.row_blocks <= (1 << 20) && t.voxel_blocks >= 1 &&
            t.voxel_blocks <= (1 << 20) &&
            (t.cross_lanes == 0 || t.cross_lanes == 1 || t.cross_lanes == 2 || t.cross_lanes == 4 ||
-            t.cross_lanes == 8);
+            t.cross_lanes == 8) &&
+           t.density_per_cu >= 0 && t.density_per_cu <= 2;
 }
 
 static void drop_graph(Batch* b) { b->graph_n = -1; }
@@ -981,7 +985,8 @@ int lvn_batch_autotune(lvn_batch* bh, int32_t reps, lvn_tuning* out) {
         };
         const std::vector<Param> params = {
             {&lvn_tuning::density_ysplit, {1, 2, 3, 6}, {0}},
-            {&lvn_tuning::persistent_per_cu, {1, 2}, {0, 5}},
+            {&lvn_tuning::density_per_cu, {1, 2}, {0}},
+            {&lvn_tuning::persistent_per_cu, {1, 2}, {5}},
             {&lvn_tuning::cross_lanes, {1, 2, 4, 8}, {5}},
             {&lvn_tuning::row_blocks, {1024, 2048, 4096, 8192, 16384, 32768}, {2, 4, 6, 8}},
             {&lvn_tuning::voxel_blocks, {1024, 2048, 4096, 8192, 16384, 32768}, {9, 11}},
