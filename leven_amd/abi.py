@@ -138,7 +138,8 @@ class Tuning(C.Structure):
         ("voxel_blocks", C.c_int32),
         ("cross_lanes", C.c_int32),
         ("density_per_cu", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("height_lines", C.c_int32),
+        ("reserved", C.c_int32 * 1),
     ]
 
 

@@ -30,6 +30,7 @@ struct KArgs {
     int cus;                 // compute units of the device
     int persist_per_cu;      // persistent workgroups per CU (edge crossing)
     int density_per_cu;      // persistent workgroups per CU (density)
+    int height_lines;        // 1: x/z-edge sample heights evaluated once per line (k_height_lines)
     const int* slot_map;     // batch entry -> arena slot, or null for the identity
     int row_blocks;          // workgroups over the batch for the bookkeeping kernels
     int voxel_blocks;        // workgroups over the batch for the per-voxel kernels

@@ -410,12 +410,93 @@ __global__ __launch_bounds__(256) void k_edge_compact(KArgs a) {
 // Edge crossing + normal
 // ---------------------------------------------------------------------------
 
+// Heights along the lines of x- and z-edges.  An x-edge (x, y, z) samples the height
+// term at (x + k/16, z) and a z-edge at (x, z + k/16): the same values for every y, and
+// caves give a line about three crossing edges.  So the interior samples k = 1..15 of
+// every line holding a crossing x- or z-edge are evaluated once, before the crossing
+// kernels, by the expressions those kernels use (bit-identical values), and read back
+// there.  Line (sel, x, z), sel 0 for x-edges and 1 for z-edges, starts at:
+__device__ __forceinline__ int hline_index(int H, int sel, int x, int z) { return ((sel * H + z) * H + x) * kLineSamples; }
+
+// Position of the j-th set bit (from 0) of m; m has more than j set bits.
+__device__ __forceinline__ int nth_bit(uint64_t m, int j) {
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const uint64_t low = m & ((1ull << w) - 1ull);
+        const int c = __popcll(low);
+        if (j >= c) {
+            j -= c;
+            m >>= w;
+            pos += w;
+        } else {
+            m = low;
+        }
+    }
+    return pos;
+}
+
+// Item = (chunk, sel, z), taken by the grid's waves in a static stride: items are
+// short (a few hundred height evaluations), and a shared atomic queue serves only about
+// 77,000 grabs per ms, which would bound this pass at 0.86 ms for 512 chunks.  The lines
+// needed are the OR over y of the chunk's edge rows (x-edge or z-edge words); the lanes
+// run over their samples only.  Chunks whose edges take no density crossings (the
+// cached-field form) are skipped.
+template <bool kCount>
+__global__ __launch_bounds__(1024) void k_height_lines(KArgs a, int chunks) {
+    __shared__ NoiseLds T;
+    load_noise_tables(T, a.perm, a.grads);
+    const Geometry& G = a.G;
+    const int H = G.H;
+    const int per_chunk = 2 * H;
+    const int total = per_chunk * chunks;
+    const int lane = threadIdx.x & 63;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    for (int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); item < total; item += waves) {
+        const int chunk = item / per_chunk;
+        const int r = item - chunk * per_chunk;
+        const int sel = r / H, z = r - sel * H;
+        const ChunkDesc& cd = a.chunks[chunk];
+        if (cd.mode == kModeCached) continue;  // uniform over the wave
+        const uint64_t* E = slot_ptr<const uint64_t>(a, chunk, a.L.e_rows);
+        const int word = sel == 0 ? 0 : 2;  // x-edge or z-edge bits of x < 64
+        uint64_t lo = 0, hi = 0;
+        for (int y = lane; y < H; y += 64) {
+            const int row = y + H * z;
+            lo |= E[4 * row + word];
+            hi |= (E[4 * row + 3] >> word) & 1ull;  // bit 0: x-edge, bit 2: z-edge at x = 64
+        }
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            lo |= __shfl_xor(lo, m);
+            hi |= __shfl_xor(hi, m);
+        }
+        float* out = slot_ptr<float>(a, chunk, a.L.h_lines) + hline_index(H, sel, 0, z);
+        const int n_lo = __popcll(lo), n = n_lo + (int)(hi & 1ull);  // hi is set only when H > 64
+        unsigned nh = 0;
+        for (int p = lane; p < n * kLineSamples; p += 64) {
+            const int j = p / kLineSamples, k = p - j * kLineSamples + 1;
+            const int x = j < n_lo ? nth_bit(lo, j) : 64;
+            const int w0x = x * cd.scale + cd.off[0], w0z = z * cd.scale + cd.off[2];
+            const float p0x = (float)w0x, p0z = (float)w0z;
+            const float p1x = (float)(w0x + (sel == 0 ? cd.scale : 0));
+            const float p1z = (float)(w0z + (sel == 1 ? cd.scale : 0));
+            const float cur = (float)k * 0.0625f;  // = the sequential scan's running sum (exact)
+            out[x * kLineSamples + (k - 1)] = height_term(a.F, T, p0x + (p1x - p0x) * cur, p0z + (p1z - p0z) * cur);
+            ++nh;
+        }
+        if constexpr (kCount) atomicAdd(&a.work[2], (unsigned long long)nh);
+    }
+}
+
 // FindEdgeIntersectionInfo, cl/density_field.cl:96-151: first arg-min of |d|
 // over t = k/16, then a central-difference normal with h = 0.001.  The height
-// term of samples that share (x, z) is evaluated once; the two end samples are
-// the stored field samples when the field is FP32.
+// term of samples that share (x, z) is evaluated once: per y-edge, and per x- or
+// z-edge line (`hl`, k_height_lines); the two end samples are the stored field
+// samples when the field is FP32.
 __device__ __forceinline__ float4 density_crossing(const KArgs& a, const NoiseLds& T, const ChunkDesc& cd, int x, int y, int z,
-                                   int axis, bool ends_known, float d_start, float d_end, unsigned& nh, unsigned& nc) {
+                                   int axis, bool ends_known, float d_start, float d_end, const float* hl, unsigned& nh,
+                                   unsigned& nc) {
     const FieldParams& Fp = a.F;
     const bool caves = Fp.kind == LVN_FIELD_TERRAIN_CAVES;
     const int w0x = x * cd.scale + cd.off[0], w0y = y * cd.scale + cd.off[1], w0z = z * cd.scale + cd.off[2];
@@ -426,6 +507,8 @@ __device__ __forceinline__ float4 density_crossing(const KArgs& a, const NoiseLd
     const float colH = axis == 1 ? height_term(Fp, T, p0x, p0z) : 0.f;
     nh += axis == 1 ? 1u : 0u;
     float minV = 3.402823466e+38f, tbest = 0.f, Hbest = colH, cur = 0.f;
+    const bool lines = a.height_lines != 0 && axis != 1;
+    float Hn = lines ? hl[0] : 0.f;  // the next interior sample's line height, loaded a sample ahead
     for (int k = 0; k <= 16; ++k) {
         const float px = p0x + (p1x - p0x) * cur;
         const float py = p0y + (p1y - p0y) * cur;
@@ -437,8 +520,13 @@ __device__ __forceinline__ float4 density_crossing(const KArgs& a, const NoiseLd
             d = fabsf(d_end);
         } else {
             if (axis != 1) {
-                H = height_term(Fp, T, px, pz);
-                ++nh;
+                if (lines && k >= 1 && k <= kLineSamples) {
+                    H = Hn;
+                    if (k < kLineSamples) Hn = hl[k];
+                } else {
+                    H = height_term(Fp, T, px, pz);
+                    ++nh;
+                }
             }
             nc += (caves && !((py - H) > Fp.cave_cut)) ? 1u : 0u;
             d = fabsf(combine(Fp, T, py - H, px, py, pz));
@@ -554,8 +642,9 @@ __device__ __forceinline__ void cross_edge(const KArgs& a, const NoiseLds& T, in
     } else {
         const float* dens = slot_ptr<float>(a, chunk, a.L.density);
         const bool ends = !a.fp16;
+        const float* hl = slot_ptr<const float>(a, chunk, a.L.h_lines) + hline_index(G.H, axis == 2 ? 1 : 0, x, z);
         unsigned nh = 0, nc = 0;
-        out = density_crossing(a, T, cd, x, y, z, axis, ends, ends ? dens[i0] : 0.f, ends ? dens[i1] : 0.f, nh, nc);
+        out = density_crossing(a, T, cd, x, y, z, axis, ends, ends ? dens[i0] : 0.f, ends ? dens[i1] : 0.f, hl, nh, nc);
         if constexpr (kCount) {
             atomicAdd(&a.work[2], (unsigned long long)nh);
             atomicAdd(&a.work[3], (unsigned long long)nc);
@@ -679,8 +768,12 @@ __device__ __forceinline__ void cross_edge_group(const KArgs& a, const NoiseLds&
     unsigned nh = 0, nc = 0;
     const float colH = axis == 1 ? height_term(Fp, T, p0x, p0z) : 0.f;
     nh += (axis == 1 && sub == 0) ? 1u : 0u;
+    const float* hl = slot_ptr<const float>(a, chunk, a.L.h_lines) + hline_index(G.H, axis == 2 ? 1 : 0, x, z);
     float minV = 3.402823466e+38f, Hbest = colH;
     int kbest = 17;
+    const int k_first = sub == 0 ? kG : sub;  // this lane's first interior sample
+    const bool lines = a.height_lines != 0 && axis != 1;
+    float Hn = (lines && k_first <= kLineSamples) ? hl[k_first - 1] : 0.f;
     for (int k = sub; k <= 16; k += kG) {
         const float cur = (float)k * 0.0625f;  // = the sequential scan's running sum (exact)
         const float px = p0x + (p1x - p0x) * cur;
@@ -693,8 +786,13 @@ __device__ __forceinline__ void cross_edge_group(const KArgs& a, const NoiseLds&
             d = fabsf(d_end);
         } else {
             if (axis != 1) {
-                H = height_term(Fp, T, px, pz);
-                ++nh;
+                if (lines && k >= 1 && k <= kLineSamples) {
+                    H = Hn;
+                    if (k + kG <= kLineSamples) Hn = hl[k + kG - 1];
+                } else {
+                    H = height_term(Fp, T, px, pz);
+                    ++nh;
+                }
             }
             nc += (caves && !((py - H) > Fp.cave_cut)) ? 1u : 0u;
             d = fabsf(combine(Fp, T, py - H, px, py, pz));
@@ -1482,6 +1580,14 @@ static void launch_cross_group(const KArgs& a, int chunks, hipStream_t s) {
 }
 
 void launch_edge_cross(const KArgs& a, int chunks, hipStream_t s) {
+    if (a.height_lines) {
+        const long line_items = 2L * a.G.H * chunks;
+        const dim3 line_grid(persistent_blocks(a, 2, line_items));  // 63 SGPRs: two workgroups per CU stay resident
+        if (a.work)
+            hipLaunchKernelGGL(k_height_lines<true>, line_grid, dim3(1024), 0, s, a, chunks);
+        else
+            hipLaunchKernelGGL(k_height_lines<false>, line_grid, dim3(1024), 0, s, a, chunks);
+    }
     const int g = cross_lanes(a, chunks);
     hipLaunchKernelGGL(k_edge_items, dim3(1), dim3(1024), 0, s, a, chunks, 64 / g);
     // the item count is on the device; size the grid for 512 edge items per chunk

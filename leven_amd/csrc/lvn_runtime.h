@@ -145,7 +145,7 @@ struct Batch {
     const void* graph_index = nullptr;
     uint64_t graph_gen = 0;
     unsigned long long* d_work = nullptr;
-    lvn_tuning tuning = {0, 2, 8192, 8192, 0, 1, {0, 0}};  // ysplit, cross_lanes 0 = by batch size
+    lvn_tuning tuning = {0, 2, 8192, 8192, 0, 1, 0, {0}};  // ysplit, cross_lanes, height_lines 0 = by batch size
     std::map<Key, int> slot_of;      // staged chunk (min, size) -> slot
     ~Batch();
 };
@@ -153,6 +153,8 @@ struct Batch {
 // Launch-shape defaults shared by every form (the batched form overrides them per batch).
 constexpr int kDefaultPersistPerCu = 2;
 constexpr int kDefaultDensityPerCu = 1;
+// Batches from this many chunks evaluate x/z-edge line heights once per line by default.
+constexpr int kHeightLinesMinChunks = 64;
 constexpr int kDefaultRowBlocks = 8192;
 constexpr int kDefaultVoxelBlocks = 8192;
 

@@ -224,6 +224,7 @@ SlotLayout make_layout(const Geometry& G, const Caps& c) {
     L.e_ids = take((uint64_t)c.edges * 4);
     L.e_data = take((uint64_t)c.edges * 16);
     L.e_order = take((uint64_t)c.edges * 4);
+    L.h_lines = take((uint64_t)2 * G.H * G.H * kLineSamples * 4);
     L.v_mask = take((uint64_t)G.voxel_segs * 8);
     L.v_count = take((uint64_t)G.voxel_segs * 4);
     L.v_base = take((uint64_t)G.voxel_segs * 4);
@@ -326,6 +327,7 @@ KArgs Ctx::kargs() const {
     a.cus = cus;
     a.persist_per_cu = kDefaultPersistPerCu;
     a.density_per_cu = kDefaultDensityPerCu;
+    a.height_lines = 0;
     a.row_blocks = kDefaultRowBlocks;
     a.voxel_blocks = kDefaultVoxelBlocks;
     return a;
@@ -610,6 +612,7 @@ static lvn_tuning effective_tuning(const Batch* b) {
     if (t.density_ysplit == 0) t.density_ysplit = b->n >= 64 ? 1 : 6;
     if (t.cross_lanes == 0) t.cross_lanes = default_cross_lanes(b->n);
     if (t.density_per_cu == 0) t.density_per_cu = kDefaultDensityPerCu;
+    if (t.height_lines == 0) t.height_lines = b->n >= kHeightLinesMinChunks ? 2 : 1;
     return t;
 }
 
@@ -618,6 +621,7 @@ static KArgs batch_args(const Batch* b, const lvn_tuning& t) {
     a.ysplit = t.density_ysplit;
     a.persist_per_cu = t.persistent_per_cu;
     a.density_per_cu = t.density_per_cu;
+    a.height_lines = t.height_lines == 2 ? 1 : 0;
     a.row_blocks = t.row_blocks;
     a.voxel_blocks = t.voxel_blocks;
     a.cross_lanes = t.cross_lanes;
@@ -650,7 +654,7 @@ static bool valid_tuning(const Batch* b, const lvn_tuning& t) {
            t.voxel_blocks <= (1 << 20) &&
            (t.cross_lanes == 0 || t.cross_lanes == 1 || t.cross_lanes == 2 || t.cross_lanes == 4 ||
             t.cross_lanes == 8) &&
-           t.density_per_cu >= 0 && t.density_per_cu <= 2;
+           t.density_per_cu >= 0 && t.density_per_cu <= 2 && t.height_lines >= 0 && t.height_lines <= 2;
 }
 
 static void drop_graph(Batch* b) { b->graph_n = -1; }
@@ -987,6 +991,7 @@ int lvn_batch_autotune(lvn_batch* bh, int32_t reps, lvn_tuning* out) {
             {&lvn_tuning::density_ysplit, {1, 2, 3, 6}, {0}},
             {&lvn_tuning::density_per_cu, {1, 2}, {0}},
             {&lvn_tuning::persistent_per_cu, {1, 2}, {5}},
+            {&lvn_tuning::height_lines, {1, 2}, {5}},
             {&lvn_tuning::cross_lanes, {1, 2, 4, 8}, {5}},
             {&lvn_tuning::row_blocks, {1024, 2048, 4096, 8192, 16384, 32768}, {2, 4, 6, 8}},
             {&lvn_tuning::voxel_blocks, {1024, 2048, 4096, 8192, 16384, 32768}, {9, 11}},

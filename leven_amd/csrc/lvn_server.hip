@@ -275,6 +275,7 @@ int Server::launch(Lane& L, const std::vector<Job>& batch, const std::vector<int
     a.ops = L.d_ops;
     a.op_index = L.d_op_index;
     a.ysplit = n >= 32 ? 1 : 6;
+    a.height_lines = n >= kHeightLinesMinChunks ? 1 : 0;
     launch_pipeline(a, n, L.stream);
     LVN_TRY(hipGetLastError());
     LVN_TRY(hipMemcpyAsync(L.h_counters, L.d_counters, sizeof(Counters) * n, hipMemcpyDeviceToHost, L.stream));

@@ -17,6 +17,10 @@ constexpr int kPermPaddedBytes = ((kPermPitch * kPermRows + 15) / 16) * 16;
 constexpr int kGradEntries = 256;
 
 
+// Interior samples (k = 1..15 of the 17) of an x- or z-edge line whose heights are
+// evaluated once per line (k_height_lines).
+constexpr int kLineSamples = 15;
+
 constexpr int kAir = LVN_MATERIAL_AIR;    // src/volume_materials.h:9
 constexpr int kNone = LVN_MATERIAL_NONE;  // src/volume_materials.h:8
 constexpr int kLeafSizeScale = 4;         // src/volume_constants.h:7-8
@@ -71,6 +75,7 @@ struct SlotLayout {
     uint64_t solid;                       // field_rows x 2 words: bit x = sample (x, y, z) is not air
     uint64_t e_rows, e_count, e_base, e_ids, e_data;  // e_rows: edge_rows x {x, y, z, x = 64} words
     uint64_t e_count_y, e_base_y, e_order;  // y-edges per row and their scan; crossing order (y-edges first)
+    uint64_t h_lines;                     // heights at k/16 (k = 1..15) along x- and z-edge lines, 2 x H x H x 15 f32
     uint64_t v_mask, v_count, v_base, v_code, v_emask, v_mat;
     uint64_t q_count, q_base, tris;
     uint64_t s_mask, s_count, s_base, seams, verts;

@@ -356,7 +356,8 @@ class Batch:
         names = [lib().lvn_batch_stage_name(i).decode() for i in range(NUM_STAGES)]
         return dict(zip(names, ms.astype(float).tolist()))
 
-    TUNING = ["density_ysplit", "persistent_per_cu", "row_blocks", "voxel_blocks", "cross_lanes", "density_per_cu"]
+    TUNING = ["density_ysplit", "persistent_per_cu", "row_blocks", "voxel_blocks", "cross_lanes", "density_per_cu",
+              "height_lines"]
 
     def tuning(self):
         t = Tuning()

@@ -255,8 +255,8 @@ def test_batched_matches_oracle_and_replays(oracle):
 
 def test_launch_shapes_do_not_change_results():
     """Every launch shape (column split, persistent workgroups per CU, workgroups over the
-    batch, lanes per edge in the crossing stage) and the autotuner's pick give
-    bit-identical meshes."""
+    batch, lanes per edge in the crossing stage, x/z-edge heights per sample or once per
+    line) and the autotuner's pick give bit-identical meshes."""
     w = workloads.terrain_512()
     setup_field(w["desc"], w["seed"])
     for op, bb in zip(w["ops"], w["aabbs"]):
@@ -268,12 +268,16 @@ def test_launch_shapes_do_not_change_results():
     b.wait()
     ref = [b.get_mesh(i) for i in range(len(pick))]
     assert sum(len(m[1]) > 0 for m in ref) >= 5
-    shapes = [dict(density_ysplit=3, persistent_per_cu=1, row_blocks=1, voxel_blocks=7, cross_lanes=2, density_per_cu=2),
-              dict(density_ysplit=11, persistent_per_cu=2, row_blocks=37, voxel_blocks=1, cross_lanes=4, density_per_cu=1),
+    shapes = [dict(density_ysplit=3, persistent_per_cu=1, row_blocks=1, voxel_blocks=7, cross_lanes=2, density_per_cu=2,
+                   height_lines=2),
+              dict(density_ysplit=11, persistent_per_cu=2, row_blocks=37, voxel_blocks=1, cross_lanes=4, density_per_cu=1,
+                   height_lines=1),
               dict(density_ysplit=66, persistent_per_cu=1, row_blocks=100000, voxel_blocks=100000, cross_lanes=8,
-                   density_per_cu=1),
+                   density_per_cu=1, height_lines=2),
               dict(density_ysplit=2, persistent_per_cu=2, row_blocks=4096, voxel_blocks=4096, cross_lanes=1,
-                   density_per_cu=2)]
+                   density_per_cu=2, height_lines=2),
+              dict(density_ysplit=1, persistent_per_cu=2, row_blocks=4096, voxel_blocks=4096, cross_lanes=1,
+                   density_per_cu=1, height_lines=1)]
     for shape in shapes:
         b.set_tuning(**shape)
         assert b.tuning() == shape
@@ -294,6 +298,8 @@ def test_launch_shapes_do_not_change_results():
         b.set_tuning(cross_lanes=3)
     with pytest.raises(RuntimeError):
         b.set_tuning(density_per_cu=3)
+    with pytest.raises(RuntimeError):
+        b.set_tuning(height_lines=3)
     mesher.clear_csg_ops()
 
 
