@@ -44,12 +44,10 @@ __device__ __forceinline__ void decode_voxel(const Geometry& G, int v, int& x, i
 
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
 
-// Dense lookup of a compacted element through its segment mask: index or -1.
-__device__ __forceinline__ int dense_index(const uint64_t* mask, const int* base, int i) {
-    const uint64_t m = mask[i >> 6];
-    const int lane = i & 63;
-    if (!((m >> lane) & 1ull)) return -1;
-    return base[i >> 6] + __popcll(m & lanemask_lt(lane));
+// Dense lookup of a compacted element through its segment's mask word m (bit `lane`)
+// and base: its index, or -1.
+__device__ __forceinline__ int dense_index_of(uint64_t m, int base, int lane) {
+    return ((m >> lane) & 1ull) ? base + __popcll(m & lanemask_lt(lane)) : -1;
 }
 
 // Work split of a chunk over the x dimension of its grid (blockIdx.y is the chunk).
@@ -1448,15 +1446,37 @@ __global__ __launch_bounds__(256) void k_emit(KArgs a) {
     const uint64_t lt = lanemask_lt(lane);
     for (int base = wave_in_chunk() << 6; base < V; base += waves_per_chunk() << 6) {
         const int vi = base + lane, seg = base >> 6;
-        const int qbits = vi < V ? (vemask[vi] >> 12) & 7 : 0;
+        // the voxel's own words, loaded together (a lane past the end reads the last voxel)
+        const int vc = min(vi, V - 1);
+        const int em = vemask[vc], code = vcode[vc], m = vmat[vc], qb = qbase[seg];
+        const uint64_t sm = smask[seg];
+        const int qbits = vi < V ? (em >> 12) & 7 : 0;
         const int nq = __popc(qbits);
         const uint64_t b0 = __ballot(nq & 1), b1 = __ballot(nq & 2);
-        const uint64_t sm = smask[seg];
         const bool seam = (sm >> lane) & 1ull;
         if (vi >= V || (!qbits && !seam)) continue;
-        const int3 p = position_for_code((uint32_t)vcode[vi], G.depth);
-        const int m = vmat[vi];
-        int qi = qbase[seg] + __popcll(b0 & lt) + 2 * __popcll(b1 & lt);
+        const int3 p = position_for_code((uint32_t)code, G.depth);
+        int qi = qb + __popcll(b0 & lt) + 2 * __popcll(b1 & lt);
+        // The nine neighbour lookups (three per axis) are loaded together, before any is
+        // used: one memory round trip instead of one per axis and step.  An index is
+        // clamped into the grid, so an axis without a quad reads a valid, unused word.
+        const int last = G.N * G.N * G.N - 1;
+        uint64_t nw[3][3];
+        int nb[3][3], nl[3][3];
+#pragma unroll
+        for (int axis = 0; axis < 3; ++axis) {
+#pragma unroll
+            for (int n = 1; n < 4; ++n) {
+                const int x = p.x + kEdgeNodeOffsets[axis][n][0];
+                const int y = p.y + kEdgeNodeOffsets[axis][n][1];
+                const int z = p.z + kEdgeNodeOffsets[axis][n][2];
+                const int i = min(x + G.N * (y + G.N * z), last);
+                nw[axis][n - 1] = vm[i >> 6];
+                nb[axis][n - 1] = vb[i >> 6];
+                nl[axis][n - 1] = i & 63;
+            }
+        }
+#pragma unroll
         for (int axis = 0; axis < 3; ++axis) {
             if (!((qbits >> axis) & 1)) continue;
             int nodes[4];
@@ -1464,10 +1484,7 @@ __global__ __launch_bounds__(256) void k_emit(KArgs a) {
             bool ok = true;
 #pragma unroll
             for (int n = 1; n < 4; ++n) {
-                const int x = p.x + kEdgeNodeOffsets[axis][n][0];
-                const int y = p.y + kEdgeNodeOffsets[axis][n][1];
-                const int z = p.z + kEdgeNodeOffsets[axis][n][2];
-                nodes[n] = dense_index(vm, vb, x + G.N * (y + G.N * z));
+                nodes[n] = dense_index_of(nw[axis][n - 1], nb[axis][n - 1], nl[axis][n - 1]);
                 ok = ok && nodes[n] >= 0;
             }
             if (!ok) {  // cannot happen: all four voxels around a sign-changing interior edge are active
@@ -1489,9 +1506,10 @@ __global__ __launch_bounds__(256) void k_emit(KArgs a) {
         if (seam) {
             const int si = sbase[seg] + __popcll(sm & lt);
             if (si < a.caps.seams) {
+                const float4 pos = verts[3 * vi], nrm = verts[3 * vi + 1];
                 reinterpret_cast<int4*>(seams)[3 * si] = make_int4(p.x, p.y, p.z, m);
-                seams[3 * si + 1] = verts[3 * vi];
-                seams[3 * si + 2] = verts[3 * vi + 1];
+                seams[3 * si + 1] = pos;
+                seams[3 * si + 2] = nrm;
             }
         }
     }
