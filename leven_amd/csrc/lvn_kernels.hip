@@ -83,6 +83,27 @@ __device__ __forceinline__ int edge_index(int H, const uint64_t* rows, const int
     return base[r] + k;
 }
 
+// edge_index from a row's four words (x, y, z edge bits of x < 64; bits of x = 64) and
+// its base, already loaded.
+__device__ __forceinline__ int edge_index_in_row(const uint64_t w[4], int base, int x, int axis) {
+    const uint64_t ex = w[0], ey = w[1], ez = w[2];
+    if (x < 64) {
+        const uint64_t bits = axis == 0 ? ex : axis == 1 ? ey : ez;
+        if (!((bits >> x) & 1ull)) return -1;
+        const uint64_t lt = lanemask_lt(x);
+        int k = __popcll(ex & lt) + __popcll(ey & lt) + __popcll(ez & lt);
+        if (axis > 0) k += (int)((ex >> x) & 1ull);
+        if (axis > 1) k += (int)((ey >> x) & 1ull);
+        return base + k;
+    }
+    const uint64_t hi = w[3];
+    if (!((hi >> axis) & 1ull)) return -1;
+    int k = __popcll(ex) + __popcll(ey) + __popcll(ez);
+    if (axis > 0) k += (int)(hi & 1ull);
+    if (axis > 1) k += (int)((hi >> 1) & 1ull);
+    return base + k;
+}
+
 // FP16 store that keeps the sign of every negative sample (an underflow to -0
 // would turn a solid sample into air under the reference's `density < 0` test,
 // cl/density_field.cl:34).
@@ -1046,15 +1067,36 @@ __device__ __forceinline__ void qef_fast(const KArgs& a, int chunk, const ChunkD
     const float4* ed = slot_ptr<float4>(a, chunk, a.L.e_data);
     const float fscale = (float)cd.scale;
 
+    // The voxel's edges start at its corners, so they lie in four edge rows, (y or y + 1,
+    // z or z + 1): those rows and bases are loaded together and every edge's index is
+    // found from them, so only the hermite-data loads remain in the accumulation loop
+    // (loading those ahead as well was slower: 99 VGPRs, 4 waves per SIMD).
+    uint64_t rw[4][4];
+    int rb[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int r = (p.y + (j & 1)) + G.H * (p.z + (j >> 1));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) rw[j][q] = erows[4 * r + q];
+        rb[j] = eb[r];
+    }
+    int ke[12];
+#pragma unroll
+    for (int e = 0; e < 12; ++e) {
+        const int e0 = kEdgeMap[e][0];
+        const int j = ((e0 >> 1) & 1) + 2 * (e0 & 1);  // row of the edge's lower end
+        const int k = ((emask >> e) & 1) ? edge_index_in_row(rw[j], rb[j], p.x + ((e0 >> 2) & 1), e >> 2) : -1;
+        ke[e] = k >= 0 && k < a.caps.edges ? k : -1;
+    }
+
     float ata0 = 0.f, ata1 = 0.f, ata2 = 0.f, ata3 = 0.f, ata4 = 0.f, ata5 = 0.f;
     Q4 atb = {0.f, 0.f, 0.f, 0.f}, mass = {0.f, 0.f, 0.f, 0.f}, nsum = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
     for (int e = 0; e < 12; ++e) {
-        if (!((emask >> e) & 1)) continue;
-        const int e0 = kEdgeMap[e][0], e1 = kEdgeMap[e][1], axis = e >> 2;
+        if (ke[e] < 0) continue;
+        const int e0 = kEdgeMap[e][0], e1 = kEdgeMap[e][1];
         const int hx = p.x + ((e0 >> 2) & 1), hy = p.y + ((e0 >> 1) & 1), hz = p.z + (e0 & 1);
-        const int k = edge_index(G.H, erows, eb, hx, hy, hz, axis);
-        if (k < 0 || k >= a.caps.edges) continue;
-        const float4 d = ed[k];
+        const float4 d = ed[ke[e]];
         const float a0x = (float)hx, a0y = (float)hy, a0z = (float)hz;
         const float a1x = (float)(p.x + ((e1 >> 2) & 1)), a1y = (float)(p.y + ((e1 >> 1) & 1)),
                     a1z = (float)(p.z + (e1 & 1));
