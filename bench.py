@@ -40,11 +40,20 @@ def flops_cave(desc):
 
 
 def stage_flops(w, desc):
-    """Algorithmic FLOPs of the density and edge-crossing stages from the kernels' work counters."""
+    """FLOPs the density and edge-crossing stages execute, from the kernels' work counters
+    (after the cave-term cut, shared heights and stored end samples)."""
     fh, fc = flops_height(desc), flops_cave(desc)
     density = w["density_height"] * fh + w["density_cave"] * fc + w["samples"] * 2
     cross = w["cross_height"] * fh + w["cross_cave"] * fc + w["edges"] * 200 + w["cross_brush"] * 20
     return density, cross
+
+
+def algorithmic_flops(chunks, edges, desc):
+    """Algorithmic FLOPs per launch: 65^3 density evaluations per chunk, and 23 per edge
+    (17 crossing samples + 6 normal samples), each a full evaluation of the field
+    (height term + cave term) at the FLOP counts above."""
+    fd = flops_height(desc) + (flops_cave(desc) if desc.kind == 1 else 0)
+    return chunks * 65 ** 3 * fd, edges * 23 * fd
 
 
 def hbm_traffic(kernel, workload):
@@ -472,7 +481,9 @@ def main():
     work = batch.work()
     d_flops, c_flops = stage_flops(work, desc)
     dom = max(("density", "edge_cross"), key=lambda k: stage_ms[k])
-    dom_flops = d_flops if dom == "density" else c_flops
+    a_d, a_c = algorithmic_flops(n, work["edges"], desc)
+    dom_flops = a_d if dom == "density" else a_c
+    exec_flops = d_flops if dom == "density" else c_flops
     achieved = dom_flops / (stage_ms[dom] * 1e-3) / 1e12
 
     # ---- plain per-chunk form ---------------------------------------------------
@@ -524,7 +535,11 @@ def main():
         "roofline": {"bound": "valu", "kernel": "k_" + dom, "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": hbm_traffic("k_" + dom, wl["name"]), "flops_per_launch": int(dom_flops),
+                     "flops_unit": "23 full field evaluations per edge" if dom == "edge_cross"
+                     else "65^3 full field evaluations per chunk",
                      "launch_ms": round(stage_ms[dom], 4),
+                     "executed_flops_per_launch": int(exec_flops),
+                     "executed_tflops": round(exec_flops / (stage_ms[dom] * 1e-3) / 1e12, 3),
                      "note": "FP32 vector-ALU bound (neither HBM nor MFMA): rocprofv3 counts 0.40 VALU "
                              "instructions per SIMD-cycle against a 0.5 ceiling, 65% of them FP32 arithmetic, "
                              "0.37 TB/s of HBM traffic (profiles/kernels_terrain512.md)"},
