@@ -394,8 +394,10 @@ __global__ __launch_bounds__(256) void k_edge_compact(KArgs a) {
     int* order = slot_ptr<int>(a, chunk, a.L.e_order);
     const int S = G.shift, cap = a.caps.edges, ey_total = a.counters[chunk].edges_y;
     for (int t = thread_in_chunk(); t < G.edge_rows; t += threads_per_chunk()) {
-        const int y = div_magic(t, G.magic_h), z = t - y * G.H;
-        const int r = y + G.H * z;
+        // row r = y + H z: consecutive lanes read consecutive rows and write adjacent
+        // stretches of the list (0.12 -> 0.08 ms against lanes a column of rows apart)
+        const int r = t;
+        const int z = div_magic(t, G.magic_h), y = t - z * G.H;
         if (count[r] == 0) continue;
         const uint64_t ex = E[4 * r], ey = E[4 * r + 1], ez = E[4 * r + 2], hi = E[4 * r + 3];
         const int row_id = (y << S) | (z << (2 * S));
