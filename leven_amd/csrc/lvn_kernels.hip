@@ -184,9 +184,10 @@ __device__ __forceinline__ int next_item(int* queue) {
     return __shfl(item, 0);
 }
 
-// Persistent: the grid fills the machine once (two 1024-thread workgroups per CU,
-// bounded by the 64 KiB LDS table), each workgroup loads the noise tables once,
-// and every wave pulls 64-column items of any chunk until the queue is empty.
+// Persistent: the grid fills the machine once (one 1024-thread workgroup per CU by
+// default: at this kernel's 93 SGPRs a SIMD holds the 4 waves of one, not the 8 of two;
+// the LDS tables would allow two), each workgroup loads the noise tables once, and
+// every wave pulls 64-column items of any chunk until the queue is empty.
 // kCount adds the work counters of lvn_batch_work (a separate instantiation, so
 // the timed kernels carry no counting code).
 template <bool kCount>

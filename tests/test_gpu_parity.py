@@ -335,6 +335,47 @@ def test_batched_mixed_lods_match_oracle(oracle):
     mesher.clear_csg_ops()
 
 
+@pytest.mark.parametrize("n", [16, 32, 64])
+def test_height_lines_match_oracle(oracle, n):
+    """x/z-edge heights evaluated once per line (k_height_lines, `height_lines` 2) and per
+    sample (1) give the oracle's meshes bit for bit, for every chunk size: caves and brushes
+    at several LODs, for 64^3 chunks the clipmap's cave chunks up to the dense LOD-5 one."""
+    if n == 64:
+        w = workloads.clipmap()
+        pick = [930, 923, 755, 748, 0]
+        mins, sizes = w["mins"][pick], w["sizes"][pick]
+    else:
+        w = workloads.terrain_512()
+        leaf = 4 * n
+        mins = np.array([(0, -leaf, 0), (leaf, -leaf, 2 * leaf), (2 * leaf, 0, leaf), (3 * leaf, -4 * leaf, 3 * leaf),
+                         (0, -2 * leaf, 4 * leaf), (4 * leaf, -2 * leaf, 0)], np.int32)
+        sizes = np.array([leaf, leaf, leaf, leaf, 2 * leaf, 2 * leaf], np.int32)
+    setup_field(w["desc"], w["seed"])
+    for op, bb in zip(w["ops"], w["aabbs"]):
+        mesher.store_csg_op(op, bb[:3], bb[3:])
+    perm = oracle.perm_table(w["seed"])
+    want = [oracle.Chunk(n).generate(w["desc"], perm, mn, int(size), ops=_ops_for_chunk(w, mn, int(size))).build()
+            for mn, size in zip(mins, sizes)]
+    b = mesher.Batch(n, len(sizes), edge_cap=0, voxel_cap=0)
+    b.set_chunks(mins, sizes)
+    tris = 0
+    for lines in (2, 1):
+        b.set_tuning(height_lines=lines)
+        b.run()
+        b.wait()
+        assert (b.counts()[:, 4] == 0).all()
+        for i, c in enumerate(want):
+            v, t, s = b.get_mesh(i)
+            assert (t == c.triangles()).all(), f"N {n} height_lines {lines} chunk {i} triangles"
+            assert (v.view(np.uint32) == c.vertices().view(np.uint32)).all(), f"N {n} chunk {i} vertices"
+            assert (s.view(np.uint32) == c.seams().view(np.uint32)).all(), f"N {n} chunk {i} seams"
+            tris += len(t)
+    print(f"N {n}: {len(sizes)} chunks, {tris // 2} triangles per pass", flush=True)
+    assert tris > 0
+    b.close()
+    mesher.clear_csg_ops()
+
+
 def test_context_follows_reinitialisation(oracle):
     """A context (and its captured pipeline) used across lvn_init calls with different
     fields and seeds meshes each chunk with the current field."""
