@@ -458,23 +458,33 @@ __device__ __forceinline__ int nth_bit(uint64_t m, int j) {
     return pos;
 }
 
-// Item = (chunk, sel, z), taken by the grid's waves in a static stride: items are
-// short (a few hundred height evaluations), and a shared atomic queue serves only about
-// 77,000 grabs per ms, which would bound this pass at 0.86 ms for 512 chunks.  The lines
+// Item = (chunk, sel, z).  Items are short (a few hundred height evaluations), and a
+// shared atomic queue serves only about 77,000 grabs per ms, which would bound this pass
+// at 0.86 ms for 512 chunks; a static stride over the waves leaves many idle (the items
+// of one wave then come from one layer of chunks, all sky or all caves).  So workgroup b
+// takes items b, b + gridDim.x, ... (chunks of every layer) and its waves share them
+// through an LDS counter.  The lines
 // needed are the OR over y of the chunk's edge rows (x-edge or z-edge words); the lanes
 // run over their samples only.  Chunks whose edges take no density crossings (the
 // cached-field form) are skipped.
 template <bool kCount>
 __global__ __launch_bounds__(1024) void k_height_lines(KArgs a, int chunks) {
     __shared__ NoiseLds T;
-    load_noise_tables(T, a.perm, a.grads);
+    __shared__ int next_local;
+    if (threadIdx.x == 0) next_local = 0;
+    load_noise_tables(T, a.perm, a.grads);  // ends with a barrier
     const Geometry& G = a.G;
     const int H = G.H;
     const int per_chunk = 2 * H;
     const int total = per_chunk * chunks;
     const int lane = threadIdx.x & 63;
-    const int waves = gridDim.x * (blockDim.x >> 6);
-    for (int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); item < total; item += waves) {
+    const int mine = (total - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;  // items of this workgroup
+    for (;;) {
+        int j = 0;
+        if (lane == 0) j = atomicAdd(&next_local, 1);
+        j = __shfl(j, 0);
+        if (j >= mine) break;
+        const int item = (int)blockIdx.x + (int)gridDim.x * j;
         const int chunk = item / per_chunk;
         const int r = item - chunk * per_chunk;
         const int sel = r / H, z = r - sel * H;
