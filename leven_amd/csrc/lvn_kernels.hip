@@ -227,37 +227,46 @@ __global__ __launch_bounds__(1024) void k_density(KArgs a, int chunks) {
 // ---------------------------------------------------------------------------
 // CSG material override (CSG_HermiteIndices + BrushMaterial,
 // cl/apply_csg_operation.cl:114-140,179-210): ops in order, the last op whose
-// brush contains a sample sets its material.  One workgroup per chunk walks
-// the ops in order over each op's bounding box of samples.
+// brush contains a sample sets its material.  A chunk's samples are split into
+// kCsgSlabs z-slabs, one workgroup each, so the chunk with the most brushes is spread
+// over several CUs; each workgroup walks the ops in order over the (x, y) columns of
+// each op's bounding box within its slab.  Every sample belongs to one workgroup and
+// sees the ops in order (a barrier after each op), as in a single pass.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_csg(KArgs a) {
+constexpr int kCsgSlabs = 8;
+
+__global__ __launch_bounds__(256) void k_csg(KArgs a) {
     const int chunk = blockIdx.y;
-    const ChunkDesc cd = a.chunks[chunk];
-    if (cd.op_count == 0) return;
+    const ChunkDesc& cd = a.chunks[chunk];
     const int F = a.G.F;
+    const int zs = (F + kCsgSlabs - 1) / kCsgSlabs;
+    const int z0 = (int)blockIdx.x * zs, z1 = min(F, z0 + zs);  // this workgroup's slab
+    if (cd.op_count == 0 || z0 >= z1) return;                    // uniform, before any barrier
     uint8_t* mat = slot_ptr<uint8_t>(a, chunk, a.L.mat);
+    const float inv = 1.f / (float)cd.scale;
     for (int k = 0; k < cd.op_count; ++k) {
         const DevOp op = a.ops[a.op_index[cd.op_start + k]];
         const float r = op.radius + 1.f;
-        const float inv = 1.f / (float)cd.scale;
         int lo[3], hi[3];
         const float o[3] = {op.ox, op.oy, op.oz};
         for (int c = 0; c < 3; ++c) {
             lo[c] = max(0, (int)floorf((o[c] - r - (float)cd.off[c]) * inv));
             hi[c] = min(F - 1, (int)ceilf((o[c] + r - (float)cd.off[c]) * inv));
         }
-        const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
-        if (nx > 0 && ny > 0 && nz > 0) {
-            const int total = nx * ny * nz;
+        lo[2] = max(lo[2], z0);
+        hi[2] = min(hi[2], z1 - 1);
+        const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1;
+        if (nx > 0 && ny > 0 && hi[2] >= lo[2]) {
             const uint8_t newm = op.type == 0 ? (uint8_t)op.material : (uint8_t)kAir;
-            for (int i = threadIdx.x; i < total; i += blockDim.x) {
-                const int x = lo[0] + i % nx;
-                const int y = lo[1] + (i / nx) % ny;
-                const int z = lo[2] + i / (nx * ny);
+            for (int i = threadIdx.x; i < nx * ny; i += blockDim.x) {
+                const int y = lo[1] + i / nx;
+                const int x = lo[0] + (i - (y - lo[1]) * nx);
                 const float px = (float)(x * cd.scale + cd.off[0]);
                 const float py = (float)(y * cd.scale + cd.off[1]);
-                const float pz = (float)(z * cd.scale + cd.off[2]);
-                if (brush_density(op, px, py, pz) <= 0.f) mat[field_index(a.G, x, y, z)] = newm;
+                for (int z = lo[2]; z <= hi[2]; ++z) {
+                    const float pz = (float)(z * cd.scale + cd.off[2]);
+                    if (brush_density(op, px, py, pz) <= 0.f) mat[field_index(a.G, x, y, z)] = newm;
+                }
             }
         }
         __syncthreads();
@@ -1623,7 +1632,7 @@ void launch_density(const KArgs& a, int chunks, hipStream_t s) {
         hipLaunchKernelGGL(k_density<false>, dim3(blocks), dim3(1024), 0, s, a, chunks);
 }
 void launch_csg(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_csg, dim3(1, chunks), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(k_csg, dim3(kCsgSlabs, chunks), dim3(256), 0, s, a);
 }
 // Workgroups per chunk for a kernel that loops over `threads` work items of 256-thread
 // workgroups: all of them for a few chunks, about `target` workgroups over a large batch.
