@@ -60,31 +60,11 @@ __device__ __forceinline__ int threads_per_chunk() { return (int)(gridDim.x * bl
 __device__ __forceinline__ uint64_t shift_next(uint64_t lo, uint64_t hi) { return (lo >> 1) | (hi << 63); }
 
 // Index of edge (x, y, z, axis) in the chunk's compacted edge list, or -1 when the
-// edge has no sign change.  The list is in slot order s = 3 (x + H y + H^2 z) + axis,
-// so the index is the row's base plus the edges of lower x in the row, plus the
-// lower axes at the same x.
-__device__ __forceinline__ int edge_index(int H, const uint64_t* rows, const int* base, int x, int y, int z, int axis) {
-    const int r = y + H * z;
-    const uint64_t ex = rows[4 * r], ey = rows[4 * r + 1], ez = rows[4 * r + 2];
-    if (x < 64) {
-        const uint64_t w = axis == 0 ? ex : axis == 1 ? ey : ez;
-        if (!((w >> x) & 1ull)) return -1;
-        const uint64_t lt = lanemask_lt(x);
-        int k = __popcll(ex & lt) + __popcll(ey & lt) + __popcll(ez & lt);
-        if (axis > 0) k += (int)((ex >> x) & 1ull);
-        if (axis > 1) k += (int)((ey >> x) & 1ull);
-        return base[r] + k;
-    }
-    const uint64_t hi = rows[4 * r + 3];  // x == 64: bit 0 x, bit 1 y, bit 2 z
-    if (!((hi >> axis) & 1ull)) return -1;
-    int k = __popcll(ex) + __popcll(ey) + __popcll(ez);
-    if (axis > 0) k += (int)(hi & 1ull);
-    if (axis > 1) k += (int)((hi >> 1) & 1ull);
-    return base[r] + k;
-}
-
-// edge_index from a row's four words (x, y, z edge bits of x < 64; bits of x = 64) and
-// its base, already loaded.
+// edge has no sign change, from its edge row r = y + H z, already loaded: the row's four
+// words w (x, y, z edge bits of x < 64; bit 0/1/2 of w[3]: the x, y, z edge at x = 64)
+// and its base.  The list is in slot order s = 3 (x + H y + H^2 z) + axis, so the index
+// is the row's base plus the edges of lower x in the row, plus the lower axes at the
+// same x.
 __device__ __forceinline__ int edge_index_in_row(const uint64_t w[4], int base, int x, int axis) {
     const uint64_t ex = w[0], ey = w[1], ez = w[2];
     if (x < 64) {
@@ -1307,6 +1287,25 @@ __device__ __forceinline__ void qef_precise(const KArgs& a, int chunk, const Chu
     const float4* ed = slot_ptr<float4>(a, chunk, a.L.e_data);
     const float fscale = (float)cd.scale;
 
+    // edge indices from the voxel's four edge rows, loaded together (as in qef_fast)
+    uint64_t rw[4][4];
+    int rb[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int r = (p.y + (j & 1)) + G.H * (p.z + (j >> 1));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) rw[j][q] = erows[4 * r + q];
+        rb[j] = eb[r];
+    }
+    int ke[12];
+#pragma unroll
+    for (int e = 0; e < 12; ++e) {
+        const int e0 = kEdgeMap[e][0];
+        const int j = ((e0 >> 1) & 1) + 2 * (e0 & 1);
+        const int k = ((emask >> e) & 1) ? edge_index_in_row(rw[j], rb[j], p.x + ((e0 >> 2) & 1), e >> 2) : -1;
+        ke[e] = k >= 0 && k < a.caps.edges ? k : -1;
+    }
+
     float px[12], py[12], pz[12], nx[12], ny[12], nz[12];
     bool ok[12];
     float cnt = 0.f;
@@ -1316,12 +1315,10 @@ __device__ __forceinline__ void qef_precise(const KArgs& a, int chunk, const Chu
     for (int e = 0; e < 12; ++e) {
         ok[e] = false;
         px[e] = py[e] = pz[e] = nx[e] = ny[e] = nz[e] = 0.f;
-        if (!((emask >> e) & 1)) continue;
-        const int e0 = kEdgeMap[e][0], e1 = kEdgeMap[e][1], axis = e >> 2;
+        if (ke[e] < 0) continue;
+        const int e0 = kEdgeMap[e][0], e1 = kEdgeMap[e][1];
         const int hx = p.x + ((e0 >> 2) & 1), hy = p.y + ((e0 >> 1) & 1), hz = p.z + (e0 & 1);
-        const int k = edge_index(G.H, erows, eb, hx, hy, hz, axis);
-        if (k < 0 || k >= a.caps.edges) continue;
-        const float4 d = ed[k];
+        const float4 d = ed[ke[e]];
         const float a0x = (float)hx, a0y = (float)hy, a0z = (float)hz;
         const float a1x = (float)(p.x + ((e1 >> 2) & 1)), a1y = (float)(p.y + ((e1 >> 1) & 1)),
                     a1z = (float)(p.z + (e1 & 1));
