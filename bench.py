@@ -540,7 +540,7 @@ def main():
                      "launch_ms": round(stage_ms[dom], 4),
                      "executed_flops_per_launch": int(exec_flops),
                      "executed_tflops": round(exec_flops / (stage_ms[dom] * 1e-3) / 1e12, 3),
-                     "note": "FP32 vector-ALU bound (neither HBM nor MFMA): rocprofv3 counts 0.39 VALU "
+                     "note": "FP32 vector-ALU bound (neither HBM nor MFMA): rocprofv3 counts 0.39-0.42 VALU "
                              "instructions per SIMD-cycle in k_edge_cross against a 0.5 ceiling, 65% of them "
                              "FP32 arithmetic, 0.47 TB/s of HBM traffic (profiles/kernels_terrain512.md); "
                              "achieved counts the method's 23 field evaluations per edge, executed_tflops "
