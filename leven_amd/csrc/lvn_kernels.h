@@ -38,6 +38,8 @@ struct KArgs {
     int pad3;
 };
 
+// Global work-queue heads (queue[1]: the edge-crossing items; the density and
+// line-height passes share their items within workgroups instead; the rest are spare).
 constexpr int kNumQueues = 8;
 
 enum ScanKind { kScanEdges = 0, kScanVoxels = 1, kScanQuadsSeams = 2 };
