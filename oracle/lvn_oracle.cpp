@@ -393,7 +393,35 @@ double fbm3_d(const uint8_t* perm, int oct, double freq, double lac, double gain
     return n;
 }
 
+// RidgedMultiFractal (cl/noise.cl:42-81) and Terrain (cl/noise.cl:205-223) in double.
+double ridged_multifractal_d(const uint8_t* perm, int oct, double lac, double gain, double offset, double x,
+                             double y) {
+    double px = x, py = y;
+    double signal = offset - std::fabs(snoise2_d(perm, px, py));
+    signal *= signal;
+    double noise = signal, frequency = 1.0;
+    for (int i = 0; i < oct; ++i) {
+        px *= lac;
+        py *= lac;
+        const double weight = std::min(std::max(signal * gain, 0.0), 1.0);
+        signal = (offset - std::fabs(snoise2_d(perm, px, py))) * weight;
+        noise += signal / frequency;
+        frequency *= lac;
+    }
+    return noise / oct;
+}
+
+double leven_terrain_d(const uint8_t* perm, double x, double z) {
+    const double qx = x / 2000.0, qy = z / 2000.0;
+    const double ridged =
+        std::min(std::max(0.8 * ridged_multifractal_d(perm, 7, 2.114352, 1.5241, 1.0, qx, qy), 0.0), 1.0);
+    const double billow = 0.5 * (0.6 * fbm2_d(perm, 4, 0.24, 1.8754, 0.433, -4.33 * qx, 7.98 * qy)) + 0.5;
+    const double b2 = 0.5 * (0.6 * fbm2_d(perm, 2, 0.63, 2.2, 0.15, qx, qy)) + 0.5;
+    return billow * ridged + b2;
+}
+
 double density_d(const lvn_field_desc& f, const uint8_t* perm, double x, double y, double z) {
+    if (f.kind == LVN_FIELD_LEVEN_TERRAIN) return y - 900.0 * leven_terrain_d(perm, x, z);
     const double a = y - (double)f.height_amplitude * fbm2_d(perm, f.height_octaves, f.height_frequency,
                                                               f.height_lacunarity, f.height_gain, x, z);
     if (f.kind == LVN_FIELD_TERRAIN_CAVES) {
@@ -1157,6 +1185,10 @@ float orc_fbm2(const uint8_t* perm, int32_t o, float f, float l, float g, float 
 float orc_fbm3(const uint8_t* perm, int32_t o, float f, float l, float g, float x, float y, float z) {
     return fbm3(perm, o, f, l, g, x, y, z);
 }
+float orc_ridged_multifractal(const uint8_t* perm, int32_t o, float l, float g, float offset, float x, float y) {
+    return ridged_multifractal(perm, o, l, g, offset, x, y);
+}
+float orc_leven_terrain(const uint8_t* perm, float x, float z) { return leven_terrain(perm, x, z); }
 float orc_density(const lvn_field_desc* d, const uint8_t* perm, float x, float y, float z) {
     return density(*d, perm, x, y, z);
 }

@@ -54,6 +54,10 @@ float orc_snoise3(const uint8_t* perm, float x, float y, float z);
 float orc_fbm2(const uint8_t* perm, int32_t octaves, float frequency, float lacunarity, float gain, float x, float y);
 float orc_fbm3(const uint8_t* perm, int32_t octaves, float frequency, float lacunarity, float gain,
                float x, float y, float z);
+/* RidgedMultiFractal (cl/noise.cl:42-81) and Terrain (cl/noise.cl:205-223) over snoise2. */
+float orc_ridged_multifractal(const uint8_t* perm, int32_t octaves, float lacunarity, float gain, float offset,
+                              float x, float y);
+float orc_leven_terrain(const uint8_t* perm, float x, float z);
 float orc_density(const lvn_field_desc* desc, const uint8_t* perm, float x, float y, float z);
 double orc_density_f64(const lvn_field_desc* desc, const uint8_t* perm, double x, double y, double z);
 float orc_brush_density(const lvn_csg_op* op, float x, float y, float z);

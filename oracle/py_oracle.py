@@ -46,6 +46,8 @@ def lib():
             "orc_snoise3": (f32, [vp, f32, f32, f32]),
             "orc_fbm2": (f32, [vp, i32, f32, f32, f32, f32, f32]),
             "orc_fbm3": (f32, [vp, i32, f32, f32, f32, f32, f32, f32]),
+            "orc_ridged_multifractal": (f32, [vp, i32, f32, f32, f32, f32, f32]),
+            "orc_leven_terrain": (f32, [vp, f32, f32]),
             "orc_density": (f32, [P(FieldDesc), vp, f32, f32, f32]),
             "orc_density_f64": (f64, [P(FieldDesc), vp, f64, f64, f64]),
             "orc_brush_density": (f32, [vp, f32, f32, f32]),

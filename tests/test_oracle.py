@@ -152,10 +152,134 @@ def test_fbm_composition(oracle):
         assert L.orc_fbm2(oracle.vptr(perm), 2, 1 / 128, 2.0, 0.5, x, y) == np.float32(s1) + np.float32(s2) * 0.5
 
 
+def _f32(v):
+    return float(np.float32(v))
+
+
+def _fbm_expected(noise, octaves, frequency, lacunarity, gain, coords):
+    """BasicFractal (cl/noise.cl:8-36): sum_i gain^i * noise(p_i), summed in float64; the
+    sample points follow the reference's float32 steps (p *= frequency, then p *= lacunarity
+    per octave), since the noise is a function of those float32 coordinates."""
+    p = [np.float32(c) * np.float32(frequency) for c in coords]
+    total = 0.0
+    for i in range(octaves):
+        total += float(gain) ** i * noise(*[float(c) for c in p])
+        p = [c * np.float32(lacunarity) for c in p]
+    return total
+
+
+def _ridged_expected(noise, octaves, lacunarity, gain, offset, coords):
+    """RidgedMultiFractal (cl/noise.cl:42-81) step by step in float64 from independently
+    evaluated snoise2 values; RIDGED_MULTI_H = 1, so exponent_i = lacunarity^-i."""
+    p = [np.float32(c) for c in coords]
+    signal = (offset - abs(noise(*[float(c) for c in p]))) ** 2
+    total = signal
+    for i in range(1, octaves + 1):
+        p = [c * np.float32(lacunarity) for c in p]
+        weight = min(max(signal * gain, 0.0), 1.0)
+        signal = (offset - abs(noise(*[float(c) for c in p]))) * weight
+        total += signal * float(lacunarity) ** -(i - 1)
+    return total / octaves
+
+
+def _terrain_expected(noise, x, z):
+    """Terrain (cl/noise.cl:205-223) in float64 from the pieces above."""
+    q = (np.float32(x) * np.float32(1 / 2000), np.float32(z) * np.float32(1 / 2000))
+    ridged = min(max(0.8 * _ridged_expected(noise, 7, 2.114352, 1.5241, 1.0, q), 0.0), 1.0)
+    billow = 0.5 * (0.6 * _fbm_expected(noise, 4, 0.24, 1.8754, 0.433, (np.float32(-4.33) * q[0], np.float32(7.98) * q[1]))) + 0.5
+    b2 = 0.5 * (0.6 * _fbm_expected(noise, 2, 0.63, 2.2, 0.15, q)) + 0.5
+    return billow * ridged + b2
+
+
+@pytest.mark.parametrize("seed", [1, -7, 93923590])
+def test_fbm_octave_sums(oracle, seed):
+    """fbm2 and fbm3 equal the octave sum built from independently evaluated simplex values
+    at lacunarities and gains away from 2 and 1/2 (a dropped lacunarity multiply, a swapped
+    gain or a frequency applied twice moves the value by O(0.1)); lacunarity 1 is the
+    geometric series s * (1 - g^n) / (1 - g)."""
+    perm = oracle.perm_table(seed)
+    L = oracle.lib()
+    rng = np.random.default_rng(abs(seed) % 1000)
+    s2 = lambda x, y: oracle.snoise2(perm, x, y)  # noqa: E731
+    s3 = lambda x, y, z: oracle.snoise3(perm, x, y, z)  # noqa: E731
+    for octaves, freq, lac, gain in [(1, 1 / 64, 2.0, 0.5), (2, 1 / 128, 1.7, 0.37), (3, 0.013, 2.31, 0.8),
+                                     (5, 1 / 64, 2.0, 0.5), (4, 0.24, 1.8754, 0.433)]:
+        for x, y, z in rng.uniform(-400, 400, (12, 3)).astype(np.float32):
+            x, y, z = float(x), float(y), float(z)
+            got2 = L.orc_fbm2(oracle.vptr(perm), octaves, freq, lac, gain, x, y)
+            want2 = _fbm_expected(s2, octaves, _f32(freq), _f32(lac), _f32(gain), (x, y))
+            assert abs(got2 - want2) <= 2e-5, (octaves, lac, gain, x, y)
+            got3 = L.orc_fbm3(oracle.vptr(perm), octaves, freq, lac, gain, x, y, z)
+            want3 = _fbm_expected(s3, octaves, _f32(freq), _f32(lac), _f32(gain), (x, y, z))
+            assert abs(got3 - want3) <= 2e-5, (octaves, lac, gain, x, y, z)
+    for x, y, z in rng.uniform(-400, 400, (8, 3)).astype(np.float32):
+        x, y, z = float(x), float(y), float(z)
+        one = oracle.snoise3(perm, _f32(x * _f32(0.02)), _f32(y * _f32(0.02)), _f32(z * _f32(0.02)))
+        got = L.orc_fbm3(oracle.vptr(perm), 6, 0.02, 1.0, 0.6, x, y, z)
+        assert abs(got - one * (1 - 0.6 ** 6) / (1 - 0.6)) <= 1e-5
+
+
+@pytest.mark.parametrize("seed", [2, -11])
+def test_ridged_multifractal_steps(oracle, seed):
+    """RidgedMultiFractal against the reference's steps (cl/noise.cl:42-81) in float64:
+    octave counts 1, 2, 3 and leven's 7, lacunarities and gains of either side of 1.
+    Gain 0 leaves only the first signal: (offset - |s(p)|)^2 / octaves, which pins the
+    final 1/octaves."""
+    perm = oracle.perm_table(seed)
+    L = oracle.lib()
+    rng = np.random.default_rng(abs(seed))
+    s2 = lambda x, y: oracle.snoise2(perm, x, y)  # noqa: E731
+    for octaves, lac, gain, offset in [(1, 2.114352, 1.5241, 1.0), (2, 1.6, 0.7, 0.8), (3, 2.5, 1.2, 1.0),
+                                       (7, 2.114352, 1.5241, 1.0)]:
+        for x, y in rng.uniform(-3, 3, (16, 2)).astype(np.float32):
+            x, y = float(x), float(y)
+            got = L.orc_ridged_multifractal(oracle.vptr(perm), octaves, lac, gain, offset, x, y)
+            want = _ridged_expected(s2, octaves, _f32(lac), _f32(gain), _f32(offset), (x, y))
+            assert abs(got - want) <= 1e-5 * max(1.0, abs(want)), (octaves, lac, gain, x, y)
+    for octaves in (1, 3, 7):
+        for x, y in rng.uniform(-3, 3, (4, 2)).astype(np.float32):
+            x, y = float(x), float(y)
+            got = L.orc_ridged_multifractal(oracle.vptr(perm), octaves, 2.0, 0.0, 1.0, x, y)
+            assert abs(got - (1.0 - abs(oracle.snoise2(perm, x, y))) ** 2 / octaves) <= 1e-6
+
+
+def test_leven_terrain_and_density_kinds(oracle):
+    """Terrain (cl/noise.cl:205-223) assembled from the pinned pieces, and each density kind
+    (DensityFunc, cl/noise.cl:225-268, and the workloads' swaps) assembled from
+    independently evaluated terms: y - 900 Terrain(x, z); y - A fbm2(x, z); and
+    max(y - A fbm2(x, z), fbm3(x, y, z) - threshold) with both branches taken."""
+    perm = oracle.perm_table(3)
+    L = oracle.lib()
+    rng = np.random.default_rng(8)
+    s2 = lambda x, y: oracle.snoise2(perm, x, y)  # noqa: E731
+    for x, y, z in rng.uniform(-6000, 6000, (40, 3)).astype(np.float32):
+        x, y, z = float(x), float(y), float(z)
+        t = L.orc_leven_terrain(oracle.vptr(perm), x, z)
+        assert abs(t - _terrain_expected(s2, x, z)) <= 1e-5, (x, z)
+        d = oracle.density(abi.field_desc(kind=abi.FIELD_LEVEN_TERRAIN), perm, x, y, z)
+        assert d == np.float32(np.float32(y) - np.float32(np.float32(900.0) * np.float32(t)))
+    hdesc = abi.field_desc(height_octaves=4, height_frequency=0.011, height_lacunarity=1.9, height_gain=0.45,
+                           height_amplitude=17.0)
+    cdesc = abi.field_desc(kind=abi.FIELD_TERRAIN_CAVES, height_octaves=3, height_frequency=1 / 100,
+                           height_lacunarity=2.2, height_gain=0.55, height_amplitude=20.0, cave_octaves=4,
+                           cave_frequency=1 / 50, cave_lacunarity=1.8, cave_gain=0.45, cave_threshold=0.25)
+    branches = set()
+    for x, y, z in rng.uniform(-200, 200, (300, 3)).astype(np.float32):
+        x, y, z = float(x), float(y) * 0.2, float(z)
+        h = L.orc_fbm2(oracle.vptr(perm), 4, 0.011, 1.9, 0.45, x, z)
+        assert oracle.density(hdesc, perm, x, y, z) == np.float32(y) - np.float32(17.0) * np.float32(h)
+        a = np.float32(y) - np.float32(20.0) * np.float32(L.orc_fbm2(oracle.vptr(perm), 3, 1 / 100, 2.2, 0.55, x, z))
+        b = np.float32(L.orc_fbm3(oracle.vptr(perm), 4, 1 / 50, 1.8, 0.45, x, y, z)) - np.float32(0.25)
+        assert oracle.density(cdesc, perm, x, y, z) == max(a, b)
+        branches.add(bool(a > b))
+    assert branches == {True, False}
+
+
 def test_density_matches_fp64_twin(oracle):
     perm = oracle.perm_table(2)
     rng = np.random.default_rng(3)
-    for desc in (abi.field_desc(), abi.field_desc(kind=abi.FIELD_TERRAIN_CAVES, height_octaves=5)):
+    for desc in (abi.field_desc(), abi.field_desc(kind=abi.FIELD_TERRAIN_CAVES, height_octaves=5),
+                 abi.field_desc(kind=abi.FIELD_LEVEN_TERRAIN)):
         for x, y, z in rng.uniform(-300, 300, (200, 3)).astype(np.float32):
             d32 = oracle.density(desc, perm, float(x), float(y), float(z))
             d64 = oracle.lib().orc_density_f64(desc, oracle.vptr(perm), float(x), float(y), float(z))
