@@ -98,15 +98,17 @@ public:
         seams.clear();
         const int32_t m[3] = {min.x, min.y, min.z};
         int32_t nv = 0, nt = 0, ns = 0;
-        std::vector<lvn_seam_node> s((size_t)voxelsPerChunk() * voxelsPerChunk() * 6);
+        // seam nodes land in the context's staging list (sized once for the largest
+        // chunk), then only the ns written are copied into the caller's vector
         const int r = lvn_generate_chunk_mesh(ctx_, m, size, reinterpret_cast<lvn_vertex*>(&meshBuffer->vertices[0]),
                                               kMaxVertices, &nv, reinterpret_cast<int32_t*>(&meshBuffer->triangles[0]),
-                                              kMaxTriangles, &nt, s.data(), (int32_t)s.size(), &ns);
+                                              kMaxTriangles, &nt, seamStaging_.data(), (int32_t)seamStaging_.size(),
+                                              &ns);
         if (r != LVN_OK) return r;
         meshBuffer->numVertices = nv;
         meshBuffer->numTriangles = nt;
         seams.resize((size_t)ns);
-        if (ns) std::memcpy(seams.data(), s.data(), sizeof(lvn_seam_node) * (size_t)ns);
+        if (ns) std::memcpy(seams.data(), seamStaging_.data(), sizeof(lvn_seam_node) * (size_t)ns);
         return LVN_OK;
     }
 
@@ -149,8 +151,12 @@ public:
     lvn_ctx* handle() const { return ctx_; }
 
 private:
-    explicit MeshGenContext(lvn_ctx* c) : ctx_(c) {}
+    // A chunk's seam nodes are its active voxels with a coordinate at 0 or N - 1: at most
+    // N^3 - (N - 2)^3 <= 6 N^2 of them.
+    explicit MeshGenContext(lvn_ctx* c)
+        : ctx_(c), seamStaging_((size_t)6 * lvn_ctx_voxels_per_chunk(c) * lvn_ctx_voxels_per_chunk(c)) {}
     lvn_ctx* ctx_;
+    std::vector<lvn_seam_node> seamStaging_;
 };
 
 }  // namespace lvn

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -35,7 +36,7 @@ void destroy_seam_state(SeamState* s);
 // (the reference keeps the same as process globals: src/compute.cpp:168-191,
 // src/compute_density_field.cpp:23-24).
 struct Global {
-    bool init = false;
+    std::atomic<bool> init{false};
     lvn_field_desc desc{};
     FieldParams fp{};
     int32_t seed = 0;
@@ -46,9 +47,15 @@ struct Global {
     std::vector<lvn_csg_op> ops;
     std::vector<std::array<int32_t, 6>> aabbs;
     uint64_t generation = 0;  // bumped when field, table pointers or default material change
-    std::mutex mu;
+    int live = 0;             // contexts, batches and servers alive (each holds the tables)
+    std::mutex mu;            // guards the tables' contents and lifetime and `live`
 };
 extern Global g_state;
+
+// Contexts, batches and servers keep the device tables alive: lvn_shutdown frees them
+// at once when none exists, else the last one's destruction does.
+void retain_global();
+void release_global();
 
 // Cache key: ivec4(min, size) as in src/compute_local.h:125,138.
 using Key = std::tuple<int32_t, int32_t, int32_t, int32_t>;
@@ -74,6 +81,7 @@ struct DevicePool {
     char* cur = nullptr;
     size_t left = 0;
     void* take(size_t bytes);  // 256-byte aligned, or null when the device is out of memory
+    void release();            // frees every block
     ~DevicePool();
 };
 
@@ -96,6 +104,7 @@ struct MeshEntry {
 };
 
 struct Ctx {
+    bool retained = false;  // holds a reference on the global tables
     int N = 0;
     uint32_t flags = 0;
     Geometry G{};
@@ -163,6 +172,9 @@ Geometry make_geometry(int n);
 Caps make_caps(const Geometry& G, int edge_cap, int voxel_cap);
 SlotLayout make_layout(const Geometry& G, const Caps& c);
 void fill_chunk_desc(ChunkDesc& cd, const int32_t* min, int32_t size, int N);
+// A chunk size the pipeline accepts for N voxels per chunk: a positive multiple of
+// N * LEAF_SIZE_SCALE (sampleScale = size / (N * 4) >= 1, src/compute_density_field.cpp:149).
+inline bool valid_chunk_size(int32_t size, int N) { return size >= N * kLeafSizeScale && size % (N * kLeafSizeScale) == 0; }
 bool aabb_overlaps(const int32_t* mn, int32_t size, const std::array<int32_t, 6>& bb);
 
 }  // namespace lvn

@@ -261,11 +261,32 @@ bool aabb_overlaps(const int32_t* mn, int32_t size, const std::array<int32_t, 6>
 
 int ensure_global() { return g_state.init ? LVN_OK : LVN_ERR_STATE; }
 
+static void free_tables_locked() {
+    if (g_state.d_perm) (void)hipFree(g_state.d_perm);
+    if (g_state.d_grads) (void)hipFree(g_state.d_grads);
+    g_state.d_perm = nullptr;
+    g_state.d_grads = nullptr;
+}
+
+void retain_global() {
+    std::lock_guard<std::mutex> lock(g_state.mu);
+    ++g_state.live;
+}
+
+void release_global() {
+    std::lock_guard<std::mutex> lock(g_state.mu);
+    if (--g_state.live == 0 && !g_state.init) free_tables_locked();
+}
+
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
 
 int Ctx::allocate(int n, uint32_t fl, int nslots, int edge_cap, int voxel_cap) {
+    if (!retained) {
+        retain_global();
+        retained = true;
+    }
     N = n;
     flags = fl;
     G = make_geometry(n);
@@ -351,6 +372,9 @@ Ctx::~Ctx() {
     if (h_ops) (void)hipHostFree(h_ops);
     if (h_op_index) (void)hipHostFree(h_op_index);
     if (stream) (void)hipStreamDestroy(stream);
+    field_pool.release();
+    mesh_pool.base.release();
+    if (retained) release_global();  // after every stream of this context has drained
 }
 
 void* DevicePool::take(size_t bytes) {
@@ -369,9 +393,14 @@ void* DevicePool::take(size_t bytes) {
     return out;
 }
 
-DevicePool::~DevicePool() {
+void DevicePool::release() {
     for (void* p : blocks) (void)hipFree(p);
+    blocks.clear();
+    cur = nullptr;
+    left = 0;
 }
+
+DevicePool::~DevicePool() { release(); }
 
 char* ClassPool::take(size_t bytes, size_t* cap) {
     size_t c = (size_t)64 << 10;
@@ -693,6 +722,8 @@ int lvn_init(const lvn_field_desc* field, int32_t noise_seed, uint32_t default_m
     g_state.desc = d;
     g_state.fp = make_field_params(g_state.desc);
     g_state.defmat = default_material;
+    // kernels still in flight on any stream read the tables: let them finish first
+    LVN_HIP(hipDeviceSynchronize());
     if (!g_state.d_perm) {
         LVN_HIP(hipMalloc(&g_state.d_perm, kPermPaddedBytes));
         LVN_HIP(hipMalloc(&g_state.d_grads, sizeof(float4) * kGradEntries));
@@ -712,18 +743,22 @@ int lvn_init(const lvn_field_desc* field, int32_t noise_seed, uint32_t default_m
 
 int lvn_shutdown(void) {
     std::lock_guard<std::mutex> lock(g_state.mu);
-    if (g_state.d_perm) (void)hipFree(g_state.d_perm);
-    if (g_state.d_grads) (void)hipFree(g_state.d_grads);
-    g_state.d_perm = nullptr;
-    g_state.d_grads = nullptr;
     g_state.ops.clear();
     g_state.aabbs.clear();
     g_state.init = false;
+    ++g_state.generation;
+    // Every launching entry point now returns LVN_ERR_STATE; contexts, batches and
+    // servers still alive keep the tables until the last of them is destroyed.
+    if (g_state.live == 0) free_tables_locked();
     return LVN_OK;
 }
 
 int lvn_set_noise_seed(int32_t seed) {
+    std::lock_guard<std::mutex> lock(g_state.mu);
     if (!g_state.init) return LVN_ERR_STATE;
+    // Work already enqueued (an asynchronous lvn_batch_run, server batches) runs with the
+    // old table, as on the reference's in-order queue: drain it before rewriting the table.
+    LVN_HIP(hipDeviceSynchronize());
     build_perm_table(seed, g_state.perm.data());
     const int r = upload_perm_table();
     if (r) return r;
@@ -788,6 +823,7 @@ int lvn_free_chunk(lvn_ctx* ctx, const int32_t min[3], int32_t size) {
 
 int lvn_chunk_has_surface(lvn_ctx* ctx, const int32_t min[3], int32_t size, int32_t* out) {
     if (!ctx || !min || !out) return LVN_ERR_INVALID;
+    if (!valid_chunk_size(size, reinterpret_cast<Ctx*>(ctx)->N)) return LVN_ERR_INVALID;
     if (!g_state.init) return LVN_ERR_STATE;
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     const Key key = make_key(min, size);
@@ -812,6 +848,7 @@ int lvn_chunk_has_surface(lvn_ctx* ctx, const int32_t min[3], int32_t size, int3
 
 int lvn_apply_csg(lvn_ctx* ctx, const lvn_csg_op* ops, int32_t n, const int32_t min[3], int32_t size) {
     if (!ctx || !min || n < 0 || (n > 0 && !ops)) return LVN_ERR_INVALID;
+    if (!valid_chunk_size(size, reinterpret_cast<Ctx*>(ctx)->N)) return LVN_ERR_INVALID;
     if (!g_state.init) return LVN_ERR_STATE;
     for (int i = 0; i < n; ++i)
         if (ops[i].material < 0 || ops[i].material > 255 || ops[i].type < 0 || ops[i].type > 1 ||
@@ -848,6 +885,7 @@ int lvn_generate_chunk_mesh(lvn_ctx* ctx, const int32_t min[3], int32_t size, lv
                             int32_t* nv, int32_t* tri, int32_t tri_cap, int32_t* nt, lvn_seam_node* s,
                             int32_t s_cap, int32_t* ns) {
     if (!ctx || !min || !nv || !nt || !ns) return LVN_ERR_INVALID;
+    if (!valid_chunk_size(size, reinterpret_cast<Ctx*>(ctx)->N)) return LVN_ERR_INVALID;
     if (!g_state.init) return LVN_ERR_STATE;
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     *nv = *nt = *ns = 0;
@@ -916,13 +954,14 @@ int lvn_batch_set_chunks(lvn_batch* bh, int32_t n, const lvn_chunk_req* reqs) {
     if (!b || n < 0 || (n > 0 && !reqs) || n > b->ctx.slots) return LVN_ERR_INVALID;
     if (!g_state.init) return LVN_ERR_STATE;
     for (int i = 0; i < n; ++i)
-        if (reqs[i].size < b->ctx.N * kLeafSizeScale || reqs[i].size % (b->ctx.N * kLeafSizeScale)) return LVN_ERR_INVALID;
+        if (!valid_chunk_size(reqs[i].size, b->ctx.N)) return LVN_ERR_INVALID;
     return stage_batch(b, n, reqs);
 }
 
 int lvn_batch_run(lvn_batch* bh) {
     Batch* b = reinterpret_cast<Batch*>(bh);
     if (!b) return LVN_ERR_INVALID;
+    if (!g_state.init) return LVN_ERR_STATE;  // the library was shut down
     if (b->n == 0) return LVN_OK;
     Ctx* c = &b->ctx;
     if (b->graph_n != b->n || b->graph_ops != c->d_ops || b->graph_index != c->d_op_index ||
@@ -959,6 +998,7 @@ const char* lvn_batch_stage_name(int32_t stage) {
 int lvn_batch_run_timed(lvn_batch* bh, float* stage_ms, int32_t n_stages) {
     Batch* b = reinterpret_cast<Batch*>(bh);
     if (!b || !stage_ms || n_stages < kNumStages) return LVN_ERR_INVALID;
+    if (!g_state.init) return LVN_ERR_STATE;  // the library was shut down
     return time_stages(b, batch_args(b), stage_ms);
 }
 
@@ -980,6 +1020,7 @@ int lvn_batch_set_tuning(lvn_batch* bh, const lvn_tuning* t) {
 int lvn_batch_autotune(lvn_batch* bh, int32_t reps, lvn_tuning* out) {
     Batch* b = reinterpret_cast<Batch*>(bh);
     if (!b || reps < 1 || reps > 100) return LVN_ERR_INVALID;
+    if (!g_state.init) return LVN_ERR_STATE;  // the library was shut down
     lvn_tuning best = effective_tuning(b);
     if (b->n > 0) {
         struct Param {
@@ -1032,6 +1073,7 @@ int lvn_batch_autotune(lvn_batch* bh, int32_t reps, lvn_tuning* out) {
 int lvn_batch_work(lvn_batch* bh, uint64_t* work) {
     Batch* b = reinterpret_cast<Batch*>(bh);
     if (!b || !work) return LVN_ERR_INVALID;
+    if (!g_state.init) return LVN_ERR_STATE;  // the library was shut down
     Ctx* c = &b->ctx;
     KArgs a = batch_args(b);
     a.work = b->d_work;
@@ -1090,6 +1132,7 @@ int lvn_batch_get_mesh(lvn_batch* bh, int32_t i, lvn_vertex* v, int32_t v_cap, i
 int lvn_debug_chunk_stages(lvn_ctx* ctx, const int32_t min[3], int32_t size, const lvn_csg_op* ops, int32_t nops,
                            lvn_stage_dump* out) {
     if (!ctx || !min || !out || nops < 0 || (nops > 0 && !ops)) return LVN_ERR_INVALID;
+    if (!valid_chunk_size(size, reinterpret_cast<Ctx*>(ctx)->N)) return LVN_ERR_INVALID;
     if (!g_state.init) return LVN_ERR_STATE;
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     std::vector<lvn_csg_op> batch(ops, ops + nops);

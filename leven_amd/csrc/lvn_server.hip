@@ -73,6 +73,7 @@ struct Entry {
     uint64_t last_use = 0;
     int nv = 0, nt = 0, ns = 0, status = 0;
     std::vector<Job> waiters; // requests for this chunk that arrived while it was in flight
+    std::vector<int64_t> tickets;  // completed mesh requests served by the current mesh
 };
 
 }  // namespace
@@ -91,8 +92,11 @@ struct Server {
     bool stop = false;
     int error = LVN_OK;
 
-    std::mutex cache_mu;           // cache + ticket_key (read by lvn_server_get_mesh)
+    std::mutex cache_mu;           // cache, ticket_key and stats (read by lvn_server_get_mesh / _stats)
     std::map<Key, Entry> cache;
+    // completed mesh request -> its chunk, while the chunk's mesh is the one it was
+    // served; dropped when that mesh is evicted or invalidated, so the map holds at most
+    // the requests served by resident meshes
     std::unordered_map<int64_t, Key> ticket_key;
 
     // scheduler-thread state
@@ -108,6 +112,8 @@ struct Server {
     int init_lane(Lane& L);
     void run();
     void finish(int64_t ticket, int kind, int status, const Entry* e, bool hit, double submit);
+    void complete_mesh(Entry& e, const Key& key, const Job& j, bool hit);  // with cache_mu held
+    void forget_tickets(Entry& e);                                          // with cache_mu held
     void admit_and_launch();
     void poll_lanes();
     void handle(const Pending& p);
@@ -181,6 +187,17 @@ void Server::finish(int64_t ticket, int kind, int status, const Entry* e, bool h
     cv.notify_all();
 }
 
+void Server::complete_mesh(Entry& e, const Key& key, const Job& j, bool hit) {
+    ticket_key[j.ticket] = key;
+    e.tickets.push_back(j.ticket);
+    finish(j.ticket, LVN_REQ_MESH, e.status, &e, hit, j.submit);
+}
+
+void Server::forget_tickets(Entry& e) {
+    for (int64_t t : e.tickets) ticket_key.erase(t);
+    e.tickets.clear();
+}
+
 // An edit stores the op and invalidates every chunk whose AABB it overlaps; a mesh
 // request is a cache hit, joins an in-flight mesh of the same chunk, or waits.
 void Server::handle(const Pending& p) {
@@ -197,6 +214,7 @@ void Server::handle(const Pending& p) {
                 ++e.version;
                 if (e.valid) {
                     e.valid = false;
+                    forget_tickets(e);
                     if (!e.in_flight) {
                         free_slots.push_back(e.slot);
                         e.slot = -1;
@@ -210,14 +228,13 @@ void Server::handle(const Pending& p) {
     const Key key = make_key(p.req.min, p.req.size);
     {
         std::lock_guard<std::mutex> lock(cache_mu);
-        ticket_key[p.ticket] = key;
         auto it = cache.find(key);
         if (it != cache.end()) {
             Entry& e = it->second;
             if (e.valid && !e.in_flight) {
                 e.last_use = ++tick;
                 ++stats[3];
-                finish(p.ticket, LVN_REQ_MESH, e.status, &e, true, p.submit);
+                complete_mesh(e, key, Job{p.ticket, key, p.submit}, true);
                 return;
             }
             if (e.in_flight) {
@@ -231,6 +248,10 @@ void Server::handle(const Pending& p) {
 
 int Server::launch(Lane& L, const std::vector<Job>& batch, const std::vector<int>& slots) {
     const int n = (int)batch.size();
+    // the tables stay allocated while this server lives, but after lvn_shutdown no new
+    // work starts; holding the global lock orders the enqueue against reseeding
+    std::lock_guard<std::mutex> glock(g_state.mu);
+    if (!g_state.init) return LVN_ERR_STATE;
     const int nops = (int)ops.size();
     std::vector<int> index;
     for (int i = 0; i < n; ++i) {
@@ -282,6 +303,7 @@ int Server::launch(Lane& L, const std::vector<Job>& batch, const std::vector<int
     LVN_TRY(hipEventRecord(L.done, L.stream));
     L.busy = true;
     L.jobs = batch;
+    std::lock_guard<std::mutex> lock(cache_mu);
     ++stats[1];
     stats[2] += n;
     stats[7] = std::max<int64_t>(stats[7], n);
@@ -308,7 +330,7 @@ void Server::admit_and_launch() {
                 if (e.valid) {  // meshed while this request waited
                     e.last_use = ++tick;
                     ++stats[3];
-                    finish(j.ticket, LVN_REQ_MESH, e.status, &e, true, j.submit);
+                    complete_mesh(e, j.key, j, true);
                     waiting.pop_front();
                     continue;
                 }
@@ -328,6 +350,7 @@ void Server::admit_and_launch() {
                     slot = victim->slot;
                     victim->slot = -1;
                     victim->valid = false;
+                    forget_tickets(*victim);
                     ++stats[4];
                 }
                 e.slot = slot;
@@ -376,8 +399,8 @@ void Server::poll_lanes() {
                 e.nt = k.edges > 0 ? 2 * k.quads : 0;
                 e.ns = k.edges > 0 ? k.seams : 0;
                 e.last_use = ++tick;
-                finish(j.ticket, LVN_REQ_MESH, e.status, &e, false, j.submit);
-                for (const Job& w : e.waiters) finish(w.ticket, LVN_REQ_MESH, e.status, &e, true, w.submit);
+                complete_mesh(e, j.key, j, false);
+                for (const Job& w : e.waiters) complete_mesh(e, j.key, w, true);
                 e.waiters.clear();
             }
             stats[6] = 0;
@@ -499,7 +522,12 @@ int lvn_server_get_mesh(lvn_server* sh, int64_t ticket, lvn_vertex* v, int32_t v
     if (!s || !nv || !nt || !ns) return LVN_ERR_INVALID;
     std::lock_guard<std::mutex> lock(s->cache_mu);
     auto tk = s->ticket_key.find(ticket);
-    if (tk == s->ticket_key.end()) return LVN_ERR_INVALID;
+    if (tk == s->ticket_key.end()) {
+        // not completed yet, an edit, or its chunk's mesh has since been evicted or
+        // invalidated: a state error for a ticket this server issued
+        std::lock_guard<std::mutex> lk(s->mu);
+        return ticket >= 1 && ticket < s->next_ticket ? LVN_ERR_STATE : LVN_ERR_INVALID;
+    }
     auto it = s->cache.find(tk->second);
     if (it == s->cache.end() || !it->second.valid || it->second.in_flight || it->second.slot < 0) return LVN_ERR_STATE;
     const Entry& e = it->second;

@@ -4,7 +4,7 @@ list and data, voxel set, codes, masks, materials, vertex positions and normals)
 Each case draws a field kind (heightfield, heightfield + caves, leven's own terrain), its
 fractal parameters (octaves 1-8, frequencies, lacunarities away from 2 so octave scaling is
 inexact, gains, amplitudes, cave threshold), a noise seed of either sign, a chunk size
-(N = 16, 32 or 64 voxels) and LOD (0-4), a position with the surface inside the chunk, and
+(N = 16, 32 or 64 voxels; a second sweep N = 4 and 8) and LOD (0-4), a position with the surface inside the chunk, and
 0-6 brushes (spheres and Y-rotated boxes, add and subtract, several materials) through
 the chunk (needs a GPU).
 """
@@ -18,7 +18,7 @@ from test_gpu_parity import compare_stages
 pytestmark = pytest.mark.gpu
 
 
-def _case(k):
+def _case(k, n_choice=(16, 32, 64), n_p=(0.4, 0.3, 0.3)):
     rng = np.random.default_rng(1000 + k)
     kind = int(rng.integers(0, 3))
     desc = abi.field_desc(kind=kind, height_octaves=int(rng.integers(1, 9)),
@@ -29,7 +29,7 @@ def _case(k):
                           cave_lacunarity=float(rng.uniform(1.7, 2.3)), cave_gain=float(rng.uniform(0.4, 0.6)),
                           cave_threshold=float(rng.uniform(-0.1, 0.5)))
     seed = int(rng.integers(-2 ** 31, 2 ** 31 - 1))
-    n = int(rng.choice([16, 32, 64], p=[0.4, 0.3, 0.3]))
+    n = int(rng.choice(list(n_choice), p=list(n_p)))
     lod = int(rng.integers(0, 5 if n < 64 else 3))
     size = n * 4 << lod
     scale = size // (n * 4)
@@ -68,9 +68,8 @@ def _ops(c, mn):
     return ops
 
 
-@pytest.mark.parametrize("k", range(24))
-def test_random_case_bit_exact(oracle, k):
-    c = _case(k)
+def _run_case(oracle, c):
+    k = c["k"]
     perm = oracle.perm_table(c["seed"])
     mn = _surface_min(oracle, c, perm)
     ops = _ops(c, mn)
@@ -83,3 +82,15 @@ def test_random_case_bit_exact(oracle, k):
     ch = oracle.Chunk(c["n"]).generate(c["desc"], perm, mn, c["size"], ops=ops).build()
     compare_stages(g, ch, c["n"], c["size"])
     ctx.close()
+
+
+@pytest.mark.parametrize("k", range(24))
+def test_random_case_bit_exact(oracle, k):
+    _run_case(oracle, _case(k))
+
+
+@pytest.mark.parametrize("k", range(100, 112))
+def test_small_chunk_case_bit_exact(oracle, k):
+    """N = 4 and 8: partial LDS staging of sample rows, CSG slabs past the chunk's z
+    range and single-segment voxel rows."""
+    _run_case(oracle, _case(k, n_choice=(4, 8), n_p=(0.5, 0.5)))
