@@ -62,8 +62,10 @@ def _ops(c, mn):
         op["material"] = int(rng.integers(1, 6)) if add else MATERIAL_AIR
         centre = lo + rng.uniform(0.15, 0.85, 3) * n * scale
         op["origin"] = [centre[0] + 0.5, centre[1] + 0.5, centre[2] + 0.5, 0.0]
-        r = rng.uniform(1.5, n * scale / 4)
-        op["dims"] = [r, r, r, 0.0] if op["brush"] == 1 else [r, rng.uniform(1.5, r), rng.uniform(1.5, r), 0.0]
+        hi = n * scale / 4  # N = 4 at LOD 0 leaves a chunk too small for 1.5-voxel brushes
+        r = rng.uniform(1.5, hi) if hi > 1.5 else rng.uniform(0.5, 1.5)
+        lo_d = min(1.5, r)
+        op["dims"] = [r, r, r, 0.0] if op["brush"] == 1 else [r, rng.uniform(lo_d, r), rng.uniform(lo_d, r), 0.0]
         op["rotate_y"] = float(rng.uniform(-np.pi, np.pi)) if op["brush"] == 0 else 0.0
     return ops
 
