@@ -334,6 +334,31 @@ int  lvn_server_get_mesh(lvn_server* s, int64_t ticket, lvn_vertex* v, int32_t v
  * results re-meshed because an edit arrived in flight, cached chunks, max batch seen. */
 int  lvn_server_stats(lvn_server* s, int64_t* stats);
 
+/* ---- allocation-free single-chunk form ------------------------------------- */
+
+/* No counterpart in the reference.  For callers that cannot allocate device memory:
+ * code whose work is recorded under stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  Meshes the chunk (min, size) fresh with the stored CSG ops
+ * that overlap it and exports it with generateChunkMesh's rules, exactly as
+ * lvn_generate_chunk_mesh does for a chunk its context has not cached, but needs no
+ * context and calls no allocator: the chunk's working arrays live in one fixed device
+ * area of the library (LVN_NOALLOC_SCRATCH_BYTES on each GPU it runs on, held until
+ * the process ends), located per device by lvn_init or by the first call on a device.
+ * Limits: voxels_per_chunk a power of two in [4, 64]; a chunk with more than
+ * LVN_NOALLOC_MAX_EDGES compacted edges or LVN_NOALLOC_MAX_VOXELS active voxels, or
+ * more than LVN_NOALLOC_MAX_OPS overlapping stored ops, returns LVN_ERR_OVERFLOW with
+ * nothing written.  v / tri / s may be host or device memory.  The call is synchronous
+ * and holds the library's global lock, so calls from one process never overlap; the
+ * area keeps the last call's contents, and each call clears it before use. */
+#define LVN_NOALLOC_MAX_EDGES     131072
+#define LVN_NOALLOC_MAX_VOXELS    65536
+#define LVN_NOALLOC_MAX_OPS       1024
+#define LVN_NOALLOC_SCRATCH_BYTES (17u << 20)
+int lvn_generate_chunk_mesh_noalloc(int32_t voxels_per_chunk, uint32_t flags, const int32_t min[3], int32_t size,
+                                    lvn_vertex* v, int32_t v_cap, int32_t* nv,
+                                    int32_t* tri, int32_t tri_cap, int32_t* nt,
+                                    lvn_seam_node* s, int32_t s_cap, int32_t* ns);
+
 /* ---- test hook: intermediate stages of one chunk --------------------------- */
 
 /* Buffers for lvn_debug_chunk_stages.  F = voxels_per_chunk + 2, H = voxels_per_chunk + 1.

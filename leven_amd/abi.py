@@ -36,6 +36,12 @@ FIELD_LEVEN_TERRAIN = 2
 CTX_FP16_DENSITY = 1
 CTX_PRECISE_QEF = 2
 
+# limits of the allocation-free single-chunk form (lvn_generate_chunk_mesh_noalloc)
+NOALLOC_MAX_EDGES = 131072
+NOALLOC_MAX_VOXELS = 65536
+NOALLOC_MAX_OPS = 1024
+NOALLOC_SCRATCH_BYTES = 17 << 20
+
 
 class CsgOp(C.Structure):
     _fields_ = [

@@ -168,6 +168,11 @@ constexpr int kDefaultRowBlocks = 8192;
 constexpr int kDefaultVoxelBlocks = 8192;
 
 DevOp make_dev_op(const lvn_csg_op& op);
+// Kernel arguments of one arena with the default launch shapes of the one-chunk forms
+// (the tables and field of the global state).
+KArgs make_kargs(char* arena, const SlotLayout& L, const Geometry& G, const Caps& caps, const ChunkDesc* chunks,
+                 Counters* counters, const DevOp* ops, const int* op_index, uint32_t flags, int* queue,
+                 int* item_prefix, int cus);
 Geometry make_geometry(int n);
 Caps make_caps(const Geometry& G, int edge_cap, int voxel_cap);
 SlotLayout make_layout(const Geometry& G, const Caps& c);

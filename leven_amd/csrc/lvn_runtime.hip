@@ -325,17 +325,19 @@ int Ctx::ensure_ops(int nops, int nidx) {
     return LVN_OK;
 }
 
-KArgs Ctx::kargs() const {
+KArgs make_kargs(char* arena, const SlotLayout& L, const Geometry& G, const Caps& caps, const ChunkDesc* chunks,
+                 Counters* counters, const DevOp* ops, const int* op_index, uint32_t flags, int* queue,
+                 int* item_prefix, int cus) {
     KArgs a;
     std::memset(&a, 0, sizeof(a));
-    a.arena = d_arena;
+    a.arena = arena;
     a.L = L;
     a.G = G;
     a.caps = caps;
-    a.chunks = d_chunks;
-    a.counters = d_counters;
-    a.ops = d_ops;
-    a.op_index = d_op_index;
+    a.chunks = chunks;
+    a.counters = counters;
+    a.ops = ops;
+    a.op_index = op_index;
     a.F = g_state.fp;
     a.perm = g_state.d_perm;
     a.grads = g_state.d_grads;
@@ -343,8 +345,8 @@ KArgs Ctx::kargs() const {
     a.fp16 = (flags & LVN_CTX_FP16_DENSITY) ? 1 : 0;
     a.ysplit = 1;
     a.precise = (flags & LVN_CTX_PRECISE_QEF) ? 1 : 0;
-    a.queue = d_queue;
-    a.item_prefix = d_item_prefix;
+    a.queue = queue;
+    a.item_prefix = item_prefix;
     a.cus = cus;
     a.persist_per_cu = kDefaultPersistPerCu;
     a.density_per_cu = kDefaultDensityPerCu;
@@ -352,6 +354,11 @@ KArgs Ctx::kargs() const {
     a.row_blocks = kDefaultRowBlocks;
     a.voxel_blocks = kDefaultVoxelBlocks;
     return a;
+}
+
+KArgs Ctx::kargs() const {
+    return make_kargs(d_arena, L, G, caps, d_chunks, d_counters, d_ops, d_op_index, flags, d_queue, d_item_prefix,
+                      cus);
 }
 
 Ctx::~Ctx() {
@@ -709,6 +716,76 @@ static int upload_perm_table() {
     LVN_HIP(hipMemcpy(g_state.d_perm, pad.data(), kPermPaddedBytes, hipMemcpyHostToDevice));
     return LVN_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Allocation-free single-chunk form (lvn_generate_chunk_mesh_noalloc)
+// ---------------------------------------------------------------------------
+
+// The device area of the allocation-free form: one chunk slot at the form's caps, its
+// descriptor, counters, op list and queue heads.  Each GPU holds its own copy, at its own
+// address; host code uses that address (noalloc_ptr), never this name.
+static __device__ __align__(256) char scratch[LVN_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+struct NoAllocDevice {
+    char* ptr = nullptr;  // address of `scratch` on the device
+    int cus = 0;
+};
+static NoAllocDevice noalloc_dev[kMaxDevices];  // guarded by g_state.mu
+// host staging of one call (guarded by g_state.mu)
+static DevOp noalloc_h_ops[LVN_NOALLOC_MAX_OPS];
+static int noalloc_h_index[LVN_NOALLOC_MAX_OPS];
+
+// Locates `scratch` on the current device once (called with g_state.mu held).
+static int noalloc_setup_locked(int* dev_out) {
+    int dev = 0;
+    LVN_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return LVN_ERR_DEVICE;
+    NoAllocDevice& d = noalloc_dev[dev];
+    if (!d.ptr) {
+        void* p = nullptr;
+        LVN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        LVN_HIP(hipDeviceGetAttribute(&d.cus, hipDeviceAttributeMultiprocessorCount, dev));
+        d.ptr = static_cast<char*>(p);
+    }
+    *dev_out = dev;
+    return LVN_OK;
+}
+
+// Clears n16 16-byte words (the area keeps the previous call's contents).
+__global__ void k_noalloc_clear(uint4* p, uint64_t n16) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Offsets of the parts of the area for one chunk of N voxels per side.
+struct NoAllocLayout {
+    Geometry G;
+    Caps caps;
+    SlotLayout L;
+    uint64_t chunk, counters, ops, index, queue, prefix, bytes;
+};
+
+static NoAllocLayout noalloc_layout(int n) {
+    NoAllocLayout t;
+    t.G = make_geometry(n);
+    t.caps = make_caps(t.G, LVN_NOALLOC_MAX_EDGES, LVN_NOALLOC_MAX_VOXELS);
+    t.L = make_layout(t.G, t.caps);
+    uint64_t o = t.L.slot_bytes;
+    auto take = [&](uint64_t bytes) {
+        const uint64_t at = o;
+        o = align256(o + bytes);
+        return at;
+    };
+    t.chunk = take(sizeof(ChunkDesc));
+    t.counters = take(sizeof(Counters));
+    t.ops = take(sizeof(DevOp) * LVN_NOALLOC_MAX_OPS);
+    t.index = take(sizeof(int) * LVN_NOALLOC_MAX_OPS);
+    t.queue = take(sizeof(int) * kNumQueues);
+    t.prefix = take(sizeof(int) * 2);
+    t.bytes = o;
+    return t;
+}
 }  // namespace lvn
 
 extern "C" {
@@ -738,7 +815,8 @@ int lvn_init(const lvn_field_desc* field, int32_t noise_seed, uint32_t default_m
     g_state.seed = noise_seed;
     g_state.init = true;
     ++g_state.generation;
-    return LVN_OK;
+    int dev = 0;
+    return noalloc_setup_locked(&dev);  // the allocation-free form's area on this device
 }
 
 int lvn_shutdown(void) {
@@ -927,6 +1005,69 @@ int lvn_generate_chunk_mesh(lvn_ctx* ctx, const int32_t min[3], int32_t size, lv
     *ns = m->S;
     if (m->V > v_cap || m->T > tri_cap || m->S > s_cap) return LVN_ERR_CAPACITY;
     return export_mesh(c, *m, v, tri, s);
+}
+
+int lvn_generate_chunk_mesh_noalloc(int32_t n, uint32_t flags, const int32_t min[3], int32_t size, lvn_vertex* v,
+                                    int32_t v_cap, int32_t* nv, int32_t* tri, int32_t tri_cap, int32_t* nt,
+                                    lvn_seam_node* s, int32_t s_cap, int32_t* ns) {
+    if (!min || !nv || !nt || !ns) return LVN_ERR_INVALID;
+    if (n < 4 || n > 64 || (n & (n - 1)) || (flags & ~(uint32_t)(LVN_CTX_FP16_DENSITY | LVN_CTX_PRECISE_QEF)))
+        return LVN_ERR_INVALID;
+    if (!valid_chunk_size(size, n)) return LVN_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(g_state.mu);  // one call at a time: every call uses the one area
+    if (!g_state.init) return LVN_ERR_STATE;
+    *nv = *nt = *ns = 0;
+    int dev = 0;
+    int r = noalloc_setup_locked(&dev);
+    if (r) return r;
+    const NoAllocDevice& d = noalloc_dev[dev];
+    const NoAllocLayout t = noalloc_layout(n);
+    if (t.bytes > LVN_NOALLOC_SCRATCH_BYTES) return LVN_ERR_INVALID;  // not reached for n <= 64
+    // the stored ops overlapping the chunk, in store order (LoadDensityField's filter,
+    // src/compute_density_field.cpp:255-263), converted as for every other form
+    int nops = 0;
+    for (size_t i = 0; i < g_state.ops.size(); ++i) {
+        if (!aabb_overlaps(min, size, g_state.aabbs[i])) continue;
+        if (nops == LVN_NOALLOC_MAX_OPS) return LVN_ERR_OVERFLOW;
+        noalloc_h_ops[nops] = make_dev_op(g_state.ops[i]);
+        noalloc_h_index[nops] = nops;
+        ++nops;
+    }
+    static ChunkDesc cd;  // host staging, guarded by g_state.mu
+    static Counters k;
+    fill_chunk_desc(cd, min, size, n);
+    cd.op_start = 0;
+    cd.op_count = nops;
+    char* base = d.ptr;
+    hipStream_t st = nullptr;
+    hipLaunchKernelGGL(k_noalloc_clear, dim3(1024), dim3(256), 0, st, reinterpret_cast<uint4*>(base), t.bytes / 16);
+    LVN_HIP(hipGetLastError());
+    if (nops) {
+        LVN_HIP(hipMemcpyAsync(base + t.ops, noalloc_h_ops, sizeof(DevOp) * nops, hipMemcpyHostToDevice, st));
+        LVN_HIP(hipMemcpyAsync(base + t.index, noalloc_h_index, sizeof(int) * nops, hipMemcpyHostToDevice, st));
+    }
+    LVN_HIP(hipMemcpyAsync(base + t.chunk, &cd, sizeof(ChunkDesc), hipMemcpyHostToDevice, st));
+    KArgs a = make_kargs(base, t.L, t.G, t.caps, reinterpret_cast<const ChunkDesc*>(base + t.chunk),
+                         reinterpret_cast<Counters*>(base + t.counters), reinterpret_cast<const DevOp*>(base + t.ops),
+                         reinterpret_cast<const int*>(base + t.index), flags, reinterpret_cast<int*>(base + t.queue),
+                         reinterpret_cast<int*>(base + t.prefix), d.cus);
+    a.ysplit = 6;  // the one-chunk forms' density column split (single_args)
+    launch_pipeline(a, 1, st);
+    LVN_HIP(hipGetLastError());
+    LVN_HIP(hipMemcpyAsync(&k, base + t.counters, sizeof(Counters), hipMemcpyDeviceToHost, st));
+    LVN_HIP(hipStreamSynchronize(st));
+    if (k.status) return LVN_ERR_OVERFLOW;
+    if (k.edges == 0) return LVN_OK;  // no octree (src/compute_octree.cpp:167-171)
+    const int V = k.quads > 0 ? k.voxels : 0, T = 2 * k.quads, S = k.seams;  // src/compute_octree.cpp:227-232
+    *nv = V;
+    *nt = T;
+    *ns = S;
+    if (V > v_cap || T > tri_cap || S > s_cap) return LVN_ERR_CAPACITY;
+    if (V && v) LVN_HIP(hipMemcpyAsync(v, base + t.L.verts, sizeof(lvn_vertex) * (size_t)V, hipMemcpyDefault, st));
+    if (T && tri) LVN_HIP(hipMemcpyAsync(tri, base + t.L.tris, 12 * (size_t)T, hipMemcpyDefault, st));
+    if (S && s) LVN_HIP(hipMemcpyAsync(s, base + t.L.seams, sizeof(lvn_seam_node) * (size_t)S, hipMemcpyDefault, st));
+    LVN_HIP(hipStreamSynchronize(st));
+    return LVN_OK;
 }
 
 int lvn_batch_create(int32_t voxels_per_chunk, int32_t max_chunks, int32_t edge_cap, int32_t voxel_cap,

@@ -27,6 +27,7 @@ EXPORTS = [
     "lvn_server_create", "lvn_server_destroy", "lvn_server_submit", "lvn_server_poll", "lvn_server_drain",
     "lvn_server_get_mesh", "lvn_server_stats", "lvn_generate_seam_mesh", "lvn_generate_seam_meshes",
     "lvn_seam_device_mesh", "lvn_batch_seam_meshes", "lvn_batch_seam_device_mesh",
+    "lvn_generate_chunk_mesh_noalloc",
 ]
 
 _lib = None
@@ -62,6 +63,8 @@ def lib():
             "lvn_chunk_has_surface": (i32, [vp, vp, i32, C.POINTER(i32)]),
             "lvn_generate_chunk_mesh": (i32, [vp, vp, i32, vp, i32, C.POINTER(i32), vp, i32, C.POINTER(i32), vp,
                                               i32, C.POINTER(i32)]),
+            "lvn_generate_chunk_mesh_noalloc": (i32, [i32, u32, vp, i32, vp, i32, C.POINTER(i32), vp, i32,
+                                                      C.POINTER(i32), vp, i32, C.POINTER(i32)]),
             "lvn_debug_chunk_stages": (i32, [vp, vp, i32, vp, i32, C.POINTER(StageDump)]),
             "lvn_batch_create": (i32, [i32, i32, i32, i32, u32, C.POINTER(vp)]),
             "lvn_batch_destroy": (None, [vp]),
@@ -138,6 +141,27 @@ def perm_table():
     out = np.zeros(65536, dtype=np.uint8)
     check(lib().lvn_get_perm_table(vptr(out)), "lvn_get_perm_table")
     return out
+
+
+def generate_chunk_mesh_noalloc(voxels_per_chunk, min_xyz, size, flags=0, v_cap=None, tri_cap=None, s_cap=None):
+    """lvn_generate_chunk_mesh_noalloc: one chunk meshed fresh in the library's fixed device
+    area (no context, no allocation).  Returns (vertices, triangles int32[T,3], seams)."""
+    n3 = voxels_per_chunk ** 3
+    v_cap = n3 if v_cap is None else v_cap
+    tri_cap = 6 * n3 if tri_cap is None else tri_cap
+    s_cap = n3 if s_cap is None else s_cap
+    v = np.zeros(max(v_cap, 1), dtype=VERTEX_DTYPE)
+    t = np.zeros((max(tri_cap, 1), 3), dtype=np.int32)
+    s = np.zeros(max(s_cap, 1), dtype=SEAM_DTYPE)
+    nv, nt, ns = C.c_int32(), C.c_int32(), C.c_int32()
+    rc = lib().lvn_generate_chunk_mesh_noalloc(int(voxels_per_chunk), int(flags), vptr(_i3(min_xyz)), int(size),
+                                               vptr(v), v_cap, C.byref(nv), vptr(t), tri_cap, C.byref(nt), vptr(s),
+                                               s_cap, C.byref(ns))
+    if rc == LVN_ERR_CAPACITY:
+        raise LvnError(rc, f"lvn_generate_chunk_mesh_noalloc (needs {nv.value} vertices, {nt.value} triangles, "
+                           f"{ns.value} seam nodes)")
+    check(rc, "lvn_generate_chunk_mesh_noalloc")
+    return v[:nv.value].copy(), t[:nt.value].copy(), s[:ns.value].copy()
 
 
 def _seam_request(hosts, sources_per_host):

@@ -56,7 +56,8 @@ def test_header_constants_match_binding():
     assert defines["LVN_NUM_STAGES"] == mesher.NUM_STAGES
     for name in ["LVN_OK", "LVN_ERR_INVALID", "LVN_ERR_NOMEM", "LVN_ERR_CAPACITY", "LVN_ERR_DEVICE",
                  "LVN_ERR_STATE", "LVN_ERR_OVERFLOW", "LVN_MATERIAL_NONE", "LVN_MATERIAL_AIR",
-                 "LVN_CTX_FP16_DENSITY", "LVN_CTX_PRECISE_QEF", "LVN_REQ_MESH", "LVN_REQ_EDIT"]:
+                 "LVN_CTX_FP16_DENSITY", "LVN_CTX_PRECISE_QEF", "LVN_REQ_MESH", "LVN_REQ_EDIT",
+                 "LVN_NOALLOC_MAX_EDGES", "LVN_NOALLOC_MAX_VOXELS", "LVN_NOALLOC_MAX_OPS"]:
         short = name[len("LVN_"):]
         assert defines[name] == getattr(abi, short if hasattr(abi, short) else name), name
 
