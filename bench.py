@@ -151,42 +151,6 @@ def precision_leg(args):
             "accuracy": "vertex error <= 1.9e-5 voxel vs FP64 (tests/test_gpu_precision.py)"}
 
 
-def clipmap_leg(args):
-    """The 6-LOD clipmap (1,890 chunks at voxel sizes 1..32; coarse cave chunks have up to
-    ~237k of their 262,144 voxels active) as one batch with full per-slot capacities,
-    autotuned."""
-    wl = workloads.clipmap()
-    mesher.init(wl["desc"], wl["seed"], wl["default_material"])
-    mesher.clear_csg_ops()
-    for op, bb in zip(wl["ops"], wl["aabbs"]):
-        mesher.store_csg_op(op, bb[:3], bb[3:])
-    n = len(wl["sizes"])
-    b = mesher.Batch(64, n, edge_cap=0, voxel_cap=0)
-    b.set_chunks(wl["mins"], wl["sizes"])
-    tuning = b.autotune(reps=1)
-    steps = max(2, args.steps // 4)
-    for _ in range(max(1, args.warmup)):
-        b.run()
-    b.wait()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        b.run()
-    b.wait()
-    elapsed = time.perf_counter() - t0
-    counts = b.counts()
-    assert (counts[:, 4] == 0).all(), "a clipmap chunk overflowed"
-    stage_ms = {k: round(v, 4) for k, v in b.run_timed().items()}
-    lods = np.log2(wl["sizes"] // 256).astype(int)
-    # active clipmap nodes (a mesh or seam nodes, src/clipmap.cpp:468) with their seam node lists
-    nodes = [(tuple(int(v) for v in wl["mins"][i]), int(wl["sizes"][i]), b.get_mesh(i)[2])
-             for i in range(n) if counts[i, 2] > 0 or counts[i, 3] > 0] if not args.no_seams else None
-    return b, nodes, {"chunks": n, "chunks_per_lod": np.bincount(lods, minlength=6).tolist(),
-            "steps": steps, "chunks_per_s": round(n * steps / elapsed, 1), "ms_per_pass": round(elapsed / steps * 1e3, 3),
-            "nonempty_chunks": int((counts[:, 2] > 0).sum()), "max_active_voxels": int(counts[:, 1].max()),
-            "max_edges": int(counts[:, 0].max()), "triangles": int(counts[:, 2].sum()),
-            "stage_ms": stage_ms, "tuning": tuning}
-
-
 def seam_leg(batch, nodes, args):
     """Seam meshes of every active clipmap node (GenerateClipmapSeamMesh, src/clipmap.cpp:573-611):
     neighbour seam node lists gathered as the reference does, all seams in one call, in two
@@ -425,16 +389,176 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n, "copies": 5}), flush=True)
 
 
+def stage_ops(wl):
+    mesher.init(wl["desc"], wl["seed"], wl["default_material"])
+    mesher.clear_csg_ops()
+    for op, bb in zip(wl["ops"], wl["aabbs"]):
+        mesher.store_csg_op(op, bb[:3], bb[3:])
+
+
+def batched_leg(wl, args, steps, edge_cap, voxel_cap, plain=True, keep=False):
+    """One workload through the batched form (one graph launch per pass over every chunk,
+    inputs staged in HBM, launch shapes autotuned outside the timed region), per-stage HIP
+    event times, work counters and the roofline of the dominant ALU stage; then the plain
+    per-chunk form (one chunk per graph launch, synchronised per chunk) over the same
+    chunks.  Returns (result dict, batch or None)."""
+    stage_ops(wl)
+    desc = wl["desc"]
+    n = len(wl["sizes"])
+    batch = mesher.Batch(64, n, edge_cap=edge_cap, voxel_cap=voxel_cap, flags=wl["flags"])
+    batch.set_chunks(wl["mins"], wl["sizes"])
+    tuning = batch.autotune(reps=1 if n > 1000 else 3) if not args.no_autotune else batch.tuning()
+    for _ in range(max(1, args.warmup)):
+        batch.run()
+    batch.wait()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run()
+    batch.wait()
+    elapsed = time.perf_counter() - t0
+    rate = n * steps / elapsed
+    counts = batch.counts()
+    assert (counts[:, 4] == 0).all(), f"a {wl['name']} chunk overflowed its slot capacity"
+    # per-stage launch durations: HIP events around each stage's launches on the batch
+    # stream, averaged over the same number of passes
+    runs = [batch.run_timed() for _ in range(steps)]
+    stage_ms = {k: sum(r[k] for r in runs) / len(runs) for k in runs[0]}
+    work = batch.work()
+    d_flops, c_flops = stage_flops(work, desc)
+    dom = max(("density", "edge_cross"), key=lambda k: stage_ms[k])
+    a_d, a_c = algorithmic_flops(n, work["edges"], desc)
+    dom_flops = a_d if dom == "density" else a_c
+    exec_flops = d_flops if dom == "density" else c_flops
+    achieved = dom_flops / (stage_ms[dom] * 1e-3) / 1e12
+    lods = np.log2(wl["sizes"] // 256).astype(int)
+    per_lod = {}
+    for lod in sorted(set(lods.tolist())):
+        sel = lods == lod
+        per_lod[str(lod)] = {"chunks": int(sel.sum()), "max_active_voxels": int(counts[sel, 1].max()),
+                             "mean_active_voxels": int(counts[sel, 1].mean())}
+    out = {"chunks": n, "steps": steps, "chunks_per_s": round(rate, 2), "ms_per_pass": round(elapsed / steps * 1e3, 4),
+           "totals": {"edges": int(counts[:, 0].sum()), "voxels": int(counts[:, 1].sum()),
+                      "triangles": int(counts[:, 2].sum()), "nonempty_chunks": int((counts[:, 2] > 0).sum()),
+                      "max_active_voxels": int(counts[:, 1].max()), "max_edges": int(counts[:, 0].max())},
+           "per_lod": per_lod,
+           "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()}, "work": work,
+           "roofline": {"bound": "valu", "kernel": "k_" + dom, "achieved": round(achieved, 3),
+                        "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                        "traffic": hbm_traffic("k_" + dom, wl["name"]), "flops_per_launch": int(dom_flops),
+                        "flops_unit": "23 full field evaluations per edge" if dom == "edge_cross"
+                        else "65^3 full field evaluations per chunk",
+                        "launch_ms": round(stage_ms[dom], 4), "executed_flops_per_launch": int(exec_flops),
+                        "executed_tflops": round(exec_flops / (stage_ms[dom] * 1e-3) / 1e12, 3)},
+           "tuning": tuning}
+    hbm = hbm_rates(stage_ms, wl["name"])
+    if hbm:
+        out["hbm_gbs"] = hbm
+    if plain:
+        # the plain per-chunk form: the same kernels, one chunk per graph launch + sync
+        pb = mesher.Batch(64, 1, edge_cap=edge_cap, voxel_cap=voxel_cap, flags=wl["flags"])
+        m = n if args.plain_chunks <= 0 else min(n, args.plain_chunks)
+        idx = np.linspace(0, n - 1, m).astype(int)
+        pb.set_chunks(wl["mins"][:1], wl["sizes"][:1])
+        pb.run()
+        pb.wait()
+        lat = []
+        t0 = time.perf_counter()
+        for i in idx:
+            ts = time.perf_counter()
+            pb.set_chunks(wl["mins"][i:i + 1], wl["sizes"][i:i + 1])
+            pb.run()
+            pb.wait()
+            lat.append(time.perf_counter() - ts)
+        plain_rate = m / (time.perf_counter() - t0)
+        pb.close()
+        lat = np.array(lat) * 1e3
+        out["plain"] = {"chunks": int(m), "chunks_per_s": round(plain_rate, 2),
+                        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                       "p99": round(float(np.percentile(lat, 99)), 3)}}
+        out["batched_over_plain"] = round(rate / plain_rate, 2)
+    if keep:
+        return out, batch
+    batch.close()
+    return out, None
+
+
+def cpu_sample(wl, count, threads):
+    """The oracle on `count` chunks of the workload spread evenly over its chunk list (in
+    the clipmap: every LOD in proportion), `threads` chunks at a time, and the first 8 of
+    them on one thread."""
+    from oracle import py_oracle
+    perm = py_oracle.perm_table(wl["seed"])
+    n = len(wl["sizes"])
+    idx = np.linspace(0, n - 1, min(n, count)).astype(int)
+    wall, secs, _ = py_oracle.mesh_chunks(wl["desc"], perm, 64, wl["mins"][idx], wl["sizes"][idx],
+                                          wl["default_material"], wl["ops"], wl["aabbs"], threads=threads)
+    one = idx[:8]
+    wall1, _, _ = py_oracle.mesh_chunks(wl["desc"], perm, 64, wl["mins"][one], wl["sizes"][one],
+                                        wl["default_material"], wl["ops"], wl["aabbs"], threads=1)
+    return dict(value=round(len(idx) / wall, 2), unit="chunks/s", cores=threads, kind="port",
+                single_thread_chunks_per_s=round(len(one) / wall1, 3),
+                sample=f"{len(idx)} of the {n} {wl['name']} chunks, evenly spaced over the chunk list, on "
+                       f"{threads} threads ({os.cpu_count()} logical CPUs visible, {_cpu_model()}); oracle = "
+                       f"C++ restatement of leven's semantics, FP32, one chunk per thread (no CPU OpenCL device "
+                       f"exists to run leven itself): {wall:.2f} s wall, p50 {np.percentile(secs, 50) * 1e3:.1f} "
+                       f"ms, p99 {np.percentile(secs, 99) * 1e3:.1f} ms per chunk; single thread "
+                       f"{len(one)} chunks in {wall1:.2f} s")
+
+
+def plain_cpu_leg(args, threads):
+    """The one-chunk heightfield workload (64^3, 6-octave 2D fBm): the plain per-chunk GPU
+    form's latency, and the oracle on one thread and on `threads` threads (that many copies
+    of the chunk, one per thread)."""
+    wl = workloads.plain_cpu()
+    stage_ops(wl)
+    pb = mesher.Batch(64, 1, edge_cap=0, voxel_cap=0, flags=wl["flags"])
+    pb.set_chunks(wl["mins"], wl["sizes"])
+    lat = []
+    for k in range(60):
+        ts = time.perf_counter()
+        pb.run()
+        pb.wait()
+        if k >= 10:
+            lat.append(time.perf_counter() - ts)
+    counts = pb.counts()
+    pb.close()
+    lat = np.array(lat) * 1e3
+    out = {"chunks": 1, "active_voxels": int(counts[0, 1]), "triangles": int(counts[0, 2]),
+           "gpu_plain_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                    "p99": round(float(np.percentile(lat, 99)), 3)}}
+    if not args.no_cpu:
+        from oracle import py_oracle
+        perm = py_oracle.perm_table(wl["seed"])
+        reps = 8
+        m1 = np.repeat(wl["mins"], reps, axis=0)
+        s1 = np.repeat(wl["sizes"], reps)
+        wall1, secs, _ = py_oracle.mesh_chunks(wl["desc"], perm, 64, m1, s1, 0, None, None, threads=1)
+        mt = np.repeat(wl["mins"], threads, axis=0)
+        st = np.repeat(wl["sizes"], threads)
+        wallt, _, _ = py_oracle.mesh_chunks(wl["desc"], perm, 64, mt, st, 0, None, None, threads=threads)
+        out["cpu_baseline"] = {"value": round(threads / wallt, 2), "unit": "chunks/s", "cores": threads,
+                               "kind": "port", "single_thread_chunks_per_s": round(reps / wall1, 2),
+                               "single_thread_latency_ms": round(float(np.median(secs)) * 1e3, 2),
+                               "sample": f"the chunk meshed {reps} times on one thread and {threads} times on "
+                                         f"{threads} threads (one copy per thread)"}
+        out["gpu_over_cpu_single_thread"] = round(out["cpu_baseline"]["single_thread_latency_ms"] /
+                                                  out["gpu_plain_latency_ms"]["p50"], 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="terrain_512", choices=sorted(workloads.WORKLOADS))
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--workload", default="clipmap", choices=["clipmap", "terrain_512"],
+                    help="the headline workload (the other is reported beside it)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--cpu-sample", type=int, default=64, help="clipmap chunks in the CPU baseline sample")
     ap.add_argument("--plain-chunks", type=int, default=0, help="chunks for the plain form (0 = all)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="length of the served request stream (0 = skip)")
-    ap.add_argument("--no-clipmap", action="store_true", help="skip the 6-LOD clipmap batch")
+    ap.add_argument("--only", default="", help="comma-separated legs to run (clipmap, terrain_512, in_range, "
+                                                 "precision, seams, serving, plain_cpu); empty = all")
     ap.add_argument("--no-seams", action="store_true", help="skip seam stitching of the clipmap")
     ap.add_argument("--no-precision", action="store_true", help="skip the precision-form batch")
     ap.add_argument("--no-autotune", action="store_true", help="time the default launch shapes")
@@ -447,123 +571,95 @@ def main():
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the build is single-GPU); got --gpus {args.gpus}", file=sys.stderr)
         sys.exit(2)
+    only = set(v for v in args.only.split(",") if v)
 
-    wl = workloads.WORKLOADS[args.workload]()
-    desc = wl["desc"]
-    n = len(wl["sizes"])
-    mesher.init(desc, wl["seed"], wl["default_material"])
-    mesher.clear_csg_ops()
-    for op, bb in zip(wl["ops"], wl["aabbs"]):
-        mesher.store_csg_op(op, bb[:3], bb[3:])
+    def want(leg):
+        return not only or leg in only
 
-    # ---- batched form ---------------------------------------------------------
-    batch = mesher.Batch(64, n, edge_cap=262144, voxel_cap=131072, flags=wl["flags"])
-    batch.set_chunks(wl["mins"], wl["sizes"])
-    # launch shapes picked by the autotuner on these chunks (outside the timed region)
-    tuning = batch.autotune(reps=3) if not args.no_autotune else batch.tuning()
-    for _ in range(args.warmup):
-        batch.run()
-    batch.wait()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.run()
-    batch.wait()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed / args.steps * 1e3
-    batched = n * args.steps / elapsed
-    counts = batch.counts()
-    assert (counts[:, 4] == 0).all(), "a chunk overflowed its slot capacity"
+    threads = max(1, min(16, os.cpu_count() or 1))
+    legs = {}
+    # ---- the 6-LOD clipmap: ~1,900 chunks at voxel sizes 1..32, full per-chunk capacities
+    clip = workloads.clipmap()
+    cb, nodes = None, None
+    if want("clipmap") or want("seams") or want("in_range"):
+        legs["clipmap"], cb = batched_leg(clip, args, max(2, args.steps // 2), 0, 0, plain=want("clipmap"),
+                                          keep=True)
+        counts = cb.counts()
+        if want("in_range"):
+            # the clipmap's chunks within BASELINE's stated range of active voxels (0 to ~60k):
+            # the coarse cave chunks above it come from the 1/64-frequency cave noise sampled
+            # every 16-32 voxels, which is nearly white noise at those voxel sizes
+            keep = counts[:, 1] <= 60000
+            sub = dict(clip, name="clipmap_in_range", mins=clip["mins"][keep], sizes=clip["sizes"][keep])
+            ecap = int(-(-int(counts[keep, 0].max()) // 4096) * 4096)  # slots sized for these chunks
+            legs["clipmap"]["in_range"], _ = batched_leg(sub, args, max(2, args.steps // 2), ecap, 60000)
+            legs["clipmap"]["in_range"]["active_voxels_max"] = 60000
+        if want("seams") and not args.no_seams:
+            n = len(clip["sizes"])
+            nodes = [(tuple(int(v) for v in clip["mins"][i]), int(clip["sizes"][i]), cb.get_mesh(i)[2])
+                     for i in range(n) if counts[i, 2] > 0 or counts[i, 3] > 0]
+            stage_ops(clip)
+            legs["seams"] = seam_leg(cb, nodes, args)
+        cb.close()
+        if want("clipmap") and not args.no_cpu:
+            legs["clipmap"]["cpu_baseline"] = cpu_sample(clip, args.cpu_sample, threads)
+            legs["clipmap"]["batched_over_cpu"] = round(legs["clipmap"]["chunks_per_s"] /
+                                                        legs["clipmap"]["cpu_baseline"]["value"], 1)
+    # ---- the 512-chunk terrain with 256 brushes (plain per-chunk configuration)
+    if want("terrain_512"):
+        t5 = workloads.terrain_512()
+        legs["terrain_512"], _ = batched_leg(t5, args, args.steps, 262144, 131072)
+        stage_ops(t5)
+        legs["terrain_512"]["dropin_generate_chunk_mesh"] = dropin_latency(t5, 128)
+        if not args.no_cpu:
+            legs["terrain_512"]["cpu_baseline"] = cpu_baseline(t5, threads)
+            legs["terrain_512"]["batched_over_cpu"] = round(legs["terrain_512"]["chunks_per_s"] /
+                                                            legs["terrain_512"]["cpu_baseline"]["value"], 1)
+    if want("plain_cpu"):
+        legs["plain_cpu"] = plain_cpu_leg(args, threads)
+    if want("precision") and not args.no_precision:
+        legs["precision"] = precision_leg(args)
+    if want("serving") and args.serve_seconds > 0:
+        legs["serving"] = serve(args)
 
-    # per-stage launch durations: HIP events around each stage's launches on the batch
-    # stream, averaged over `steps` passes of the same pipeline
-    runs = [batch.run_timed() for _ in range(args.steps)]
-    stage_ms = {k: sum(r[k] for r in runs) / len(runs) for k in runs[0]}
-    work = batch.work()
-    d_flops, c_flops = stage_flops(work, desc)
-    dom = max(("density", "edge_cross"), key=lambda k: stage_ms[k])
-    a_d, a_c = algorithmic_flops(n, work["edges"], desc)
-    dom_flops = a_d if dom == "density" else a_c
-    exec_flops = d_flops if dom == "density" else c_flops
-    achieved = dom_flops / (stage_ms[dom] * 1e-3) / 1e12
-
-    # ---- plain per-chunk form ---------------------------------------------------
-    plain = mesher.Batch(64, 1, edge_cap=262144, voxel_cap=131072, flags=wl["flags"])
-    m = n if args.plain_chunks <= 0 else min(n, args.plain_chunks)
-    idx = np.linspace(0, n - 1, m).astype(int)
-    plain.set_chunks(wl["mins"][:1], wl["sizes"][:1])
-    plain.run()
-    plain.wait()
-    lat = []
-    t0 = time.perf_counter()
-    for i in idx:
-        ts = time.perf_counter()
-        plain.set_chunks(wl["mins"][i:i + 1], wl["sizes"][i:i + 1])
-        plain.run()
-        plain.wait()
-        lat.append(time.perf_counter() - ts)
-    plain_rate = m / (time.perf_counter() - t0)
-    lat = np.array(lat) * 1e3
-    batch.close()
-    plain.close()
-    dropin = dropin_latency(wl, min(n, 128))
-
+    head = legs.get(args.workload)
+    if head is None:  # a partial run (--only) without the headline leg
+        print(json.dumps({"metric": METRIC, "partial": True, **legs}), flush=True)
+        return
+    hw = clip if args.workload == "clipmap" else workloads.terrain_512()
+    roof = dict(head["roofline"])
+    roof["note"] = ("FP32 vector-ALU bound (neither HBM nor MFMA): achieved counts the method's field "
+                    "evaluations (23 per edge, 65^3 per chunk), executed_tflops what the kernels evaluate after "
+                    "the cave cut and shared heights; counters in profiles/kernels_" +
+                    ("clipmap" if args.workload == "clipmap" else "terrain512") + ".md")
     out = {
         "metric": METRIC,
-        "value": round(batched, 2),
+        "value": head["chunks_per_s"],
         "unit": "chunks/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": head["steps"],
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": head["ms_per_pass"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": wl["name"], "chunks": n, "voxels_per_chunk": 64,
-                   "field": "max(y - 24 fbm2, fbm3 - 0.3), 5+5 octaves" if desc.kind == 1 else "heightfield",
-                   "csg_ops": int(len(wl["ops"])), "form": "batched (one launch per stage over all chunks, hipGraph)"},
-        "plain_chunks_per_s": round(plain_rate, 2),
-        "batched_over_plain": round(batched / plain_rate, 2),
-        "plain_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
-                             "p99": round(float(np.percentile(lat, 99)), 3)},
-        "dropin_generate_chunk_mesh": dropin,
-        "totals": {"edges": int(counts[:, 0].sum()), "voxels": int(counts[:, 1].sum()),
-                   "triangles": int(counts[:, 2].sum()), "nonempty_chunks": int((counts[:, 2] > 0).sum())},
-        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
-        "work": work,
-        "roofline": {"bound": "valu", "kernel": "k_" + dom, "achieved": round(achieved, 3),
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": hbm_traffic("k_" + dom, wl["name"]), "flops_per_launch": int(dom_flops),
-                     "flops_unit": "23 full field evaluations per edge" if dom == "edge_cross"
-                     else "65^3 full field evaluations per chunk",
-                     "launch_ms": round(stage_ms[dom], 4),
-                     "executed_flops_per_launch": int(exec_flops),
-                     "executed_tflops": round(exec_flops / (stage_ms[dom] * 1e-3) / 1e12, 3),
-                     "note": "FP32 vector-ALU bound (neither HBM nor MFMA): rocprofv3 counts 0.39-0.42 VALU "
-                             "instructions per SIMD-cycle in k_edge_cross against a 0.5 ceiling, 65% of them "
-                             "FP32 arithmetic, 0.47 TB/s of HBM traffic (profiles/kernels_terrain512.md); "
-                             "achieved counts the method's 23 field evaluations per edge, executed_tflops "
-                             "what the kernels evaluate after the cave cut and shared heights"},
-        "tuning": tuning,
+        "config": {"workload": hw["name"], "chunks": len(hw["sizes"]), "voxels_per_chunk": 64,
+                   "lods": "voxel sizes 1..32 (6-LOD clipmap selection around a seeded camera)"
+                   if args.workload == "clipmap" else "LOD0",
+                   "field": "max(y - 24 fbm2, fbm3 - 0.3), 5+5 octaves", "csg_ops": int(len(hw["ops"])),
+                   "form": "batched (one launch per stage over all chunks, hipGraph)"},
+        "plain_chunks_per_s": head.get("plain", {}).get("chunks_per_s"),
+        "batched_over_plain": head.get("batched_over_plain"),
+        "batched_over_cpu": head.get("batched_over_cpu"),
+        "roofline": roof,
+        "cpu_baseline": head.get("cpu_baseline"),
     }
-    hbm = hbm_rates(stage_ms, wl["name"])
-    if hbm:
-        out["hbm_gbs"] = hbm
-    if not args.no_precision:
-        out["precision"] = precision_leg(args)
-    if not args.no_clipmap:
-        cb, nodes, out["clipmap"] = clipmap_leg(args)
-        if nodes:
-            out["seams"] = seam_leg(cb, nodes, args)
-        cb.close()
-    if args.serve_seconds > 0:
-        out["serving"] = serve(args)
-        out["p99_chunk_latency_ms"] = out["serving"]["latency_ms"]["p99"]
-    if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
-        out["cpu_baseline"] = cpu_baseline(wl, threads)
-        out["batched_over_cpu"] = round(batched / out["cpu_baseline"]["value"], 1)
+    if "serving" in legs:
+        out["p99_chunk_latency_ms"] = legs["serving"]["latency_ms"]["p99"]
+    for k, v in legs.items():
+        out[k] = v
     print(json.dumps(out), flush=True)
 
 

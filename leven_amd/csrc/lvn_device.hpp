@@ -21,12 +21,13 @@ namespace lvn {
 // wrapped ones plus 0 or 1, is one add away from the cell's base index with no wrap
 // masks per corner (REPEAT addressing, cl/simplex.cl:59).  Each byte holds the texel's
 // value plus one, mod 256, and byte y * 258 holds texel (0, y): snoise3's second read,
-// at x = value with 255 wrapping to 0, is then row + stored byte with no compare, and
-// the gradient table, indexed by the stored byte, replaces the "& 15".  70,416 bytes
-// per workgroup.
+// at x = value with 255 wrapping to 0, is then row + stored byte with no compare.  The
+// 16-entry gradient table is indexed by the stored byte's low four bits (rotated by one
+// entry, so no subtraction is needed): lanes that need the same gradient read the same
+// 16 bytes.  66,576 bytes per workgroup.
 struct NoiseLds {
     uint8_t perm[kPermPaddedBytes];
-    float4 grad[kGradEntries];  // entry w: gradient (w - 1) & 15
+    float4 grad[kGradEntries];  // entry j: gradient (j - 1) & 15, for stored bytes w with w & 15 == j
 };
 
 // Cooperative copy of the tables into LDS (all threads of the block call it).
@@ -73,7 +74,7 @@ __device__ __forceinline__ float recip_1_2(float d) {
 // value then adds +0 to a sum that is never -0), or the ridged form's fabsf.
 __device__ __forceinline__ float corner2(const NoiseLds& T, int texel, float fx, float fy) {
     const float r = fmaxf(0.5f - fmaf(fy, fy, fx * fx), 0.f);
-    const float4 gr = T.grad[T.perm[texel]];
+    const float4 gr = T.grad[T.perm[texel] & 15];
     const float t = r * r;
     return (t * t) * fmaf(gr.y, fy, gr.x * fx);
 }
@@ -106,7 +107,7 @@ __device__ __forceinline__ float snoise2(const NoiseLds& T, float px, float py) 
 __device__ __forceinline__ float corner3(const NoiseLds& T, int texel, int zrow, float fx, float fy, float fz) {
     const int w = T.perm[texel];
     const float r = fmaxf(0.6f - fmaf(fz, fz, fmaf(fy, fy, fx * fx)), 0.f);
-    const float4 gr = T.grad[T.perm[zrow + w]];
+    const float4 gr = T.grad[T.perm[zrow + w] & 15];
     const float t = r * r;
     return (t * t) * fmaf(gr.z, fz, fmaf(gr.y, fy, gr.x * fx));
 }

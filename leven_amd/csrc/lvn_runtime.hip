@@ -85,8 +85,9 @@ void build_perm_table(int32_t seed, uint8_t* out) {
         }
 }
 
-// UNORM8 decode of the gradient bytes g*64+64, then *4-1 (cl/simplex.cl:124), one entry
-// per stored table byte w = value + 1: the gradient of value w - 1, i.e. (w + 15) & 15.
+// UNORM8 decode of the gradient bytes g*64+64, then *4-1 (cl/simplex.cl:124).  Entry j
+// serves every stored table byte w = value + 1 with w & 15 == j: the gradient of value
+// w - 1, i.e. (j + 15) & 15.
 void build_grads(float4* out) {
     for (int w = 0; w < kGradEntries; ++w) {
         const int i = (w + 15) & 15;

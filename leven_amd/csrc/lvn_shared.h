@@ -10,11 +10,13 @@ namespace lvn {
 // The permutation table in device memory and LDS.  Each byte holds its texel's value
 // plus one (mod 256).  Rows of 258 bytes: texel (x, y) at y * 258 + 1 + x for x, y in
 // 0..256 (x or y = 256 repeating 0), and byte y * 258 holds texel (0, y) once more.
-// 257 rows, padded to 16 bytes.  The gradient table has one entry per stored byte.
+// 257 rows, padded to 16 bytes.  The gradient table has 16 entries indexed by the stored
+// byte's low four bits: entry j holds gradient (j - 1) & 15, the gradient of every value
+// w - 1 with w & 15 == j.
 constexpr int kPermPitch = 258;
 constexpr int kPermRows = 257;
 constexpr int kPermPaddedBytes = ((kPermPitch * kPermRows + 15) / 16) * 16;
-constexpr int kGradEntries = 256;
+constexpr int kGradEntries = 16;
 
 
 // Interior samples (k = 1..15 of the 17) of an x- or z-edge line whose heights are

@@ -14,7 +14,8 @@ from .abi import (CSG_OP_DTYPE, ERROR_NAMES, LVN_ERR_CAPACITY, LVN_OK, REQ_EDIT,
                   vptr)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblvn_mesher.so")
+# LVN_LIB_PATH selects another build of the library (e.g. to compare two builds)
+LIB_PATH = os.environ.get("LVN_LIB_PATH") or os.path.join(HERE, "liblvn_mesher.so")
 
 # every symbol declared in include/lvn_mesher.h
 EXPORTS = [
