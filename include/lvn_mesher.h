@@ -243,7 +243,8 @@ typedef struct lvn_tuning {
     int32_t persistent_per_cu;  /* persistent workgroups per CU in the edge-crossing stage, 1..2 */
     int32_t row_blocks;         /* workgroups over the batch for the edge / voxel bookkeeping kernels */
     int32_t voxel_blocks;       /* workgroups over the batch for the per-voxel kernels (QEF, emit) */
-    int32_t cross_lanes;        /* lanes sharing one edge in the edge-crossing stage: 1, 2, 4 or 8 */
+    int32_t cross_lanes;        /* lanes sharing one edge in the edge-crossing stage: 1, 2, 4 or 8, or 16:
+                                   sample-parallel (16 lanes per edge in the search, 6 in the normal) */
     int32_t density_per_cu;     /* persistent workgroups per CU in the density stage, 1..2 (0: 1) */
     int32_t height_lines;       /* heights of x/z-edge samples: 1 per sample, 2 once per line (0: 2 for
                                    batches of 64 chunks or more, else 1) */

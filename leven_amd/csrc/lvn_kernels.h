@@ -34,7 +34,7 @@ struct KArgs {
     const int* slot_map;     // batch entry -> arena slot, or null for the identity
     int row_blocks;          // workgroups over the batch for the bookkeeping kernels
     int voxel_blocks;        // workgroups over the batch for the per-voxel kernels
-    int cross_lanes;         // lanes per edge in the crossing stage (1, 2, 4, 8), 0 = by batch size
+    int cross_lanes;         // lanes per edge in the crossing stage (1, 2, 4, 8; 16 = sample-parallel), 0 = by batch size
     int pad3;
 };
 
@@ -65,6 +65,9 @@ void launch_emit(const KArgs& a, int chunks, hipStream_t s);
 
 // Lanes per edge in the crossing stage when the launch shapes leave it open.
 int default_cross_lanes(int chunks);
+// cross_lanes value of the sample-parallel crossing kernel (16 lanes per edge in the
+// search, 6 in the normal).
+constexpr int kCrossSampleParallel = 16;
 
 // Whole pipeline for chunk slots [0, chunks), counters cleared first.
 void launch_pipeline(const KArgs& a, int chunks, hipStream_t s);

@@ -134,6 +134,7 @@ __device__ __forceinline__ void density_column(const KArgs& a, const NoiseLds& T
     const float wx = (float)(x * cd.scale + cd.off[0]);
     const float wz = (float)(z * cd.scale + cd.off[2]);
     const float H = height_term(a.F, T, wx, wz);
+    if (seg == 0) slot_ptr<float>(a, chunk, a.L.h_cols)[col] = H;  // read by the sample-parallel crossing
     float* dens = slot_ptr<float>(a, chunk, a.L.density);
     __half* dens16 = slot_ptr<__half>(a, chunk, a.L.density);
     unsigned caves = 0;
@@ -523,6 +524,36 @@ __global__ __launch_bounds__(1024) void k_height_lines(KArgs a, int chunks) {
     }
 }
 
+// The normal of FindEdgeIntersectionInfo (cl/density_field.cl:131-150): central
+// differences with h = 0.001 at the arg-min sample t of the edge p0 -> p1, whose height
+// term `Hbest` is reused when it was evaluated there.
+__device__ __forceinline__ float4 crossing_normal(const FieldParams& Fp, const NoiseLds& T, int axis, bool ends_known,
+                                                  float tbest, float Hbest, float p0x, float p0y, float p0z, float p1x,
+                                                  float p1y, float p1z, unsigned& nh, unsigned& nc) {
+    const bool caves = Fp.kind == LVN_FIELD_TERRAIN_CAVES;
+    const float px = p0x + (p1x - p0x) * tbest;
+    const float py = p0y + (p1y - p0y) * tbest;
+    const float pz = p0z + (p1z - p0z) * tbest;
+    const float h = 0.001f;
+    // The height at the arg-min sample is only known when it was evaluated there.
+    const bool hb_known = axis == 1 || !(ends_known && (tbest == 0.f || tbest == 1.f));
+    const float H0 = hb_known ? Hbest : height_term(Fp, T, px, pz);
+    const float Hxp = height_term(Fp, T, px + h, pz);
+    const float Hxm = height_term(Fp, T, px - h, pz);
+    const float Hzp = height_term(Fp, T, px, pz + h);
+    const float Hzm = height_term(Fp, T, px, pz - h);
+    const float dx = combine(Fp, T, py - Hxp, px + h, py, pz) - combine(Fp, T, py - Hxm, px - h, py, pz);
+    const float dy = combine(Fp, T, (py + h) - H0, px, py + h, pz) - combine(Fp, T, (py - h) - H0, px, py - h, pz);
+    const float dz = combine(Fp, T, py - Hzp, px, py, pz + h) - combine(Fp, T, py - Hzm, px, py, pz - h);
+    nh += hb_known ? 4u : 5u;
+    if (caves) {
+        nc += !((py - Hxp) > Fp.cave_cut) + !((py - Hxm) > Fp.cave_cut) + !(((py + h) - H0) > Fp.cave_cut) +
+              !(((py - h) - H0) > Fp.cave_cut) + !((py - Hzp) > Fp.cave_cut) + !((py - Hzm) > Fp.cave_cut);
+    }
+    const float3 n = normalize_or_zero(dx, dy, dz);
+    return make_float4(n.x, n.y, n.z, tbest);
+}
+
 // FindEdgeIntersectionInfo, cl/density_field.cl:96-151: first arg-min of |d|
 // over t = k/16, then a central-difference normal with h = 0.001.  The height
 // term of samples that share (x, z) is evaluated once: per y-edge, and per x- or
@@ -572,27 +603,7 @@ __device__ __forceinline__ float4 density_crossing(const KArgs& a, const NoiseLd
         }
         cur = cur + 0.0625f;
     }
-    const float px = p0x + (p1x - p0x) * tbest;
-    const float py = p0y + (p1y - p0y) * tbest;
-    const float pz = p0z + (p1z - p0z) * tbest;
-    const float h = 0.001f;
-    // The height at the arg-min sample is only known when it was evaluated there.
-    const bool hb_known = axis == 1 || !(ends_known && (tbest == 0.f || tbest == 1.f));
-    const float H0 = hb_known ? Hbest : height_term(Fp, T, px, pz);
-    const float Hxp = height_term(Fp, T, px + h, pz);
-    const float Hxm = height_term(Fp, T, px - h, pz);
-    const float Hzp = height_term(Fp, T, px, pz + h);
-    const float Hzm = height_term(Fp, T, px, pz - h);
-    const float dx = combine(Fp, T, py - Hxp, px + h, py, pz) - combine(Fp, T, py - Hxm, px - h, py, pz);
-    const float dy = combine(Fp, T, (py + h) - H0, px, py + h, pz) - combine(Fp, T, (py - h) - H0, px, py - h, pz);
-    const float dz = combine(Fp, T, py - Hzp, px, py, pz + h) - combine(Fp, T, py - Hzm, px, py, pz - h);
-    nh += hb_known ? 4u : 5u;
-    if (caves) {
-        nc += !((py - Hxp) > Fp.cave_cut) + !((py - Hxm) > Fp.cave_cut) + !(((py + h) - H0) > Fp.cave_cut) +
-              !(((py - h) - H0) > Fp.cave_cut) + !((py - Hzp) > Fp.cave_cut) + !((py - Hzm) > Fp.cave_cut);
-    }
-    const float3 n = normalize_or_zero(dx, dy, dz);
-    return make_float4(n.x, n.y, n.z, tbest);
+    return crossing_normal(Fp, T, axis, ends_known, tbest, Hbest, p0x, p0y, p0z, p1x, p1y, p1z, nh, nc);
 }
 
 // FindEdgeIntersectionInfo of the CSG program, cl/apply_csg_operation.cl:443-477,
@@ -910,6 +921,186 @@ __global__ __attribute__((amdgpu_waves_per_eu(6, 8))) __launch_bounds__(kCrossTh
         }
         const int i = edge_of_item(a, lo, item - a.item_prefix[lo], lane / kG, kPerItem);
         if (i >= 0) cross_edge_group<kCount, kG>(a, T, lo, i, lane & (kG - 1));
+    }
+}
+
+// Sample-parallel search.  Lanes of different edges evaluate points a voxel or more
+// apart (32 voxels in a coarse chunk), so their noise-table reads hit unrelated LDS
+// addresses and conflict; lanes of one edge evaluate points a fraction of a voxel apart
+// and mostly read the same table entries.  A wave takes a 64-edge item: each lane first
+// loads its own edge (edges with brush data or a cached list are finished there by the
+// one-lane path, cross_edge); the search then runs four edges at a time, 16 lanes each
+// (lane s evaluates interior sample s + 1, lane 15 the end samples, read from the stored
+// field when it is FP32), reduced over the row to the first arg-min (smallest |d|, then
+// smallest k: the sequential scan's rule); finally each lane computes its own edge's
+// normal from the search's result (crossing_normal).  Every value comes from the same
+// expression as in density_crossing, so results are identical.  A y-edge's height term is
+// its column's, stored by the density pass.
+constexpr int kSpWaves = kCrossThreads / 64;
+
+// v from the lane kN places further round its row of 16 (DPP row_ror).
+template <int kN>
+__device__ __forceinline__ int row_ror(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, 0x120 + kN, 0xf, 0xf, false);
+}
+
+// One step of the row's arg-min: keep the smaller (|d| bits, k) pair and its height.
+template <int kN>
+__device__ __forceinline__ void argmin_step(uint32_t& d, int& k, float& H) {
+    const uint32_t od = (uint32_t)row_ror<kN>((int)d);
+    const int ok = row_ror<kN>(k);
+    const float oh = __int_as_float(row_ror<kN>(__float_as_int(H)));
+    const bool take = od < d || (od == d && ok < k);
+    d = take ? od : d;
+    k = take ? ok : k;
+    H = take ? oh : H;
+}
+
+template <bool kCount>
+__global__ __attribute__((amdgpu_waves_per_eu(6, 8))) __launch_bounds__(kCrossThreads) void k_edge_cross_sp(KArgs a,
+                                                                                                             int chunks) {
+    __shared__ NoiseLds T;
+    __shared__ int sp_k[kSpWaves][64];    // per wave: kbest of each edge of the item
+    __shared__ float sp_h[kSpWaves][64];  // and the height term there
+    load_noise_tables(T, a.perm, a.grads);
+    const int total = a.item_prefix[chunks];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane >> 4, sub = lane & 15;
+    const FieldParams& Fp = a.F;
+    const bool caves = Fp.kind == LVN_FIELD_TERRAIN_CAVES;
+    const bool ends = !a.fp16;
+    const Geometry& G = a.G;
+    unsigned nh = 0, nc = 0, ne = 0;
+    for (;;) {
+        const int item = next_item(&a.queue[1]);
+        if (item >= total) break;
+        int lo = 0, hi = chunks;  // last chunk with item_prefix[c] <= item
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (a.item_prefix[mid] <= item) lo = mid; else hi = mid;
+        }
+        const int chunk = lo;
+        const ChunkDesc& cd = a.chunks[chunk];
+        // ---- each lane loads its own edge
+        int ei = edge_of_item(a, chunk, item - a.item_prefix[chunk], lane, 64);
+        int exyz = 0;
+        float eH = 0.f, eS = 0.f, eE = 0.f;
+        if (ei >= 0) {
+            const int id = slot_ptr<int>(a, chunk, a.L.e_ids)[ei];
+            const int axis = id & 3, hidx = id >> 2, msk = (1 << G.shift) - 1;
+            const int x = hidx & msk, y = (hidx >> G.shift) & msk, z = (hidx >> (2 * G.shift)) & msk;
+            if (x >= G.H || y >= G.H || z >= G.H || axis > 2) {  // never produced by k_edge_compact
+                atomicOr(&a.counters[chunk].status, kStatusLookupMiss);
+                ei = -1;
+            } else {
+                const int i0 = field_index(G, x, y, z);
+                const int i1 = i0 + (axis == 0 ? 1 : axis == 1 ? G.F : G.F * G.F);
+                const uint8_t* mat = slot_ptr<uint8_t>(a, chunk, a.L.mat);
+                const uint8_t* orig = slot_ptr<uint8_t>(a, chunk, a.L.orig);
+                const bool touched = cd.op_count > 0 && (mat[i0] != orig[i0] || mat[i1] != orig[i1]);
+                if (touched || cd.mode == kModeCached) {
+                    cross_edge<kCount>(a, T, chunk, ei);
+                    ei = -1;
+                } else {
+                    exyz = x | (y << 8) | (z << 16) | (axis << 24);
+                    eH = axis == 1 ? slot_ptr<const float>(a, chunk, a.L.h_cols)[x + G.F * z] : 0.f;
+                    if (ends) {
+                        const float* dens = slot_ptr<const float>(a, chunk, a.L.density);
+                        eS = dens[i0];
+                        eE = dens[i1];
+                    }
+                }
+            }
+        }
+        // ---- search: four edges at a time, 16 lanes each
+        for (int r = 0; r < 16; ++r) {
+            const int src = 4 * r + row;
+            // every lane takes part in the shuffles: a lane of a row that skips the round
+            // may be the source of another row's edge
+            const int i = __shfl(ei, src);
+            const int xyz = __shfl(exyz, src);
+            const float colH = __shfl(eH, src);
+            const float dS = __shfl(eS, src), dE = __shfl(eE, src);
+            if (i < 0) continue;  // uniform over the row
+            const int x = xyz & 255, y = (xyz >> 8) & 255, z = (xyz >> 16) & 255, axis = xyz >> 24;
+            const int w0x = x * cd.scale + cd.off[0], w0y = y * cd.scale + cd.off[1], w0z = z * cd.scale + cd.off[2];
+            const float p0x = (float)w0x, p0y = (float)w0y, p0z = (float)w0z;
+            const float p1x = (float)(w0x + (axis == 0 ? cd.scale : 0));
+            const float p1y = (float)(w0y + (axis == 1 ? cd.scale : 0));
+            const float p1z = (float)(w0z + (axis == 2 ? cd.scale : 0));
+            const bool lines = a.height_lines != 0 && axis != 1;
+            // the lane's candidate: (|d| bits, k, H); none = (FLT_MAX, 17, colH)
+            uint32_t md = 0x7f7fffffu;
+            int mk = 17;
+            float mh = colH;
+            const int nk = sub < 15 ? 1 : 2;
+            for (int q = 0; q < nk; ++q) {
+                const int k = sub < 15 ? sub + 1 : 16 * q;
+                const float cur = (float)k * 0.0625f;  // = the sequential scan's running sum (exact)
+                float d, H = colH;
+                if (ends && k == 0) {
+                    d = fabsf(dS);
+                } else if (ends && k == 16) {
+                    d = fabsf(dE);
+                } else {
+                    const float px = p0x + (p1x - p0x) * cur;
+                    const float py = p0y + (p1y - p0y) * cur;
+                    const float pz = p0z + (p1z - p0z) * cur;
+                    if (axis != 1) {
+                        if (lines && k >= 1 && k <= kLineSamples) {
+                            H = slot_ptr<const float>(a, chunk, a.L.h_lines)[hline_index(G.H, axis == 2 ? 1 : 0, x, z) +
+                                                                             (k - 1)];
+                        } else {
+                            H = height_term(Fp, T, px, pz);
+                            ++nh;
+                        }
+                    }
+                    nc += (caves && !((py - H) > Fp.cave_cut)) ? 1u : 0u;
+                    d = fabsf(combine(Fp, T, py - H, px, py, pz));
+                }
+                // the scan takes a sample only when |d| < its running minimum (from FLT_MAX):
+                // lane 15 tries sample 0 before sample 16
+                if (d < __uint_as_float(md)) {
+                    md = __float_as_uint(d);
+                    mk = k;
+                    mh = H;
+                }
+            }
+            // |d| >= 0, so its bits order like its values
+            argmin_step<1>(md, mk, mh);
+            argmin_step<2>(md, mk, mh);
+            argmin_step<4>(md, mk, mh);
+            argmin_step<8>(md, mk, mh);
+            if (sub == 0) {
+                sp_k[wave][src] = mk == 17 ? 0 : mk;  // no sample taken: the scan keeps t = 0
+                sp_h[wave][src] = mh;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // ---- normal: each lane its own edge
+        if (ei >= 0) {
+            const int kb = sp_k[wave][lane];
+            const float hb = sp_h[wave][lane];
+            const int x = exyz & 255, y = (exyz >> 8) & 255, z = (exyz >> 16) & 255, axis = exyz >> 24;
+            const int w0x = x * cd.scale + cd.off[0], w0y = y * cd.scale + cd.off[1], w0z = z * cd.scale + cd.off[2];
+            const float p1x = (float)(w0x + (axis == 0 ? cd.scale : 0));
+            const float p1y = (float)(w0y + (axis == 1 ? cd.scale : 0));
+            const float p1z = (float)(w0z + (axis == 2 ? cd.scale : 0));
+            slot_ptr<float4>(a, chunk, a.L.e_data)[ei] =
+                crossing_normal(Fp, T, axis, ends, (float)kb * 0.0625f, hb, (float)w0x, (float)w0y, (float)w0z, p1x, p1y,
+                                p1z, nh, nc);
+            ++ne;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if constexpr (kCount) {
+        atomicAdd(&a.work[2], (unsigned long long)nh);
+        atomicAdd(&a.work[3], (unsigned long long)nc);
+        atomicAdd(&a.work[6], (unsigned long long)ne);
     }
 }
 
@@ -1683,12 +1874,20 @@ void launch_edge_cross(const KArgs& a, int chunks, hipStream_t s) {
             hipLaunchKernelGGL(k_height_lines<false>, line_grid, dim3(1024), 0, s, a, chunks);
     }
     const int g = cross_lanes(a, chunks);
-    hipLaunchKernelGGL(k_edge_items, dim3(1), dim3(1024), 0, s, a, chunks, 64 / g);
+    hipLaunchKernelGGL(k_edge_items, dim3(1), dim3(1024), 0, s, a, chunks, g == kCrossSampleParallel ? 64 : 64 / g);
     // the item count is on the device; size the grid for 512 edge items per chunk
     switch (g) {
     case 2: launch_cross_group<2>(a, chunks, s); return;
     case 4: launch_cross_group<4>(a, chunks, s); return;
     case 8: launch_cross_group<8>(a, chunks, s); return;
+    case kCrossSampleParallel: {
+        const dim3 grid(persistent_blocks(a, a.persist_per_cu, 512L * chunks));
+        if (a.work)
+            hipLaunchKernelGGL(k_edge_cross_sp<true>, grid, dim3(kCrossThreads), 0, s, a, chunks);
+        else
+            hipLaunchKernelGGL(k_edge_cross_sp<false>, grid, dim3(kCrossThreads), 0, s, a, chunks);
+        return;
+    }
     default: break;
     }
     const dim3 grid(persistent_blocks(a, a.persist_per_cu, 512L * chunks));

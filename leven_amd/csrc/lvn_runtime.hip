@@ -226,6 +226,7 @@ SlotLayout make_layout(const Geometry& G, const Caps& c) {
     L.e_data = take((uint64_t)c.edges * 16);
     L.e_order = take((uint64_t)c.edges * 4);
     L.h_lines = take((uint64_t)2 * G.H * G.H * kLineSamples * 4);
+    L.h_cols = take((uint64_t)G.F * G.F * 4);
     L.v_mask = take((uint64_t)G.voxel_segs * 8);
     L.v_count = take((uint64_t)G.voxel_segs * 4);
     L.v_base = take((uint64_t)G.voxel_segs * 4);
@@ -690,7 +691,7 @@ static bool valid_tuning(const Batch* b, const lvn_tuning& t) {
            t.persistent_per_cu <= 2 && t.row_blocks >= 1 && t.row_blocks <= (1 << 20) && t.voxel_blocks >= 1 &&
            t.voxel_blocks <= (1 << 20) &&
            (t.cross_lanes == 0 || t.cross_lanes == 1 || t.cross_lanes == 2 || t.cross_lanes == 4 ||
-            t.cross_lanes == 8) &&
+            t.cross_lanes == 8 || t.cross_lanes == kCrossSampleParallel) &&
            t.density_per_cu >= 0 && t.density_per_cu <= 2 && t.height_lines >= 0 && t.height_lines <= 2;
 }
 
@@ -1175,7 +1176,7 @@ int lvn_batch_autotune(lvn_batch* bh, int32_t reps, lvn_tuning* out) {
             {&lvn_tuning::density_per_cu, {1, 2}, {0}},
             {&lvn_tuning::persistent_per_cu, {1, 2}, {5}},
             {&lvn_tuning::height_lines, {1, 2}, {5}},
-            {&lvn_tuning::cross_lanes, {1, 2, 4, 8}, {5}},
+            {&lvn_tuning::cross_lanes, {1, 2, 4, 8, kCrossSampleParallel}, {5}},
             {&lvn_tuning::row_blocks, {1024, 2048, 4096, 8192, 16384, 32768}, {2, 4, 6, 8}},
             {&lvn_tuning::voxel_blocks, {1024, 2048, 4096, 8192, 16384, 32768}, {9, 11}},
         };

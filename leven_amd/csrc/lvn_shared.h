@@ -78,6 +78,7 @@ struct SlotLayout {
     uint64_t e_rows, e_count, e_base, e_ids, e_data;  // e_rows: edge_rows x {x, y, z, x = 64} words
     uint64_t e_count_y, e_base_y, e_order;  // y-edges per row and their scan; crossing order (y-edges first)
     uint64_t h_lines;                     // heights at k/16 (k = 1..15) along x- and z-edge lines, 2 x H x H x 15 f32
+    uint64_t h_cols;                      // the height term of each field column (x, z), F x F f32 (fresh fields)
     uint64_t v_mask, v_count, v_base, v_code, v_emask, v_mat;
     uint64_t q_count, q_base, tris;
     uint64_t s_mask, s_count, s_base, seams, verts;

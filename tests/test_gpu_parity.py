@@ -255,8 +255,8 @@ def test_batched_matches_oracle_and_replays(oracle):
 
 def test_launch_shapes_do_not_change_results():
     """Every launch shape (column split, persistent workgroups per CU, workgroups over the
-    batch, lanes per edge in the crossing stage, x/z-edge heights per sample or once per
-    line) and the autotuner's pick give bit-identical meshes."""
+    batch, lanes per edge in the crossing stage including the sample-parallel kernel, x/z-edge
+    heights per sample or once per line) and the autotuner's pick give bit-identical meshes."""
     w = workloads.terrain_512()
     setup_field(w["desc"], w["seed"])
     for op, bb in zip(w["ops"], w["aabbs"]):
@@ -277,6 +277,11 @@ def test_launch_shapes_do_not_change_results():
               dict(density_ysplit=2, persistent_per_cu=2, row_blocks=4096, voxel_blocks=4096, cross_lanes=1,
                    density_per_cu=2, height_lines=2),
               dict(density_ysplit=1, persistent_per_cu=2, row_blocks=4096, voxel_blocks=4096, cross_lanes=1,
+                   density_per_cu=1, height_lines=1),
+              # sample-parallel crossing, with x/z-edge heights per line and per sample
+              dict(density_ysplit=1, persistent_per_cu=2, row_blocks=4096, voxel_blocks=4096, cross_lanes=16,
+                   density_per_cu=1, height_lines=2),
+              dict(density_ysplit=6, persistent_per_cu=1, row_blocks=512, voxel_blocks=4096, cross_lanes=16,
                    density_per_cu=1, height_lines=1)]
     for shape in shapes:
         b.set_tuning(**shape)
@@ -294,6 +299,19 @@ def test_launch_shapes_do_not_change_results():
         assert all((x.view(np.uint32) == y.view(np.uint32)).all() for x, y in zip(b.get_mesh(i), ref[i]))
     with pytest.raises(RuntimeError):
         b.set_tuning(persistent_per_cu=3)
+    # FP16 storage: the end samples are evaluated (by the row's last lane in the
+    # sample-parallel kernel) instead of read from the field
+    b16 = mesher.Batch(64, len(pick), edge_cap=262144, voxel_cap=131072, flags=CTX_FP16_DENSITY)
+    b16.set_chunks(w["mins"][pick], w["sizes"][pick])
+    meshes = []
+    for lanes in (1, 16):
+        b16.set_tuning(**dict(b16.tuning(), cross_lanes=lanes))
+        b16.run()
+        b16.wait()
+        meshes.append([b16.get_mesh(i) for i in range(len(pick))])
+    for i in range(len(pick)):
+        assert all((x.view(np.uint32) == y.view(np.uint32)).all() for x, y in zip(meshes[0][i], meshes[1][i]))
+    b16.close()
     with pytest.raises(RuntimeError):
         b.set_tuning(cross_lanes=3)
     with pytest.raises(RuntimeError):
