@@ -335,6 +335,53 @@ int  lvn_server_get_mesh(lvn_server* s, int64_t ticket, lvn_vertex* v, int32_t v
  * results re-meshed because an edit arrived in flight, cached chunks, max batch seen. */
 int  lvn_server_stats(lvn_server* s, int64_t* stats);
 
+/* ---- request producer: the camera-driven clipmap (src/clipmap.cpp) ----------- */
+
+/* leven's clipmap as a request producer: which chunks to mesh as the camera moves, which
+ * meshes went stale, which chunks are empty, which seams to regenerate.  Host-side state
+ * only; the caller meshes with a context, a batch or a server.  Six LODs: chunk sizes
+ * 4 * voxels_per_chunk * 2^lod world units, LOD_ACTIVE_DISTANCES as in src/clipmap.cpp:33-41.
+ * The roots are the LOD_MAX_NODE_SIZE cells tiling [world_min, world_max) from world_min
+ * (leven: one power-of-two cube around the world bounds, src/clipmap.cpp:797-818); a node's
+ * children are made when the selection first descends into it. */
+typedef struct lvn_clipmap lvn_clipmap;
+
+#define LVN_CLIPMAP_TO_MESH   0   /* chunks the last update wants meshed, in selection order */
+#define LVN_CLIPMAP_RELEASED  1   /* nodes the last update released: their meshes and octrees are stale */
+#define LVN_CLIPMAP_SELECTED  2   /* every node the last update selected */
+#define LVN_CLIPMAP_ACTIVE    3   /* every node holding a mesh or seam nodes now */
+#define LVN_CLIPMAP_TOUCHED   4   /* nodes the last edit touched */
+
+/* Replaces Clipmap::initialise / constructTree (src/clipmap.cpp:797-852); world units. */
+int  lvn_clipmap_create(int32_t voxels_per_chunk, const int32_t world_min[3], const int32_t world_max[3],
+                        lvn_clipmap** out);
+void lvn_clipmap_destroy(lvn_clipmap* c);
+/* Replaces the selection half of Clipmap::update (src/clipmap.cpp:1176-1252):
+ * SelectActiveClipmapNodes (:1084-1126) around `camera` (world units), ReleaseInvalidatedNodes
+ * (:1130-1151) and the frustum filter (AABBInsideFrustum, src/frustum.cpp:30-52) with six
+ * planes (a, b, c, d), a box being outside when all 8 corners give a x + b y + c z + d < 0;
+ * frustum == NULL keeps every node.  counts = {chunks to mesh, nodes released, nodes selected};
+ * lvn_clipmap_list reads the lists until the next update. */
+int lvn_clipmap_update(lvn_clipmap* c, const float camera[3], const float* frustum, int32_t counts[3]);
+/* Copies list `which` (LVN_CLIPMAP_*) as chunk requests; LVN_ERR_CAPACITY with *n set when it
+ * does not fit. */
+int lvn_clipmap_list(lvn_clipmap* c, int32_t which, lvn_chunk_req* out, int32_t cap, int32_t* n);
+/* Replaces the rest of Clipmap::update (src/clipmap.cpp:1254-1342) once the last update's
+ * chunks are meshed: results[3 i .. 3 i + 2] = status, triangles, seam nodes of chunk i of the
+ * to-mesh list.  A chunk with triangles or seam nodes becomes active; one with neither, or
+ * whose meshing failed, becomes empty with everything below it (:1280-1287).  Then the seams
+ * to regenerate (:1306-1324), each host with its sources gathered as GenerateClipmapSeamMesh
+ * does (:579-597), count = -1 sources: the lists where the chunks' meshes lie, ready for
+ * lvn_batch_seam_meshes / lvn_generate_seam_meshes.  counts = {seam hosts, sources}. */
+int lvn_clipmap_commit(lvn_clipmap* c, int32_t n, const int32_t* results, int32_t counts[2]);
+int lvn_clipmap_seams(lvn_clipmap* c, lvn_seam_req* reqs, int32_t req_cap, lvn_seam_source* sources,
+                      int32_t src_cap);
+/* Replaces the node half of Clipmap::processCSGOperationsImpl (src/clipmap.cpp:1655-1705):
+ * every node up to LOD_MAX_NODE_SIZE that the op's AABB (world units) overlaps is invalidated
+ * (released and re-meshed by the next update) and loses its empty mark; *n = nodes touched
+ * (list LVN_CLIPMAP_TOUCHED). */
+int lvn_clipmap_edit(lvn_clipmap* c, const int32_t aabb_min[3], const int32_t aabb_max[3], int32_t* n);
+
 /* ---- allocation-free single-chunk form ------------------------------------- */
 
 /* No counterpart in the reference.  For callers that cannot allocate device memory:

@@ -36,6 +36,9 @@ FIELD_LEVEN_TERRAIN = 2
 CTX_FP16_DENSITY = 1
 CTX_PRECISE_QEF = 2
 
+# lists of the clipmap request producer (lvn_clipmap_list)
+CLIPMAP_TO_MESH, CLIPMAP_RELEASED, CLIPMAP_SELECTED, CLIPMAP_ACTIVE, CLIPMAP_TOUCHED = 0, 1, 2, 3, 4
+
 # limits of the allocation-free single-chunk form (lvn_generate_chunk_mesh_noalloc)
 NOALLOC_MAX_EDGES = 131072
 NOALLOC_MAX_VOXELS = 65536
@@ -241,7 +244,9 @@ def ptr(arr, ctype):
 
 
 def vptr(arr):
+    """void pointer to a contiguous numpy array (or None).  The pointer object keeps the array
+    alive, so a temporary array passed straight into a call is not freed before the call."""
     if arr is None:
         return None
     assert arr.flags["C_CONTIGUOUS"]
-    return C.c_void_p(arr.ctypes.data)
+    return arr.ctypes.data_as(C.c_void_p)

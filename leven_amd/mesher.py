@@ -10,8 +10,8 @@ import os
 import numpy as np
 
 from .abi import (CSG_OP_DTYPE, ERROR_NAMES, LVN_ERR_CAPACITY, LVN_OK, REQ_EDIT, REQ_MESH, RESULT_DTYPE, SEAM_DTYPE,
-                  VERTEX_DTYPE, CsgOp, FieldDesc, Request, SeamOut, SeamReq, StageDump, Tuning, ptr, seam_sources,
-                  vptr)
+                  VERTEX_DTYPE, CsgOp, FieldDesc, Request, SeamOut, SeamReq, SeamSource, StageDump, Tuning, ptr,
+                  seam_sources, vptr)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # LVN_LIB_PATH selects another build of the library (e.g. to compare two builds)
@@ -29,6 +29,8 @@ EXPORTS = [
     "lvn_server_get_mesh", "lvn_server_stats", "lvn_generate_seam_mesh", "lvn_generate_seam_meshes",
     "lvn_seam_device_mesh", "lvn_batch_seam_meshes", "lvn_batch_seam_device_mesh",
     "lvn_generate_chunk_mesh_noalloc",
+    "lvn_clipmap_create", "lvn_clipmap_destroy", "lvn_clipmap_update", "lvn_clipmap_list", "lvn_clipmap_commit",
+    "lvn_clipmap_seams", "lvn_clipmap_edit",
 ]
 
 _lib = None
@@ -67,6 +69,13 @@ def lib():
             "lvn_generate_chunk_mesh_noalloc": (i32, [i32, u32, vp, i32, vp, i32, C.POINTER(i32), vp, i32,
                                                       C.POINTER(i32), vp, i32, C.POINTER(i32)]),
             "lvn_debug_chunk_stages": (i32, [vp, vp, i32, vp, i32, C.POINTER(StageDump)]),
+            "lvn_clipmap_create": (i32, [i32, vp, vp, C.POINTER(vp)]),
+            "lvn_clipmap_destroy": (None, [vp]),
+            "lvn_clipmap_update": (i32, [vp, vp, vp, vp]),
+            "lvn_clipmap_list": (i32, [vp, i32, vp, i32, C.POINTER(i32)]),
+            "lvn_clipmap_commit": (i32, [vp, i32, vp, vp]),
+            "lvn_clipmap_seams": (i32, [vp, vp, i32, vp, i32]),
+            "lvn_clipmap_edit": (i32, [vp, vp, vp, C.POINTER(i32)]),
             "lvn_batch_create": (i32, [i32, i32, i32, i32, u32, C.POINTER(vp)]),
             "lvn_batch_destroy": (None, [vp]),
             "lvn_batch_set_chunks": (i32, [vp, i32, vp]),
@@ -340,6 +349,64 @@ def chunk_reqs(mins, sizes):
     mins = np.ascontiguousarray(mins, dtype=np.int32).reshape(-1, 3)
     sizes = np.ascontiguousarray(sizes, dtype=np.int32).reshape(-1, 1)
     return np.ascontiguousarray(np.hstack([mins, sizes]), dtype=np.int32)
+
+
+class Clipmap:
+    """The clipmap request producer (lvn_clipmap_*): leven's Clipmap::update as chunk and seam
+    requests.  Host-side only; runs without a GPU."""
+
+    def __init__(self, world_min, world_max, voxels_per_chunk=64):
+        h = C.c_void_p()
+        check(lib().lvn_clipmap_create(int(voxels_per_chunk), vptr(_i3(world_min)), vptr(_i3(world_max)),
+                                       C.byref(h)), "lvn_clipmap_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lvn_clipmap_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def update(self, camera, frustum=None):
+        """Returns {to_mesh, released, selected} as (mins int32[n,3], sizes int32[n]) pairs."""
+        cam = np.ascontiguousarray(camera, dtype=np.float32).reshape(3)
+        fr = None if frustum is None else np.ascontiguousarray(frustum, dtype=np.float32).reshape(24)
+        counts = np.zeros(3, dtype=np.int32)
+        check(lib().lvn_clipmap_update(self._h, vptr(cam), vptr(fr) if fr is not None else None, vptr(counts)),
+              "lvn_clipmap_update")
+        return {name: self.list(k) for k, name in enumerate(("to_mesh", "released", "selected"))}
+
+    def list(self, which):
+        n = C.c_int32()
+        lib().lvn_clipmap_list(self._h, int(which), None, 0, C.byref(n))
+        out = np.zeros((max(n.value, 1), 4), dtype=np.int32)
+        check(lib().lvn_clipmap_list(self._h, int(which), vptr(out), len(out), C.byref(n)), "lvn_clipmap_list")
+        out = out[:n.value]
+        return out[:, :3].copy(), out[:, 3].copy()
+
+    def commit(self, results):
+        """results: int32[n, 3] = (status, triangles, seam nodes) per chunk of the last update's
+        to-mesh list.  Returns (SeamReq array, SeamSource array) of the seams to regenerate."""
+        r = np.ascontiguousarray(results, dtype=np.int32).reshape(-1, 3)
+        counts = np.zeros(2, dtype=np.int32)
+        check(lib().lvn_clipmap_commit(self._h, len(r), vptr(r) if len(r) else None, vptr(counts)),
+              "lvn_clipmap_commit")
+        reqs = (SeamReq * max(1, int(counts[0])))()
+        srcs = (SeamSource * max(1, int(counts[1])))()
+        check(lib().lvn_clipmap_seams(self._h, C.cast(reqs, C.c_void_p), len(reqs), C.cast(srcs, C.c_void_p),
+                                      len(srcs)), "lvn_clipmap_seams")
+        return reqs, srcs, int(counts[0]), int(counts[1])
+
+    def edit(self, aabb_min, aabb_max):
+        n = C.c_int32()
+        check(lib().lvn_clipmap_edit(self._h, vptr(_i3(aabb_min)), vptr(_i3(aabb_max)), C.byref(n)),
+              "lvn_clipmap_edit")
+        return self.list(4)
 
 
 class Batch:

@@ -141,10 +141,12 @@ def clipmap(seed=4):
     cam = np.array([rng.uniform(-1024, 1024), rng.uniform(-64, 64), rng.uniform(-1024, 1024)])
     R = LOD_MAX_NODE_SIZE
     # 28 x 2 x 28 root nodes: the rule keeps ~330 nodes of LOD 0-4 near the camera, the
-    # rest of the ~1900 selected chunks are LOD-5 tiles (voxel size 32).
-    sel = clipmap_select(cam, (-14 * R, -R, -14 * R), (14 * R, R, 14 * R))
-    mins = np.array([s[0] for s in sel], np.int32)
-    sizes = np.array([s[1] for s in sel], np.int32)
+    # rest of the ~1900 selected chunks are LOD-5 tiles (voxel size 32).  The selection is
+    # the library's request producer (lvn_clipmap_update); clipmap_select above restates it.
+    from . import mesher
+    cm = mesher.Clipmap((-14 * R, -R, -14 * R), (14 * R, R, 14 * R))
+    mins, sizes = cm.update(cam)["selected"]
+    cm.close()
     ops, aabbs = make_ops(256, seed=2)
     return dict(name="clipmap", desc=terrain_desc(), seed=2, default_material=0, mins=mins, sizes=sizes,
                 ops=ops, aabbs=aabbs, flags=0, camera=cam)
@@ -170,9 +172,13 @@ def serving(seed=5, duration=2.0, rate=4000.0, edit_fraction=0.05, speed_voxels=
     def selection(t):
         k = int(t / step)
         if k not in selections:
+            from . import mesher
             cam = cam0 + vel * (k * step)
             c = (int(cam[0]) // R) * R, (int(cam[2]) // R) * R
-            selections[k] = clipmap_select(cam, (c[0] - 2 * R, -R, c[1] - 2 * R), (c[0] + 2 * R, R, c[1] + 2 * R))
+            cm = mesher.Clipmap((c[0] - 2 * R, -R, c[1] - 2 * R), (c[0] + 2 * R, R, c[1] + 2 * R))
+            mins, sizes = cm.update(cam)["selected"]
+            cm.close()
+            selections[k] = [(tuple(int(v) for v in m), int(sz)) for m, sz in zip(mins, sizes)]
         return selections[k]
 
     events = []

@@ -1,0 +1,184 @@
+"""The clipmap request producer (lvn_clipmap_*, leven's Clipmap::update, src/clipmap.cpp)
+through the C ABI.  Host code only: these tests need the built library but no GPU.
+
+Pinned against the Python restatement of the selection rule (workloads.clipmap_select)
+and of GenerateClipmapSeamMesh's source gathering (workloads.seam_sources), and by the
+update cycle's invariants: a second update with the camera still meshes nothing, a
+moved camera releases exactly the active nodes that are no longer selected, an empty
+chunk is never meshed again (nor is anything below it), an edit re-meshes the selected
+nodes it touches, and the frustum filter falls back to the nodes outside it."""
+import os
+
+import numpy as np
+import pytest
+
+from leven_amd import abi, mesher, workloads
+from leven_amd.abi import SEAM_DTYPE
+
+if not os.path.exists(mesher.LIB_PATH):
+    pytest.skip("library not built (python -m leven_amd.build)", allow_module_level=True)
+
+R = workloads.LOD_MAX_NODE_SIZE
+C0 = workloads.CHUNK_SIZE
+
+
+def _pairs(mins, sizes):
+    return [(tuple(int(v) for v in m), int(s)) for m, s in zip(mins, sizes)]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_selection_matches_restatement(seed):
+    rng = np.random.default_rng(100 + seed)
+    cam = np.array([rng.uniform(-2 * R, 2 * R), rng.uniform(-R / 2, R / 2), rng.uniform(-2 * R, 2 * R)])
+    wmin, wmax = (-2 * R, -R, -2 * R), (2 * R, R, 2 * R)
+    cm = mesher.Clipmap(wmin, wmax)
+    u = cm.update(cam)
+    sel = _pairs(*u["selected"])
+    assert sel == workloads.clipmap_select(cam, wmin, wmax)  # same nodes, same order
+    assert _pairs(*u["to_mesh"]) == sel and len(u["released"][1]) == 0  # nothing meshed yet
+    cm.close()
+
+
+def _commit_all(cm, to_mesh, seams_of):
+    """Pretend every chunk meshed: (status, triangles, seam nodes) from seams_of(min, size)."""
+    res = np.zeros((len(to_mesh), 3), np.int32)
+    for k, (m, s) in enumerate(to_mesh):
+        ns = seams_of(m, s)
+        res[k] = (0, 2 * ns, ns)
+    return cm.commit(res)
+
+
+def _seams_of(m, s):
+    # a chunk holds a surface when it straddles y = 0; its seam node count is a function of it
+    return 0 if not (m[1] <= 0 < m[1] + s) else 1 + (abs(m[0]) // C0 + abs(m[2]) // C0) % 7
+
+
+def test_update_cycle_and_releases():
+    wmin, wmax = (-2 * R, -R, -2 * R), (2 * R, R, 2 * R)
+    cm = mesher.Clipmap(wmin, wmax)
+    cam = np.array([100.0, 20.0, -300.0])
+    u = cm.update(cam)
+    first = _pairs(*u["to_mesh"])
+    _commit_all(cm, first, _seams_of)
+    active = set(_pairs(*cm.list(abi.CLIPMAP_ACTIVE)))
+    assert active == {p for p in first if _seams_of(*p) > 0}
+    # the same camera: nothing to mesh, nothing released
+    u2 = cm.update(cam)
+    assert len(u2["to_mesh"][1]) == 0 and len(u2["released"][1]) == 0
+    # a moved camera: exactly the active nodes no longer selected are released; the newly
+    # selected nodes are meshed unless they lie in (or under) a chunk found empty
+    cam2 = cam + np.array([3000.0, 0.0, 2500.0])
+    u3 = cm.update(cam2)
+    sel3 = set(_pairs(*u3["selected"]))
+    assert set(_pairs(*u3["released"])) == active - sel3
+    empty = {p for p in first if _seams_of(*p) == 0}
+
+    def under_empty(p):
+        (m, s) = p
+        return any(all(e[0][a] <= m[a] and m[a] + s <= e[0][a] + e[1] for a in range(3)) for e in empty)
+
+    expect = [p for p in _pairs(*u3["selected"]) if p not in (active & sel3) and not under_empty(p)]
+    assert _pairs(*u3["to_mesh"]) == expect
+    cm.close()
+
+
+def test_seam_sources_match_restatement():
+    wmin, wmax = (-R, -R, -R), (R, R, R)
+    cm = mesher.Clipmap(wmin, wmax)
+    u = cm.update(np.array([-700.0, 35.0, 420.0]))
+    to_mesh = _pairs(*u["to_mesh"])
+    reqs, srcs, nreq, nsrc = _commit_all(cm, to_mesh, _seams_of)
+    # every active node's neighbourhood includes itself, so every active node is a host
+    nodes = [(m, s, np.zeros(_seams_of(m, s), SEAM_DTYPE)) for (m, s) in to_mesh if _seams_of(m, s) > 0]
+    assert nreq == len(nodes)
+    expect = {(tuple(m), s): [(tuple(d["min"]), d["size"], d["neighbour"]) for d in ss]
+              for (m, s, _), ss in zip(nodes, workloads.seam_sources(nodes))}
+    got = {}
+    for i in range(nreq):
+        q = reqs[i]
+        got[(tuple(q.host_min), q.host_size)] = [
+            (tuple(srcs[j].min), srcs[j].size, srcs[j].neighbour) for j in range(q.first_source,
+                                                                                 q.first_source + q.n_sources)]
+        assert all(srcs[j].count == -1 and not srcs[j].nodes
+                   for j in range(q.first_source, q.first_source + q.n_sources))
+    assert got == expect
+    cm.close()
+
+
+def test_seam_hosts_of_a_new_chunk():
+    """A chunk meshed later regenerates the seams of the active nodes under its eight
+    negative-side neighbour regions, itself included (src/clipmap.cpp:1306-1324)."""
+    cm = mesher.Clipmap((-R, -R, -R), (R, R, R))
+    cam = np.array([10.0, 10.0, 10.0])
+    u = cm.update(cam)
+    to_mesh = _pairs(*u["to_mesh"])
+    # mesh every chunk but one LOD0 chunk straddling y = 0; then report that one
+    target = next(p for p in to_mesh if p[1] == C0 and _seams_of(*p) > 0 and p[0][0] > 0 and p[0][2] > 0)
+    res = np.array([(0, 2 * _seams_of(*p), _seams_of(*p)) if p != target else (-6, 0, 0) for p in to_mesh],
+                   np.int32)
+    cm.commit(res)  # the failed chunk becomes empty (ConstructClipmapNodeData leaves it without data)
+    assert target not in set(_pairs(*cm.list(abi.CLIPMAP_ACTIVE)))
+    cm.edit(np.array(target[0]) + 8, np.array(target[0]) + 16)  # touch it: no longer empty
+    u2 = cm.update(cam)
+    assert target in _pairs(*u2["to_mesh"])
+    m2 = _pairs(*u2["to_mesh"])
+    reqs, srcs, nreq, _ = cm.commit(np.array([(0, 2, 3)] * len(m2), np.int32))
+    hosts = {(tuple(reqs[i].host_min), reqs[i].host_size) for i in range(nreq)}
+    active = set(_pairs(*cm.list(abi.CLIPMAP_ACTIVE)))
+    expect = set()
+    for (m, s) in m2:
+        for i in range(8):
+            off = workloads.CHILD_MIN_OFFSETS[i]
+            nmin = tuple(m[a] - off[a] * s for a in range(3))
+            for (am, asz) in active:
+                if all(am[a] <= nmin[a] < am[a] + asz for a in range(3)) or \
+                        all(nmin[a] <= am[a] < nmin[a] + s for a in range(3)):
+                    expect.add((am, asz))
+    assert hosts == expect and target in hosts
+    cm.close()
+
+
+def test_edit_remeshes_touched_selected_nodes():
+    cm = mesher.Clipmap((-R, -R, -R), (R, R, R))
+    cam = np.array([50.0, 5.0, 50.0])
+    u = cm.update(cam)
+    _commit_all(cm, _pairs(*u["to_mesh"]), _seams_of)
+    active = set(_pairs(*cm.list(abi.CLIPMAP_ACTIVE)))
+    lo, hi = np.array([100, -30, 100]), np.array([140, 10, 150])
+    touched = set(_pairs(*cm.edit(lo, hi)))
+    assert touched and all(s <= R for _, s in touched)
+    for (m, s) in touched:  # every touched node overlaps the (closed) box
+        assert all(m[a] <= hi[a] and m[a] + s >= lo[a] for a in range(3))
+    u2 = cm.update(cam)
+    assert set(_pairs(*u2["released"])) == touched  # invalidated nodes are released
+    assert set(_pairs(*u2["to_mesh"])) == touched & set(_pairs(*u2["selected"]))
+    assert touched & active  # the edit touched meshed chunks
+    cm.close()
+
+
+def test_frustum_filter_and_fallback():
+    cm = mesher.Clipmap((-R, -R, -R), (R, R, R))
+    cam = np.array([0.0, 0.0, 0.0])
+    # keep x >= 0 only (a plane x >= 0; the other five accept everything)
+    planes = np.array([[1, 0, 0, 0]] + [[0, 0, 0, 1]] * 5, np.float32)
+    u = cm.update(cam, planes)
+    sel = _pairs(*u["selected"])
+    inside = [p for p in sel if p[0][0] + p[1] >= 0]  # some corner at x >= 0
+    assert _pairs(*u["to_mesh"]) == inside and len(inside) < len(sel)
+    cm.commit(np.zeros((len(inside), 3), np.int32))  # all empty
+    # nothing left inside the frustum: the update meshes the nodes outside it
+    u2 = cm.update(cam, planes)
+    assert _pairs(*u2["to_mesh"]) == [p for p in sel if p not in set(inside)]
+    cm.close()
+
+
+def test_rejects_bad_arguments():
+    with pytest.raises(mesher.LvnError):
+        mesher.Clipmap((0, 0, 0), (0, 10, 10))
+    with pytest.raises(mesher.LvnError):
+        mesher.Clipmap((0, 0, 0), (10, 10, 10), voxels_per_chunk=48)
+    cm = mesher.Clipmap((-R, -R, -R), (R, R, R))
+    cm.update(np.zeros(3))
+    with pytest.raises(mesher.LvnError):  # results must cover the to-mesh list
+        cm.commit(np.zeros((1, 3), np.int32))
+    cm.close()
