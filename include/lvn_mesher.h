@@ -335,6 +335,50 @@ int  lvn_server_get_mesh(lvn_server* s, int64_t ticket, lvn_vertex* v, int32_t v
  * results re-meshed because an edit arrived in flight, cached chunks, max batch seen. */
 int  lvn_server_stats(lvn_server* s, int64_t* stats);
 
+/* ---- post-mesh simplification (ngMeshSimplifier, src/ng_mesh_simplify.cpp) ---- */
+
+/* = MeshSimplificationOptions (src/ng_mesh_simplify.h:6-31), plus the generator seed (the
+ * reference seeds with 42, src/ng_mesh_simplify.cpp:195-196).  The result is what the
+ * reference computes when built with libstdc++, with one change of arithmetic.  Its random
+ * candidate edges: a std::mt19937 seeded anew in every iteration and
+ * std::uniform_int_distribution<int>(0, E - 1), whose mapping the C++ library chooses;
+ * libstdc++ (GCC 11 and later) takes Lemire's method (the high word of u * E, drawing again
+ * while the low word is below 2^32 mod E), and so does this library; MSVC's, which leven
+ * was built with, differs.  The change: the SSE QEF's Givens cosine (src/qef_simd.h:206)
+ * takes _mm_rsqrt_ps, an approximation whose bits depend on the processor; here it is the
+ * IEEE 1 / sqrt.  The rest of that solver is computed lane for lane as written, including
+ * rotateq_xy's slips (:228-274: row 0's diagonal stored twice, y formed from cc * u and
+ * cc + v and stored into the off-diagonal entries, the third diagonal never rotated) and the
+ * pseudo-inverse's product with V where V^T is meant (:371-382). */
+typedef struct lvn_simplify_options {
+    float    edge_fraction;      /* 0.125: share of the candidate edges drawn per iteration */
+    int32_t  max_iterations;     /* 10 */
+    float    target_percentage;  /* 0.05: stop at this share of the input triangles */
+    float    max_error;          /* collapse cost limit (cost = 1 / QEF error + degree penalty) */
+    float    max_edge_size;      /* world units */
+    float    min_angle_cosine;   /* of the two endpoint normals */
+    uint32_t seed;               /* 42 */
+    int32_t  reserved;
+} lvn_simplify_options;
+
+/* Replaces ngMeshSimplifier (src/ng_mesh_simplify.cpp:438-539) on one host mesh, in place:
+ * vertices[0 .. *num_vertices), triangles[0 .. 3 * *num_triangles) (indices), positions
+ * taken relative to offset (leven passes the chunk centre, w = 0) while simplifying.  A mesh
+ * under 100 vertices or 100 triangles is left as it is.  stats (NULL or 2 ints) =
+ * iterations run, edges collapsed.  Runs on the context's stream and waits. */
+int lvn_simplify_mesh(lvn_ctx* ctx, lvn_vertex* vertices, int32_t* num_vertices, int32_t* triangles,
+                      int32_t* num_triangles, const float offset[4], const lvn_simplify_options* options,
+                      int32_t* stats);
+/* Simplifies every staged chunk's mesh in its slot, as ConstructClipmapNodeData does
+ * (src/clipmap.cpp:453-464): offset = the chunk centre min + size / 2, and max_error and
+ * max_edge_size given per voxel (leven's meshMaxError and meshMaxEdgeLen, src/options.h)
+ * scaled by the chunk's voxel size LEAF_SIZE_SCALE * (size / (LEAF_SIZE_SCALE * N)).  One
+ * workgroup per chunk, every chunk in one launch.  lvn_batch_get_mesh then returns the
+ * simplified meshes until the pipeline runs again; a second call simplifies them again.
+ * results (NULL or 4 ints per chunk) = vertices, triangles, iterations run, edges collapsed
+ * (iterations = LVN_ERR_OVERFLOW for a chunk that overflowed, which is left alone). */
+int lvn_batch_simplify(lvn_batch* b, const lvn_simplify_options* options, int32_t* results);
+
 /* ---- request producer: the camera-driven clipmap (src/clipmap.cpp) ----------- */
 
 /* leven's clipmap as a request producer: which chunks to mesh as the camera moves, which

@@ -172,6 +172,41 @@ class SeamReq(C.Structure):
     ]
 
 
+class SimplifyOptions(C.Structure):
+    _fields_ = [
+        ("edge_fraction", C.c_float),
+        ("max_iterations", C.c_int32),
+        ("target_percentage", C.c_float),
+        ("max_error", C.c_float),
+        ("max_edge_size", C.c_float),
+        ("min_angle_cosine", C.c_float),
+        ("seed", C.c_uint32),
+        ("reserved", C.c_int32),
+    ]
+
+
+def simplify_options_per_voxel(max_error=5.0, max_edge_len=2.5, min_cos_angle=0.7, **kw):
+    """Options for lvn_batch_simplify: leven's meshMaxError, meshMaxEdgeLen and meshMinCosAngle
+    (src/options.h:14-16), which the library scales per chunk by its voxel size."""
+    o = SimplifyOptions(edge_fraction=0.125, max_iterations=10, target_percentage=0.05, max_error=max_error,
+                        max_edge_size=max_edge_len, min_angle_cosine=min_cos_angle, seed=42, reserved=0)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def simplify_options(chunk_size, max_error=5.0, max_edge_len=2.5, min_cos_angle=0.7, **kw):
+    """MeshSimplificationOptions as leven sets them for a chunk of `chunk_size` world units
+    (src/clipmap.cpp:453-462): error and edge limits scale with the voxel size
+    4 * chunk_size / 256 (options.h defaults 5, 2.5, 0.7)."""
+    leaf = 4.0 * (chunk_size // 256)
+    o = SimplifyOptions(edge_fraction=0.125, max_iterations=10, target_percentage=0.05, max_error=max_error * leaf,
+                        max_edge_size=max_edge_len * leaf, min_angle_cosine=min_cos_angle, seed=42, reserved=0)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
 class SeamOut(C.Structure):
     _fields_ = [
         ("v", C.c_void_p),
@@ -188,6 +223,7 @@ class SeamOut(C.Structure):
 assert C.sizeof(Request) == 96
 assert C.sizeof(Result) == 48
 assert C.sizeof(SeamSource) == 32 and C.sizeof(SeamReq) == 40 and C.sizeof(SeamOut) == 40
+assert C.sizeof(SimplifyOptions) == 32
 
 
 def seam_sources(sources):

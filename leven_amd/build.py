@@ -13,7 +13,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblvn_mesher.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["lvn_kernels.hip", "lvn_runtime.hip", "lvn_server.hip", "lvn_seams.hip", "lvn_clipmap.cpp"]
+SOURCES = ["lvn_kernels.hip", "lvn_runtime.hip", "lvn_server.hip", "lvn_seams.hip", "lvn_clipmap.cpp",
+           "lvn_simplify.hip"]
 
 # -ffp-contract=off: the kernels follow the reference's operation order with fused
 # multiply-adds only where written as fmaf(), as the IEEE FP32 restatement does,

@@ -31,6 +31,9 @@ namespace lvn {
 // Seam-stitching buffers of a context (lvn_seams.hip).
 struct SeamState;
 void destroy_seam_state(SeamState* s);
+// Simplifier buffers of a context (lvn_simplify.hip).
+struct SimpState;
+void destroy_simp_state(SimpState* s);
 
 // Global state set by lvn_init / lvn_set_noise_seed / lvn_store_csg_op
 // (the reference keeps the same as process globals: src/compute.cpp:168-191,
@@ -136,6 +139,7 @@ struct Ctx {
     uint64_t graph_gen = 0;              // global generation the graph was captured with
     float* h_verts = nullptr;   // pinned read-back staging for one chunk's mesh (caps sized)
     SeamState* seams = nullptr; // seam stitching buffers, made on first use
+    SimpState* simp = nullptr;  // simplifier buffers, made on first use
 
     int allocate(int n, uint32_t flags, int nslots, int edge_cap, int voxel_cap);
     int ensure_ops(int nops, int nidx);
@@ -156,6 +160,9 @@ struct Batch {
     unsigned long long* d_work = nullptr;
     lvn_tuning tuning = {0, 2, 8192, 8192, 0, 1, 0, {0}};  // ysplit, cross_lanes, height_lines 0 = by batch size
     std::map<Key, int> slot_of;      // staged chunk (min, size) -> slot
+    // lvn_batch_simplify's per-slot (vertices, triangles), valid until the pipeline runs again
+    std::vector<int32_t> simp_counts;
+    bool simplified = false;
     ~Batch();
 };
 

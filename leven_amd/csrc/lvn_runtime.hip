@@ -369,6 +369,7 @@ Ctx::~Ctx() {
     if (graph) (void)hipGraphDestroy(graph);
     if (h_verts) (void)hipHostFree(h_verts);
     if (seams) destroy_seam_state(seams);
+    if (simp) destroy_simp_state(simp);
     if (d_arena) (void)hipFree(d_arena);
     if (d_chunks) (void)hipFree(d_chunks);
     if (d_counters) (void)hipFree(d_counters);
@@ -638,6 +639,7 @@ static int stage_batch(Batch* b, int n, const lvn_chunk_req* reqs) {
     LVN_HIP(hipMemcpyAsync(c->d_chunks, c->h_chunks, sizeof(ChunkDesc) * n, hipMemcpyHostToDevice, c->stream));
     LVN_HIP(hipStreamSynchronize(c->stream));
     b->n = n;
+    b->simplified = false;
     b->slot_of.clear();
     for (int i = 0; i < n; ++i) b->slot_of[make_key(reqs[i].min, reqs[i].size)] = i;
     return LVN_OK;
@@ -671,6 +673,7 @@ static KArgs batch_args(const Batch* b) { return batch_args(b, effective_tuning(
 // One pass of the pipeline without the graph, each stage bracketed by HIP events.
 static int time_stages(Batch* b, const KArgs& a, float* stage_ms) {
     Ctx* c = &b->ctx;
+    b->simplified = false;
     hipEvent_t ev[kNumStages + 1];
     for (auto& e : ev) LVN_HIP(hipEventCreate(&e));
     launch_clear_counters(a, std::max(b->n, 1), c->stream);
@@ -1123,6 +1126,7 @@ int lvn_batch_run(lvn_batch* bh) {
         b->graph_index = c->d_op_index;
         b->graph_gen = g_state.generation;
     }
+    b->simplified = false;
     LVN_HIP(hipGraphLaunch(b->exec, c->stream));
     return LVN_OK;
 }
@@ -1221,6 +1225,7 @@ int lvn_batch_work(lvn_batch* bh, uint64_t* work) {
     KArgs a = batch_args(b);
     a.work = b->d_work;
     LVN_HIP(hipMemsetAsync(b->d_work, 0, sizeof(unsigned long long) * 8, c->stream));
+    b->simplified = false;
     if (b->n) launch_pipeline(a, b->n, c->stream);
     unsigned long long h[8];
     LVN_HIP(hipMemcpyAsync(h, b->d_work, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -1258,8 +1263,9 @@ int lvn_batch_get_mesh(lvn_batch* bh, int32_t i, lvn_vertex* v, int32_t v_cap, i
     LVN_HIP(hipStreamSynchronize(c->stream));
     k = c->h_counters[i];
     if (k.status) return LVN_ERR_OVERFLOW;
-    const int V = k.edges > 0 && k.quads > 0 ? k.voxels : 0;
-    const int T = k.edges > 0 ? 2 * k.quads : 0;
+    const bool simp = b->simplified && (size_t)(2 * i + 1) < b->simp_counts.size();
+    const int V = simp ? b->simp_counts[2 * i] : k.edges > 0 && k.quads > 0 ? k.voxels : 0;
+    const int T = simp ? b->simp_counts[2 * i + 1] : k.edges > 0 ? 2 * k.quads : 0;
     const int S = k.edges > 0 ? k.seams : 0;
     *nv = V;
     *nt = T;

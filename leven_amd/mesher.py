@@ -10,8 +10,8 @@ import os
 import numpy as np
 
 from .abi import (CSG_OP_DTYPE, ERROR_NAMES, LVN_ERR_CAPACITY, LVN_OK, REQ_EDIT, REQ_MESH, RESULT_DTYPE, SEAM_DTYPE,
-                  VERTEX_DTYPE, CsgOp, FieldDesc, Request, SeamOut, SeamReq, SeamSource, StageDump, Tuning, ptr,
-                  seam_sources, vptr)
+                  VERTEX_DTYPE, CsgOp, FieldDesc, Request, SeamOut, SeamReq, SeamSource, SimplifyOptions, StageDump,
+                  Tuning, ptr, seam_sources, vptr)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # LVN_LIB_PATH selects another build of the library (e.g. to compare two builds)
@@ -30,7 +30,7 @@ EXPORTS = [
     "lvn_seam_device_mesh", "lvn_batch_seam_meshes", "lvn_batch_seam_device_mesh",
     "lvn_generate_chunk_mesh_noalloc",
     "lvn_clipmap_create", "lvn_clipmap_destroy", "lvn_clipmap_update", "lvn_clipmap_list", "lvn_clipmap_commit",
-    "lvn_clipmap_seams", "lvn_clipmap_edit",
+    "lvn_clipmap_seams", "lvn_clipmap_edit", "lvn_simplify_mesh", "lvn_batch_simplify",
 ]
 
 _lib = None
@@ -76,6 +76,9 @@ def lib():
             "lvn_clipmap_commit": (i32, [vp, i32, vp, vp]),
             "lvn_clipmap_seams": (i32, [vp, vp, i32, vp, i32]),
             "lvn_clipmap_edit": (i32, [vp, vp, vp, C.POINTER(i32)]),
+            "lvn_simplify_mesh": (i32, [vp, vp, C.POINTER(i32), vp, C.POINTER(i32), vp, C.POINTER(SimplifyOptions),
+                                        vp]),
+            "lvn_batch_simplify": (i32, [vp, C.POINTER(SimplifyOptions), vp]),
             "lvn_batch_create": (i32, [i32, i32, i32, i32, u32, C.POINTER(vp)]),
             "lvn_batch_destroy": (None, [vp]),
             "lvn_batch_set_chunks": (i32, [vp, i32, vp]),
@@ -286,6 +289,18 @@ class Context:
         context's cached chunk at (min, size).  Returns (vertices, triangles int32[T, 3])."""
         return self.seam_meshes([(host_min, host_size, colour)], [sources])[0]
 
+    def simplify_mesh(self, vertices, triangles, offset, options):
+        """ngMeshSimplifier on one mesh (lvn_simplify_mesh).  Returns (vertices, triangles int32[T, 3],
+        (iterations, collapses)); the inputs are not modified."""
+        v = np.ascontiguousarray(vertices, dtype=VERTEX_DTYPE).copy()
+        t = np.ascontiguousarray(np.asarray(triangles, dtype=np.int32).reshape(-1, 3)).copy()
+        nv, nt = C.c_int32(len(v)), C.c_int32(len(t))
+        off = np.ascontiguousarray(offset, dtype=np.float32).reshape(4)
+        stats = np.zeros(2, dtype=np.int32)
+        check(lib().lvn_simplify_mesh(self._h, vptr(v), C.byref(nv), vptr(t), C.byref(nt), vptr(off),
+                                      C.byref(options), vptr(stats)), "lvn_simplify_mesh")
+        return v[:nv.value].copy(), t[:nt.value].copy(), (int(stats[0]), int(stats[1]))
+
     def seam_meshes(self, hosts, sources_per_host, v_caps=None, tri_caps=None):
         """Many seams in one call: hosts = [(min, size, colour)], sources_per_host = [[source dicts]].
         Returns [(vertices, triangles)]; raises LvnError on the first failing seam."""
@@ -490,6 +505,14 @@ class Batch:
         check(lib().lvn_batch_get_mesh(self._h, int(i), vptr(v), len(v), C.byref(nv), vptr(t), len(t), C.byref(nt),
                                        vptr(s), len(s), C.byref(ns)), "lvn_batch_get_mesh")
         return v[:nv.value].copy(), t[:nt.value].copy(), s[:ns.value].copy()
+
+    def simplify(self, options):
+        """Simplify every staged chunk's mesh in its slot (lvn_batch_simplify) with per-voxel
+        limits (abi.simplify_options_per_voxel).  Returns int32 [n, 4]: vertices, triangles,
+        iterations, collapses; get_mesh then returns the simplified meshes."""
+        out = np.zeros((max(self.n, 1), 4), dtype=np.int32)
+        check(lib().lvn_batch_simplify(self._h, C.byref(options), vptr(out)), "lvn_batch_simplify")
+        return out[: self.n]
 
     def seam_meshes(self, hosts, sources_per_host):
         """Seams of this batch's chunks (sources with nodes=None are staged chunks, read from

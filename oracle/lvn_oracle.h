@@ -121,6 +121,12 @@ int orc_seam_mesh_leaves(const int32_t host_min[3], int32_t host_size, const lvn
                          int32_t* leaf_min_size);
 /* n seams, one seam per thread on `threads` threads; totals[0..2] += vertices, triangles,
  * failed seams.  Returns wall seconds. */
+/* ngMeshSimplifier restated (lvn_oracle_simplify.cpp): one mesh, offset = the chunk centre
+ * (w 0).  Outputs hold at most nv vertices and nt triangles.  stats = iterations run,
+ * collapses, candidate edges left, interior edges found.  Returns LVN_OK. */
+int orc_simplify_mesh(const lvn_vertex* vin, int32_t nv, const int32_t* tin, int32_t nt, const float offset[4],
+                      const lvn_simplify_options* opt, lvn_vertex* vout, int32_t* nv_out, int32_t* tout,
+                      int32_t* nt_out, int32_t* stats);
 double orc_seam_meshes(int32_t n, const lvn_seam_req* reqs, const lvn_seam_source* sources,
                        int32_t voxels_per_chunk, int32_t threads, int64_t* totals);
 

@@ -76,6 +76,8 @@ def lib():
             "orc_seam_mesh": (i32, [vp, i32, vp, i32, vp, i32, vp, i32, vp, vp, i32, vp]),
             "orc_seam_meshes": (f64, [i32, vp, vp, i32, i32, vp]),
             "orc_seam_mesh_leaves": (i32, [vp, i32, vp, i32, i32, i32, vp, vp]),
+            "orc_simplify_mesh": (i32, [vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp]),
+            "orc_simplify_qef": (C.c_float, [vp, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -313,3 +315,27 @@ def seam_meshes(reqs, sources, voxels_per_chunk=64, threads=1):
     wall = lib().orc_seam_meshes(len(reqs), C.cast(reqs, C.c_void_p), C.cast(sources, C.c_void_p),
                                  int(voxels_per_chunk), int(threads), vptr(totals))
     return wall, totals
+
+
+def simplify_qef(p0, n0, p1, n1):
+    """The simplifier's QEF on two endpoints (4-vectors): (position float32[4], error)."""
+    a = [np.ascontiguousarray(x, dtype=np.float32).reshape(4) for x in (p0, n0, p1, n1)]
+    out = np.zeros(4, dtype=np.float32)
+    err = lib().orc_simplify_qef(vptr(a[0]), vptr(a[1]), vptr(a[2]), vptr(a[3]), vptr(out))
+    return out, np.float32(err)
+
+
+def simplify_mesh(vertices, triangles, offset, options):
+    """ngMeshSimplifier restated: (vertices VERTEX_DTYPE, triangles int32 (T, 3), stats) of the
+    simplified mesh; offset = the chunk centre (4 floats, w 0); options an abi.SimplifyOptions."""
+    v = np.ascontiguousarray(vertices, dtype=VERTEX_DTYPE)
+    t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+    off = np.ascontiguousarray(offset, dtype=np.float32).reshape(4)
+    vo = np.zeros(max(len(v), 1), dtype=VERTEX_DTYPE)
+    to = np.zeros((max(len(t), 1), 3), dtype=np.int32)
+    nv, nt = C.c_int32(), C.c_int32()
+    stats = np.zeros(4, dtype=np.int32)
+    rc = lib().orc_simplify_mesh(vptr(v), len(v), vptr(t), len(t), vptr(off), C.byref(options), vptr(vo),
+                                 C.byref(nv), vptr(to), C.byref(nt), vptr(stats))
+    assert rc == 0
+    return vo[:nv.value].copy(), to[:nt.value].copy(), stats
