@@ -19,7 +19,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from leven_amd import mesher, workloads  # noqa: E402
+from leven_amd import abi, mesher, workloads  # noqa: E402
 
 METRIC = "64^3 chunks meshed/sec (batched vs plain, same GPU) + p99 chunk latency"
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak (fused multiply-adds)
@@ -482,6 +482,53 @@ def batched_leg(wl, args, steps, edge_cap, voxel_cap, plain=True, keep=False):
     return out, None
 
 
+def simplify_leg(batch, wl, args):
+    """Post-mesh simplification (ngMeshSimplifier, as ConstructClipmapNodeData runs it on every
+    chunk mesh with leven's options scaled by voxel size) over the meshed batch: one
+    lvn_batch_simplify call per pass, each pass after a fresh meshing pass; then the
+    pipeline and the simplifier together, and the oracle on a sample of the meshes."""
+    opts = abi.simplify_options_per_voxel()
+    n = len(wl["sizes"])
+    batch.run()
+    batch.wait()
+    counts = batch.counts()
+    idx = np.linspace(0, n - 1, min(n, args.cpu_sample)).astype(int)
+    sample = [batch.get_mesh(i)[:2] for i in idx]
+    res = batch.simplify(opts)  # warm-up (buffers grow here)
+    times, both = [], []
+    steps = max(2, args.steps // 2)
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        batch.run()
+        batch.wait()
+        t1 = time.perf_counter()
+        res = batch.simplify(opts)
+        t2 = time.perf_counter()
+        times.append(t2 - t1)
+        both.append(t2 - t0)
+    ms = float(np.median(times)) * 1e3
+    out = {"chunks": n, "steps": steps, "ms_per_pass": round(ms, 3), "chunks_per_s": round(n / (ms * 1e-3), 1),
+           "mesh_and_simplify_chunks_per_s": round(n / float(np.median(both)), 1),
+           "triangles_in": int(counts[:, 2].sum()), "triangles_out": int(res[:, 1].sum()),
+           "vertices_out": int(res[:, 0].sum()), "collapses": int(res[:, 3].sum()),
+           "iterations_max": int(res[:, 2].max()),
+           "options": "leven's: max error 5 and max edge 2.5 voxel sizes, min cos 0.7, 1/8 of the edges per "
+                      "iteration, 10 iterations, target 5 %, seed 42"}
+    if not args.no_cpu:
+        from oracle import py_oracle
+        secs = []
+        for i, (v, t) in zip(idx, sample):
+            off = np.append(wl["mins"][i] + np.float32(wl["sizes"][i] / 2.0), 0).astype(np.float32)
+            t0 = time.perf_counter()
+            py_oracle.simplify_mesh(v, t, off, abi.simplify_options(int(wl["sizes"][i])))
+            secs.append(time.perf_counter() - t0)
+        out["cpu_baseline"] = {"value": round(len(idx) / sum(secs), 2), "unit": "chunks/s", "cores": 1,
+                               "kind": "port", "sample": f"the meshes of {len(idx)} of the {n} chunks, evenly "
+                                                         f"spaced, simplified by the oracle on one thread"}
+        out["gpu_over_cpu_single_thread"] = round(out["chunks_per_s"] / out["cpu_baseline"]["value"], 1)
+    return out
+
+
 def cpu_sample(wl, count, threads):
     """The oracle on `count` chunks of the workload spread evenly over its chunk list (in
     the clipmap: every LOD in proportion), `threads` chunks at a time, and the first 8 of
@@ -558,7 +605,7 @@ def main():
     ap.add_argument("--plain-chunks", type=int, default=0, help="chunks for the plain form (0 = all)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="length of the served request stream (0 = skip)")
     ap.add_argument("--only", default="", help="comma-separated legs to run (clipmap, terrain_512, in_range, "
-                                                 "precision, seams, serving, plain_cpu); empty = all")
+                                                 "precision, seams, simplify, serving, plain_cpu); empty = all")
     ap.add_argument("--no-seams", action="store_true", help="skip seam stitching of the clipmap")
     ap.add_argument("--no-precision", action="store_true", help="skip the precision-form batch")
     ap.add_argument("--no-autotune", action="store_true", help="time the default launch shapes")
@@ -581,7 +628,7 @@ def main():
     # ---- the 6-LOD clipmap: ~1,900 chunks at voxel sizes 1..32, full per-chunk capacities
     clip = workloads.clipmap()
     cb, nodes = None, None
-    if want("clipmap") or want("seams") or want("in_range"):
+    if want("clipmap") or want("seams") or want("in_range") or want("simplify"):
         legs["clipmap"], cb = batched_leg(clip, args, max(2, args.steps // 2), 0, 0, plain=want("clipmap"),
                                           keep=True)
         counts = cb.counts()
@@ -594,6 +641,8 @@ def main():
             ecap = int(-(-int(counts[keep, 0].max()) // 4096) * 4096)  # slots sized for these chunks
             legs["clipmap"]["in_range"], _ = batched_leg(sub, args, max(2, args.steps // 2), ecap, 60000)
             legs["clipmap"]["in_range"]["active_voxels_max"] = 60000
+        if want("simplify"):
+            legs["simplify"] = simplify_leg(cb, clip, args)
         if want("seams") and not args.no_seams:
             n = len(clip["sizes"])
             nodes = [(tuple(int(v) for v in clip["mins"][i]), int(clip["sizes"][i]), cb.get_mesh(i)[2])
