@@ -826,10 +826,10 @@ struct DevBuf {
     size_t bytes = 0;
     int ensure(size_t need) {
         if (need <= bytes) return LVN_OK;
+        const size_t want = std::max(need, bytes + bytes / 2);  // grow by half at least
         if (p) LVN_HIP(hipFree(p));
         p = nullptr;
         bytes = 0;
-        const size_t want = std::max(need, bytes + bytes / 2);
         if (hipMalloc(&p, want) != hipSuccess) {
             p = nullptr;
             return LVN_ERR_NOMEM;
@@ -847,10 +847,10 @@ struct HostBuf {
     size_t bytes = 0;
     int ensure(size_t need) {
         if (need <= bytes) return LVN_OK;
+        const size_t want = std::max(need, bytes + bytes / 2);  // grow by half at least
         if (p) LVN_HIP(hipHostFree(p));
         p = nullptr;
         bytes = 0;
-        const size_t want = std::max(need, bytes + bytes / 2);
         if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
             p = nullptr;
             return LVN_ERR_NOMEM;

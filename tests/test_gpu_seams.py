@@ -10,7 +10,7 @@ import itertools
 import numpy as np
 import pytest
 
-from leven_amd import mesher, workloads
+from leven_amd import abi, mesher, workloads
 from leven_amd.abi import LVN_ERR_CAPACITY, LVN_ERR_INVALID, LVN_OK, SEAM_DTYPE, VERTEX_DTYPE
 from oracle import py_oracle
 
@@ -91,6 +91,14 @@ def test_mixed_lod_seams_match_oracle(ctx, mixed_lod):
     sizes = {s for _, s, _ in mixed_lod.nodes}
     assert sizes == {256, 512}
     _check_scene(ctx, mixed_lod)
+
+
+def test_mixed_lod_plane_seams_match_oracle(ctx):
+    """The coarse-fine sheet whose triangle count the CPU tests fix in closed form."""
+    nodes = [((0, -512, 0), 512), ((0, 0, 0), 512)]
+    nodes += [((512 + 256 * x, -512 + 256 * y, 256 * z), 256)
+              for x, y, z in itertools.product(range(2), range(4), range(2))]
+    _check_scene(ctx, Scene(abi.field_desc(height_octaves=0), SEED, nodes))
 
 
 def test_cached_sources(ctx, equal_lod):

@@ -17,7 +17,7 @@ import itertools
 import numpy as np
 import pytest
 
-from leven_amd import workloads
+from leven_amd import abi, workloads
 from leven_amd.abi import LVN_ERR_INVALID, LVN_OK, SEAM_DTYPE
 from oracle import py_oracle
 
@@ -97,3 +97,57 @@ def test_seam_sources_follow_find_active_nodes():
                    (4, (512, 0, 0), 512), (5, (512, 0, 0), 512), (6, (512, 0, 0), 512), (7, (512, 0, 0), 512)]
     # the coarse node's own seam takes the finer nodes inside its +x... regions: none here
     assert sorted(d["neighbour"] for d in srcs[1]) == [0]
+
+
+def _mixed_lod_nodes():
+    """A 2-LOD block 1024 x 1024 x 512 world units: two stacked LOD1 chunks (voxel size 8)
+    over x in [0, 512), sixteen LOD0 chunks (voxel size 4) over x in [512, 1024)."""
+    nodes = [((0, -512, 0), 512), ((0, 0, 0), 512)]
+    nodes += [((512 + 256 * x, -512 + 256 * y, 256 * z), 256)
+              for x, y, z in itertools.product(range(2), range(4), range(2))]
+    return nodes
+
+
+def _union_triangles(scene):
+    """Chunk and seam triangles of a scene by vertex positions (each once, rotated so the
+    smallest position leads; the reference draws some seam quads in two seams)."""
+    tris = {}
+
+    def add(pos, ts):
+        for t in ts:
+            k = (pos[t[0]], pos[t[1]], pos[t[2]])
+            r = min(range(3), key=lambda i: k[i])
+            tris[k[r:] + k[:r]] = True
+
+    for c in scene.chunks:
+        add([tuple(p[:3].tolist()) for p in c.vertices()["pos"]], c.triangles())
+    for (mn, size, _), srcs in zip(scene.nodes, scene.sources()):
+        rc, v, t = py_oracle.seam_mesh(mn, size, srcs)
+        assert rc == LVN_OK
+        add([tuple(p[:3].tolist()) for p in v["pos"]], t)
+    return list(tris)
+
+
+def test_seam_oracle_mixed_lod_plane_closed_form():
+    """The sheet d = y across a coarse-fine face.  Every lattice point of the x-z plane
+    strictly inside the block has one sign-changing y-edge, contoured at the finest of its
+    four surrounding cells: 63^2 coarse and 127^2 fine interior points give two triangles
+    each; on the face x = 512 the 63 points shared with the coarse lattice have four
+    distinct cells (two triangles) and the 64 fine points between them have two of their
+    cells in one coarse leaf (one triangle and one degenerate).  The union of chunk and
+    seam meshes has no boundary away from the block's outer faces, one winding throughout."""
+    scene = Scene(abi.field_desc(height_octaves=0), 2, _mixed_lod_nodes())
+    tris = _union_triangles(scene)
+    real = [k for k in tris if len(set(k)) == 3]
+    assert len(real) == 2 * (63 ** 2 + 127 ** 2) + 2 * 63 + 64
+    assert len(tris) - len(real) == 64
+    assert all(p[1] == 0.0 for k in real for p in k)  # the sheet lies on y = 0
+    chain = {}
+    for k in real:
+        for a, b in ((k[0], k[1]), (k[1], k[2]), (k[2], k[0])):
+            chain[(a, b)] = chain.get((a, b), 0) + 1
+            chain[(b, a)] = chain.get((b, a), 0) - 1
+    open_edges = [e for e, m in chain.items() if m > 0]
+    assert open_edges and all(min(p[0], p[2], 1024 - p[0], 512 - p[2]) < 8 for e in open_edges for p in e)
+    normals = {int(np.sign(np.cross(np.subtract(k[1], k[0]), np.subtract(k[2], k[0]))[1])) for k in real}
+    assert len(normals) == 1
