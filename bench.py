@@ -122,33 +122,44 @@ def cpu_baseline(wl, threads):
 
 
 def precision_leg(args):
-    """The precision form on the 512-chunk field: FP16 density storage, compensated FP32
-    QEF with FP64 re-solves, 1024 Y-rotated boxes; same batched pipeline."""
+    """The precision form on the 512-chunk field with 1024 Y-rotated boxes, against the
+    FP32 form on the same chunks: compensated FP32 QEF with FP64 re-solves near the
+    truncation thresholds, with and without FP16 density storage; same batched pipeline."""
+    from leven_amd.abi import CTX_FP16_DENSITY, CTX_PRECISE_QEF
     wl = workloads.precision()
     mesher.init(wl["desc"], wl["seed"], wl["default_material"])
     mesher.clear_csg_ops()
     for op, bb in zip(wl["ops"], wl["aabbs"]):
         mesher.store_csg_op(op, bb[:3], bb[3:])
     n = len(wl["sizes"])
-    b = mesher.Batch(64, n, edge_cap=262144, voxel_cap=131072, flags=wl["flags"])
-    b.set_chunks(wl["mins"], wl["sizes"])
-    tuning = b.autotune(reps=1)
-    for _ in range(max(1, args.warmup)):
-        b.run()
-    b.wait()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        b.run()
-    b.wait()
-    elapsed = time.perf_counter() - t0
-    counts = b.counts()
-    assert (counts[:, 4] == 0).all(), "a precision-form chunk overflowed"
-    stage_ms = {k: round(v, 4) for k, v in b.run_timed().items()}
-    b.close()
-    return {"chunks": n, "csg_ops": int(len(wl["ops"])), "chunks_per_s": round(n * args.steps / elapsed, 1),
-            "ms_per_step": round(elapsed / args.steps * 1e3, 3), "vertices": int(counts[:, 1].sum()),
-            "fp64_resolved": int(counts[:, 5].sum()), "stage_ms": stage_ms, "tuning": tuning,
-            "accuracy": "vertex error <= 1.9e-5 voxel vs FP64 (tests/test_gpu_precision.py)"}
+    out = {"chunks": n, "csg_ops": int(len(wl["ops"]))}
+    for name, flags in (("fp32", 0), ("precise_qef", CTX_PRECISE_QEF),
+                        ("precise_qef_fp16_density", CTX_PRECISE_QEF | CTX_FP16_DENSITY)):
+        b = mesher.Batch(64, n, edge_cap=262144, voxel_cap=131072, flags=flags)
+        b.set_chunks(wl["mins"], wl["sizes"])
+        tuning = b.autotune(reps=1)
+        for _ in range(max(1, args.warmup)):
+            b.run()
+        b.wait()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.run()
+        b.wait()
+        elapsed = time.perf_counter() - t0
+        counts = b.counts()
+        assert (counts[:, 4] == 0).all(), "a precision-form chunk overflowed"
+        stage_ms = {k: round(v, 4) for k, v in b.run_timed().items()}
+        b.close()
+        out[name] = {"chunks_per_s": round(n * args.steps / elapsed, 1), "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+                     "vertices": int(counts[:, 1].sum()), "fp64_resolved": int(counts[:, 5].sum()),
+                     "qef_ms": stage_ms["qef"], "density_ms": stage_ms["density"], "tuning": tuning}
+    base = out["fp32"]["chunks_per_s"]
+    for name in ("precise_qef", "precise_qef_fp16_density"):
+        out[name]["over_fp32"] = round(out[name]["chunks_per_s"] / base, 3)
+    out["accuracy"] = ("vertex error vs the FP64 solve of the same hermite data <= 1e-4 voxel; vs the FP64 "
+                       "pipeline end to end see tests/test_gpu_precision.py")
+    mesher.clear_csg_ops()
+    return out
 
 
 def seam_leg(batch, nodes, args):

@@ -1465,9 +1465,16 @@ __device__ __forceinline__ void qef_pinv_solve(const T A[6], const T r[3], T x[3
     }
 }
 
-__device__ __forceinline__ bool near_threshold(float s) {
+// The truncated pseudo-inverse keeps eigenvalues in [0.1, 10] (svd_invdet, cl/qef.cl:107-109),
+// so the retained ones have a condition number of at most 100 and FP32 solves them well;
+// what FP32 can get wrong is which eigenvalues are retained.  FP32 Jacobi's eigenvalues
+// are within a small multiple of eps * |A|_F of the exact ones (backward stable, about 30
+// rotations here): an eigenvalue that close to 0.1 or 10 is re-solved in FP64.
+constexpr float kThresholdWindow = 256.f * 5.9604645e-8f;  // 256 eps, times |A|_F
+
+__device__ __forceinline__ bool near_threshold(float s, float w) {
     const float a = fabsf(s);
-    return fabsf(a - 0.1f) < 1e-4f || fabsf(a - 10.f) < 1e-2f;
+    return fabsf(a - 0.1f) <= w || fabsf(a - 10.f) <= w;
 }
 
 __device__ __forceinline__ void qef_precise(const KArgs& a, int chunk, const ChunkDesc& cd, int3 p, int emask, float4& out,
@@ -1550,10 +1557,9 @@ __device__ __forceinline__ void qef_precise(const KArgs& a, int chunk, const Chu
 #pragma unroll
     for (int i = 0; i < 3; ++i) r[i] = rs[i] + rc[i];
     qef_pinv_solve<float>(A, r, x, sig);
-    const float smax = fmaxf(fmaxf(fabsf(sig[0]), fabsf(sig[1])), fabsf(sig[2]));
-    const float smin = fminf(fminf(fabsf(sig[0]), fabsf(sig[1])), fabsf(sig[2]));
-    if (smin == 0.f || smax > 1e6f * smin || near_threshold(sig[0]) || near_threshold(sig[1]) ||
-        near_threshold(sig[2])) {
+    const float normf = sqrtf((A[0] * A[0] + A[3] * A[3] + A[5] * A[5]) + 2.f * (A[1] * A[1] + A[2] * A[2] + A[4] * A[4]));
+    const float w = kThresholdWindow * normf;
+    if (near_threshold(sig[0], w) || near_threshold(sig[1], w) || near_threshold(sig[2], w)) {
         double n = 0, mxd = 0, myd = 0, mzd = 0;
 #pragma unroll
         for (int e = 0; e < 12; ++e) {

@@ -16,7 +16,7 @@ import math
 
 import numpy as np
 
-from .abi import (CSG_OP_DTYPE, CTX_FP16_DENSITY, CTX_PRECISE_QEF, FIELD_HEIGHTFIELD, FIELD_TERRAIN_CAVES,
+from .abi import (CSG_OP_DTYPE, CTX_PRECISE_QEF, FIELD_HEIGHTFIELD, FIELD_TERRAIN_CAVES,
                   MATERIAL_AIR, field_desc)
 
 LEAF_SIZE_SCALE = 4
@@ -99,7 +99,7 @@ def precision():
     """terrain_512's chunks and field, 1024 seeded Y-rotated boxes; FP16 density + precise QEF."""
     w = terrain_512()
     ops, aabbs = make_ops(1024, seed=3, spheres=0, rotated=True)
-    w.update(name="precision", ops=ops, aabbs=aabbs, flags=CTX_FP16_DENSITY | CTX_PRECISE_QEF)
+    w.update(name="precision", ops=ops, aabbs=aabbs, flags=CTX_PRECISE_QEF)
     return w
 
 

@@ -866,6 +866,105 @@ struct orc_chunk {
         return {dx / len, dy / len, dz / len};
     }
 
+    // FP64 twin of density_edge's crossing: the first arg-min of |d| over the 17 samples
+    // t = k / 16 of the FP64 density.
+    double crossing_f64(int x, int y, int z, int axis) const {
+        const double p0[3] = {(double)(x * scale + off[0]), (double)(y * scale + off[1]), (double)(z * scale + off[2])};
+        double best = HUGE_VAL, t = 0.0;
+        for (int i = 0; i <= 16; ++i) {
+            double p[3] = {p0[0], p0[1], p0[2]};
+            p[axis] += (double)scale * (i / 16.0);
+            const double d = std::fabs(density_d(desc, perm.data(), p[0], p[1], p[2]));
+            if (d < best) {
+                best = d;
+                t = i / 16.0;
+            }
+        }
+        return t;
+    }
+
+    Vec3d density_normal_f64_at(int x, int y, int z, int axis, double t) const {
+        const double p0[3] = {(double)(x * scale + off[0]), (double)(y * scale + off[1]), (double)(z * scale + off[2])};
+        double p[3] = {p0[0], p0[1], p0[2]};
+        p[axis] += (double)scale * t;
+        const double h = 0.001;
+        const double dx = density_d(desc, perm.data(), p[0] + h, p[1], p[2]) - density_d(desc, perm.data(), p[0] - h, p[1], p[2]);
+        const double dy = density_d(desc, perm.data(), p[0], p[1] + h, p[2]) - density_d(desc, perm.data(), p[0], p[1] - h, p[2]);
+        const double dz = density_d(desc, perm.data(), p[0], p[1], p[2] + h) - density_d(desc, perm.data(), p[0], p[1], p[2] - h);
+        const double len = std::sqrt(dx * dx + dy * dy + dz * dz);
+        if (len == 0.0) return {0.0, 0.0, 0.0};
+        return {dx / len, dy / len, dz / len};
+    }
+
+    // The whole vertex computation in FP64 on this chunk's topology: every density edge's
+    // crossing and normal from the FP64 field (brush edges keep their data), then the QEF
+    // in FP64.  pos4 / normal4 per vertex; *moved = edges whose FP64 crossing picks another
+    // of the 17 samples than the FP32 one.
+    void fp64_pipeline(double* pos4, double* normal4, int32_t* moved) const {
+        const int H = g.H;
+        std::vector<int32_t> slot((size_t)H * H * H * 3, -1);
+        for (size_t k = 0; k < edgeIds.size(); ++k) {
+            const int id = edgeIds[k], a = id & 3, h = id >> 2;
+            const int hx = h & g.mask, hy = (h >> g.shift) & g.mask, hz = (h >> (2 * g.shift)) & g.mask;
+            slot[(((size_t)hz * H + hy) * H + hx) * 3 + a] = (int32_t)k;
+        }
+        std::vector<double> et(edgeIds.size(), -1.0);
+        std::vector<Vec3d> en(edgeIds.size());
+        int32_t changed = 0;
+        for (size_t k = 0; k < edgeIds.size(); ++k) {
+            const int id = edgeIds[k], a = id & 3, h = id >> 2;
+            const int hx = h & g.mask, hy = (h >> g.shift) & g.mask, hz = (h >> (2 * g.shift)) & g.mask;
+            const float* ed = &edgeData[k * 4];
+            if (edgeBrush[k]) {
+                et[k] = ed[3];
+                en[k] = Vec3d{ed[0], ed[1], ed[2]};
+            } else {
+                et[k] = crossing_f64(hx, hy, hz, a);
+                en[k] = density_normal_f64_at(hx, hy, hz, a, et[k]);
+                changed += et[k] != (double)ed[3];
+            }
+        }
+        if (moved) *moved = changed;
+        const int V = (int)vcode.size();
+        for (int v = 0; v < V; ++v) {
+            int p[3];
+            position_for_code((uint32_t)vcode[v], g.depth, p);
+            Qef<double> q;
+            double n[3] = {0.0, 0.0, 0.0};
+            double count = 0.0;
+            for (int i = 0; i < 12; ++i) {
+                if (!((vmask[v] >> i) & 1)) continue;
+                const int e0 = kEdgeMap[i][0], axis = i / 4;
+                const int hx = p[0] + kChild[e0][0], hy = p[1] + kChild[e0][1], hz = p[2] + kChild[e0][2];
+                const int k = slot[(((size_t)hz * H + hy) * H + hx) * 3 + axis];
+                if (k < 0) continue;
+                V4<double> pd, nd;
+                const double a0[3] = {(double)hx, (double)hy, (double)hz};
+                pd.x = (double)scale * (a0[0] + (axis == 0 ? et[k] : 0.0));
+                pd.y = (double)scale * (a0[1] + (axis == 1 ? et[k] : 0.0));
+                pd.z = (double)scale * (a0[2] + (axis == 2 ? et[k] : 0.0));
+                pd.w = 0.0;
+                nd.x = en[k].x;
+                nd.y = en[k].y;
+                nd.z = en[k].z;
+                nd.w = 0.0;
+                qef_add_point(q, nd, pd);
+                n[0] += en[k].x;
+                n[1] += en[k].y;
+                n[2] += en[k].z;
+                count += 1.0;
+            }
+            qef_finish_mass(q);
+            const V4<double> x = qef_solve(q);
+            pos4[4 * v + 0] = x.x * 4.0 + (double)min[0];
+            pos4[4 * v + 1] = x.y * 4.0 + (double)min[1];
+            pos4[4 * v + 2] = x.z * 4.0 + (double)min[2];
+            pos4[4 * v + 3] = 1.0;
+            for (int c = 0; c < 3; ++c) normal4[4 * v + c] = count > 0 ? n[c] / count : 0.0;
+            normal4[4 * v + 3] = 0.0;
+        }
+    }
+
     bool solid(int idx) const { return mat[idx] != kAir; }
 
     // Sign-changing edges of the hermite grid in ascending id order
@@ -1288,6 +1387,9 @@ void orc_chunk_get_positions_f64(const orc_chunk* c, double* pos4) {
 }
 void orc_chunk_get_normals_f64(const orc_chunk* c, double* normal4) {
     std::copy(c->vnorm64.begin(), c->vnorm64.end(), normal4);
+}
+void orc_chunk_fp64_pipeline(const orc_chunk* c, double* pos4, double* normal4, int32_t* moved) {
+    c->fp64_pipeline(pos4, normal4, moved);
 }
 void orc_chunk_get_triangles(const orc_chunk* c, int32_t* tri3) { std::copy(c->tris.begin(), c->tris.end(), tri3); }
 void orc_chunk_get_vertices(const orc_chunk* c, lvn_vertex* v) { std::copy(c->verts.begin(), c->verts.end(), v); }

@@ -102,6 +102,11 @@ void orc_chunk_get_positions_f64(const orc_chunk* c, double* pos4);
 /* FP64 vertex normals: the mean of the edges' FP64 central-difference normals at the
  * same crossings (brush edges keep their FP32 normal). */
 void orc_chunk_get_normals_f64(const orc_chunk* c, double* normal4);
+/* The vertex computation end to end in FP64 on the chunk's topology: each density edge's
+ * crossing (17-sample search) and central-difference normal from the FP64 field (brush
+ * edges keep their FP32 data), the QEF in FP64.  *moved = edges whose FP64 crossing is
+ * another sample than the FP32 one. */
+void orc_chunk_fp64_pipeline(const orc_chunk* c, double* pos4, double* normal4, int32_t* moved);
 void orc_chunk_get_triangles(const orc_chunk* c, int32_t* tri3);
 void orc_chunk_get_vertices(const orc_chunk* c, lvn_vertex* v);
 void orc_chunk_get_seams(const orc_chunk* c, lvn_seam_node* s);

@@ -69,6 +69,7 @@ def lib():
             "orc_chunk_get_positions": (None, [vp, vp, vp]),
             "orc_chunk_get_positions_f64": (None, [vp, vp]),
             "orc_chunk_get_normals_f64": (None, [vp, vp]),
+            "orc_chunk_fp64_pipeline": (None, [vp, vp, vp, vp]),
             "orc_chunk_get_triangles": (None, [vp, vp]),
             "orc_chunk_get_vertices": (None, [vp, vp]),
             "orc_chunk_get_seams": (None, [vp, vp]),
@@ -241,6 +242,16 @@ class Chunk:
         nrm = np.zeros((v, 4), dtype=np.float64)
         lib().orc_chunk_get_normals_f64(self._h, vptr(nrm))
         return nrm
+
+    def fp64_pipeline(self):
+        """(positions, normals, moved crossings) of the vertex computation in FP64 end to end
+        on this chunk's topology."""
+        v = self.counts()["voxels"]
+        pos = np.zeros((v, 4), dtype=np.float64)
+        nrm = np.zeros((v, 4), dtype=np.float64)
+        moved = C.c_int32(0)
+        lib().orc_chunk_fp64_pipeline(self._h, vptr(pos), vptr(nrm), C.byref(moved))
+        return pos, nrm, int(moved.value)
 
     def triangles(self):
         t = self.counts()["quads"] * 2
