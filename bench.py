@@ -327,13 +327,13 @@ def dropin_latency(wl, count):
 
 
 def serve_burst():
-    """Capacity of the served form: the 512 terrain_512 chunks submitted at once to a
-    fresh server (all cache misses), meshed by continuous batching on 3 lanes."""
+    """Capacity of the served form: the 512 terrain_512 chunks submitted at once
+    (lvn_server_submit_many) to a fresh server (all cache misses), meshed by continuous
+    batching on 3 lanes."""
     wl = workloads.terrain_512()
     server = mesher.Server(64, hbm_budget=64 << 30, max_batch=64, lanes=3, edge_cap=0, voxel_cap=0)
     t0 = time.perf_counter()
-    for mn, size in zip(wl["mins"], wl["sizes"]):
-        server.submit_mesh(mn, int(size))
+    server.submit_meshes(wl["mins"], wl["sizes"])
     server.drain()
     wall = time.perf_counter() - t0
     res = server.poll(len(wl["sizes"]) + 16)

@@ -323,6 +323,9 @@ int  lvn_server_create(int32_t voxels_per_chunk, uint64_t hbm_budget, int32_t ma
 void lvn_server_destroy(lvn_server* s);
 /* Non-blocking.  An edit completes when it is recorded; later mesh requests see it. */
 int  lvn_server_submit(lvn_server* s, const lvn_request* req, int64_t* ticket);
+/* n requests at once (one lock, one wake-up of the scheduler), tickets[i] for reqs[i];
+ * nothing is submitted when one of them is invalid. */
+int  lvn_server_submit_many(lvn_server* s, int32_t n, const lvn_request* reqs, int64_t* tickets);
 /* Moves up to cap completed results into out (completion order); *n = number moved. */
 int  lvn_server_poll(lvn_server* s, lvn_result* out, int32_t cap, int32_t* n);
 /* Blocks until every submitted request has completed. */

@@ -428,8 +428,10 @@ void Server::run() {
         for (const Pending& p : batch) handle(p);
         poll_lanes();
         admit_and_launch();
+        // while batches are in flight, poll their events without sleeping: a sleep's
+        // granularity (tens of microseconds) would hold finished lanes idle
         const bool busy = std::any_of(lanes.begin(), lanes.end(), [](const Lane& L) { return L.busy; });
-        if (busy) std::this_thread::sleep_for(std::chrono::microseconds(20));
+        if (busy) std::this_thread::yield();
     }
 }
 
@@ -474,23 +476,33 @@ int lvn_server_create(int32_t voxels_per_chunk, uint64_t hbm_budget, int32_t max
 
 void lvn_server_destroy(lvn_server* sh) { delete reinterpret_cast<Server*>(sh); }
 
-int lvn_server_submit(lvn_server* sh, const lvn_request* req, int64_t* ticket) {
-    Server* s = reinterpret_cast<Server*>(sh);
-    if (!s || !req || !ticket) return LVN_ERR_INVALID;
-    if (req->kind == LVN_REQ_MESH) {
-        if (req->size < s->arena.N * kLeafSizeScale || req->size % (s->arena.N * kLeafSizeScale)) return LVN_ERR_INVALID;
-    } else if (req->kind == LVN_REQ_EDIT) {
+static bool valid_request(const Server* s, const lvn_request* req) {
+    if (req->kind == LVN_REQ_MESH)
+        return req->size >= s->arena.N * kLeafSizeScale && req->size % (s->arena.N * kLeafSizeScale) == 0;
+    if (req->kind == LVN_REQ_EDIT) {
         const lvn_csg_op& o = req->op;
-        if (o.material < 0 || o.material > 255 || o.type < 0 || o.type > 1 || o.brush < 0 || o.brush > 1)
-            return LVN_ERR_INVALID;
-    } else {
-        return LVN_ERR_INVALID;
+        return o.material >= 0 && o.material <= 255 && o.type >= 0 && o.type <= 1 && o.brush >= 0 && o.brush <= 1;
     }
+    return false;
+}
+
+int lvn_server_submit(lvn_server* sh, const lvn_request* req, int64_t* ticket) {
+    return lvn_server_submit_many(sh, 1, req, ticket);
+}
+
+int lvn_server_submit_many(lvn_server* sh, int32_t n, const lvn_request* reqs, int64_t* tickets) {
+    Server* s = reinterpret_cast<Server*>(sh);
+    if (!s || n < 0 || (n > 0 && (!reqs || !tickets))) return LVN_ERR_INVALID;
+    for (int32_t i = 0; i < n; ++i)
+        if (!valid_request(s, &reqs[i])) return LVN_ERR_INVALID;
     std::lock_guard<std::mutex> lock(s->mu);
     if (s->error) return s->error;
-    *ticket = s->next_ticket++;
-    s->incoming.push_back(Pending{*ticket, *req, now_s()});
-    ++s->outstanding;
+    const double t = now_s();
+    for (int32_t i = 0; i < n; ++i) {
+        tickets[i] = s->next_ticket++;
+        s->incoming.push_back(Pending{tickets[i], reqs[i], t});
+        ++s->outstanding;
+    }
     s->cv.notify_all();
     return LVN_OK;
 }

@@ -26,7 +26,7 @@ EXPORTS = [
     "lvn_batch_run_timed", "lvn_batch_stage_name", "lvn_batch_counts", "lvn_batch_get_mesh", "lvn_batch_work",
     "lvn_batch_get_tuning", "lvn_batch_set_tuning", "lvn_batch_autotune",
     "lvn_server_create", "lvn_server_destroy", "lvn_server_submit", "lvn_server_poll", "lvn_server_drain",
-    "lvn_server_get_mesh", "lvn_server_stats", "lvn_generate_seam_mesh", "lvn_generate_seam_meshes",
+    "lvn_server_get_mesh", "lvn_server_stats", "lvn_server_submit_many", "lvn_generate_seam_mesh", "lvn_generate_seam_meshes",
     "lvn_seam_device_mesh", "lvn_batch_seam_meshes", "lvn_batch_seam_device_mesh",
     "lvn_generate_chunk_mesh_noalloc",
     "lvn_clipmap_create", "lvn_clipmap_destroy", "lvn_clipmap_update", "lvn_clipmap_list", "lvn_clipmap_commit",
@@ -96,6 +96,7 @@ def lib():
             "lvn_server_create": (i32, [i32, C.c_uint64, i32, i32, i32, i32, u32, C.POINTER(vp)]),
             "lvn_server_destroy": (None, [vp]),
             "lvn_server_submit": (i32, [vp, C.POINTER(Request), C.POINTER(C.c_int64)]),
+            "lvn_server_submit_many": (i32, [vp, i32, vp, vp]),
             "lvn_server_poll": (i32, [vp, vp, i32, C.POINTER(i32)]),
             "lvn_server_drain": (i32, [vp]),
             "lvn_server_get_mesh": (i32, [vp, C.c_int64, vp, i32, C.POINTER(i32), vp, i32, C.POINTER(i32), vp, i32,
@@ -561,6 +562,21 @@ class Server:
         t = C.c_int64()
         check(lib().lvn_server_submit(self._h, C.byref(r), C.byref(t)), "lvn_server_submit")
         return t.value
+
+    def submit_meshes(self, mins, sizes):
+        """Mesh requests for many chunks in one call; returns their tickets (int64 array)."""
+        mins = np.asarray(mins, dtype=np.int64).reshape(-1, 3)
+        n = len(mins)
+        reqs = (Request * max(n, 1))()
+        for k in range(n):
+            reqs[k].kind = REQ_MESH
+            for i in range(3):
+                reqs[k].min[i] = int(mins[k][i])
+            reqs[k].size = int(sizes[k])
+        tickets = np.zeros(max(n, 1), dtype=np.int64)
+        check(lib().lvn_server_submit_many(self._h, n, C.cast(reqs, C.c_void_p), vptr(tickets)),
+              "lvn_server_submit_many")
+        return tickets[:n]
 
     def submit_edit(self, op, aabb):
         r = Request()

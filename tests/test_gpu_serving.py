@@ -150,3 +150,28 @@ def test_eviction_under_small_budget():
         assert _same(s.get_mesh(t), b.get_mesh(int(i))), f"chunk {i} read back"
     s.close()
     mesher.clear_csg_ops()
+
+
+def test_submit_many_equals_one_by_one():
+    """A burst submitted in one call (lvn_server_submit_many) serves the same meshes as
+    requests submitted one at a time; an invalid request rejects the whole call."""
+    w = _setup()
+    pick = list(range(0, 512, 37))
+    mins, sizes = w["mins"][pick], w["sizes"][pick]
+    s1 = mesher.Server(64, hbm_budget=8 << 30, max_batch=8, lanes=2)
+    one = [s1.submit_mesh(m, int(s)) for m, s in zip(mins, sizes)]
+    s1.drain()
+    s2 = mesher.Server(64, hbm_budget=8 << 30, max_batch=8, lanes=2)
+    many = s2.submit_meshes(mins, sizes)
+    assert len(many) == len(pick) and len(set(many.tolist())) == len(pick)
+    s2.drain()
+    res = s2.poll(64)
+    assert len(res) == len(pick) and (res["status"] == 0).all()
+    for a, b in zip(one, many):
+        assert _same(s1.get_mesh(a), s2.get_mesh(int(b)))
+    with pytest.raises(mesher.LvnError):
+        s2.submit_meshes(np.zeros((2, 3), np.int64), [256, 100])  # 100 is not a chunk size
+    assert len(s2.poll(64)) == 0
+    s1.close()
+    s2.close()
+    mesher.clear_csg_ops()
