@@ -175,3 +175,47 @@ def test_submit_many_equals_one_by_one():
     s1.close()
     s2.close()
     mesher.clear_csg_ops()
+
+
+def test_poisson_stream_replay():
+    """The seeded serving stream (Poisson arrivals along a camera path, 5% edits each with
+    its forced re-mesh requests) replayed through a server at full speed: every request
+    succeeds, and every chunk whose last served mesh is still current equals the batched
+    form meshed with all the stream's ops stored (the ops a current mesh does not hold
+    cannot overlap it: an overlapping edit would have invalidated it)."""
+    wl = workloads.serving(duration=0.25)
+    mesher.init(wl["desc"], wl["seed"], wl["default_material"])
+    mesher.clear_csg_ops()
+    for op, bb in zip(wl["ops"], wl["aabbs"]):
+        mesher.store_csg_op(op, bb[:3], bb[3:])
+    server = mesher.Server(64, hbm_budget=32 << 30, max_batch=64, lanes=3, edge_cap=0, voxel_cap=0)
+    last = {}
+    edits = 0
+    for t, kind, payload in wl["events"]:
+        if kind == "mesh":
+            last[(tuple(int(v) for v in payload[0]), int(payload[1]))] = server.submit_mesh(payload[0], payload[1])
+        else:
+            server.submit_edit(payload[0], payload[1])
+            mesher.store_csg_op(payload[0], payload[1][:3], payload[1][3:])
+            edits += 1
+    server.drain()
+    res = server.poll(len(wl["events"]) + 16)
+    assert len(res) == len(wl["events"]) and (res["status"] == 0).all() and edits > 0
+    current = {}
+    for chunk, ticket in last.items():
+        try:
+            current[chunk] = server.get_mesh(ticket, v_cap=262144, tri_cap=6 * 262144, s_cap=262144)
+        except mesher.LvnError:
+            pass  # invalidated by a later edit
+    assert len(current) >= len(last) // 2
+    chunks = sorted(current)
+    b = mesher.Batch(64, len(chunks), edge_cap=0, voxel_cap=0)
+    b.set_chunks(np.array([m for m, _ in chunks], np.int32), np.array([s for _, s in chunks], np.int32))
+    b.run()
+    b.wait()
+    for i, c in enumerate(chunks):
+        assert _same(current[c], b.get_mesh(i)), f"chunk {c}"
+    print(f"{len(wl['events'])} requests, {edits} edits, {len(chunks)} current chunks checked", flush=True)
+    b.close()
+    server.close()
+    mesher.clear_csg_ops()
