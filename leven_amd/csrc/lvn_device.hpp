@@ -9,6 +9,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "lvn_shared.h"
@@ -23,12 +24,20 @@ namespace lvn {
 // value plus one, mod 256, and byte y * 258 holds texel (0, y): snoise3's second read,
 // at x = value with 255 wrapping to 0, is then row + stored byte with no compare.  The
 // 16-entry gradient table is indexed by the stored byte's low four bits (rotated by one
-// entry, so no subtraction is needed): lanes that need the same gradient read the same
-// 16 bytes.  66,576 bytes per workgroup.
-struct NoiseLds {
+// entry, so no subtraction is needed).  The gradients are stored as (x, y) pairs, 8 bytes
+// an entry, and z components apart, the block starting on a bank row: the 16 pairs fill
+// the 32 banks once, so a wave's 8-byte reads never meet two addresses in one bank
+// (equal gradients share an address; whole 16-byte entries put entries 8 apart in the
+// same banks).  snoise2 reads a pair, snoise3 a pair and a z.  66,624 bytes per workgroup.
+constexpr int kGradPad = (128 - kPermPaddedBytes % 128) % 128;
+
+struct alignas(128) NoiseLds {
     uint8_t perm[kPermPaddedBytes];
-    float4 grad[kGradEntries];  // entry j: gradient (j - 1) & 15, for stored bytes w with w & 15 == j
+    uint8_t pad[kGradPad];
+    float2 gxy[kGradEntries];  // entry j: gradient (j - 1) & 15
+    float gz[kGradEntries];
 };
+static_assert(offsetof(NoiseLds, gxy) % 128 == 0, "the gradient block starts on a bank row");
 
 // Cooperative copy of the tables into LDS (all threads of the block call it).
 __device__ __forceinline__ void load_noise_tables(NoiseLds& lds, const uint8_t* __restrict__ perm,
@@ -36,7 +45,11 @@ __device__ __forceinline__ void load_noise_tables(NoiseLds& lds, const uint8_t* 
     const uint4* src = reinterpret_cast<const uint4*>(perm);
     uint4* dst = reinterpret_cast<uint4*>(lds.perm);
     for (int i = threadIdx.x; i < kPermPaddedBytes / 16; i += blockDim.x) dst[i] = src[i];
-    for (int i = threadIdx.x; i < kGradEntries; i += blockDim.x) lds.grad[i] = grads[i];
+    for (int i = threadIdx.x; i < kGradEntries; i += blockDim.x) {
+        const float4 g = grads[i];
+        lds.gxy[i] = make_float2(g.x, g.y);
+        lds.gz[i] = g.z;
+    }
     __syncthreads();
 }
 
@@ -74,9 +87,9 @@ __device__ __forceinline__ float recip_1_2(float d) {
 // value then adds +0 to a sum that is never -0), or the ridged form's fabsf.
 __device__ __forceinline__ float corner2(const NoiseLds& T, int texel, float fx, float fy) {
     const float r = fmaxf(0.5f - fmaf(fy, fy, fx * fx), 0.f);
-    const float4 gr = T.grad[T.perm[texel] & 15];
+    const float2 g = T.gxy[T.perm[texel] & 15];
     const float t = r * r;
-    return (t * t) * fmaf(gr.y, fy, gr.x * fx);
+    return (t * t) * fmaf(g.y, fy, g.x * fx);
 }
 
 // snoise2, cl/simplex.cl:99-157.
@@ -107,9 +120,10 @@ __device__ __forceinline__ float snoise2(const NoiseLds& T, float px, float py) 
 __device__ __forceinline__ float corner3(const NoiseLds& T, int texel, int zrow, float fx, float fy, float fz) {
     const int w = T.perm[texel];
     const float r = fmaxf(0.6f - fmaf(fz, fz, fmaf(fy, fy, fx * fx)), 0.f);
-    const float4 gr = T.grad[T.perm[zrow + w] & 15];
+    const int j = T.perm[zrow + w] & 15;
+    const float2 g = T.gxy[j];
     const float t = r * r;
-    return (t * t) * fmaf(gr.z, fz, fmaf(gr.y, fy, gr.x * fx));
+    return (t * t) * fmaf(T.gz[j], fz, fmaf(g.y, fy, g.x * fx));
 }
 
 // snoise3, cl/simplex.cl:159-230 (simplex ranking of :74-97).
