@@ -405,12 +405,26 @@ struct EdgeOp {
     }
     using Item = int;
     static __device__ int limit(const SMesh& M) { return M.nv; }
+    // the count pass keeps each vertex's count (the bucket cursors are free by now) for
+    // the write pass, which then walks each bucket once
     static __device__ int count(const SG& g, const SMesh& M, int i, Item&) {
-        return walk(g, M, i, [](int, int) {});
+        const int kept = walk(g, M, i, [](int, int) {});
+        g.ucnt[M.vbase + i] = kept;
+        return kept;
     }
     static __device__ void emit(const SG& g, const SMesh& M, int i, int pos, const Item&) {
         int2* out = g.edge_a + M.ebase + pos;
         walk(g, M, i, [&](int q, int mn) { out[q] = make_int2(mn, i); });
+    }
+};
+
+// The write pass of the candidate edges: counts kept by EdgeOp's count pass.
+struct EdgeEmitOp {
+    using Item = int;
+    static __device__ int limit(const SMesh& M) { return M.nv; }
+    static __device__ int count(const SG& g, const SMesh& M, int i, Item&) { return g.ucnt[M.vbase + i]; }
+    static __device__ void emit(const SG& g, const SMesh& M, int i, int pos, const Item& it) {
+        EdgeOp::emit(g, M, i, pos, it);
     }
 };
 
@@ -1132,7 +1146,7 @@ static int run_group(SimpState& st, const SimpJob* jobs, int n, const lvn_simpli
     hipLaunchKernelGGL(k_bucket_sort, GV, B, 0, stream, g);
     hipLaunchKernelGGL(k_tile_count<EdgeOp>, GV, B, 0, stream, g, g.vt, g.nvt);
     hipLaunchKernelGGL((k_mesh_scan<&SMesh::vt0, &SMesh::vtn, &SMesh::E, &SMesh::one>), GW, B, 0, stream, g);
-    hipLaunchKernelGGL(k_tile_emit<EdgeOp>, GV, B, 0, stream, g, g.vt, g.nvt);
+    hipLaunchKernelGGL(k_tile_emit<EdgeEmitOp>, GV, B, 0, stream, g, g.vt, g.nvt);
     LVN_HIP(hipGetLastError());
     for (int it = 0; it < opt.max_iterations; ++it) {
         LVN_HIP(hipMemsetAsync(st.d_active, 0, sizeof(int), stream));
