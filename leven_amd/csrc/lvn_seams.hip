@@ -6,9 +6,11 @@
 // mesh (Octree_GenerateMesh, src/octree.cpp:520-550: GenerateVertexIndices
 // :196-231, ContourCellProc / FaceProc / EdgeProc / ProcessEdge :235-474).
 //
-// One workgroup stitches one seam; a launch covers every seam of a request.
-// The reference builds the octree with a hash map per level and contours it by
-// recursion; here each phase is data-parallel over the workgroup:
+// Two forms of one pass: one workgroup stitches one seam (k_seam, a launch covers every
+// seam of a request), or each step below runs as one launch over the elements of all
+// seams (k_sg_*: a seam spread over the whole GPU, for calls of few seams).  The
+// reference builds the octree with a hash map per level and contours it by recursion;
+// here each phase is data-parallel:
 //   1. select: every offered seam node is filtered and its leaf key
 //      (level << 48 | Morton code in the root) inserted into an open-addressing
 //      table in HBM (a repeated node keeps the last offer, as the reference's
@@ -28,6 +30,7 @@
 #include "lvn_runtime.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -816,6 +819,645 @@ __global__ __launch_bounds__(256) void k_seam_pack(const SeamJobDev* jobs, const
 }
 
 // ---------------------------------------------------------------------------
+// The seam pass as grid-wide steps over all seams of a launch
+// ---------------------------------------------------------------------------
+//
+// The steps of one seam above, each run as one launch over the elements of every seam.
+// Work is cut into tiles of 1024 elements that belong to one seam (offered nodes of one
+// list, hash slots, leaves, nodes, the nodes of one level, pending calls); a tile list
+// is made on the device from each seam's counts, so lists whose length only the device
+// knows (a level's nodes, a round's calls) need no read-back.  A seam's counts, level
+// starts, call-list state and status live in its SeamRun record.  Counters shared by a
+// tile's threads are reserved once per tile (LDS histograms, workgroup scans) or once per
+// wave, so the per-seam atomics stay few.
+
+constexpr int kSgTile = 1024;
+constexpr int kSgThreads = 256;
+constexpr int kSgPer = kSgTile / kSgThreads;
+constexpr int kSgGrid = 2048;  // workgroups of the launches whose tile count only the device knows
+constexpr int kSgLevels = 33;
+
+struct SeamRun {
+    int nodes, nleaves, nnodes, status;
+    int root, linked, rounds, stage;  // stage: calls reserved in the staging list this round
+    int n, more, pad0, pad1;  // pending calls, 1 = calls still expand (2 = one was written this round)
+    int tile0, ntiles;        // this seam's tiles in the current round's list
+    int lev[kSgLevels + 1];   // node counts per level (lev[l + 1]), then level starts
+    int fill[kSgLevels];      // placement cursors per level
+};
+
+// Tile list sizes and counters of one grid pass, in one int array.
+enum : int {
+    kSgCntHash = 0,
+    kSgCntOffered = 1,
+    kSgCntLeaf = 2,
+    kSgCntNode = 3,
+    kSgCntFinal = 4,
+    kSgCntLevel = 8,                          // + level
+    kSgCntRound = kSgCntLevel + kSgLevels,        // + round: tiles of the round's list
+    kSgCntActive = kSgCntRound + kMaxRounds + 1,  // + round: seams whose calls still expand
+    kSgCounters = kSgCntActive + kMaxRounds,
+};
+
+struct SeamGrid {
+    const SeamJobDev* jobs;
+    const SeamSrcDev* srcs;
+    char* scratch;
+    SeamRun* run;
+    int* cnt;
+    lvn_vertex* out_v;
+    int* out_t;
+    int m;
+};
+
+__device__ __forceinline__ SeamScratch sg_scratch(const SeamGrid& g, int s) {
+    return seam_scratch(g.scratch + g.jobs[s].scratch, g.jobs[s]);
+}
+
+// Loop over the tiles of a list: tl = (seam, first element, source, 0).
+#define LVN_SG_TILES(tiles, ntiles)                     \
+    for (int tk = blockIdx.x; tk < (ntiles); tk += gridDim.x) \
+        if (const int4 tl = (tiles)[tk]; true)
+
+// Exclusive scan over the workgroup; *total = the sum.
+__device__ __forceinline__ int sg_scan(int v, int* total) {
+    __shared__ int wsum[kSgThreads / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int x = wave_inclusive(v);
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kSgThreads / 64; ++i) {
+        base += i < w ? wsum[i] : 0;
+        tot += wsum[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return base + x - v;
+}
+
+// `cnt` consecutive indices of *counter for each thread, reserved with one atomic per
+// workgroup; returns the thread's first.
+__device__ __forceinline__ int sg_reserve(int cnt, int* counter) {
+    __shared__ int base;
+    int total;
+    const int excl = sg_scan(cnt, &total);
+    if (threadIdx.x == 0) base = total ? atomicAdd(counter, total) : 0;
+    __syncthreads();
+    const int b = base;
+    __syncthreads();
+    return b + excl;
+}
+
+__device__ __forceinline__ int lane_rank(uint64_t mask) {
+    return __popcll(mask & ((1ull << (threadIdx.x & 63)) - 1));
+}
+
+// Tile lists (one workgroup of 1024 threads scans the seams; workgroup b of a level list
+// makes level b's list at tiles + b * stride).  what: 0 a level's nodes, 1 a round's
+// pending calls of the seams still expanding, 2 the final quads, 3 hash slots, 4 leaves,
+// 5 nodes, 6 offered nodes (one list per source).
+__global__ __launch_bounds__(1024) void k_sg_tiles(SeamGrid g, int what, int4* tiles, int stride, int* total) {
+    __shared__ int sw[kSeamWaves];
+    __shared__ int carry;
+    const int l = blockIdx.x;
+    int4* out = tiles + (size_t)l * stride;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int b = 0; b < g.m; b += 1024) {
+        const int s = b + threadIdx.x;
+        int cnt = 0;
+        if (s < g.m) {
+            const SeamRun& R = g.run[s];
+            const SeamJobDev& J = g.jobs[s];
+            auto nt = [](int n) { return (n + kSgTile - 1) / kSgTile; };
+            switch (what) {
+            case 0: cnt = R.status == 0 ? nt(R.lev[l + 1] - R.lev[l]) : 0; break;
+            case 1: cnt = R.status == 0 && R.linked && R.more ? nt(R.n) : 0; break;
+            case 2: cnt = R.status == 0 && R.linked ? nt(R.n) : 0; break;
+            case 3: cnt = nt(J.hash_cap); break;
+            case 4: cnt = nt(J.leaf_cap); break;
+            case 5: cnt = nt(J.node_cap); break;
+            default:
+                for (int si = 0; si < J.src_count; ++si) cnt += nt(g.srcs[J.src_begin + si].count);
+            }
+        }
+        int tot;
+        const int at = carry + block_exclusive(cnt, sw, &tot);
+        if (s < g.m && what == 6) {
+            const SeamJobDev& J = g.jobs[s];
+            int k = at;
+            for (int si = 0; si < J.src_count; ++si)
+                for (int j = 0; j < g.srcs[J.src_begin + si].count; j += kSgTile) out[k++] = make_int4(s, j, si, 0);
+        } else if (s < g.m) {
+            for (int t = 0; t < cnt; ++t) out[at + t] = make_int4(s, t * kSgTile, 0, 0);
+            if (what == 1) {
+                g.run[s].tile0 = at;
+                g.run[s].ntiles = cnt;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) total[l] = carry;
+}
+
+// Seam records and round counters cleared; every hash slot emptied.
+__global__ __launch_bounds__(kSgThreads) void k_sg_init(SeamGrid g, const int4* tiles) {
+    for (int s = blockIdx.x * kSgThreads + threadIdx.x; s < g.m; s += gridDim.x * kSgThreads) {
+        SeamRun& R = g.run[s];
+        R.nodes = R.nleaves = R.nnodes = R.status = 0;
+        R.root = -1;
+        R.linked = R.rounds = R.stage = 0;
+        R.n = R.more = R.pad0 = R.pad1 = 0;
+        R.tile0 = R.ntiles = 0;
+        for (int k = 0; k <= kSgLevels; ++k) R.lev[k] = 0;
+        for (int k = 0; k < kSgLevels; ++k) R.fill[k] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < kMaxRounds) g.cnt[kSgCntActive + threadIdx.x] = 0;
+    LVN_SG_TILES(tiles, g.cnt[kSgCntHash]) {
+        const SeamJobDev& J = g.jobs[tl.x];
+        SeamScratch S = sg_scratch(g, tl.x);
+        for (int r = 0; r < kSgPer; ++r) {
+            const int i = tl.y + threadIdx.x + r * kSgThreads;
+            if (i >= J.hash_cap) continue;
+            S.keys[i] = kEmptyKey;
+            S.pick[i] = 0;
+            S.vals[i] = -1;
+            S.leaf[i] = 0;
+        }
+    }
+}
+
+// 1. SelectSeamNodes (src/clipmap.cpp:546-569) over the offered nodes of one list.
+__global__ __launch_bounds__(kSgThreads) void k_sg_select(SeamGrid g, const int4* tiles) {
+    LVN_SG_TILES(tiles, g.cnt[kSgCntOffered]) {
+        const int s = tl.x, si = tl.z;
+        const SeamJobDev& J = g.jobs[s];
+        SeamRun& R = g.run[s];
+        SeamScratch S = sg_scratch(g, s);
+        const SeamSrcDev src = g.srcs[J.src_begin + si];
+        const int leaf_size = src.size / J.vpc;
+        const int root_size = 2 * J.host_size;
+        const int root_levels = __builtin_ctz((unsigned)root_size);
+        const int mask = J.hash_cap - 1;
+        for (int r = 0; r < kSgPer; ++r) {
+            const int j = tl.y + threadIdx.x + r * kSgThreads;
+            if (j >= src.count) continue;
+            int mn[3];
+            if (!select_leaf(J, src, src.nodes[j], mn)) continue;
+            bool aligned = pow2(leaf_size) && leaf_size < root_size;
+            for (int a = 0; a < 3; ++a) aligned &= ((mn[a] - J.host_min[a]) & (leaf_size - 1)) == 0;
+            if (!aligned) {
+                set_status(&R.status, LVN_ERR_INVALID);
+                continue;
+            }
+            const int level = root_levels - __builtin_ctz((unsigned)leaf_size);
+            const uint64_t key = (uint64_t)level << 48 |
+                                 morton((mn[0] - J.host_min[0]) / leaf_size, (mn[1] - J.host_min[1]) / leaf_size,
+                                        (mn[2] - J.host_min[2]) / leaf_size, level);
+            bool fresh;
+            const int slot = hash_insert(S.keys, mask, key, &fresh);
+            if (slot < 0) {
+                set_status(&R.status, LVN_ERR_OVERFLOW);
+                continue;
+            }
+            atomicMax(reinterpret_cast<unsigned long long*>(S.pick + slot),
+                      (unsigned long long)((uint64_t)(si + 1) << 32 | (uint32_t)j));
+        }
+    }
+}
+
+// Leaves: one node per distinct key, carrying the last offer.
+__global__ __launch_bounds__(kSgThreads) void k_sg_leaves(SeamGrid g, const int4* tiles) {
+    LVN_SG_TILES(tiles, g.cnt[kSgCntHash]) {
+        const SeamJobDev& J = g.jobs[tl.x];
+        SeamRun& R = g.run[tl.x];
+        SeamScratch S = sg_scratch(g, tl.x);
+        const int i0 = tl.y + threadIdx.x * kSgPer;
+        uint64_t key[kSgPer];
+        int cnt = 0;
+#pragma unroll
+        for (int r = 0; r < kSgPer; ++r) {
+            key[r] = i0 + r < J.hash_cap ? S.keys[i0 + r] : kEmptyKey;
+            cnt += key[r] != kEmptyKey;
+        }
+        int idx = sg_reserve(cnt, &R.nodes);
+#pragma unroll
+        for (int r = 0; r < kSgPer; ++r) {
+            if (key[r] == kEmptyKey) continue;
+            const int i = i0 + r;
+            if (idx >= J.node_cap) {
+                set_status(&R.status, LVN_ERR_OVERFLOW);
+                continue;
+            }
+            const uint64_t p = S.pick[i];
+            const SeamSrcDev src = g.srcs[J.src_begin + (int)(p >> 32) - 1];
+            const lvn_seam_node& sn = src.nodes[(uint32_t)p];
+            SNode n;
+            n.size = src.size / J.vpc;
+            for (int a = 0; a < 3; ++a) n.min[a] = sn.local_min_and_material[a] * n.size + src.min[a];
+            n.parent = -1;
+            n.level = (int)(key[r] >> 48);
+            n.mat = sn.local_min_and_material[3];
+            n.count = 1;
+            n.offset = 0;
+            n.type = kNodeLeaf;
+            n.code = key[r] & ((1ull << 48) - 1);
+            n.payload = p;
+            S.nodes[idx] = n;
+            for (int c = 0; c < 8; ++c) S.ch[8 * idx + c] = -1;
+            S.shape[idx] = node_shape(n.min, J.host_min, n.size, false, n.mat);
+            S.vals[i] = idx;
+            S.leaf[i] = 1;
+            ++idx;
+        }
+    }
+}
+
+// Per seam between steps.  0: the leaves are counted; 1: the ancestors too; 2: level
+// counts become level starts and the root is found; 3: the contouring list starts with
+// the root's cell call.
+__global__ void k_sg_seam(SeamGrid g, int step) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.m) return;
+    const SeamJobDev& J = g.jobs[s];
+    SeamRun& R = g.run[s];
+    if (step == 0) {
+        R.nleaves = min(R.nodes, J.node_cap);
+    } else if (step == 1) {
+        R.nnodes = R.nleaves ? min(R.nodes, J.node_cap) : 0;
+    } else if (step == 2) {
+        for (int l = 1; l <= kSgLevels; ++l) R.lev[l] += R.lev[l - 1];
+        for (int l = 0; l < kSgLevels; ++l) R.fill[l] = R.lev[l];
+        if (R.status == 0 && R.nleaves > 0) {
+            SeamScratch S = sg_scratch(g, s);
+            const int slot = hash_find(S.keys, J.hash_cap - 1, 0ull);
+            R.root = slot < 0 ? -1 : S.vals[slot];
+        }
+    } else {
+        R.linked = R.status == 0 && R.nleaves > 0 && R.root >= 0;
+        R.n = 0;
+        if (R.linked) {
+            SeamScratch S = sg_scratch(g, s);
+            if (S.nodes[R.root].type == kNodeInternal && ((S.shape[R.root] >> 56) & 1) == 0) {
+                S.list0[0] = Call{{R.root, -1, -1, -1}, kCallCell};
+                R.n = 1;
+            }
+        }
+        R.more = R.n > 0;
+    }
+}
+
+// 2. Ancestors (ConstructParents, src/octree.cpp:29-88): a leaf inserts its parent chain
+// up to the first parent already present, a wave's lanes one level per step so that the
+// wave reserves the new nodes with one atomic.  A parent key held by a leaf means two
+// selected nodes overlap.
+__global__ __launch_bounds__(kSgThreads) void k_sg_ancestors(SeamGrid g, const int4* tiles) {
+    LVN_SG_TILES(tiles, g.cnt[kSgCntLeaf]) {
+        const SeamJobDev& J = g.jobs[tl.x];
+        SeamRun& R = g.run[tl.x];
+        if (R.status != 0) continue;
+        SeamScratch S = sg_scratch(g, tl.x);
+        const int mask = J.hash_cap - 1;
+        const int root_size = 2 * J.host_size;
+        const int nleaves = R.nleaves;
+        for (int r = 0; r < kSgPer; ++r) {
+            const int i = tl.y + threadIdx.x + r * kSgThreads;
+            const bool leaf = i < nleaves;
+            const int level = leaf ? S.nodes[i].level : 0;
+            const uint64_t code = leaf ? S.nodes[i].code : 0;
+            int l = level - 1;
+            bool act = leaf && l >= 0;
+            while (__ballot(act)) {
+                bool fresh = false;
+                int slot = -1;
+                uint64_t c = 0;
+                if (act) {
+                    c = code >> (3 * (level - l));
+                    slot = hash_insert(S.keys, mask, (uint64_t)l << 48 | c, &fresh);
+                    if (slot < 0) {
+                        set_status(&R.status, LVN_ERR_OVERFLOW);
+                        act = false;
+                    } else if (!fresh) {
+                        if (load_l2(S.leaf + slot)) set_status(&R.status, LVN_ERR_INVALID);
+                        act = false;
+                    }
+                }
+                const uint64_t fm = __ballot(fresh);
+                if (!fm) continue;
+                const int leader = __ffsll((unsigned long long)fm) - 1;
+                int base = 0;
+                if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(&R.nodes, __popcll(fm));
+                base = __shfl(base, leader, 64);
+                if (!fresh) continue;
+                const int idx = base + lane_rank(fm);
+                if (idx >= J.node_cap) {
+                    set_status(&R.status, LVN_ERR_OVERFLOW);
+                    act = false;
+                    continue;
+                }
+                SNode n;
+                n.size = root_size >> l;
+                int xyz[3];
+                unmorton(c, l, xyz);
+                for (int a = 0; a < 3; ++a) n.min[a] = J.host_min[a] + xyz[a] * n.size;
+                n.parent = -1;
+                n.level = l;
+                n.mat = 0;
+                n.count = 0;
+                n.offset = 0;
+                n.type = kNodeInternal;
+                n.code = c;
+                n.payload = 0;
+                S.nodes[idx] = n;
+                for (int q = 0; q < 8; ++q) S.ch[8 * idx + q] = -1;
+                S.shape[idx] = node_shape(n.min, J.host_min, n.size, true, 0);
+                S.vals[slot] = idx;
+                --l;
+                act = l >= 0;
+            }
+        }
+    }
+}
+
+// 3. Links to parents, and the nodes of each level counted (an LDS histogram per tile).
+__global__ __launch_bounds__(kSgThreads) void k_sg_links(SeamGrid g, const int4* tiles) {
+    __shared__ int hist[kSgLevels];
+    __shared__ int skip;
+    LVN_SG_TILES(tiles, g.cnt[kSgCntNode]) {
+        const SeamJobDev& J = g.jobs[tl.x];
+        SeamRun& R = g.run[tl.x];
+        // the status can change while the tile runs (set by other tiles): one thread
+        // decides for the workgroup, whose barriers follow
+        if (threadIdx.x == 0) skip = load_l2(&R.status) != 0 || tl.y >= R.nnodes;
+        if (threadIdx.x < kSgLevels) hist[threadIdx.x] = 0;
+        __syncthreads();
+        if (skip) {
+            __syncthreads();
+            continue;
+        }
+        SeamScratch S = sg_scratch(g, tl.x);
+        const int nnodes = R.nnodes;
+        for (int r = 0; r < kSgPer; ++r) {
+            const int i = tl.y + threadIdx.x + r * kSgThreads;
+            if (i >= nnodes) continue;
+            const int level = S.nodes[i].level;
+            atomicAdd(&hist[level], 1);
+            if (level == 0) continue;
+            const uint64_t c = S.nodes[i].code;
+            const int slot = hash_find(S.keys, J.hash_cap - 1, (uint64_t)(level - 1) << 48 | (c >> 3));
+            const int p = slot < 0 ? -1 : S.vals[slot];
+            if (p < 0) {  // every parent was inserted: not reached
+                set_status(&R.status, LVN_ERR_OVERFLOW);
+                continue;
+            }
+            S.nodes[i].parent = p;
+            S.ch[8 * p + (c & 7)] = i;
+        }
+        __syncthreads();
+        if (threadIdx.x < kSgLevels && hist[threadIdx.x]) atomicAdd(&R.lev[threadIdx.x + 1], hist[threadIdx.x]);
+        __syncthreads();
+    }
+}
+
+// Nodes bucketed by level: a tile reserves its share of each level once.
+__global__ __launch_bounds__(kSgThreads) void k_sg_order(SeamGrid g, const int4* tiles) {
+    __shared__ int hist[kSgLevels], base[kSgLevels];
+    LVN_SG_TILES(tiles, g.cnt[kSgCntNode]) {
+        SeamRun& R = g.run[tl.x];
+        if (R.status != 0 || tl.y >= R.nnodes) continue;
+        SeamScratch S = sg_scratch(g, tl.x);
+        if (threadIdx.x < kSgLevels) hist[threadIdx.x] = 0;
+        __syncthreads();
+        const int nnodes = R.nnodes;
+        int lv[kSgPer];
+#pragma unroll
+        for (int r = 0; r < kSgPer; ++r) {
+            const int i = tl.y + threadIdx.x + r * kSgThreads;
+            lv[r] = i < nnodes ? S.nodes[i].level : -1;
+            if (lv[r] >= 0) atomicAdd(&hist[lv[r]], 1);
+        }
+        __syncthreads();
+        if (threadIdx.x < kSgLevels) {
+            base[threadIdx.x] = hist[threadIdx.x] ? atomicAdd(&R.fill[threadIdx.x], hist[threadIdx.x]) : 0;
+            hist[threadIdx.x] = 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kSgPer; ++r)
+            if (lv[r] >= 0) S.order[base[lv[r]] + atomicAdd(&hist[lv[r]], 1)] = tl.y + threadIdx.x + r * kSgThreads;
+        __syncthreads();
+    }
+}
+
+// One level: leaf counts added to the parents (deepest level first), or depth-first ranks
+// handed from internal nodes to their children in child order (top level first).
+__global__ __launch_bounds__(kSgThreads) void k_sg_level(SeamGrid g, const int4* tiles, int l, int ranks) {
+    LVN_SG_TILES(tiles, g.cnt[kSgCntLevel + l]) {
+        SeamRun& R = g.run[tl.x];
+        if (ranks && !R.linked) continue;
+        SeamScratch S = sg_scratch(g, tl.x);
+        const int first = R.lev[l], end = R.lev[l + 1];
+        for (int r = 0; r < kSgPer; ++r) {
+            const int k = first + tl.y + threadIdx.x + r * kSgThreads;
+            if (k >= end) continue;
+            const int i = S.order[k];
+            if (!ranks) {
+                atomicAdd(&S.nodes[S.nodes[i].parent].count, S.nodes[i].count);
+            } else if (S.nodes[i].type == kNodeInternal) {
+                int off = S.nodes[i].offset;
+                for (int c = 0; c < 8; ++c) {
+                    const int ch = S.ch[8 * i + c];
+                    if (ch < 0) continue;
+                    S.nodes[ch].offset = off;
+                    off += S.nodes[ch].count;
+                }
+            }
+        }
+    }
+}
+
+// GenerateVertexIndices (src/octree.cpp:196-231): vertex = rank of the leaf.
+__global__ __launch_bounds__(kSgThreads) void k_sg_vertices(SeamGrid g, const int4* tiles) {
+    LVN_SG_TILES(tiles, g.cnt[kSgCntLeaf]) {
+        const SeamJobDev& J = g.jobs[tl.x];
+        SeamRun& R = g.run[tl.x];
+        if (!R.linked) continue;
+        SeamScratch S = sg_scratch(g, tl.x);
+        const int nleaves = R.nleaves;
+        for (int r = 0; r < kSgPer; ++r) {
+            const int i = tl.y + threadIdx.x + r * kSgThreads;
+            if (i >= nleaves) continue;
+            const uint64_t p = S.nodes[i].payload;
+            const lvn_seam_node& sn = g.srcs[J.src_begin + (int)(p >> 32) - 1].nodes[(uint32_t)p];
+            lvn_vertex v;
+            v.pos[0] = sn.pos[0];
+            v.pos[1] = sn.pos[1];
+            v.pos[2] = sn.pos[2];
+            v.pos[3] = 0.f;
+            v.normal[0] = sn.normal[0];
+            v.normal[1] = sn.normal[1];
+            v.normal[2] = sn.normal[2];
+            v.normal[3] = 0.f;
+            v.colour[0] = J.colour[0];
+            v.colour[1] = J.colour[1];
+            v.colour[2] = J.colour[2];
+            v.colour[3] = (float)(S.nodes[i].mat >> 8);
+            g.out_v[J.v_off + S.nodes[i].offset] = v;
+        }
+    }
+}
+
+// 4. One contouring round: every pending call replaced by the calls it makes.  Thread t
+// of a tile expands calls t * 4 .. t * 4 + 3 of it: it counts their calls, the workgroup
+// scans the counts, reserves the tile's block of the staging list (list1) with one
+// atomic and writes the calls at once, their nodes still in cache.  The blocks are put
+// in call order by k_sg_round_copy after k_sg_round_next has scanned each seam's tile
+// counts in tile order.
+__global__ __launch_bounds__(kSgThreads) void k_sg_round(SeamGrid g, const int4* tiles, int round, int* tsum,
+                                                          int* tbase) {
+    __shared__ int base;
+    LVN_SG_TILES(tiles, g.cnt[kSgCntRound + round]) {
+        const SeamJobDev& J = g.jobs[tl.x];
+        SeamRun& R = g.run[tl.x];
+        SeamScratch S = sg_scratch(g, tl.x);
+        const Call* A = S.list0;
+        const int n = R.n;
+        const int i0 = tl.y + threadIdx.x * kSgPer;
+        int cnt = 0;
+        for (int q = 0; q < kSgPer && i0 + q < n; ++q) {
+            Expander e{S.ch, S.shape, S.list1, 0, 0, 0, false};
+            e.expand(A[i0 + q]);
+            cnt += e.k;
+        }
+        int total;
+        int at = sg_scan(cnt, &total);
+        if (threadIdx.x == 0) {
+            base = total ? atomicAdd(&R.stage, total) : 0;
+            tsum[tk] = total;
+            tbase[tk] = base;
+        }
+        __syncthreads();
+        at += base;
+        __syncthreads();  // base is rewritten by the next tile
+        if (cnt == 0 || at + cnt > J.list_cap) continue;  // the seam's total then exceeds the cap
+        bool m = false;
+        for (int q = 0; q < kSgPer && i0 + q < n; ++q) {
+            Expander w{S.ch, S.shape, S.list1, at, 0, 0, true};
+            w.expand(A[i0 + q]);
+            at += w.k;
+            m |= w.more != 0;
+        }
+        if (m) R.more = 2;
+    }
+}
+
+// After a round, per seam (one workgroup of 1024 threads, a thread per seam): the tile
+// counts of the round scanned in tile order into the blocks' places in the next list, the
+// seam's round closed (the next list's length; the seam stays in the rounds while a
+// written call still expands, counted in the round's active counter; an overflow of the
+// list or of the round limit ends it), and the next round's tile list made.
+__global__ __launch_bounds__(1024) void k_sg_round_next(SeamGrid g, int round, const int* tsum, int* toff,
+                                                         int4* next) {
+    __shared__ int sw[kSeamWaves];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    int active = 0;
+    for (int b = 0; b < g.m; b += 1024) {
+        const int s = b + threadIdx.x;
+        int cnt = 0;
+        if (s < g.m) {
+            SeamRun& R = g.run[s];
+            const SeamJobDev& J = g.jobs[s];
+            if (R.status == 0 && R.linked && R.more && R.ntiles > 0) {
+                int sum = 0;
+                for (int t = 0; t < R.ntiles; ++t) {
+                    const int v = tsum[R.tile0 + t];
+                    toff[R.tile0 + t] = sum;
+                    sum += v;
+                }
+                R.rounds += 1;
+                R.stage = 0;
+                if (sum > J.list_cap || R.rounds >= kMaxRounds) {
+                    set_status(&R.status, LVN_ERR_OVERFLOW);
+                } else {
+                    R.n = sum;
+                    R.more = R.more == 2;
+                    active += R.more;
+                }
+            }
+            cnt = R.status == 0 && R.linked && R.more ? (R.n + kSgTile - 1) / kSgTile : 0;
+        }
+        int tot;
+        const int at = carry + block_exclusive(cnt, sw, &tot);
+        if (s < g.m) {
+            for (int t = 0; t < cnt; ++t) next[at + t] = make_int4(s, t * kSgTile, 0, 0);
+            g.run[s].tile0 = at;
+            g.run[s].ntiles = cnt;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) g.cnt[kSgCntRound + round + 1] = carry;
+    if (active) atomicAdd(&g.cnt[kSgCntActive + round], active);
+}
+
+// The round's blocks copied from the staging list into the seam's list in call order.
+__global__ __launch_bounds__(kSgThreads) void k_sg_round_copy(SeamGrid g, const int4* tiles, int round,
+                                                               const int* tsum, const int* tbase, const int* toff) {
+    LVN_SG_TILES(tiles, g.cnt[kSgCntRound + round]) {
+        if (g.run[tl.x].status != 0) continue;
+        SeamScratch S = sg_scratch(g, tl.x);
+        const int n = tsum[tk];
+        const Call* src = S.list1 + tbase[tk];
+        Call* dst = S.list0 + toff[tk];
+        for (int i = threadIdx.x; i < n; i += kSgThreads) dst[i] = src[i];
+    }
+}
+
+// Triangles of the quads, in list order (ContourProcessEdge's two triangles), and each
+// seam's result.
+__global__ __launch_bounds__(kSgThreads) void k_sg_triangles(SeamGrid g, const int4* tiles, SeamResDev* res) {
+    LVN_SG_TILES(tiles, g.cnt[kSgCntFinal]) {
+        const SeamJobDev& J = g.jobs[tl.x];
+        SeamRun& R = g.run[tl.x];
+        SeamScratch S = sg_scratch(g, tl.x);
+        const Call* Q = S.list0;
+        const int n = R.n;
+        for (int r = 0; r < kSgPer; ++r) {
+            const int i = tl.y + threadIdx.x + r * kSgThreads;
+            if (i >= n) continue;
+            const Call c = Q[i];
+            int v[4];
+            for (int j = 0; j < 4; ++j) v[j] = S.nodes[c.n[j]].offset;
+            int* t = g.out_t + 3 * (J.t_off + 2 * (uint64_t)i);
+            if (c.meta & 16) {
+                t[0] = v[0]; t[1] = v[3]; t[2] = v[1];
+                t[3] = v[0]; t[4] = v[2]; t[5] = v[3];
+            } else {
+                t[0] = v[0]; t[1] = v[1]; t[2] = v[3];
+                t[3] = v[0]; t[4] = v[3]; t[5] = v[2];
+            }
+        }
+    }
+    for (int s = blockIdx.x * kSgThreads + threadIdx.x; s < g.m; s += gridDim.x * kSgThreads) {
+        const SeamRun& R = g.run[s];
+        const bool ok = R.status == 0 && R.linked;
+        SeamResDev r;
+        r.status = R.status;
+        r.nv = ok ? R.nleaves : 0;
+        r.nt = ok ? 2 * R.n : 0;
+        r.rounds = R.rounds;
+        res[s] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 
@@ -877,7 +1519,8 @@ struct SeamLoc {
 // attempt writes its own output buffers, so every finished seam stays addressable.
 struct SeamState {
     DevBuf in, jobs, srcs, scratch, res, offs, pv, pt, counts;
-    HostBuf h_in, h_jobs, h_srcs, h_res, h_pv, h_pt, h_counts;
+    DevBuf g_run, g_tiles, g_tsum, g_cnt;  // grid pass: seam records, tile lists, round scans, counters
+    HostBuf h_in, h_jobs, h_srcs, h_res, h_pv, h_pt, h_counts, h_active;
     std::vector<std::unique_ptr<DevBuf>> out_v, out_t;  // per attempt
     std::vector<SeamLoc> loc;                            // per seam of the last call
 };
@@ -946,6 +1589,115 @@ int resolve_batch(void* owner, const lvn_seam_source& q, SeamSrcDev& d) {
     d.nodes = reinterpret_cast<const lvn_seam_node*>(c.d_arena + (uint64_t)it->second * c.L.slot_bytes + c.L.seams);
     d.count = k.seams;
     return LVN_OK;
+}
+
+// The grid-wide seam pass over m seams whose jobs (caps, scratch and output offsets) are
+// in device memory as d_jobs and on the host as h_jobs.  Launch sequence: tile lists of
+// the sizes the host knows, clear, select, leaves, ancestors, links, level buckets, leaf
+// counts (deepest level first), ranks (top level first), vertices, contouring rounds,
+// triangles.  The rounds run in batches without a read-back; after a batch the host reads
+// how many seams still expand.  A call descends at least one level per round, so the
+// first batch (deepest possible level + 3 rounds) normally finishes every seam.
+int run_seam_grid(Ctx* c, SeamState& st, int m, const SeamJobDev* h_jobs, const std::vector<SeamSrcDev>& srcs,
+                  const SeamJobDev* d_jobs, lvn_vertex* ov, int* ot, SeamResDev* d_res) {
+    auto nt = [](int64_t n) { return (n + kSgTile - 1) / kSgTile; };
+    int64_t th = 0, to = 0, tl = 0, tn = 0, tr = 0;
+    int lmax = 0;
+    for (int k = 0; k < m; ++k) {
+        const SeamJobDev& J = h_jobs[k];
+        th += nt(J.hash_cap);
+        tl += nt(J.leaf_cap);
+        tn += nt(J.node_cap);
+        tr += nt(J.list_cap);
+        const int root_size = 2 * J.host_size, root_levels = __builtin_ctz((unsigned)root_size);
+        for (int si = 0; si < J.src_count; ++si) {
+            const SeamSrcDev& q = srcs[J.src_begin + si];
+            to += nt(q.count);
+            const int leaf = q.size / J.vpc;
+            if (host_pow2(leaf) && leaf < root_size) lmax = std::max(lmax, root_levels - __builtin_ctz((unsigned)leaf));
+        }
+    }
+    lmax = std::min(lmax, kSgLevels - 1);
+    const int64_t o_off = th, o_leaf = o_off + to, o_node = o_leaf + tl, o_lev = o_node + tn,
+                  o_round = o_lev + (lmax + 1) * tn, o_fin = o_round + 2 * tr, total = o_fin + tr;
+    int r;
+    if ((r = st.g_run.ensure(m * sizeof(SeamRun))) || (r = st.g_tiles.ensure(std::max<int64_t>(total, 1) * sizeof(int4))) ||
+        (r = st.g_tsum.ensure(3 * std::max<int64_t>(tr, 1) * sizeof(int))) || (r = st.g_cnt.ensure(kSgCounters * sizeof(int))) ||
+        (r = st.h_active.ensure(sizeof(int))))
+        return r;
+    int* cnt = static_cast<int*>(st.g_cnt.p);
+    int* tsum = static_cast<int*>(st.g_tsum.p);
+    int* toff = tsum + std::max<int64_t>(tr, 1);
+    int* tbase = toff + std::max<int64_t>(tr, 1);
+    int4* T = static_cast<int4*>(st.g_tiles.p);
+    SeamGrid g{d_jobs, static_cast<const SeamSrcDev*>(st.srcs.p), static_cast<char*>(st.scratch.p),
+               static_cast<SeamRun*>(st.g_run.p), cnt, ov, ot, m};
+    const hipStream_t sm = c->stream;
+    auto grid = [](int64_t tiles) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(tiles, 1 << 20))); };
+    const dim3 per_seam((m + kSgThreads - 1) / kSgThreads), block(kSgThreads), dev_grid(kSgGrid);
+#define LVN_SG(...)                              \
+    do {                                          \
+        hipLaunchKernelGGL(__VA_ARGS__);          \
+        LVN_HIP(hipGetLastError());               \
+    } while (0)
+    LVN_SG(k_sg_tiles, dim3(1), dim3(1024), 0, sm, g, 3, T, 0, cnt + kSgCntHash);
+    LVN_SG(k_sg_tiles, dim3(1), dim3(1024), 0, sm, g, 6, T + o_off, 0, cnt + kSgCntOffered);
+    LVN_SG(k_sg_tiles, dim3(1), dim3(1024), 0, sm, g, 4, T + o_leaf, 0, cnt + kSgCntLeaf);
+    LVN_SG(k_sg_tiles, dim3(1), dim3(1024), 0, sm, g, 5, T + o_node, 0, cnt + kSgCntNode);
+    LVN_SG(k_sg_init, grid(std::max<int64_t>(th, per_seam.x)), block, 0, sm, g, T);
+    LVN_SG(k_sg_select, grid(to), block, 0, sm, g, T + o_off);
+    LVN_SG(k_sg_leaves, grid(th), block, 0, sm, g, T);
+    LVN_SG(k_sg_seam, per_seam, block, 0, sm, g, 0);
+    LVN_SG(k_sg_ancestors, grid(tl), block, 0, sm, g, T + o_leaf);
+    LVN_SG(k_sg_seam, per_seam, block, 0, sm, g, 1);
+    LVN_SG(k_sg_links, grid(tn), block, 0, sm, g, T + o_node);
+    LVN_SG(k_sg_seam, per_seam, block, 0, sm, g, 2);
+    LVN_SG(k_sg_order, grid(tn), block, 0, sm, g, T + o_node);
+    LVN_SG(k_sg_tiles, dim3(lmax + 1), dim3(1024), 0, sm, g, 0, T + o_lev, (int)tn, cnt + kSgCntLevel);
+    for (int l = lmax; l >= 1; --l) LVN_SG(k_sg_level, dev_grid, block, 0, sm, g, T + o_lev + l * tn, l, 0);
+    LVN_SG(k_sg_seam, per_seam, block, 0, sm, g, 3);
+    for (int l = 0; l < lmax; ++l) LVN_SG(k_sg_level, dev_grid, block, 0, sm, g, T + o_lev + l * tn, l, 1);
+    LVN_SG(k_sg_vertices, grid(tl), block, 0, sm, g, T + o_leaf);
+    // round r reads tile list r & 1 and makes list (r + 1) & 1
+    LVN_SG(k_sg_tiles, dim3(1), dim3(1024), 0, sm, g, 1, T + o_round, 0, cnt + kSgCntRound);
+    int round = 0;
+    for (int batch = lmax + 3; round < kMaxRounds; batch = 2) {
+        for (int k = 0; k < batch && round < kMaxRounds; ++k, ++round) {
+            int4* cur = T + o_round + (round & 1) * tr;
+            int4* nxt = T + o_round + ((round + 1) & 1) * tr;
+            LVN_SG(k_sg_round, dev_grid, block, 0, sm, g, cur, round, tsum, tbase);
+            LVN_SG(k_sg_round_next, dim3(1), dim3(1024), 0, sm, g, round, tsum, toff, nxt);
+            LVN_SG(k_sg_round_copy, dev_grid, block, 0, sm, g, cur, round, tsum, tbase, toff);
+        }
+        LVN_HIP(hipMemcpyAsync(st.h_active.p, cnt + kSgCntActive + round - 1, sizeof(int), hipMemcpyDeviceToHost, sm));
+        LVN_HIP(hipStreamSynchronize(sm));
+        if (*static_cast<const int*>(st.h_active.p) == 0) break;
+    }
+    LVN_SG(k_sg_tiles, dim3(1), dim3(1024), 0, sm, g, 2, T + o_fin, 0, cnt + kSgCntFinal);
+    LVN_SG(k_sg_triangles, dev_grid, block, 0, sm, g, T + o_fin, d_res);
+#undef LVN_SG
+    return LVN_OK;
+}
+
+// Which seam pass runs m seams: the grid-wide steps spread each seam over the whole GPU
+// and win for calls of up to a few hundred seams (measured on the clipmap's seams: 2.5x for
+// its largest seam alone, 1.5x for 256 of them); one workgroup per seam keeps the higher
+// throughput of whole clipmaps (1,798 seams: 33.6 against 38.7 ms) and has the shorter
+// fixed cost for one small seam.  LVN_SEAM_PATH=grid or =workgroup in the environment
+// overrides the choice.
+bool seam_pass_grid(int m, const std::vector<SeamJobDev>& jobs, const std::vector<SeamSrcDev>& srcs) {
+    if (const char* e = getenv("LVN_SEAM_PATH")) {
+        if (!strcmp(e, "grid")) return true;
+        if (!strcmp(e, "workgroup")) return false;
+    }
+    if (m > 320) return false;
+    int64_t largest = 0;
+    for (const SeamJobDev& J : jobs) {
+        int64_t offered = 0;
+        for (int si = 0; si < J.src_count; ++si) offered += srcs[J.src_begin + si].count;
+        largest = std::max(largest, offered);
+    }
+    return largest >= 16384;
 }
 
 int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* sources, lvn_seam_out* outs,
@@ -1075,10 +1827,16 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
         const auto* d_jobs = static_cast<const SeamJobDev*>(st.jobs.p);
         auto* d_res = static_cast<SeamResDev*>(st.res.p);
         auto* d_offs = static_cast<uint64_t*>(st.offs.p);
-        hipLaunchKernelGGL(k_seam, dim3(m), dim3(kSeamThreads), 0, c->stream, d_jobs,
-                           static_cast<const SeamSrcDev*>(st.srcs.p), static_cast<char*>(st.scratch.p),
-                           static_cast<lvn_vertex*>(ov.p), static_cast<int*>(ot.p), d_res);
-        LVN_HIP(hipGetLastError());
+        if (seam_pass_grid(m, jobs, srcs)) {
+            if ((r = run_seam_grid(c, st, m, jobs.data(), srcs, d_jobs, static_cast<lvn_vertex*>(ov.p),
+                                   static_cast<int*>(ot.p), d_res)))
+                return r;
+        } else {
+            hipLaunchKernelGGL(k_seam, dim3(m), dim3(kSeamThreads), 0, c->stream, d_jobs,
+                               static_cast<const SeamSrcDev*>(st.srcs.p), static_cast<char*>(st.scratch.p),
+                               static_cast<lvn_vertex*>(ov.p), static_cast<int*>(ot.p), d_res);
+            LVN_HIP(hipGetLastError());
+        }
         hipLaunchKernelGGL(k_seam_offsets, dim3(1), dim3(1024), 0, c->stream, d_res, m, d_offs);
         LVN_HIP(hipGetLastError());
         char* h_res = static_cast<char*>(st.h_res.p);

@@ -5,7 +5,9 @@ layout), offered to each host node as GenerateClipmapSeamMesh gathers them
 (src/clipmap.cpp:579-597).  The cached-source form reads the lists of chunks that the
 context meshed itself.
 """
+import contextlib
 import itertools
+import os
 
 import numpy as np
 import pytest
@@ -61,21 +63,39 @@ def _colour(i):
     return (0.25 * (i % 4), 0.5, 1.0 - 0.125 * (i % 8))
 
 
+PATHS = ("grid", "workgroup")  # the two forms of the seam pass (LVN_SEAM_PATH)
+
+
+@contextlib.contextmanager
+def seam_path(path):
+    """Run the seam calls inside with the given pass (the library reads LVN_SEAM_PATH per call)."""
+    old = os.environ.get("LVN_SEAM_PATH")
+    os.environ["LVN_SEAM_PATH"] = path
+    try:
+        yield
+    finally:
+        if old is None:
+            del os.environ["LVN_SEAM_PATH"]
+        else:
+            os.environ["LVN_SEAM_PATH"] = old
+
+
 def _check_scene(ctx, scene):
     srcs = scene.sources()
     hosts = [(mn, size, _colour(i)) for i, (mn, size, _) in enumerate(scene.nodes)]
-    got = ctx.seam_meshes(hosts, srcs)
-    tris = 0
-    for i, ((mn, size, colour), s) in enumerate(zip(hosts, srcs)):
-        rc, v, t = py_oracle.seam_mesh(mn, size, s, colour=colour)
-        assert rc == LVN_OK
-        gv, gt = got[i]
-        print(f"seam {i} host {mn} size {size}: {len(v)} vertices, {len(t)} triangles", flush=True)
-        assert len(gv) == len(v) and len(gt) == len(t), f"seam {i}"
-        assert gv.tobytes() == v.tobytes(), f"seam {i} vertices"
-        assert (gt == t).all(), f"seam {i} triangles"
-        tris += len(t)
+    want = [py_oracle.seam_mesh(mn, size, s, colour=colour) for (mn, size, colour), s in zip(hosts, srcs)]
+    tris = sum(len(t) for _, _, t in want)
     assert tris > 0
+    for path in PATHS:
+        with seam_path(path):
+            got = ctx.seam_meshes(hosts, srcs)
+        for i, ((mn, size, _), (rc, v, t)) in enumerate(zip(hosts, want)):
+            assert rc == LVN_OK
+            gv, gt = got[i]
+            print(f"{path}: seam {i} host {mn} size {size}: {len(v)} vertices, {len(t)} triangles", flush=True)
+            assert len(gv) == len(v) and len(gt) == len(t), f"{path}: seam {i}"
+            assert gv.tobytes() == v.tobytes(), f"{path}: seam {i} vertices"
+            assert (gt == t).all(), f"{path}: seam {i} triangles"
     return got
 
 
@@ -190,15 +210,18 @@ def test_clipmap_region_with_brushes():
              if counts[i, 2] > 0 or counts[i, 3] > 0]
     srcs = workloads.seam_sources(nodes)
     hosts = [(mn, size, _colour(i)) for i, (mn, size, _) in enumerate(nodes)]
-    got = b.seam_meshes(hosts, [[dict(x, nodes=None) for x in s] for s in srcs])
-    tris = 0
-    for i, ((mn, size, colour), s) in enumerate(zip(hosts, srcs)):
-        rc, v, t = py_oracle.seam_mesh(mn, size, s, colour=colour)
-        assert rc == LVN_OK
-        assert got[i][0].tobytes() == v.tobytes() and (got[i][1] == t).all(), f"seam {i} host {mn} size {size}"
-        tris += len(t)
-    print(f"{len(nodes)} seams, {tris} triangles", flush=True)
-    assert tris > 1000
+    want = [py_oracle.seam_mesh(mn, size, s, colour=colour) for (mn, size, colour), s in zip(hosts, srcs)]
+    for path in PATHS:
+        with seam_path(path):
+            got = b.seam_meshes(hosts, [[dict(x, nodes=None) for x in s] for s in srcs])
+        tris = 0
+        for i, ((mn, size, _), (rc, v, t)) in enumerate(zip(hosts, want)):
+            assert rc == LVN_OK
+            assert got[i][0].tobytes() == v.tobytes() and (got[i][1] == t).all(), \
+                f"{path}: seam {i} host {mn} size {size}"
+            tris += len(t)
+        print(f"{path}: {len(nodes)} seams, {tris} triangles", flush=True)
+        assert tris > 1000
     b.close()
     mesher.clear_csg_ops()
 
@@ -209,16 +232,18 @@ def test_seam_errors(ctx, equal_lod):
     # empty
     v, t = ctx.seam_mesh(mn, size, [])
     assert len(v) == 0 and len(t) == 0
-    # a list offered twice: each node once
-    v0, t0 = ctx.seam_mesh(mn, size, srcs)
-    v1, t1 = ctx.seam_mesh(mn, size, srcs + srcs)
-    assert v0.tobytes() == v1.tobytes() and (t0 == t1).all()
-    # overlapping offers (a coarser node over one of the host's leaves): rejected by both
     bad = srcs + [overlapping_offer(mn, size, srcs)]
     rc, _, _ = py_oracle.seam_mesh(mn, size, bad)
-    with pytest.raises(mesher.LvnError) as e:
-        ctx.seam_mesh(mn, size, bad)
-    assert e.value.code == rc == LVN_ERR_INVALID
+    for path in PATHS:
+        with seam_path(path):
+            # a list offered twice: each node once
+            v0, t0 = ctx.seam_mesh(mn, size, srcs)
+            v1, t1 = ctx.seam_mesh(mn, size, srcs + srcs)
+            assert v0.tobytes() == v1.tobytes() and (t0 == t1).all(), path
+            # overlapping offers (a coarser node over one of the host's leaves): rejected by both
+            with pytest.raises(mesher.LvnError) as e:
+                ctx.seam_mesh(mn, size, bad)
+            assert e.value.code == rc == LVN_ERR_INVALID, path
     # caller buffers too small: LVN_ERR_CAPACITY with the needed counts
     with pytest.raises(mesher.LvnError) as e:
         _seam_capacity(ctx, mn, size, srcs, len(v0), len(t0))
