@@ -6,9 +6,9 @@
 // mesh (Octree_GenerateMesh, src/octree.cpp:520-550: GenerateVertexIndices
 // :196-231, ContourCellProc / FaceProc / EdgeProc / ProcessEdge :235-474).
 //
-// Two forms of one pass: one workgroup stitches one seam (k_seam, a launch covers every
-// seam of a request), or each step below runs as one launch over the elements of all
-// seams (k_sg_*: a seam spread over the whole GPU, for calls of few seams).  The
+// Two forms of one pass: each step below runs as one launch over the elements of all
+// seams of a request (k_sg_*: every seam spread over the whole GPU), or one workgroup
+// stitches one seam (k_seam, for requests of small seams only).  The
 // reference builds the octree with a hash map per level and contours it by recursion;
 // here each phase is data-parallel:
 //   1. select: every offered seam node is filtered and its leaf key
@@ -97,11 +97,19 @@ __device__ inline uint64_t node_shape(const int* world_min, const int* host_min,
            (uint64_t)(corners & 0xff) << 48 | (uint64_t)one_chunk << 56;
 }
 
-// A pending call of the recursion: node indices and meta = kind | dir << 2 | flip << 4.
-struct Call {
-    int n[4];
-    int meta;
+// A pending call of the recursion in 16 bytes: four node indices (-1 = none; the first is
+// always a node), the first carrying meta = kind | dir << 2 | flip << 4 in its top 5 bits.
+// Node indices stay below 2^27 (checked when a seam's caps are set).
+constexpr int kCallNodeBits = 27;
+constexpr int kCallNodeMask = (1 << kCallNodeBits) - 1;
+struct alignas(16) Call {
+    int w[4];
+    __device__ int node(int j) const { return j == 0 ? (w[0] & kCallNodeMask) : w[j]; }
+    __device__ int meta() const { return (int)((unsigned)w[0] >> kCallNodeBits); }
 };
+__device__ inline Call make_call(int a, int b, int c, int d, int meta) {
+    return Call{{a | meta << kCallNodeBits, b, c, d}};
+}
 
 constexpr int kSeamThreads = 1024;
 constexpr int kSeamWaves = kSeamThreads / 64;
@@ -265,7 +273,7 @@ struct Expander {
 
     __device__ __forceinline__ void push(int a, int b, int c, int d, int meta) {
         if (write) {
-            dst[at + k] = Call{{a, b, c, d}, meta};
+            dst[at + k] = make_call(a, b, c, d, meta);
             if ((meta & 3) != kCallQuad) more = 1;
         }
         ++k;
@@ -322,15 +330,15 @@ struct Expander {
         int r0[8], r1[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            r0[k] = child(c.n[0], k);
-            r1[k] = child(c.n[1], k);
+            r0[k] = child(c.node(0), k);
+            r1[k] = child(c.node(1), k);
         }
-        const bool b0 = internal(c.n[0]), b1 = internal(c.n[1]);
+        const bool b0 = internal(c.node(0)), b1 = internal(c.node(1));
         int ch0[8], ch1[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            ch0[k] = b0 ? r0[k] : c.n[0];
-            ch1[k] = b1 ? r1[k] : c.n[1];
+            ch0[k] = b0 ? r0[k] : c.node(0);
+            ch1[k] = b1 ? r1[k] : c.node(1);
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) face(ch0[kFaceFaces[D][i][0]], ch1[kFaceFaces[D][i][1]], kFaceFaces[D][i][2]);
@@ -351,26 +359,26 @@ struct Expander {
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) r[i][j] = child(c.n[j], kEdgeEdges[D][i][j]);
+            for (int j = 0; j < 4; ++j) r[i][j] = child(c.node(j), kEdgeEdges[D][i][j]);
         bool br[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) br[j] = internal(c.n[j]);
+        for (int j = 0; j < 4; ++j) br[j] = internal(c.node(j));
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             int sub[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) sub[j] = br[j] ? r[i][j] : c.n[j];
+            for (int j = 0; j < 4; ++j) sub[j] = br[j] ? r[i][j] : c.node(j);
             edge(sub[0], sub[1], sub[2], sub[3], kEdgeEdges[D][i][4]);
         }
     }
     __device__ __forceinline__ void expand(const Call& c) {
-        const int kind = c.meta & 3, dir = (c.meta >> 2) & 3;
+        const int kind = c.meta() & 3, dir = (c.meta() >> 2) & 3;
         if (kind == kCallQuad) {
-            push(c.n[0], c.n[1], c.n[2], c.n[3], c.meta);
+            push(c.node(0), c.node(1), c.node(2), c.node(3), c.meta());
         } else if (kind == kCallCell) {  // ContourCellProc, src/octree.cpp:422-474
             int ch[8];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) ch[i] = child(c.n[0], i);
+            for (int i = 0; i < 8; ++i) ch[i] = child(c.node(0), i);
 #pragma unroll
             for (int i = 0; i < 8; ++i)
                 if (ch[i] >= 0 && cell_emits(ch[i])) push(ch[i], -1, -1, -1, kCallCell);
@@ -667,7 +675,7 @@ __global__ __launch_bounds__(kSeamThreads) __attribute__((amdgpu_waves_per_eu(8,
     if (tid == 0) {
         s_n = 0;
         if (linked && N[root].type == kNodeInternal && ((S.shape[root] >> 56) & 1) == 0) {
-            S.list0[0] = Call{{root, -1, -1, -1}, kCallCell};
+            S.list0[0] = make_call(root, -1, -1, -1, kCallCell);
             s_n = 1;
         }
     }
@@ -729,9 +737,9 @@ __global__ __launch_bounds__(kSeamThreads) __attribute__((amdgpu_waves_per_eu(8,
     for (int i = tid; i < nq; i += kSeamThreads) {
         const Call c = Q[i];
         int v[4];
-        for (int j = 0; j < 4; ++j) v[j] = N[c.n[j]].offset;
+        for (int j = 0; j < 4; ++j) v[j] = N[c.node(j)].offset;
         int* t = out_t + 3 * (J.t_off + 2 * (uint64_t)i);
-        if (c.meta & 16) {
+        if (c.meta() & 16) {
             t[0] = v[0]; t[1] = v[3]; t[2] = v[1];
             t[3] = v[0]; t[4] = v[2]; t[5] = v[3];
         } else {
@@ -834,7 +842,8 @@ __global__ __launch_bounds__(256) void k_seam_pack(const SeamJobDev* jobs, const
 constexpr int kSgTile = 1024;
 constexpr int kSgThreads = 256;
 constexpr int kSgPer = kSgTile / kSgThreads;
-constexpr int kSgGrid = 2048;  // workgroups of the launches whose tile count only the device knows
+constexpr int kSgGrid = 2048;  // workgroups of the tile launches
+constexpr int kSgStage = 3072;  // calls a round tile stages in LDS (48 KB)
 constexpr int kSgLevels = 33;
 
 struct SeamRun {
@@ -874,10 +883,12 @@ __device__ __forceinline__ SeamScratch sg_scratch(const SeamGrid& g, int s) {
     return seam_scratch(g.scratch + g.jobs[s].scratch, g.jobs[s]);
 }
 
-// Loop over the tiles of a list: tl = (seam, first element, source, 0).
-#define LVN_SG_TILES(tiles, ntiles)                     \
-    for (int tk = blockIdx.x; tk < (ntiles); tk += gridDim.x) \
-        if (const int4 tl = (tiles)[tk]; true)
+// Loop over the tiles of a list: tl = (seam, first element, source, 0); padding tiles
+// (seam -1) are skipped by the whole workgroup.  gridDim.x is a multiple of 8, so a
+// workgroup keeps to the tiles of its XCD.
+#define LVN_SG_TILES(tiles, ntiles)                                \
+    for (int tk = blockIdx.x; tk < (ntiles); tk += gridDim.x)      \
+        if (const int4 tl = (tiles)[tk]; tl.x >= 0)
 
 // Exclusive scan over the workgroup; *total = the sum.
 __device__ __forceinline__ int sg_scan(int v, int* total) {
@@ -914,54 +925,71 @@ __device__ __forceinline__ int lane_rank(uint64_t mask) {
     return __popcll(mask & ((1ull << (threadIdx.x & 63)) - 1));
 }
 
-// Tile lists (one workgroup of 1024 threads scans the seams; workgroup b of a level list
-// makes level b's list at tiles + b * stride).  what: 0 a level's nodes, 1 a round's
+// Tile lists, placed by XCD.  Workgroups go to the eight XCDs in turn (workgroup b to XCD
+// b mod 8), and every tile launch loops over its list with a stride that is a multiple of
+// 8, so tile k runs on XCD k mod 8.  A list is eight interleaved sub-lists, one per XCD
+// (sub-list x at k = 8p + x), and seam s's tiles all go to sub-list s mod 8: a seam's hash
+// table, nodes and call lists are then touched by one XCD and share its L2.  Seams come
+// largest first, so the sub-lists are about equally long; the shorter ones are padded with
+// empty tiles (seam -1).  Within a sub-list a seam's tiles are consecutive, in order:
+// tile t of seam s is at k = 8 (tile0 + t) + s mod 8.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int sg_tile_index(int seam, int p) { return p * kXcds + (seam & (kXcds - 1)); }
+
+// Pads the sub-lists of `out` to the longest (fill[x] = length of sub-list x); returns the
+// list's length.  Called by every thread of the workgroup.
+__device__ int sg_pad(int4* out, const int* fill) {
+    int longest = 0;
+#pragma unroll
+    for (int x = 0; x < kXcds; ++x) longest = max(longest, fill[x]);
+    for (int k = threadIdx.x; k < kXcds * longest; k += blockDim.x)
+        if (k / kXcds >= fill[k % kXcds]) out[k] = make_int4(-1, 0, 0, 0);
+    return kXcds * longest;
+}
+
+// Tile lists (one workgroup of 1024 threads, a thread per seam; workgroup b of a level
+// list makes level b's list at tiles + b * stride).  what: 0 a level's nodes, 1 a round's
 // pending calls of the seams still expanding, 2 the final quads, 3 hash slots, 4 leaves,
 // 5 nodes, 6 offered nodes (one list per source).
 __global__ __launch_bounds__(1024) void k_sg_tiles(SeamGrid g, int what, int4* tiles, int stride, int* total) {
-    __shared__ int sw[kSeamWaves];
-    __shared__ int carry;
+    __shared__ int fill[kXcds];
     const int l = blockIdx.x;
     int4* out = tiles + (size_t)l * stride;
-    if (threadIdx.x == 0) carry = 0;
+    if (threadIdx.x < kXcds) fill[threadIdx.x] = 0;
     __syncthreads();
-    for (int b = 0; b < g.m; b += 1024) {
-        const int s = b + threadIdx.x;
+    for (int s = threadIdx.x; s < g.m; s += blockDim.x) {
+        const SeamRun& R = g.run[s];
+        const SeamJobDev& J = g.jobs[s];
+        auto nt = [](int n) { return (n + kSgTile - 1) / kSgTile; };
         int cnt = 0;
-        if (s < g.m) {
-            const SeamRun& R = g.run[s];
-            const SeamJobDev& J = g.jobs[s];
-            auto nt = [](int n) { return (n + kSgTile - 1) / kSgTile; };
-            switch (what) {
-            case 0: cnt = R.status == 0 ? nt(R.lev[l + 1] - R.lev[l]) : 0; break;
-            case 1: cnt = R.status == 0 && R.linked && R.more ? nt(R.n) : 0; break;
-            case 2: cnt = R.status == 0 && R.linked ? nt(R.n) : 0; break;
-            case 3: cnt = nt(J.hash_cap); break;
-            case 4: cnt = nt(J.leaf_cap); break;
-            case 5: cnt = nt(J.node_cap); break;
-            default:
-                for (int si = 0; si < J.src_count; ++si) cnt += nt(g.srcs[J.src_begin + si].count);
-            }
+        switch (what) {
+        case 0: cnt = R.status == 0 ? nt(R.lev[l + 1] - R.lev[l]) : 0; break;
+        case 1: cnt = R.status == 0 && R.linked && R.more ? nt(R.n) : 0; break;
+        case 2: cnt = R.status == 0 && R.linked ? nt(R.n) : 0; break;
+        case 3: cnt = nt(J.hash_cap); break;
+        case 4: cnt = nt(J.leaf_cap); break;
+        case 5: cnt = nt(J.node_cap); break;
+        default:
+            for (int si = 0; si < J.src_count; ++si) cnt += nt(g.srcs[J.src_begin + si].count);
         }
-        int tot;
-        const int at = carry + block_exclusive(cnt, sw, &tot);
-        if (s < g.m && what == 6) {
-            const SeamJobDev& J = g.jobs[s];
-            int k = at;
+        const int p0 = cnt ? atomicAdd(&fill[s & (kXcds - 1)], cnt) : 0;
+        if (what == 6) {
+            int p = p0;
             for (int si = 0; si < J.src_count; ++si)
-                for (int j = 0; j < g.srcs[J.src_begin + si].count; j += kSgTile) out[k++] = make_int4(s, j, si, 0);
-        } else if (s < g.m) {
-            for (int t = 0; t < cnt; ++t) out[at + t] = make_int4(s, t * kSgTile, 0, 0);
+                for (int j = 0; j < g.srcs[J.src_begin + si].count; j += kSgTile)
+                    out[sg_tile_index(s, p++)] = make_int4(s, j, si, 0);
+        } else {
+            for (int t = 0; t < cnt; ++t) out[sg_tile_index(s, p0 + t)] = make_int4(s, t * kSgTile, 0, 0);
             if (what == 1) {
-                g.run[s].tile0 = at;
+                g.run[s].tile0 = p0;
                 g.run[s].ntiles = cnt;
             }
         }
-        __syncthreads();
-        if (threadIdx.x == 0) carry += tot;
-        __syncthreads();
     }
-    if (threadIdx.x == 0) total[l] = carry;
+    __syncthreads();
+    const int n = sg_pad(out, fill);
+    if (threadIdx.x == 0) total[l] = n;
 }
 
 // Seam records and round counters cleared; every hash slot emptied.
@@ -1103,7 +1131,7 @@ __global__ void k_sg_seam(SeamGrid g, int step) {
         if (R.linked) {
             SeamScratch S = sg_scratch(g, s);
             if (S.nodes[R.root].type == kNodeInternal && ((S.shape[R.root] >> 56) & 1) == 0) {
-                S.list0[0] = Call{{R.root, -1, -1, -1}, kCallCell};
+                S.list0[0] = make_call(R.root, -1, -1, -1, kCallCell);
                 R.n = 1;
             }
         }
@@ -1205,7 +1233,7 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_links(SeamGrid g, const int4*
             const int i = tl.y + threadIdx.x + r * kSgThreads;
             if (i >= nnodes) continue;
             const int level = S.nodes[i].level;
-            atomicAdd(&hist[level], 1);
+            if (i >= R.nleaves) atomicAdd(&hist[level], 1);  // internal nodes (created after the leaves)
             if (level == 0) continue;
             const uint64_t c = S.nodes[i].code;
             const int slot = hash_find(S.keys, J.hash_cap - 1, (uint64_t)(level - 1) << 48 | (c >> 3));
@@ -1223,21 +1251,22 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_links(SeamGrid g, const int4*
     }
 }
 
-// Nodes bucketed by level: a tile reserves its share of each level once.
+// Internal nodes (indices nleaves .. nnodes - 1) bucketed by level: a tile reserves its
+// share of each level once.
 __global__ __launch_bounds__(kSgThreads) void k_sg_order(SeamGrid g, const int4* tiles) {
     __shared__ int hist[kSgLevels], base[kSgLevels];
     LVN_SG_TILES(tiles, g.cnt[kSgCntNode]) {
         SeamRun& R = g.run[tl.x];
-        if (R.status != 0 || tl.y >= R.nnodes) continue;
+        if (R.status != 0 || tl.y >= R.nnodes || tl.y + kSgTile <= R.nleaves) continue;
         SeamScratch S = sg_scratch(g, tl.x);
         if (threadIdx.x < kSgLevels) hist[threadIdx.x] = 0;
         __syncthreads();
-        const int nnodes = R.nnodes;
+        const int nnodes = R.nnodes, nleaves = R.nleaves;
         int lv[kSgPer];
 #pragma unroll
         for (int r = 0; r < kSgPer; ++r) {
             const int i = tl.y + threadIdx.x + r * kSgThreads;
-            lv[r] = i < nnodes ? S.nodes[i].level : -1;
+            lv[r] = i >= nleaves && i < nnodes ? S.nodes[i].level : -1;
             if (lv[r] >= 0) atomicAdd(&hist[lv[r]], 1);
         }
         __syncthreads();
@@ -1253,8 +1282,9 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_order(SeamGrid g, const int4*
     }
 }
 
-// One level: leaf counts added to the parents (deepest level first), or depth-first ranks
-// handed from internal nodes to their children in child order (top level first).
+// One level's internal nodes: leaf counts gathered from the children (a leaf counts 1;
+// deepest level first), or depth-first ranks handed to the children in child order (top
+// level first).
 __global__ __launch_bounds__(kSgThreads) void k_sg_level(SeamGrid g, const int4* tiles, int l, int ranks) {
     LVN_SG_TILES(tiles, g.cnt[kSgCntLevel + l]) {
         SeamRun& R = g.run[tl.x];
@@ -1265,15 +1295,24 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_level(SeamGrid g, const int4*
             const int k = first + tl.y + threadIdx.x + r * kSgThreads;
             if (k >= end) continue;
             const int i = S.order[k];
+            int ch[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) ch[c] = S.ch[8 * i + c];
+            int cnt[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) cnt[c] = ch[c] >= 0 ? S.nodes[ch[c]].count : 0;
             if (!ranks) {
-                atomicAdd(&S.nodes[S.nodes[i].parent].count, S.nodes[i].count);
-            } else if (S.nodes[i].type == kNodeInternal) {
+                int sum = 0;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) sum += cnt[c];
+                S.nodes[i].count = sum;
+            } else {
                 int off = S.nodes[i].offset;
+#pragma unroll
                 for (int c = 0; c < 8; ++c) {
-                    const int ch = S.ch[8 * i + c];
-                    if (ch < 0) continue;
-                    S.nodes[ch].offset = off;
-                    off += S.nodes[ch].count;
+                    if (ch[c] < 0) continue;
+                    S.nodes[ch[c]].offset = off;
+                    off += cnt[c];
                 }
             }
         }
@@ -1320,6 +1359,7 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_vertices(SeamGrid g, const in
 __global__ __launch_bounds__(kSgThreads) void k_sg_round(SeamGrid g, const int4* tiles, int round, int* tsum,
                                                           int* tbase) {
     __shared__ int base;
+    __shared__ Call stage[kSgStage];
     LVN_SG_TILES(tiles, g.cnt[kSgCntRound + round]) {
         const SeamJobDev& J = g.jobs[tl.x];
         SeamRun& R = g.run[tl.x];
@@ -1334,24 +1374,34 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_round(SeamGrid g, const int4*
             cnt += e.k;
         }
         int total;
-        int at = sg_scan(cnt, &total);
+        const int at = sg_scan(cnt, &total);
         if (threadIdx.x == 0) {
             base = total ? atomicAdd(&R.stage, total) : 0;
             tsum[tk] = total;
             tbase[tk] = base;
         }
         __syncthreads();
-        at += base;
+        const int b0 = base;
         __syncthreads();  // base is rewritten by the next tile
-        if (cnt == 0 || at + cnt > J.list_cap) continue;  // the seam's total then exceeds the cap
+        if (total == 0 || b0 + total > J.list_cap) continue;  // (the seam's total then exceeds its cap)
+        // the tile's block is written from LDS in whole lines when it fits there
+        const bool in_lds = total <= kSgStage;
+        Call* dst = in_lds ? stage : S.list1 + b0;
+        int w_at = at;
         bool m = false;
         for (int q = 0; q < kSgPer && i0 + q < n; ++q) {
-            Expander w{S.ch, S.shape, S.list1, at, 0, 0, true};
+            Expander w{S.ch, S.shape, dst, w_at, 0, 0, true};
             w.expand(A[i0 + q]);
-            at += w.k;
+            w_at += w.k;
             m |= w.more != 0;
         }
         if (m) R.more = 2;
+        if (in_lds) {
+            __syncthreads();
+            Call* out = S.list1 + b0;
+            for (int i = threadIdx.x; i < total; i += kSgThreads) out[i] = stage[i];
+            __syncthreads();
+        }
     }
 }
 
@@ -1362,48 +1412,40 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_round(SeamGrid g, const int4*
 // list or of the round limit ends it), and the next round's tile list made.
 __global__ __launch_bounds__(1024) void k_sg_round_next(SeamGrid g, int round, const int* tsum, int* toff,
                                                          int4* next) {
-    __shared__ int sw[kSeamWaves];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
+    __shared__ int fill[kXcds];
+    if (threadIdx.x < kXcds) fill[threadIdx.x] = 0;
     __syncthreads();
     int active = 0;
-    for (int b = 0; b < g.m; b += 1024) {
-        const int s = b + threadIdx.x;
-        int cnt = 0;
-        if (s < g.m) {
-            SeamRun& R = g.run[s];
-            const SeamJobDev& J = g.jobs[s];
-            if (R.status == 0 && R.linked && R.more && R.ntiles > 0) {
-                int sum = 0;
-                for (int t = 0; t < R.ntiles; ++t) {
-                    const int v = tsum[R.tile0 + t];
-                    toff[R.tile0 + t] = sum;
-                    sum += v;
-                }
-                R.rounds += 1;
-                R.stage = 0;
-                if (sum > J.list_cap || R.rounds >= kMaxRounds) {
-                    set_status(&R.status, LVN_ERR_OVERFLOW);
-                } else {
-                    R.n = sum;
-                    R.more = R.more == 2;
-                    active += R.more;
-                }
+    for (int s = threadIdx.x; s < g.m; s += blockDim.x) {
+        SeamRun& R = g.run[s];
+        const SeamJobDev& J = g.jobs[s];
+        if (R.status == 0 && R.linked && R.more && R.ntiles > 0) {
+            int sum = 0;
+            for (int t = 0; t < R.ntiles; ++t) {
+                const int k = sg_tile_index(s, R.tile0 + t);
+                const int v = tsum[k];
+                toff[k] = sum;
+                sum += v;
             }
-            cnt = R.status == 0 && R.linked && R.more ? (R.n + kSgTile - 1) / kSgTile : 0;
+            R.rounds += 1;
+            R.stage = 0;
+            if (sum > J.list_cap || R.rounds >= kMaxRounds) {
+                set_status(&R.status, LVN_ERR_OVERFLOW);
+            } else {
+                R.n = sum;
+                R.more = R.more == 2;
+                active += R.more;
+            }
         }
-        int tot;
-        const int at = carry + block_exclusive(cnt, sw, &tot);
-        if (s < g.m) {
-            for (int t = 0; t < cnt; ++t) next[at + t] = make_int4(s, t * kSgTile, 0, 0);
-            g.run[s].tile0 = at;
-            g.run[s].ntiles = cnt;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) carry += tot;
-        __syncthreads();
+        const int cnt = R.status == 0 && R.linked && R.more ? (R.n + kSgTile - 1) / kSgTile : 0;
+        const int p0 = cnt ? atomicAdd(&fill[s & (kXcds - 1)], cnt) : 0;
+        for (int t = 0; t < cnt; ++t) next[sg_tile_index(s, p0 + t)] = make_int4(s, t * kSgTile, 0, 0);
+        R.tile0 = p0;
+        R.ntiles = cnt;
     }
-    if (threadIdx.x == 0) g.cnt[kSgCntRound + round + 1] = carry;
+    __syncthreads();
+    const int n = sg_pad(next, fill);
+    if (threadIdx.x == 0) g.cnt[kSgCntRound + round + 1] = n;
     if (active) atomicAdd(&g.cnt[kSgCntActive + round], active);
 }
 
@@ -1434,9 +1476,9 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_triangles(SeamGrid g, const i
             if (i >= n) continue;
             const Call c = Q[i];
             int v[4];
-            for (int j = 0; j < 4; ++j) v[j] = S.nodes[c.n[j]].offset;
+            for (int j = 0; j < 4; ++j) v[j] = S.nodes[c.node(j)].offset;
             int* t = g.out_t + 3 * (J.t_off + 2 * (uint64_t)i);
-            if (c.meta & 16) {
+            if (c.meta() & 16) {
                 t[0] = v[0]; t[1] = v[3]; t[2] = v[1];
                 t[3] = v[0]; t[4] = v[2]; t[5] = v[3];
             } else {
@@ -1556,13 +1598,19 @@ void parallel_ranges(int n, size_t bytes, F&& f) {
 // Caps of one seam from its selected nodes; `grow` scales them after an overflow.  A
 // seam's octree has about 1.4 nodes per leaf (the leaves lie on faces) and at most ~3
 // quads per leaf.
-void seam_caps(SeamJobDev& J, int selected, int grow) {
-    J.leaf_cap = std::max(selected, 1);
-    J.node_cap = (2 * J.leaf_cap + 64) * grow;
-    J.list_cap = (6 * J.leaf_cap + 1024) * grow;
-    int h = 1;
-    while (h < J.node_cap + J.node_cap / 2) h <<= 1;
-    J.hash_cap = h;
+// Returns false when the caps exceed what the pass indexes (node indices below 2^27, call
+// lists and hash tables below 2^30 entries).
+bool seam_caps(SeamJobDev& J, int selected, int grow) {
+    const int64_t leaf = std::max(selected, 1);
+    const int64_t node = (2 * leaf + 64) * grow, list = (6 * leaf + 1024) * grow;
+    int64_t h = 1;
+    while (h < node + node / 2) h <<= 1;
+    if (node > kCallNodeMask || list >= (1 << 30) || h >= (1 << 30)) return false;
+    J.leaf_cap = (int)leaf;
+    J.node_cap = (int)node;
+    J.list_cap = (int)list;
+    J.hash_cap = (int)h;
+    return true;
 }
 
 // Device location and length of a source given as count = -1 without a list: a cached
@@ -1600,40 +1648,47 @@ int resolve_batch(void* owner, const lvn_seam_source& q, SeamSrcDev& d) {
 // first batch (deepest possible level + 3 rounds) normally finishes every seam.
 int run_seam_grid(Ctx* c, SeamState& st, int m, const SeamJobDev* h_jobs, const std::vector<SeamSrcDev>& srcs,
                   const SeamJobDev* d_jobs, lvn_vertex* ov, int* ot, SeamResDev* d_res) {
+    // list lengths: eight interleaved sub-lists, seam k in sub-list k mod 8, each padded to
+    // the longest (bounds from the seams' caps)
     auto nt = [](int64_t n) { return (n + kSgTile - 1) / kSgTile; };
-    int64_t th = 0, to = 0, tl = 0, tn = 0, tr = 0;
+    int64_t xh[kXcds] = {}, xo[kXcds] = {}, xl[kXcds] = {}, xn[kXcds] = {}, xr[kXcds] = {};
     int lmax = 0;
     for (int k = 0; k < m; ++k) {
         const SeamJobDev& J = h_jobs[k];
-        th += nt(J.hash_cap);
-        tl += nt(J.leaf_cap);
-        tn += nt(J.node_cap);
-        tr += nt(J.list_cap);
+        const int x = k & (kXcds - 1);
+        xh[x] += nt(J.hash_cap);
+        xl[x] += nt(J.leaf_cap);
+        xn[x] += nt(J.node_cap);
+        xr[x] += nt(J.list_cap);
         const int root_size = 2 * J.host_size, root_levels = __builtin_ctz((unsigned)root_size);
         for (int si = 0; si < J.src_count; ++si) {
             const SeamSrcDev& q = srcs[J.src_begin + si];
-            to += nt(q.count);
+            xo[x] += nt(q.count);
             const int leaf = q.size / J.vpc;
             if (host_pow2(leaf) && leaf < root_size) lmax = std::max(lmax, root_levels - __builtin_ctz((unsigned)leaf));
         }
     }
+    auto padded = [](const int64_t* v) { return kXcds * std::max<int64_t>(1, *std::max_element(v, v + kXcds)); };
+    const int64_t th = padded(xh), to = padded(xo), tl = padded(xl), tn = padded(xn), tr = padded(xr);
     lmax = std::min(lmax, kSgLevels - 1);
     const int64_t o_off = th, o_leaf = o_off + to, o_node = o_leaf + tl, o_lev = o_node + tn,
                   o_round = o_lev + (lmax + 1) * tn, o_fin = o_round + 2 * tr, total = o_fin + tr;
     int r;
-    if ((r = st.g_run.ensure(m * sizeof(SeamRun))) || (r = st.g_tiles.ensure(std::max<int64_t>(total, 1) * sizeof(int4))) ||
-        (r = st.g_tsum.ensure(3 * std::max<int64_t>(tr, 1) * sizeof(int))) || (r = st.g_cnt.ensure(kSgCounters * sizeof(int))) ||
+    if ((r = st.g_run.ensure(m * sizeof(SeamRun))) || (r = st.g_tiles.ensure(total * sizeof(int4))) ||
+        (r = st.g_tsum.ensure(3 * tr * sizeof(int))) || (r = st.g_cnt.ensure(kSgCounters * sizeof(int))) ||
         (r = st.h_active.ensure(sizeof(int))))
         return r;
     int* cnt = static_cast<int*>(st.g_cnt.p);
     int* tsum = static_cast<int*>(st.g_tsum.p);
-    int* toff = tsum + std::max<int64_t>(tr, 1);
-    int* tbase = toff + std::max<int64_t>(tr, 1);
+    int* toff = tsum + tr;
+    int* tbase = toff + tr;
     int4* T = static_cast<int4*>(st.g_tiles.p);
     SeamGrid g{d_jobs, static_cast<const SeamSrcDev*>(st.srcs.p), static_cast<char*>(st.scratch.p),
                static_cast<SeamRun*>(st.g_run.p), cnt, ov, ot, m};
     const hipStream_t sm = c->stream;
-    auto grid = [](int64_t tiles) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(tiles, 1 << 20))); };
+    // tile launches: a multiple of 8 workgroups (see k_sg_tiles), looping over their lists
+    static_assert(kSgGrid % kXcds == 0, "tile launches keep to one XCD per workgroup");
+    auto grid = [](int64_t tiles) { return dim3((unsigned)std::min<int64_t>(tiles, kSgGrid)); };
     const dim3 per_seam((m + kSgThreads - 1) / kSgThreads), block(kSgThreads), dev_grid(kSgGrid);
 #define LVN_SG(...)                              \
     do {                                          \
@@ -1644,7 +1699,7 @@ int run_seam_grid(Ctx* c, SeamState& st, int m, const SeamJobDev* h_jobs, const 
     LVN_SG(k_sg_tiles, dim3(1), dim3(1024), 0, sm, g, 6, T + o_off, 0, cnt + kSgCntOffered);
     LVN_SG(k_sg_tiles, dim3(1), dim3(1024), 0, sm, g, 4, T + o_leaf, 0, cnt + kSgCntLeaf);
     LVN_SG(k_sg_tiles, dim3(1), dim3(1024), 0, sm, g, 5, T + o_node, 0, cnt + kSgCntNode);
-    LVN_SG(k_sg_init, grid(std::max<int64_t>(th, per_seam.x)), block, 0, sm, g, T);
+    LVN_SG(k_sg_init, dev_grid, block, 0, sm, g, T);
     LVN_SG(k_sg_select, grid(to), block, 0, sm, g, T + o_off);
     LVN_SG(k_sg_leaves, grid(th), block, 0, sm, g, T);
     LVN_SG(k_sg_seam, per_seam, block, 0, sm, g, 0);
@@ -1654,7 +1709,7 @@ int run_seam_grid(Ctx* c, SeamState& st, int m, const SeamJobDev* h_jobs, const 
     LVN_SG(k_sg_seam, per_seam, block, 0, sm, g, 2);
     LVN_SG(k_sg_order, grid(tn), block, 0, sm, g, T + o_node);
     LVN_SG(k_sg_tiles, dim3(lmax + 1), dim3(1024), 0, sm, g, 0, T + o_lev, (int)tn, cnt + kSgCntLevel);
-    for (int l = lmax; l >= 1; --l) LVN_SG(k_sg_level, dev_grid, block, 0, sm, g, T + o_lev + l * tn, l, 0);
+    for (int l = lmax - 1; l >= 1; --l) LVN_SG(k_sg_level, dev_grid, block, 0, sm, g, T + o_lev + l * tn, l, 0);
     LVN_SG(k_sg_seam, per_seam, block, 0, sm, g, 3);
     for (int l = 0; l < lmax; ++l) LVN_SG(k_sg_level, dev_grid, block, 0, sm, g, T + o_lev + l * tn, l, 1);
     LVN_SG(k_sg_vertices, grid(tl), block, 0, sm, g, T + o_leaf);
@@ -1679,18 +1734,17 @@ int run_seam_grid(Ctx* c, SeamState& st, int m, const SeamJobDev* h_jobs, const 
     return LVN_OK;
 }
 
-// Which seam pass runs m seams: the grid-wide steps spread each seam over the whole GPU
-// and win for calls of up to a few hundred seams (measured on the clipmap's seams: 2.5x for
-// its largest seam alone, 1.5x for 256 of them); one workgroup per seam keeps the higher
-// throughput of whole clipmaps (1,798 seams: 33.6 against 38.7 ms) and has the shorter
-// fixed cost for one small seam.  LVN_SEAM_PATH=grid or =workgroup in the environment
-// overrides the choice.
-bool seam_pass_grid(int m, const std::vector<SeamJobDev>& jobs, const std::vector<SeamSrcDev>& srcs) {
+// Which seam pass runs a call: the grid-wide steps (measured on the clipmap's seams: its
+// largest seam alone 3.1 against 7.8 ms, 512 seams 10.2 against 11.6 ms, all 1,798 seams
+// 32.6 against 33.4 ms) unless every seam is small, where one workgroup per seam has the
+// shorter fixed cost (a seam of 6,440 offered nodes: 0.20 against 0.38 ms; 36,606: 1.17
+// against 0.80 ms).  LVN_SEAM_PATH=grid or =workgroup in the environment overrides the
+// choice.
+bool seam_pass_grid(const std::vector<SeamJobDev>& jobs, const std::vector<SeamSrcDev>& srcs) {
     if (const char* e = getenv("LVN_SEAM_PATH")) {
         if (!strcmp(e, "grid")) return true;
         if (!strcmp(e, "workgroup")) return false;
     }
-    if (m > 320) return false;
     int64_t largest = 0;
     for (const SeamJobDev& J : jobs) {
         int64_t offered = 0;
@@ -1802,13 +1856,22 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
         }
         DevBuf& ov = *st.out_v[attempt];
         DevBuf& ot = *st.out_t[attempt];
+        std::vector<SeamJobDev> jobs;
+        {  // a seam whose caps outgrow the pass stays LVN_ERR_OVERFLOW
+            std::vector<int> fit;
+            for (int i : todo) {
+                SeamJobDev J = base[i];
+                if (!seam_caps(J, selected[i], grow[i])) continue;
+                fit.push_back(i);
+                jobs.push_back(J);
+            }
+            todo.swap(fit);
+        }
         const int m = (int)todo.size();
-        std::vector<SeamJobDev> jobs(m);
+        if (m == 0) break;
         uint64_t scratch = 0, v_off = 0, t_off = 0;
         for (int k = 0; k < m; ++k) {
             SeamJobDev& J = jobs[k];
-            J = base[todo[k]];
-            seam_caps(J, selected[todo[k]], grow[todo[k]]);
             J.scratch = scratch;
             J.v_off = v_off;
             J.t_off = t_off;
@@ -1827,7 +1890,7 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
         const auto* d_jobs = static_cast<const SeamJobDev*>(st.jobs.p);
         auto* d_res = static_cast<SeamResDev*>(st.res.p);
         auto* d_offs = static_cast<uint64_t*>(st.offs.p);
-        if (seam_pass_grid(m, jobs, srcs)) {
+        if (seam_pass_grid(jobs, srcs)) {
             if ((r = run_seam_grid(c, st, m, jobs.data(), srcs, d_jobs, static_cast<lvn_vertex*>(ov.p),
                                    static_cast<int*>(ot.p), d_res)))
                 return r;
