@@ -883,6 +883,24 @@ __device__ __forceinline__ SeamScratch sg_scratch(const SeamGrid& g, int s) {
     return seam_scratch(g.scratch + g.jobs[s].scratch, g.jobs[s]);
 }
 
+// The grid pass's nodes, in the node region of a seam's scratch as three dense arrays:
+// level << 48 | Morton code; (leaf count, depth-first leaf rank); for leaves the offer,
+// (source + 1) << 32 | index.  Leaves are nodes 0 .. nleaves - 1, internal nodes follow.
+struct SgNodes {
+    uint64_t* key;
+    int2* co;
+    uint64_t* offer;
+};
+static_assert(sizeof(SNode) >= 24, "the grid pass's node arrays fit the node region");
+__device__ __forceinline__ SgNodes sg_nodes(const SeamScratch& S, int node_cap) {
+    SgNodes q;
+    q.key = reinterpret_cast<uint64_t*>(S.nodes);
+    q.co = reinterpret_cast<int2*>(q.key + node_cap);
+    q.offer = reinterpret_cast<uint64_t*>(q.co + node_cap);
+    return q;
+}
+constexpr uint64_t kCodeMask = (1ull << 48) - 1;
+
 // Loop over the tiles of a list: tl = (seam, first element, source, 0); padding tiles
 // (seam -1) are skipped by the whole workgroup.  gridDim.x is a multiple of 8, so a
 // workgroup keeps to the tiles of its XCD.
@@ -1064,6 +1082,7 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_leaves(SeamGrid g, const int4
         const SeamJobDev& J = g.jobs[tl.x];
         SeamRun& R = g.run[tl.x];
         SeamScratch S = sg_scratch(g, tl.x);
+        const SgNodes Q = sg_nodes(S, J.node_cap);
         const int i0 = tl.y + threadIdx.x * kSgPer;
         uint64_t key[kSgPer];
         int cnt = 0;
@@ -1084,20 +1103,14 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_leaves(SeamGrid g, const int4
             const uint64_t p = S.pick[i];
             const SeamSrcDev src = g.srcs[J.src_begin + (int)(p >> 32) - 1];
             const lvn_seam_node& sn = src.nodes[(uint32_t)p];
-            SNode n;
-            n.size = src.size / J.vpc;
-            for (int a = 0; a < 3; ++a) n.min[a] = sn.local_min_and_material[a] * n.size + src.min[a];
-            n.parent = -1;
-            n.level = (int)(key[r] >> 48);
-            n.mat = sn.local_min_and_material[3];
-            n.count = 1;
-            n.offset = 0;
-            n.type = kNodeLeaf;
-            n.code = key[r] & ((1ull << 48) - 1);
-            n.payload = p;
-            S.nodes[idx] = n;
+            const int size = src.size / J.vpc;
+            int mn[3];
+            for (int a = 0; a < 3; ++a) mn[a] = sn.local_min_and_material[a] * size + src.min[a];
+            Q.key[idx] = key[r];
+            Q.co[idx] = make_int2(1, 0);
+            Q.offer[idx] = p;
             for (int c = 0; c < 8; ++c) S.ch[8 * idx + c] = -1;
-            S.shape[idx] = node_shape(n.min, J.host_min, n.size, false, n.mat);
+            S.shape[idx] = node_shape(mn, J.host_min, size, false, sn.local_min_and_material[3]);
             S.vals[i] = idx;
             S.leaf[i] = 1;
             ++idx;
@@ -1130,7 +1143,7 @@ __global__ void k_sg_seam(SeamGrid g, int step) {
         R.n = 0;
         if (R.linked) {
             SeamScratch S = sg_scratch(g, s);
-            if (S.nodes[R.root].type == kNodeInternal && ((S.shape[R.root] >> 56) & 1) == 0) {
+            if (R.root >= R.nleaves && ((S.shape[R.root] >> 56) & 1) == 0) {  // an internal root
                 S.list0[0] = make_call(R.root, -1, -1, -1, kCallCell);
                 R.n = 1;
             }
@@ -1149,14 +1162,16 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_ancestors(SeamGrid g, const i
         SeamRun& R = g.run[tl.x];
         if (R.status != 0) continue;
         SeamScratch S = sg_scratch(g, tl.x);
+        const SgNodes Q = sg_nodes(S, J.node_cap);
         const int mask = J.hash_cap - 1;
         const int root_size = 2 * J.host_size;
         const int nleaves = R.nleaves;
         for (int r = 0; r < kSgPer; ++r) {
             const int i = tl.y + threadIdx.x + r * kSgThreads;
             const bool leaf = i < nleaves;
-            const int level = leaf ? S.nodes[i].level : 0;
-            const uint64_t code = leaf ? S.nodes[i].code : 0;
+            const uint64_t lk = leaf ? Q.key[i] : 0;
+            const int level = (int)(lk >> 48);
+            const uint64_t code = lk & kCodeMask;
             int l = level - 1;
             bool act = leaf && l >= 0;
             while (__ballot(act)) {
@@ -1187,22 +1202,14 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_ancestors(SeamGrid g, const i
                     act = false;
                     continue;
                 }
-                SNode n;
-                n.size = root_size >> l;
-                int xyz[3];
+                const int size = root_size >> l;
+                int xyz[3], mn[3];
                 unmorton(c, l, xyz);
-                for (int a = 0; a < 3; ++a) n.min[a] = J.host_min[a] + xyz[a] * n.size;
-                n.parent = -1;
-                n.level = l;
-                n.mat = 0;
-                n.count = 0;
-                n.offset = 0;
-                n.type = kNodeInternal;
-                n.code = c;
-                n.payload = 0;
-                S.nodes[idx] = n;
+                for (int a = 0; a < 3; ++a) mn[a] = J.host_min[a] + xyz[a] * size;
+                Q.key[idx] = (uint64_t)l << 48 | c;
+                Q.co[idx] = make_int2(0, 0);
                 for (int q = 0; q < 8; ++q) S.ch[8 * idx + q] = -1;
-                S.shape[idx] = node_shape(n.min, J.host_min, n.size, true, 0);
+                S.shape[idx] = node_shape(mn, J.host_min, size, true, 0);
                 S.vals[slot] = idx;
                 --l;
                 act = l >= 0;
@@ -1228,21 +1235,22 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_links(SeamGrid g, const int4*
             continue;
         }
         SeamScratch S = sg_scratch(g, tl.x);
+        const SgNodes Q = sg_nodes(S, J.node_cap);
         const int nnodes = R.nnodes;
         for (int r = 0; r < kSgPer; ++r) {
             const int i = tl.y + threadIdx.x + r * kSgThreads;
             if (i >= nnodes) continue;
-            const int level = S.nodes[i].level;
+            const uint64_t k = Q.key[i];
+            const int level = (int)(k >> 48);
             if (i >= R.nleaves) atomicAdd(&hist[level], 1);  // internal nodes (created after the leaves)
             if (level == 0) continue;
-            const uint64_t c = S.nodes[i].code;
+            const uint64_t c = k & kCodeMask;
             const int slot = hash_find(S.keys, J.hash_cap - 1, (uint64_t)(level - 1) << 48 | (c >> 3));
             const int p = slot < 0 ? -1 : S.vals[slot];
             if (p < 0) {  // every parent was inserted: not reached
                 set_status(&R.status, LVN_ERR_OVERFLOW);
                 continue;
             }
-            S.nodes[i].parent = p;
             S.ch[8 * p + (c & 7)] = i;
         }
         __syncthreads();
@@ -1259,6 +1267,7 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_order(SeamGrid g, const int4*
         SeamRun& R = g.run[tl.x];
         if (R.status != 0 || tl.y >= R.nnodes || tl.y + kSgTile <= R.nleaves) continue;
         SeamScratch S = sg_scratch(g, tl.x);
+        const SgNodes Q = sg_nodes(S, g.jobs[tl.x].node_cap);
         if (threadIdx.x < kSgLevels) hist[threadIdx.x] = 0;
         __syncthreads();
         const int nnodes = R.nnodes, nleaves = R.nleaves;
@@ -1266,7 +1275,7 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_order(SeamGrid g, const int4*
 #pragma unroll
         for (int r = 0; r < kSgPer; ++r) {
             const int i = tl.y + threadIdx.x + r * kSgThreads;
-            lv[r] = i >= nleaves && i < nnodes ? S.nodes[i].level : -1;
+            lv[r] = i >= nleaves && i < nnodes ? (int)(Q.key[i] >> 48) : -1;
             if (lv[r] >= 0) atomicAdd(&hist[lv[r]], 1);
         }
         __syncthreads();
@@ -1290,6 +1299,7 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_level(SeamGrid g, const int4*
         SeamRun& R = g.run[tl.x];
         if (ranks && !R.linked) continue;
         SeamScratch S = sg_scratch(g, tl.x);
+        const SgNodes Q = sg_nodes(S, g.jobs[tl.x].node_cap);
         const int first = R.lev[l], end = R.lev[l + 1];
         for (int r = 0; r < kSgPer; ++r) {
             const int k = first + tl.y + threadIdx.x + r * kSgThreads;
@@ -1300,18 +1310,18 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_level(SeamGrid g, const int4*
             for (int c = 0; c < 8; ++c) ch[c] = S.ch[8 * i + c];
             int cnt[8];
 #pragma unroll
-            for (int c = 0; c < 8; ++c) cnt[c] = ch[c] >= 0 ? S.nodes[ch[c]].count : 0;
+            for (int c = 0; c < 8; ++c) cnt[c] = ch[c] >= 0 ? Q.co[ch[c]].x : 0;
             if (!ranks) {
                 int sum = 0;
 #pragma unroll
                 for (int c = 0; c < 8; ++c) sum += cnt[c];
-                S.nodes[i].count = sum;
+                Q.co[i].x = sum;
             } else {
-                int off = S.nodes[i].offset;
+                int off = Q.co[i].y;
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
                     if (ch[c] < 0) continue;
-                    S.nodes[ch[c]].offset = off;
+                    Q.co[ch[c]].y = off;
                     off += cnt[c];
                 }
             }
@@ -1326,11 +1336,12 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_vertices(SeamGrid g, const in
         SeamRun& R = g.run[tl.x];
         if (!R.linked) continue;
         SeamScratch S = sg_scratch(g, tl.x);
+        const SgNodes Q = sg_nodes(S, J.node_cap);
         const int nleaves = R.nleaves;
         for (int r = 0; r < kSgPer; ++r) {
             const int i = tl.y + threadIdx.x + r * kSgThreads;
             if (i >= nleaves) continue;
-            const uint64_t p = S.nodes[i].payload;
+            const uint64_t p = Q.offer[i];
             const lvn_seam_node& sn = g.srcs[J.src_begin + (int)(p >> 32) - 1].nodes[(uint32_t)p];
             lvn_vertex v;
             v.pos[0] = sn.pos[0];
@@ -1344,8 +1355,8 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_vertices(SeamGrid g, const in
             v.colour[0] = J.colour[0];
             v.colour[1] = J.colour[1];
             v.colour[2] = J.colour[2];
-            v.colour[3] = (float)(S.nodes[i].mat >> 8);
-            g.out_v[J.v_off + S.nodes[i].offset] = v;
+            v.colour[3] = (float)(sn.local_min_and_material[3] >> 8);
+            g.out_v[J.v_off + Q.co[i].y] = v;
         }
     }
 }
@@ -1469,14 +1480,15 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_triangles(SeamGrid g, const i
         const SeamJobDev& J = g.jobs[tl.x];
         SeamRun& R = g.run[tl.x];
         SeamScratch S = sg_scratch(g, tl.x);
-        const Call* Q = S.list0;
+        const SgNodes Q = sg_nodes(S, J.node_cap);
+        const Call* quads = S.list0;
         const int n = R.n;
         for (int r = 0; r < kSgPer; ++r) {
             const int i = tl.y + threadIdx.x + r * kSgThreads;
             if (i >= n) continue;
-            const Call c = Q[i];
+            const Call c = quads[i];
             int v[4];
-            for (int j = 0; j < 4; ++j) v[j] = S.nodes[c.node(j)].offset;
+            for (int j = 0; j < 4; ++j) v[j] = Q.co[c.node(j)].y;
             int* t = g.out_t + 3 * (J.t_off + 2 * (uint64_t)i);
             if (c.meta() & 16) {
                 t[0] = v[0]; t[1] = v[3]; t[2] = v[1];
