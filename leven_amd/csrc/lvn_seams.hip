@@ -1577,6 +1577,10 @@ struct SeamState {
     HostBuf h_in, h_jobs, h_srcs, h_res, h_pv, h_pt, h_counts, h_active;
     std::vector<std::unique_ptr<DevBuf>> out_v, out_t;  // per attempt
     std::vector<SeamLoc> loc;                            // per seam of the last call
+    std::vector<hipEvent_t> ev;                          // ends of the download chunks
+    ~SeamState() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
 };
 
 void destroy_seam_state(SeamState* s) { delete s; }
@@ -1606,7 +1610,27 @@ void parallel_ranges(int n, size_t bytes, F&& f) {
     for (auto& t : pool) t.join();
 }
 
-// Caps of one seam from the number of offered nodes; `grow` scales them after an overflow.
+// Ends of consecutive ranges of [0, n) holding about `target` bytes each (bytes(k) of
+// element k), at most 32 ranges: host copies of one range overlap the DMA of the next.
+template <class F>
+std::vector<int> byte_chunks(int n, F&& bytes, size_t target) {
+    size_t total = 0;
+    for (int k = 0; k < n; ++k) total += bytes(k);
+    target = std::max(target, total / 32 + 1);
+    std::vector<int> ends;
+    size_t acc = 0;
+    for (int k = 0; k < n; ++k) {
+        acc += bytes(k);
+        if (acc >= target) {
+            ends.push_back(k + 1);
+            acc = 0;
+        }
+    }
+    if (ends.empty() || ends.back() != n) ends.push_back(n);
+    return ends;
+}
+constexpr size_t kCopyChunk = 128u << 20;
+
 // Caps of one seam from its selected nodes; `grow` scales them after an overflow.  A
 // seam's octree has about 1.4 nodes per leaf (the leaves lie on faces) and at most ~3
 // quads per leaf.
@@ -1794,14 +1818,28 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
     if (r) return r;
     auto* h_in = static_cast<lvn_seam_node*>(st.h_in.p);
     auto* d_in = static_cast<lvn_seam_node*>(st.in.p);
-    {
+    {  // lists staged in pinned memory in chunks, each uploaded while the next is staged
         std::vector<std::pair<const lvn_seam_node*, std::pair<uint64_t, size_t>>> lists(uploaded.begin(),
                                                                                          uploaded.end());
-        parallel_ranges((int)lists.size(), in_nodes * sizeof(lvn_seam_node), [&](int lo, int hi) {
-            for (int k = lo; k < hi; ++k)
-                std::memcpy(h_in + lists[k].second.first, lists[k].first,
-                            lists[k].second.second * sizeof(lvn_seam_node));
-        });
+        std::sort(lists.begin(), lists.end(), [](const auto& a, const auto& b) { return a.second.first < b.second.first; });
+        const auto ends = byte_chunks((int)lists.size(), [&](int k) { return lists[k].second.second * sizeof(lvn_seam_node); },
+                                      kCopyChunk);
+        int k0 = 0;
+        for (int k1 : ends) {
+            if (k1 == k0) continue;
+            size_t bytes = 0;
+            for (int k = k0; k < k1; ++k) bytes += lists[k].second.second * sizeof(lvn_seam_node);
+            parallel_ranges(k1 - k0, bytes, [&](int lo, int hi) {
+                for (int k = k0 + lo; k < k0 + hi; ++k)
+                    std::memcpy(h_in + lists[k].second.first, lists[k].first,
+                                lists[k].second.second * sizeof(lvn_seam_node));
+            });
+            const uint64_t first = lists[k0].second.first;
+            const uint64_t end = lists[k1 - 1].second.first + lists[k1 - 1].second.second;
+            LVN_HIP(hipMemcpyAsync(d_in + first, h_in + first, (end - first) * sizeof(lvn_seam_node),
+                                   hipMemcpyHostToDevice, c->stream));
+            k0 = k1;
+        }
     }
     srcs.resize(std::max(total_sources, 1));
     for (int s = 0; s < total_sources; ++s) {
@@ -1818,8 +1856,6 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
             if (rr) return rr;
         }
     }
-    if (in_nodes)
-        LVN_HIP(hipMemcpyAsync(d_in, h_in, in_nodes * sizeof(lvn_seam_node), hipMemcpyHostToDevice, c->stream));
     if ((r = st.srcs.ensure(srcs.size() * sizeof(SeamSrcDev))) || (r = st.h_srcs.ensure(srcs.size() * sizeof(SeamSrcDev))))
         return r;
     std::memcpy(st.h_srcs.p, srcs.data(), srcs.size() * sizeof(SeamSrcDev));
@@ -1922,21 +1958,6 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
         const auto* rs = reinterpret_cast<const SeamResDev*>(h_res);
         const auto* offs = reinterpret_cast<const uint64_t*>(h_res + m * sizeof(SeamResDev));
         const uint64_t tv = offs[2 * m], tt = offs[2 * m + 1];
-        if (host_out) {  // pack every seam's mesh back to back and read it once
-            if ((r = st.pv.ensure(std::max<uint64_t>(tv, 1) * sizeof(lvn_vertex))) ||
-                (r = st.pt.ensure(std::max<uint64_t>(tt, 1) * 12)) ||
-                (r = st.h_pv.ensure(std::max<uint64_t>(tv, 1) * sizeof(lvn_vertex))) ||
-                (r = st.h_pt.ensure(std::max<uint64_t>(tt, 1) * 12)))
-                return r;
-            hipLaunchKernelGGL(k_seam_pack, dim3(m), dim3(256), 0, c->stream, d_jobs, d_res, d_offs,
-                               static_cast<const lvn_vertex*>(ov.p), static_cast<const int*>(ot.p),
-                               static_cast<lvn_vertex*>(st.pv.p), static_cast<int*>(st.pt.p));
-            LVN_HIP(hipGetLastError());
-            if (tv)
-                LVN_HIP(hipMemcpyAsync(st.h_pv.p, st.pv.p, tv * sizeof(lvn_vertex), hipMemcpyDeviceToHost, c->stream));
-            if (tt) LVN_HIP(hipMemcpyAsync(st.h_pt.p, st.pt.p, tt * 12, hipMemcpyDeviceToHost, c->stream));
-            LVN_HIP(hipStreamSynchronize(c->stream));
-        }
         std::vector<int> again;
         for (int k = 0; k < m; ++k) {
             const int i = todo[k];
@@ -1958,18 +1979,60 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
             l.t_off = jobs[k].t_off;
             if ((o.v && o.nv > o.v_cap) || (o.tri && o.nt > o.tri_cap)) o.status = LVN_ERR_CAPACITY;
         }
-        if (host_out)
-            parallel_ranges(m, (tv * sizeof(lvn_vertex) + tt * 12), [&](int lo, int hi) {
-                for (int k = lo; k < hi; ++k) {
-                    const lvn_seam_out& o = outs[todo[k]];
-                    if (rs[k].status != LVN_OK || o.status != LVN_OK) continue;
-                    if (o.nv && o.v)
-                        std::memcpy(o.v, static_cast<const lvn_vertex*>(st.h_pv.p) + offs[2 * k],
-                                    o.nv * sizeof(lvn_vertex));
-                    if (o.nt && o.tri)
-                        std::memcpy(o.tri, static_cast<const int*>(st.h_pt.p) + 3 * offs[2 * k + 1], (size_t)o.nt * 12);
-                }
-            });
+        if (host_out) {
+            // every seam's mesh packed back to back, read back in chunks of seams; each chunk
+            // is copied into the callers' buffers while the next one is in flight
+            if ((r = st.pv.ensure(std::max<uint64_t>(tv, 1) * sizeof(lvn_vertex))) ||
+                (r = st.pt.ensure(std::max<uint64_t>(tt, 1) * 12)) ||
+                (r = st.h_pv.ensure(std::max<uint64_t>(tv, 1) * sizeof(lvn_vertex))) ||
+                (r = st.h_pt.ensure(std::max<uint64_t>(tt, 1) * 12)))
+                return r;
+            hipLaunchKernelGGL(k_seam_pack, dim3(m), dim3(256), 0, c->stream, d_jobs, d_res, d_offs,
+                               static_cast<const lvn_vertex*>(ov.p), static_cast<const int*>(ot.p),
+                               static_cast<lvn_vertex*>(st.pv.p), static_cast<int*>(st.pt.p));
+            LVN_HIP(hipGetLastError());
+            auto seam_bytes = [&](int k) {
+                return (offs[2 * k + 2] - offs[2 * k]) * sizeof(lvn_vertex) + (offs[2 * k + 3] - offs[2 * k + 1]) * 12;
+            };
+            const auto ends = byte_chunks(m, seam_bytes, kCopyChunk);
+            while (st.ev.size() < ends.size()) {
+                hipEvent_t e;
+                LVN_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                st.ev.push_back(e);
+            }
+            auto* hv = static_cast<lvn_vertex*>(st.h_pv.p);
+            auto* ht = static_cast<int*>(st.h_pt.p);
+            const auto* dv = static_cast<const lvn_vertex*>(st.pv.p);
+            const auto* dt = static_cast<const int*>(st.pt.p);
+            int k0 = 0;
+            for (size_t ci = 0; ci < ends.size(); ++ci) {
+                const int k1 = ends[ci];
+                const uint64_t v0 = offs[2 * k0], v1 = offs[2 * k1], t0 = offs[2 * k0 + 1], t1 = offs[2 * k1 + 1];
+                if (v1 > v0)
+                    LVN_HIP(hipMemcpyAsync(hv + v0, dv + v0, (v1 - v0) * sizeof(lvn_vertex), hipMemcpyDeviceToHost,
+                                           c->stream));
+                if (t1 > t0)
+                    LVN_HIP(hipMemcpyAsync(ht + 3 * t0, dt + 3 * t0, (t1 - t0) * 12, hipMemcpyDeviceToHost, c->stream));
+                LVN_HIP(hipEventRecord(st.ev[ci], c->stream));
+                k0 = k1;
+            }
+            k0 = 0;
+            for (size_t ci = 0; ci < ends.size(); ++ci) {
+                const int k1 = ends[ci];
+                LVN_HIP(hipEventSynchronize(st.ev[ci]));
+                size_t bytes = 0;
+                for (int k = k0; k < k1; ++k) bytes += seam_bytes(k);
+                parallel_ranges(k1 - k0, bytes, [&](int lo, int hi) {
+                    for (int k = k0 + lo; k < k0 + hi; ++k) {
+                        const lvn_seam_out& o = outs[todo[k]];
+                        if (rs[k].status != LVN_OK || o.status != LVN_OK) continue;
+                        if (o.nv && o.v) std::memcpy(o.v, hv + offs[2 * k], o.nv * sizeof(lvn_vertex));
+                        if (o.nt && o.tri) std::memcpy(o.tri, ht + 3 * offs[2 * k + 1], (size_t)o.nt * 12);
+                    }
+                });
+                k0 = k1;
+            }
+        }
         todo.swap(again);
     }
     int first = LVN_OK;
