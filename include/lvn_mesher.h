@@ -334,6 +334,12 @@ int  lvn_server_drain(lvn_server* s);
  * since been evicted or invalidated by an edit). */
 int  lvn_server_get_mesh(lvn_server* s, int64_t ticket, lvn_vertex* v, int32_t v_cap, int32_t* nv,
                          int32_t* tri, int32_t tri_cap, int32_t* nt, lvn_seam_node* sn, int32_t s_cap, int32_t* ns);
+/* Launch shapes of the server's batches, as lvn_batch_set_tuning (for instance the result
+ * of lvn_batch_autotune on a batch of the expected chunks; zero density_ysplit,
+ * cross_lanes, density_per_cu or height_lines = the automatic choice by batch size).
+ * Applies to batches launched after the call; meshes do not depend on it.  No counterpart
+ * in the reference (its chunks are meshed one by one). */
+int  lvn_server_set_tuning(lvn_server* s, const lvn_tuning* t);
 /* stats[0..7] = slots, batches launched, chunks meshed, cache hits, evictions,
  * results re-meshed because an edit arrived in flight, cached chunks, max batch seen. */
 int  lvn_server_stats(lvn_server* s, int64_t* stats);

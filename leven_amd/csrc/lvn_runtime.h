@@ -188,5 +188,7 @@ void fill_chunk_desc(ChunkDesc& cd, const int32_t* min, int32_t size, int N);
 // N * LEAF_SIZE_SCALE (sampleScale = size / (N * 4) >= 1, src/compute_density_field.cpp:149).
 inline bool valid_chunk_size(int32_t size, int N) { return size >= N * kLeafSizeScale && size % (N * kLeafSizeScale) == 0; }
 bool aabb_overlaps(const int32_t* mn, int32_t size, const std::array<int32_t, 6>& bb);
+// Launch shapes within their documented ranges (F = density samples per axis).
+bool tuning_valid(const lvn_tuning& t, int F);
 
 }  // namespace lvn

@@ -689,14 +689,16 @@ static int time_stages(Batch* b, const KArgs& a, float* stage_ms) {
     return LVN_OK;
 }
 
-static bool valid_tuning(const Batch* b, const lvn_tuning& t) {
-    return t.density_ysplit >= 0 && t.density_ysplit <= b->ctx.G.F && t.persistent_per_cu >= 1 &&
+bool tuning_valid(const lvn_tuning& t, int F) {
+    return t.density_ysplit >= 0 && t.density_ysplit <= F && t.persistent_per_cu >= 1 &&
            t.persistent_per_cu <= 2 && t.row_blocks >= 1 && t.row_blocks <= (1 << 20) && t.voxel_blocks >= 1 &&
            t.voxel_blocks <= (1 << 20) &&
            (t.cross_lanes == 0 || t.cross_lanes == 1 || t.cross_lanes == 2 || t.cross_lanes == 4 ||
             t.cross_lanes == 8 || t.cross_lanes == kCrossSampleParallel) &&
            t.density_per_cu >= 0 && t.density_per_cu <= 2 && t.height_lines >= 0 && t.height_lines <= 2;
 }
+
+static bool valid_tuning(const Batch* b, const lvn_tuning& t) { return tuning_valid(t, b->ctx.G.F); }
 
 static void drop_graph(Batch* b) { b->graph_n = -1; }
 

@@ -55,10 +55,11 @@ struct Lane {
     int* d_item_prefix = nullptr;
     DevOp* d_ops = nullptr;
     DevOp* h_ops = nullptr;
+    int4* d_boxes = nullptr;  // per op: AABB min, AABB max
+    int4* h_boxes = nullptr;
     int ops_cap = 0;
-    int* d_op_index = nullptr;
-    int* h_op_index = nullptr;
-    int idx_cap = 0;
+    int* d_op_index = nullptr;  // per chunk of the batch: its overlapping ops (k_op_index)
+    int64_t idx_cap = 0;
     bool busy = false;
     std::vector<Job> jobs;
     std::vector<Key> keys;
@@ -92,6 +93,9 @@ struct Server {
     bool stop = false;
     int error = LVN_OK;
 
+    std::mutex tuning_mu;          // launch shapes (lvn_server_set_tuning), read at each launch
+    lvn_tuning tuning = {0, kDefaultPersistPerCu, kDefaultRowBlocks, kDefaultVoxelBlocks, 0, kDefaultDensityPerCu, 0, {0}};
+
     std::mutex cache_mu;           // cache, ticket_key and stats (read by lvn_server_get_mesh / _stats)
     std::map<Key, Entry> cache;
     // completed mesh request -> its chunk, while the chunk's mesh is the one it was
@@ -121,6 +125,48 @@ struct Server {
     ~Server();
 };
 
+// The stored CSG ops of each chunk of a lane's batch, in stored order: op k is listed for
+// chunk i when its AABB overlaps the chunk's (aabb_overlaps, src/aabb.h:26-35: touching
+// counts).  One workgroup per chunk, ballots compact the hits in order; chunk i's list
+// goes to index[i * nops ..], its start and length into the chunk's descriptor.
+constexpr int kOpIndexThreads = 256;
+__global__ __launch_bounds__(kOpIndexThreads) void k_op_index(ChunkDesc* chunks, const int4* boxes, int nops,
+                                                              int* index) {
+    __shared__ int wsum[kOpIndexThreads / 64];
+    __shared__ int carry;
+    ChunkDesc& cd = chunks[blockIdx.x];
+    const int m0 = cd.min[0], m1 = cd.min[1], m2 = cd.min[2], size = cd.size;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int* out = index + (size_t)blockIdx.x * nops;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int b = 0; b < nops; b += kOpIndexThreads) {
+        const int k = b + threadIdx.x;
+        bool hit = false;
+        if (k < nops) {
+            const int4 lo = boxes[2 * k], hi = boxes[2 * k + 1];
+            hit = !(m0 + size < lo.x || m1 + size < lo.y || m2 + size < lo.z || m0 > hi.x || m1 > hi.y || m2 > hi.z);
+        }
+        const uint64_t mask = __ballot(hit);
+        if (lane == 0) wsum[w] = __popcll(mask);
+        __syncthreads();
+        int base = carry, total = 0;
+#pragma unroll
+        for (int i = 0; i < kOpIndexThreads / 64; ++i) {
+            base += i < w ? wsum[i] : 0;
+            total += wsum[i];
+        }
+        if (hit) out[base + __popcll(mask & ((1ull << lane) - 1))] = k;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        cd.op_start = blockIdx.x * nops;
+        cd.op_count = carry;
+    }
+}
+
 #define LVN_TRY(call)                                    \
     do {                                                 \
         if ((call) != hipSuccess) return LVN_ERR_DEVICE; \
@@ -138,6 +184,15 @@ int Server::init_lane(Lane& L) {
     LVN_TRY(hipHostMalloc(&L.h_slot_map, sizeof(int) * B, hipHostMallocDefault));
     LVN_TRY(hipMalloc(&L.d_queue, sizeof(int) * kNumQueues));
     LVN_TRY(hipMalloc(&L.d_item_prefix, sizeof(int) * (B + 1)));
+    // op lists sized up front (1,024 stored ops, op lists of a full batch with up to 256
+    // stored ops) so that a lane's first batches allocate nothing; more ops grow them
+    L.ops_cap = 1024;
+    L.idx_cap = std::max<int64_t>(4096, 256 * (int64_t)B);
+    LVN_TRY(hipMalloc(&L.d_ops, sizeof(DevOp) * L.ops_cap));
+    LVN_TRY(hipHostMalloc(&L.h_ops, sizeof(DevOp) * L.ops_cap, hipHostMallocDefault));
+    LVN_TRY(hipMalloc(&L.d_boxes, sizeof(int4) * 2 * L.ops_cap));
+    LVN_TRY(hipHostMalloc(&L.h_boxes, sizeof(int4) * 2 * L.ops_cap, hipHostMallocDefault));
+    LVN_TRY(hipMalloc(&L.d_op_index, sizeof(int) * L.idx_cap));
     return LVN_OK;
 }
 
@@ -161,8 +216,9 @@ Server::~Server() {
         if (L.d_item_prefix) (void)hipFree(L.d_item_prefix);
         if (L.d_ops) (void)hipFree(L.d_ops);
         if (L.h_ops) (void)hipHostFree(L.h_ops);
+        if (L.d_boxes) (void)hipFree(L.d_boxes);
+        if (L.h_boxes) (void)hipHostFree(L.h_boxes);
         if (L.d_op_index) (void)hipFree(L.d_op_index);
-        if (L.h_op_index) (void)hipHostFree(L.h_op_index);
         if (L.stream) (void)hipStreamDestroy(L.stream);
     }
 }
@@ -253,40 +309,50 @@ int Server::launch(Lane& L, const std::vector<Job>& batch, const std::vector<int
     std::lock_guard<std::mutex> glock(g_state.mu);
     if (!g_state.init) return LVN_ERR_STATE;
     const int nops = (int)ops.size();
-    std::vector<int> index;
     for (int i = 0; i < n; ++i) {
         const int32_t mn[3] = {std::get<0>(batch[i].key), std::get<1>(batch[i].key), std::get<2>(batch[i].key)};
-        const int size = std::get<3>(batch[i].key);
         ChunkDesc& cd = L.h_chunks[i];
-        fill_chunk_desc(cd, mn, size, arena.N);
-        cd.op_start = (int)index.size();
-        for (int k = 0; k < nops; ++k)
-            if (aabb_overlaps(mn, size, aabbs[k])) index.push_back(k);
-        cd.op_count = (int)index.size() - cd.op_start;
+        fill_chunk_desc(cd, mn, std::get<3>(batch[i].key), arena.N);
+        cd.op_start = cd.op_count = 0;  // set by k_op_index
         L.h_slot_map[i] = slots[i];
     }
-    if (std::max(nops, 1) > L.ops_cap) {
-        if (L.d_ops) LVN_TRY(hipFree(L.d_ops));
-        if (L.h_ops) LVN_TRY(hipHostFree(L.h_ops));
-        L.ops_cap = std::max(2 * L.ops_cap, std::max(nops, 64));
+    if (nops > L.ops_cap) {
+        LVN_TRY(hipFree(L.d_ops));
+        LVN_TRY(hipHostFree(L.h_ops));
+        LVN_TRY(hipFree(L.d_boxes));
+        LVN_TRY(hipHostFree(L.h_boxes));
+        L.d_ops = nullptr;
+        L.h_ops = nullptr;
+        L.d_boxes = L.h_boxes = nullptr;
+        L.ops_cap = std::max(2 * L.ops_cap, nops);
         LVN_TRY(hipMalloc(&L.d_ops, sizeof(DevOp) * L.ops_cap));
         LVN_TRY(hipHostMalloc(&L.h_ops, sizeof(DevOp) * L.ops_cap, hipHostMallocDefault));
+        LVN_TRY(hipMalloc(&L.d_boxes, sizeof(int4) * 2 * L.ops_cap));
+        LVN_TRY(hipHostMalloc(&L.h_boxes, sizeof(int4) * 2 * L.ops_cap, hipHostMallocDefault));
     }
-    if (std::max((int)index.size(), 1) > L.idx_cap) {
-        if (L.d_op_index) LVN_TRY(hipFree(L.d_op_index));
-        if (L.h_op_index) LVN_TRY(hipHostFree(L.h_op_index));
-        L.idx_cap = std::max(2 * L.idx_cap, std::max((int)index.size(), 256));
+    if ((int64_t)n * nops > L.idx_cap) {
+        LVN_TRY(hipFree(L.d_op_index));
+        L.d_op_index = nullptr;
+        L.idx_cap = std::max(2 * L.idx_cap, (int64_t)n * nops);
         LVN_TRY(hipMalloc(&L.d_op_index, sizeof(int) * L.idx_cap));
-        LVN_TRY(hipHostMalloc(&L.h_op_index, sizeof(int) * L.idx_cap, hipHostMallocDefault));
     }
-    for (int k = 0; k < nops; ++k) L.h_ops[k] = make_dev_op(ops[k]);
-    if (!index.empty()) std::memcpy(L.h_op_index, index.data(), sizeof(int) * index.size());
-    if (nops) LVN_TRY(hipMemcpyAsync(L.d_ops, L.h_ops, sizeof(DevOp) * nops, hipMemcpyHostToDevice, L.stream));
-    if (!index.empty())
-        LVN_TRY(hipMemcpyAsync(L.d_op_index, L.h_op_index, sizeof(int) * index.size(), hipMemcpyHostToDevice,
-                               L.stream));
+    for (int k = 0; k < nops; ++k) {
+        L.h_ops[k] = make_dev_op(ops[k]);
+        const auto& bb = aabbs[k];
+        L.h_boxes[2 * k] = make_int4(bb[0], bb[1], bb[2], 0);
+        L.h_boxes[2 * k + 1] = make_int4(bb[3], bb[4], bb[5], 0);
+    }
+    if (nops) {
+        LVN_TRY(hipMemcpyAsync(L.d_ops, L.h_ops, sizeof(DevOp) * nops, hipMemcpyHostToDevice, L.stream));
+        LVN_TRY(hipMemcpyAsync(L.d_boxes, L.h_boxes, sizeof(int4) * 2 * nops, hipMemcpyHostToDevice, L.stream));
+    }
     LVN_TRY(hipMemcpyAsync(L.d_chunks, L.h_chunks, sizeof(ChunkDesc) * n, hipMemcpyHostToDevice, L.stream));
     LVN_TRY(hipMemcpyAsync(L.d_slot_map, L.h_slot_map, sizeof(int) * n, hipMemcpyHostToDevice, L.stream));
+    if (nops) {
+        hipLaunchKernelGGL(k_op_index, dim3(n), dim3(kOpIndexThreads), 0, L.stream, L.d_chunks,
+                           static_cast<const int4*>(L.d_boxes), nops, L.d_op_index);
+        LVN_TRY(hipGetLastError());
+    }
     KArgs a = arena.kargs();
     a.chunks = L.d_chunks;
     a.counters = L.d_counters;
@@ -295,8 +361,19 @@ int Server::launch(Lane& L, const std::vector<Job>& batch, const std::vector<int
     a.item_prefix = L.d_item_prefix;
     a.ops = L.d_ops;
     a.op_index = L.d_op_index;
-    a.ysplit = n >= 32 ? 1 : 6;
-    a.height_lines = n >= kHeightLinesMinChunks ? 1 : 0;
+    lvn_tuning t;
+    {
+        std::lock_guard<std::mutex> tl(tuning_mu);
+        t = tuning;
+    }
+    // zero fields: the automatic choice by batch size
+    a.ysplit = t.density_ysplit ? t.density_ysplit : n >= 32 ? 1 : 6;
+    a.height_lines = t.height_lines ? (t.height_lines == 2 ? 1 : 0) : (n >= kHeightLinesMinChunks ? 1 : 0);
+    a.persist_per_cu = t.persistent_per_cu;
+    a.density_per_cu = t.density_per_cu ? t.density_per_cu : kDefaultDensityPerCu;
+    a.row_blocks = t.row_blocks;
+    a.voxel_blocks = t.voxel_blocks;
+    a.cross_lanes = t.cross_lanes;
     launch_pipeline(a, n, L.stream);
     LVN_TRY(hipGetLastError());
     LVN_TRY(hipMemcpyAsync(L.h_counters, L.d_counters, sizeof(Counters) * n, hipMemcpyDeviceToHost, L.stream));
@@ -555,6 +632,14 @@ int lvn_server_get_mesh(lvn_server* sh, int64_t ticket, lvn_vertex* v, int32_t v
     if (e.nt && tri && hipMemcpy(tri, slot + c.L.tris, 12 * (size_t)e.nt, hipMemcpyDeviceToHost)) return LVN_ERR_DEVICE;
     if (e.ns && sn && hipMemcpy(sn, slot + c.L.seams, sizeof(lvn_seam_node) * (size_t)e.ns, hipMemcpyDeviceToHost))
         return LVN_ERR_DEVICE;
+    return LVN_OK;
+}
+
+int lvn_server_set_tuning(lvn_server* sh, const lvn_tuning* t) {
+    Server* s = reinterpret_cast<Server*>(sh);
+    if (!s || !t || !tuning_valid(*t, s->arena.G.F)) return LVN_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(s->tuning_mu);
+    s->tuning = *t;
     return LVN_OK;
 }
 

@@ -26,7 +26,7 @@ EXPORTS = [
     "lvn_batch_run_timed", "lvn_batch_stage_name", "lvn_batch_counts", "lvn_batch_get_mesh", "lvn_batch_work",
     "lvn_batch_get_tuning", "lvn_batch_set_tuning", "lvn_batch_autotune",
     "lvn_server_create", "lvn_server_destroy", "lvn_server_submit", "lvn_server_poll", "lvn_server_drain",
-    "lvn_server_get_mesh", "lvn_server_stats", "lvn_server_submit_many", "lvn_generate_seam_mesh", "lvn_generate_seam_meshes",
+    "lvn_server_get_mesh", "lvn_server_stats", "lvn_server_submit_many", "lvn_server_set_tuning", "lvn_generate_seam_mesh", "lvn_generate_seam_meshes",
     "lvn_seam_device_mesh", "lvn_batch_seam_meshes", "lvn_batch_seam_device_mesh",
     "lvn_generate_chunk_mesh_noalloc",
     "lvn_clipmap_create", "lvn_clipmap_destroy", "lvn_clipmap_update", "lvn_clipmap_list", "lvn_clipmap_commit",
@@ -102,6 +102,7 @@ def lib():
             "lvn_server_get_mesh": (i32, [vp, C.c_int64, vp, i32, C.POINTER(i32), vp, i32, C.POINTER(i32), vp, i32,
                                           C.POINTER(i32)]),
             "lvn_server_stats": (i32, [vp, vp]),
+            "lvn_server_set_tuning": (i32, [vp, vp]),
             "lvn_generate_seam_mesh": (i32, [vp, vp, i32, vp, i32, vp, vp, i32, C.POINTER(i32), vp, i32,
                                              C.POINTER(i32)]),
             "lvn_generate_seam_meshes": (i32, [vp, i32, vp, vp, vp]),
@@ -598,6 +599,13 @@ class Server:
 
     def drain(self):
         check(lib().lvn_server_drain(self._h), "lvn_server_drain")
+
+    def set_tuning(self, tuning):
+        """Launch shapes for the batches launched from now on (a dict as Batch.tuning())."""
+        t = Tuning()
+        for k in Batch.TUNING:
+            setattr(t, k, int(tuning.get(k, 0)))
+        check(lib().lvn_server_set_tuning(self._h, C.byref(t)), "lvn_server_set_tuning")
 
     def stats(self):
         s = np.zeros(8, dtype=np.int64)

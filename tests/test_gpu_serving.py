@@ -75,6 +75,30 @@ def test_served_alone_equals_in_mix_and_batched():
     mesher.clear_csg_ops()
 
 
+def test_tuned_server_serves_the_same_meshes():
+    """Launch shapes set on a server (as the autotuner picks them) change no mesh, and a
+    shape outside its range is rejected."""
+    w = _setup()
+    meshes = []
+    for tuning in (None, dict(density_ysplit=3, persistent_per_cu=1, row_blocks=4096, voxel_blocks=32768,
+                              cross_lanes=2, density_per_cu=2, height_lines=1)):
+        s = mesher.Server(64, hbm_budget=8 << 30, max_batch=32, lanes=2)
+        if tuning:
+            s.set_tuning(tuning)
+        tickets = [s.submit_mesh(w["mins"][i], int(w["sizes"][i])) for i in range(0, 512, 7)]
+        s.drain()
+        meshes.append([s.get_mesh(t) for t in tickets])
+        s.close()
+    assert sum(len(m[1]) > 0 for m in meshes[0]) > 10
+    for a, b in zip(*meshes):
+        assert _same(a, b)
+    s = mesher.Server(64, hbm_budget=1 << 30, max_batch=8, lanes=1)
+    with pytest.raises(mesher.LvnError):
+        s.set_tuning(dict(persistent_per_cu=3, row_blocks=8192, voxel_blocks=8192))
+    s.close()
+    mesher.clear_csg_ops()
+
+
 def test_edit_forces_remesh(oracle):
     w = _setup()
     s = mesher.Server(64, hbm_budget=2 << 30, max_batch=8, lanes=2)
