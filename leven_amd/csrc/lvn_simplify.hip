@@ -1244,6 +1244,7 @@ int lvn_simplify_mesh(lvn_ctx* ctx, lvn_vertex* vertices, int32_t* num_vertices,
     for (int64_t i = 0; i < 3 * (int64_t)nt; ++i)
         if (triangles[i] < 0 || triangles[i] >= nv) return LVN_ERR_INVALID;
     if (stats) stats[0] = stats[1] = 0;
+    if (!g_state.init) return LVN_ERR_STATE;  // the library was shut down
     if (nt < 100 || nv < 100) return LVN_OK;  // unchanged (src/ng_mesh_simplify.cpp:443-446)
     SimpState* st = simp_state(c);
     if (!st) return LVN_ERR_NOMEM;
