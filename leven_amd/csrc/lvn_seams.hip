@@ -17,12 +17,12 @@
 //      child-slot assignment does);
 //   2. ancestors: each leaf inserts its chain of parents, stopping at the first
 //      one already present, so every internal node is created exactly once;
-//   3. links, subtree leaf counts (bottom-up atomics) and depth-first leaf ranks
-//      (top-down prefix over the eight children, level by level): the rank is the
-//      vertex index GenerateVertexIndices assigns;
+//   3. links, subtree leaf counts (bottom-up, level by level) and depth-first leaf
+//      ranks (top-down prefix over the eight children, level by level): the rank is
+//      the vertex index GenerateVertexIndices assigns;
 //   4. contouring as a breadth-first expansion of the recursion's calls: each
 //      round replaces every pending cell / face / edge call by the calls it makes,
-//      in call order, at positions given by a workgroup prefix sum; a finished
+//      in call order, at positions given by prefix sums over the calls; a finished
 //      quad is carried forward in place.  The list therefore stays in depth-first
 //      call order and its final content is the reference's triangle order.
 // Everything is integer bookkeeping; vertex data is copied from the seam nodes.
@@ -832,7 +832,7 @@ __global__ __launch_bounds__(256) void k_seam_pack(const SeamJobDev* jobs, const
 //
 // The steps of one seam above, each run as one launch over the elements of every seam.
 // Work is cut into tiles of 1024 elements that belong to one seam (offered nodes of one
-// list, hash slots, leaves, nodes, the nodes of one level, pending calls); a tile list
+// list, hash slots, leaves, nodes, the internal nodes of one level, pending calls); a tile list
 // is made on the device from each seam's counts, so lists whose length only the device
 // knows (a level's nodes, a round's calls) need no read-back.  A seam's counts, level
 // starts, call-list state and status live in its SeamRun record.  Counters shared by a
@@ -849,9 +849,9 @@ constexpr int kSgLevels = 33;
 struct SeamRun {
     int nodes, nleaves, nnodes, status;
     int root, linked, rounds, stage;  // stage: calls reserved in the staging list this round
-    int n, more, pad0, pad1;  // pending calls, 1 = calls still expand (2 = one was written this round)
+    int n, more;              // pending calls; 1 = calls still expand (2 = one was written this round)
     int tile0, ntiles;        // this seam's tiles in the current round's list
-    int lev[kSgLevels + 1];   // node counts per level (lev[l + 1]), then level starts
+    int lev[kSgLevels + 1];   // internal nodes per level (lev[l + 1]), then level starts
     int fill[kSgLevels];      // placement cursors per level
 };
 
@@ -1017,7 +1017,7 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_init(SeamGrid g, const int4* 
         R.nodes = R.nleaves = R.nnodes = R.status = 0;
         R.root = -1;
         R.linked = R.rounds = R.stage = 0;
-        R.n = R.more = R.pad0 = R.pad1 = 0;
+        R.n = R.more = 0;
         R.tile0 = R.ntiles = 0;
         for (int k = 0; k <= kSgLevels; ++k) R.lev[k] = 0;
         for (int k = 0; k < kSgLevels; ++k) R.fill[k] = 0;
