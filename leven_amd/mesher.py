@@ -508,6 +508,17 @@ class Batch:
                                        vptr(s), len(s), C.byref(ns)), "lvn_batch_get_mesh")
         return v[:nv.value].copy(), t[:nt.value].copy(), s[:ns.value].copy()
 
+    def get_seam_nodes(self, i, counts=None):
+        """Chunk i's seam nodes only (lvn_batch_get_mesh without vertex or triangle buffers);
+        counts = self.counts() when already at hand."""
+        k = (self.counts() if counts is None else counts)[i]
+        s = np.zeros(max(int(k[3]), 1), dtype=SEAM_DTYPE)
+        nv, nt, ns = C.c_int32(), C.c_int32(), C.c_int32()
+        big = 2 ** 31 - 1
+        check(lib().lvn_batch_get_mesh(self._h, int(i), None, big, C.byref(nv), None, big, C.byref(nt), vptr(s),
+                                       len(s), C.byref(ns)), "lvn_batch_get_mesh")
+        return s[:ns.value].copy()
+
     def simplify(self, options):
         """Simplify every staged chunk's mesh in its slot (lvn_batch_simplify) with per-voxel
         limits (abi.simplify_options_per_voxel).  Returns int32 [n, 4]: vertices, triangles,

@@ -133,7 +133,7 @@ int orc_simplify_mesh(const lvn_vertex* vin, int32_t nv, const int32_t* tin, int
                       const lvn_simplify_options* opt, lvn_vertex* vout, int32_t* nv_out, int32_t* tout,
                       int32_t* nt_out, int32_t* stats);
 double orc_seam_meshes(int32_t n, const lvn_seam_req* reqs, const lvn_seam_source* sources,
-                       int32_t voxels_per_chunk, int32_t threads, int64_t* totals);
+                       int32_t voxels_per_chunk, int32_t threads, int64_t* totals, int64_t* per_seam);
 
 /* ---- many chunks, timed (CPU baseline) ------------------------------------ */
 /* Meshes n chunks, one chunk per thread on `threads` threads.  Stored ops are filtered per

@@ -362,8 +362,21 @@ extern "C" int orc_seam_mesh_leaves(const int32_t host_min[3], int32_t host_size
                      0x7fffffff, &nt, leaf_min_size);
 }
 
+// Position-weighted checksum of a mesh's bytes as 32-bit words (vertices, then
+// triangles): sum of word[i] * (i + 1) mod 2^64.
+static uint64_t mesh_checksum(const lvn_vertex* v, int32_t nv, const int32_t* t, int32_t nt) {
+    uint64_t c = 0, i = 0;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(v);
+    for (size_t k = 0; k < (size_t)nv * (sizeof(lvn_vertex) / 4); ++k) c += (uint64_t)w[k] * ++i;
+    w = reinterpret_cast<const uint32_t*>(t);
+    for (size_t k = 0; k < 3 * (size_t)nt; ++k) c += (uint64_t)w[k] * ++i;
+    return c;
+}
+
+// Many seams on `threads` host threads (the CPU baseline); totals += (vertices, triangles,
+// failed seams); per_seam (optional, 4 per seam) = status, vertices, triangles, checksum.
 extern "C" double orc_seam_meshes(int32_t n, const lvn_seam_req* reqs, const lvn_seam_source* sources,
-                                  int32_t voxels_per_chunk, int32_t threads, int64_t* totals) {
+                                  int32_t voxels_per_chunk, int32_t threads, int64_t* totals, int64_t* per_seam) {
     std::atomic<int> next{0};
     std::atomic<int64_t> verts{0}, tris{0}, failed{0};
     auto worker = [&]() {
@@ -386,6 +399,12 @@ extern "C" double orc_seam_meshes(int32_t n, const lvn_seam_req* reqs, const lvn
             if (rc != LVN_OK) failed.fetch_add(1);
             verts.fetch_add(nv);
             tris.fetch_add(nt);
+            if (per_seam) {
+                per_seam[4 * (size_t)i] = rc;
+                per_seam[4 * (size_t)i + 1] = nv;
+                per_seam[4 * (size_t)i + 2] = nt;
+                per_seam[4 * (size_t)i + 3] = rc == LVN_OK ? (int64_t)mesh_checksum(v.data(), nv, t.data(), nt) : 0;
+            }
         }
     };
     const auto t0 = std::chrono::steady_clock::now();

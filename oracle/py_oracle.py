@@ -75,7 +75,7 @@ def lib():
             "orc_chunk_get_seams": (None, [vp, vp]),
             "orc_mesh_chunks": (f64, [P(FieldDesc), vp, i32, i32, vp, vp, u32, vp, i32, vp, i32, vp, vp]),
             "orc_seam_mesh": (i32, [vp, i32, vp, i32, vp, i32, vp, i32, vp, vp, i32, vp]),
-            "orc_seam_meshes": (f64, [i32, vp, vp, i32, i32, vp]),
+            "orc_seam_meshes": (f64, [i32, vp, vp, i32, i32, vp, vp]),
             "orc_seam_mesh_leaves": (i32, [vp, i32, vp, i32, i32, i32, vp, vp]),
             "orc_simplify_mesh": (i32, [vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp]),
             "orc_simplify_qef": (C.c_float, [vp, vp, vp, vp, vp]),
@@ -319,13 +319,23 @@ def seam_mesh_leaves(host_min, host_size, sources, voxels_per_chunk=64):
     return out[:n.value].copy()
 
 
-def seam_meshes(reqs, sources, voxels_per_chunk=64, threads=1):
+def seam_meshes(reqs, sources, voxels_per_chunk=64, threads=1, per_seam=False):
     """Time the oracle over many seams (CPU baseline): reqs is a ctypes SeamReq array, sources
-    a ctypes SeamSource array.  Returns (wall_s, totals [vertices, triangles, failed])."""
+    a ctypes SeamSource array.  Returns (wall_s, totals [vertices, triangles, failed]), and
+    with per_seam an int64 (n, 4) array of status, vertices, triangles and mesh_checksum."""
     totals = np.zeros(3, dtype=np.int64)
+    each = np.zeros((len(reqs), 4), dtype=np.int64) if per_seam else None
     wall = lib().orc_seam_meshes(len(reqs), C.cast(reqs, C.c_void_p), C.cast(sources, C.c_void_p),
-                                 int(voxels_per_chunk), int(threads), vptr(totals))
-    return wall, totals
+                                 int(voxels_per_chunk), int(threads), vptr(totals), vptr(each))
+    return (wall, totals, each) if per_seam else (wall, totals)
+
+
+def mesh_checksum(vertices, triangles):
+    """The oracle's per-seam checksum of a mesh: the vertex bytes then the triangle bytes as
+    32-bit words w, sum of w[i] * (i + 1) mod 2^64."""
+    w = np.concatenate([np.frombuffer(np.ascontiguousarray(vertices).tobytes(), dtype=np.uint32),
+                        np.frombuffer(np.ascontiguousarray(triangles, dtype=np.int32).tobytes(), dtype=np.uint32)])
+    return int((w.astype(np.uint64) * np.arange(1, len(w) + 1, dtype=np.uint64)).sum(dtype=np.uint64))
 
 
 def simplify_qef(p0, n0, p1, n1):

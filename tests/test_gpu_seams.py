@@ -226,6 +226,48 @@ def test_clipmap_region_with_brushes():
     mesher.clear_csg_ops()
 
 
+def test_clipmap_seams_all_match_oracle():
+    """Every seam of the 6-LOD clipmap (1,798 seams; the largest offers ~1M nodes and a few
+    outgrow their first scratch caps, so they are run again with larger ones), read from the
+    clipmap batch's slots, in both forms of the pass: status, counts and the checksum of each
+    mesh's bytes equal the oracle's on the same lists."""
+    from leven_amd.mesher import _seam_request
+    wl = workloads.clipmap()
+    mesher.init(wl["desc"], wl["seed"], wl["default_material"])
+    mesher.clear_csg_ops()
+    for op, bb in zip(wl["ops"], wl["aabbs"]):
+        mesher.store_csg_op(op, bb[:3], bb[3:])
+    n = len(wl["sizes"])
+    b = mesher.Batch(64, n, edge_cap=0, voxel_cap=0, flags=wl["flags"])
+    b.set_chunks(wl["mins"], wl["sizes"])
+    b.run()
+    b.wait()
+    counts = b.counts()
+    assert (counts[:, 4] == 0).all()
+    nodes = [(tuple(int(v) for v in wl["mins"][i]), int(wl["sizes"][i]), b.get_seam_nodes(i, counts))
+             for i in range(n) if counts[i, 2] > 0 or counts[i, 3] > 0]
+    srcs = workloads.seam_sources(nodes)
+    hosts = [(mn, size, _colour(i)) for i, (mn, size, _) in enumerate(nodes)]
+    reqs, src, keep = _seam_request(hosts, srcs)
+    _, totals, want = py_oracle.seam_meshes(reqs, src, threads=16, per_seam=True)
+    assert totals[2] == 0 and (want[:, 0] == 0).all()
+    assert len(hosts) > 1500 and totals[1] > 10_000_000  # the whole clipmap: 1,798 seams, ~33M triangles
+    print(f"{len(hosts)} seams, {totals[0]} vertices, {totals[1]} triangles", flush=True)
+    slot_srcs = [[dict(d, nodes=None) for d in s] for s in srcs]
+    for path in PATHS:
+        with seam_path(path):
+            got = b.seam_meshes_on_device(hosts, slot_srcs)
+        for i in range(len(hosts)):
+            assert got[i] == (want[i, 1], want[i, 2]), f"{path}: seam {i} counts"
+            pv, pt, nv, nt = b.seam_device_mesh(i)
+            v = np.frombuffer(_d2h(pv, 48 * nv), dtype=VERTEX_DTYPE)
+            t = np.frombuffer(_d2h(pt, 12 * nt), dtype=np.int32).reshape(-1, 3)
+            assert py_oracle.mesh_checksum(v, t) == int(want[i, 3]) & (2 ** 64 - 1), f"{path}: seam {i} bytes"
+        print(f"{path}: all seams equal the oracle", flush=True)
+    b.close()
+    mesher.clear_csg_ops()
+
+
 def test_seam_errors(ctx, equal_lod):
     mn, size, _ = equal_lod.nodes[0]
     srcs = equal_lod.sources()[0]

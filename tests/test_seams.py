@@ -85,6 +85,28 @@ def test_seam_oracle_edge_cases(block):
     assert rc == LVN_ERR_INVALID
 
 
+def test_oracle_per_seam_results_and_checksum(block):
+    """The oracle's many-seam call reports each seam's status, counts and the checksum of its
+    mesh bytes (the GPU tests compare whole clipmaps by it): equal to one call per seam and
+    to the checksum computed here; a changed word changes it."""
+    from leven_amd.mesher import _seam_request
+    hosts = [(mn, size, (0.25 * i, 0.5, 1.0)) for i, (mn, size, _) in enumerate(block.nodes)]
+    srcs = block.sources()
+    reqs, src, keep = _seam_request(hosts, srcs)
+    _, totals, each = py_oracle.seam_meshes(reqs, src, threads=4, per_seam=True)
+    tris = 0
+    for i, ((mn, size, colour), s) in enumerate(zip(hosts, srcs)):
+        rc, v, t = py_oracle.seam_mesh(mn, size, s, colour=colour)
+        assert tuple(int(x) for x in each[i, :3]) == (rc, len(v), len(t))
+        assert int(each[i, 3]) & (2 ** 64 - 1) == py_oracle.mesh_checksum(v, t)
+        tris += len(t)
+        if len(t):
+            t2 = t.copy()
+            t2[-1, 2] ^= 1
+            assert py_oracle.mesh_checksum(v, t2) != py_oracle.mesh_checksum(v, t)
+    assert tris > 0 and tuple(totals) == (each[:, 1].sum(), each[:, 2].sum(), 0)
+
+
 def test_seam_sources_follow_find_active_nodes():
     """Neighbour lists of a mixed-LOD layout: a coarse node is offered for every neighbour
     region it covers, finer nodes for the region they lie in."""
