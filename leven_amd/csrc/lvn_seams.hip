@@ -757,7 +757,9 @@ __global__ __launch_bounds__(kSeamThreads) __attribute__((amdgpu_waves_per_eu(8,
     }
 }
 
-// Selected seam nodes of each seam, to size its scratch.
+// Selected seam nodes of each seam, to size its scratch: workgroup (s, b) of gridDim.y per
+// seam counts every gridDim.y-th 256-node slice of seam s's lists into counts[s] (zeroed
+// by the caller), so one big seam is counted by many CUs.
 __global__ __launch_bounds__(256) void k_seam_count(const SeamJobDev* __restrict__ jobs,
                                                     const SeamSrcDev* __restrict__ srcs, int* counts) {
     const SeamJobDev J = jobs[blockIdx.x];
@@ -765,16 +767,17 @@ __global__ __launch_bounds__(256) void k_seam_count(const SeamJobDev* __restrict
     if (threadIdx.x == 0) s_count = 0;
     __syncthreads();
     int mine = 0;
+    const int stride = gridDim.y * blockDim.x;
     for (int si = 0; si < J.src_count; ++si) {
         const SeamSrcDev src = srcs[J.src_begin + si];
-        for (int j = threadIdx.x; j < src.count; j += blockDim.x) {
+        for (int j = blockIdx.y * blockDim.x + threadIdx.x; j < src.count; j += stride) {
             int mn[3];
             mine += select_leaf(J, src, src.nodes[j], mn) ? 1 : 0;
         }
     }
     atomicAdd(&s_count, mine);
     __syncthreads();
-    if (threadIdx.x == 0) counts[blockIdx.x] = s_count;
+    if (threadIdx.x == 0 && s_count) atomicAdd(&counts[blockIdx.x], s_count);
 }
 
 // Exclusive offsets of every seam's vertices and triangles in the packed output
@@ -815,16 +818,22 @@ __global__ __launch_bounds__(1024) void k_seam_offsets(const SeamResDev* res, in
     }
 }
 
-// Copies each seam's vertices and triangles into the packed output.
+// Copies each seam's vertices and triangles into the packed output (gridDim.y workgroups
+// per seam).
 __global__ __launch_bounds__(256) void k_seam_pack(const SeamJobDev* jobs, const SeamResDev* res,
                                                    const uint64_t* offs, const lvn_vertex* out_v, const int* out_t,
                                                    lvn_vertex* pv, int* pt) {
     const SeamJobDev& J = jobs[blockIdx.x];
     const SeamResDev r = res[blockIdx.x];
     const uint64_t ov = offs[2 * blockIdx.x], ot = offs[2 * blockIdx.x + 1];
-    for (int i = threadIdx.x; i < r.nv; i += blockDim.x) pv[ov + i] = out_v[J.v_off + i];
-    for (int i = threadIdx.x; i < 3 * r.nt; i += blockDim.x) pt[3 * ot + i] = out_t[3 * J.t_off + i];
+    const int first = blockIdx.y * blockDim.x + threadIdx.x, stride = gridDim.y * blockDim.x;
+    for (int i = first; i < r.nv; i += stride) pv[ov + i] = out_v[J.v_off + i];
+    for (int i = first; i < 3 * r.nt; i += stride) pt[3 * ot + i] = out_t[3 * J.t_off + i];
 }
+
+// Workgroups per seam for the count and pack kernels: one CU's worth is too little for a
+// request of few, large seams.
+inline int per_seam_blocks(int m) { return std::max(1, std::min(256, 4096 / std::max(m, 1))); }
 
 // ---------------------------------------------------------------------------
 // The seam pass as grid-wide steps over all seams of a launch
@@ -850,7 +859,7 @@ struct SeamRun {
     int nodes, nleaves, nnodes, status;
     int root, linked, rounds, stage;  // stage: calls reserved in the staging list this round
     int n, more;              // pending calls; 1 = calls still expand (2 = one was written this round)
-    int tile0, ntiles;        // this seam's tiles in the current round's list
+    int tbase[8], ntiles;     // this seam's tiles in the current round's list (first position per sub-list)
     int lev[kSgLevels + 1];   // internal nodes per level (lev[l + 1]), then level starts
     int fill[kSgLevels];      // placement cursors per level
 };
@@ -877,6 +886,7 @@ struct SeamGrid {
     lvn_vertex* out_v;
     int* out_t;
     int m;
+    int spread;  // tiles of each seam dealt over every XCD (requests of few seams)
 };
 
 __device__ __forceinline__ SeamScratch sg_scratch(const SeamGrid& g, int s) {
@@ -946,14 +956,34 @@ __device__ __forceinline__ int lane_rank(uint64_t mask) {
 // Tile lists, placed by XCD.  Workgroups go to the eight XCDs in turn (workgroup b to XCD
 // b mod 8), and every tile launch loops over its list with a stride that is a multiple of
 // 8, so tile k runs on XCD k mod 8.  A list is eight interleaved sub-lists, one per XCD
-// (sub-list x at k = 8p + x), and seam s's tiles all go to sub-list s mod 8: a seam's hash
-// table, nodes and call lists are then touched by one XCD and share its L2.  Seams come
-// largest first, so the sub-lists are about equally long; the shorter ones are padded with
-// empty tiles (seam -1).  Within a sub-list a seam's tiles are consecutive, in order:
-// tile t of seam s is at k = 8 (tile0 + t) + s mod 8.
+// (sub-list x at k = 8p + x).  Local placement puts all of seam s's tiles in sub-list
+// s mod 8: its hash table, nodes and call lists are then touched by one XCD and share its
+// L2 (seams come largest first, so the sub-lists are about equally long).  A request of few
+// seams would leave XCDs idle that way, so it spreads them: tile t of seam s goes to
+// sub-list (s + t) mod 8.  Shorter sub-lists are padded with empty tiles (seam -1).  A
+// seam's tiles keep their order: its tiles in one sub-list are consecutive, from base[x].
 constexpr int kXcds = 8;
+constexpr int kSpreadBelow = 64;  // requests of fewer seams spread each seam over every XCD
 
-__device__ __forceinline__ int sg_tile_index(int seam, int p) { return p * kXcds + (seam & (kXcds - 1)); }
+// Reserves seam s's cnt tiles in the sub-lists (fill = their lengths so far, in LDS):
+// base[x] = the seam's first position in sub-list x.
+__device__ __forceinline__ void sg_place(bool spread, int s, int cnt, int* fill, int* base) {
+    if (!spread) {
+        base[s & (kXcds - 1)] = cnt ? atomicAdd(&fill[s & (kXcds - 1)], cnt) : 0;
+        return;
+    }
+#pragma unroll
+    for (int x = 0; x < kXcds; ++x) {
+        const int nx = cnt / kXcds + (((x - s) & (kXcds - 1)) < (cnt & (kXcds - 1)) ? 1 : 0);
+        base[x] = nx ? atomicAdd(&fill[x], nx) : 0;
+    }
+}
+// List index of tile t of seam s.
+__device__ __forceinline__ int sg_tile_index(bool spread, int s, int t, const int* base) {
+    if (!spread) return (base[s & (kXcds - 1)] + t) * kXcds + (s & (kXcds - 1));
+    const int x = (s + t) & (kXcds - 1);
+    return (base[x] + t / kXcds) * kXcds + x;
+}
 
 // Pads the sub-lists of `out` to the longest (fill[x] = length of sub-list x); returns the
 // list's length.  Called by every thread of the workgroup.
@@ -991,16 +1021,17 @@ __global__ __launch_bounds__(1024) void k_sg_tiles(SeamGrid g, int what, int4* t
         default:
             for (int si = 0; si < J.src_count; ++si) cnt += nt(g.srcs[J.src_begin + si].count);
         }
-        const int p0 = cnt ? atomicAdd(&fill[s & (kXcds - 1)], cnt) : 0;
+        int base[kXcds];
+        sg_place(g.spread, s, cnt, fill, base);
         if (what == 6) {
-            int p = p0;
+            int t = 0;
             for (int si = 0; si < J.src_count; ++si)
                 for (int j = 0; j < g.srcs[J.src_begin + si].count; j += kSgTile)
-                    out[sg_tile_index(s, p++)] = make_int4(s, j, si, 0);
+                    out[sg_tile_index(g.spread, s, t++, base)] = make_int4(s, j, si, 0);
         } else {
-            for (int t = 0; t < cnt; ++t) out[sg_tile_index(s, p0 + t)] = make_int4(s, t * kSgTile, 0, 0);
+            for (int t = 0; t < cnt; ++t) out[sg_tile_index(g.spread, s, t, base)] = make_int4(s, t * kSgTile, 0, 0);
             if (what == 1) {
-                g.run[s].tile0 = p0;
+                for (int x = 0; x < kXcds; ++x) g.run[s].tbase[x] = base[x];
                 g.run[s].ntiles = cnt;
             }
         }
@@ -1018,7 +1049,8 @@ __global__ __launch_bounds__(kSgThreads) void k_sg_init(SeamGrid g, const int4* 
         R.root = -1;
         R.linked = R.rounds = R.stage = 0;
         R.n = R.more = 0;
-        R.tile0 = R.ntiles = 0;
+        for (int x = 0; x < 8; ++x) R.tbase[x] = 0;
+        R.ntiles = 0;
         for (int k = 0; k <= kSgLevels; ++k) R.lev[k] = 0;
         for (int k = 0; k < kSgLevels; ++k) R.fill[k] = 0;
     }
@@ -1433,7 +1465,7 @@ __global__ __launch_bounds__(1024) void k_sg_round_next(SeamGrid g, int round, c
         if (R.status == 0 && R.linked && R.more && R.ntiles > 0) {
             int sum = 0;
             for (int t = 0; t < R.ntiles; ++t) {
-                const int k = sg_tile_index(s, R.tile0 + t);
+                const int k = sg_tile_index(g.spread, s, t, R.tbase);
                 const int v = tsum[k];
                 toff[k] = sum;
                 sum += v;
@@ -1449,9 +1481,10 @@ __global__ __launch_bounds__(1024) void k_sg_round_next(SeamGrid g, int round, c
             }
         }
         const int cnt = R.status == 0 && R.linked && R.more ? (R.n + kSgTile - 1) / kSgTile : 0;
-        const int p0 = cnt ? atomicAdd(&fill[s & (kXcds - 1)], cnt) : 0;
-        for (int t = 0; t < cnt; ++t) next[sg_tile_index(s, p0 + t)] = make_int4(s, t * kSgTile, 0, 0);
-        R.tile0 = p0;
+        int base[kXcds];
+        sg_place(g.spread, s, cnt, fill, base);
+        for (int t = 0; t < cnt; ++t) next[sg_tile_index(g.spread, s, t, base)] = make_int4(s, t * kSgTile, 0, 0);
+        for (int x = 0; x < kXcds; ++x) R.tbase[x] = base[x];
         R.ntiles = cnt;
     }
     __syncthreads();
@@ -1686,23 +1719,32 @@ int run_seam_grid(Ctx* c, SeamState& st, int m, const SeamJobDev* h_jobs, const 
                   const SeamJobDev* d_jobs, lvn_vertex* ov, int* ot, SeamResDev* d_res) {
     // list lengths: eight interleaved sub-lists, seam k in sub-list k mod 8, each padded to
     // the longest (bounds from the seams' caps)
+    const bool spread = m < kSpreadBelow;
     auto nt = [](int64_t n) { return (n + kSgTile - 1) / kSgTile; };
     int64_t xh[kXcds] = {}, xo[kXcds] = {}, xl[kXcds] = {}, xn[kXcds] = {}, xr[kXcds] = {};
+    auto add = [spread](int64_t* xs, int s, int64_t cnt) {  // sub-list lengths as sg_place makes them
+        if (!spread) {
+            xs[s & (kXcds - 1)] += cnt;
+            return;
+        }
+        for (int x = 0; x < kXcds; ++x) xs[x] += cnt / kXcds + (((x - s) & (kXcds - 1)) < cnt % kXcds ? 1 : 0);
+    };
     int lmax = 0;
     for (int k = 0; k < m; ++k) {
         const SeamJobDev& J = h_jobs[k];
-        const int x = k & (kXcds - 1);
-        xh[x] += nt(J.hash_cap);
-        xl[x] += nt(J.leaf_cap);
-        xn[x] += nt(J.node_cap);
-        xr[x] += nt(J.list_cap);
+        add(xh, k, nt(J.hash_cap));
+        add(xl, k, nt(J.leaf_cap));
+        add(xn, k, nt(J.node_cap));
+        add(xr, k, nt(J.list_cap));
         const int root_size = 2 * J.host_size, root_levels = __builtin_ctz((unsigned)root_size);
+        int64_t offered = 0;
         for (int si = 0; si < J.src_count; ++si) {
             const SeamSrcDev& q = srcs[J.src_begin + si];
-            xo[x] += nt(q.count);
+            offered += nt(q.count);
             const int leaf = q.size / J.vpc;
             if (host_pow2(leaf) && leaf < root_size) lmax = std::max(lmax, root_levels - __builtin_ctz((unsigned)leaf));
         }
+        add(xo, k, offered);
     }
     auto padded = [](const int64_t* v) { return kXcds * std::max<int64_t>(1, *std::max_element(v, v + kXcds)); };
     const int64_t th = padded(xh), to = padded(xo), tl = padded(xl), tn = padded(xn), tr = padded(xr);
@@ -1720,7 +1762,7 @@ int run_seam_grid(Ctx* c, SeamState& st, int m, const SeamJobDev* h_jobs, const 
     int* tbase = toff + tr;
     int4* T = static_cast<int4*>(st.g_tiles.p);
     SeamGrid g{d_jobs, static_cast<const SeamSrcDev*>(st.srcs.p), static_cast<char*>(st.scratch.p),
-               static_cast<SeamRun*>(st.g_run.p), cnt, ov, ot, m};
+               static_cast<SeamRun*>(st.g_run.p), cnt, ov, ot, m, spread ? 1 : 0};
     const hipStream_t sm = c->stream;
     // tile launches: a multiple of 8 workgroups (see k_sg_tiles), looping over their lists
     static_assert(kSgGrid % kXcds == 0, "tile launches keep to one XCD per workgroup");
@@ -1879,7 +1921,9 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
         return r;
     std::memcpy(st.h_jobs.p, base.data(), n * sizeof(SeamJobDev));
     LVN_HIP(hipMemcpyAsync(st.jobs.p, st.h_jobs.p, n * sizeof(SeamJobDev), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_seam_count, dim3(n), dim3(256), 0, c->stream, static_cast<const SeamJobDev*>(st.jobs.p),
+    LVN_HIP(hipMemsetAsync(st.counts.p, 0, n * sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_seam_count, dim3(n, per_seam_blocks(n)), dim3(256), 0, c->stream,
+                       static_cast<const SeamJobDev*>(st.jobs.p),
                        static_cast<const SeamSrcDev*>(st.srcs.p), static_cast<int*>(st.counts.p));
     LVN_HIP(hipGetLastError());
     LVN_HIP(hipMemcpyAsync(st.h_counts.p, st.counts.p, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1987,7 +2031,7 @@ int run_seams(Ctx* c, int n, const lvn_seam_req* reqs, const lvn_seam_source* so
                 (r = st.h_pv.ensure(std::max<uint64_t>(tv, 1) * sizeof(lvn_vertex))) ||
                 (r = st.h_pt.ensure(std::max<uint64_t>(tt, 1) * 12)))
                 return r;
-            hipLaunchKernelGGL(k_seam_pack, dim3(m), dim3(256), 0, c->stream, d_jobs, d_res, d_offs,
+            hipLaunchKernelGGL(k_seam_pack, dim3(m, per_seam_blocks(m)), dim3(256), 0, c->stream, d_jobs, d_res, d_offs,
                                static_cast<const lvn_vertex*>(ov.p), static_cast<const int*>(ot.p),
                                static_cast<lvn_vertex*>(st.pv.p), static_cast<int*>(st.pt.p));
             LVN_HIP(hipGetLastError());
