@@ -25,19 +25,20 @@ namespace lvn {
 // at x = value with 255 wrapping to 0, is then row + stored byte with no compare.  The
 // 16-entry gradient table is indexed by the stored byte's low four bits (rotated by one
 // entry, so no subtraction is needed).  The gradients are stored as (x, y) pairs, 8 bytes
-// an entry, and z components apart, the block starting on a bank row: the 16 pairs fill
-// the 32 banks once, so a wave's 8-byte reads never meet two addresses in one bank
-// (equal gradients share an address; whole 16-byte entries put entries 8 apart in the
-// same banks).  snoise2 reads a pair, snoise3 a pair and a z.  66,624 bytes per workgroup.
-constexpr int kGradPad = (128 - kPermPaddedBytes % 128) % 128;
-
+// an entry, and z components at the same 8-byte stride, the two blocks on bank rows: the
+// 16 pairs fill the 32 banks once and the 16 z values take the even banks once, so a
+// wave's reads never meet two addresses in one bank (equal gradients share an address;
+// whole 16-byte entries put entries 8 apart in the same banks).  The gradient blocks
+// come first: their offsets then fit the LDS instructions' 16-bit immediate, and a
+// corner's two reads share one address, its index times 8.  snoise2 reads a pair,
+// snoise3 a pair and a z.  66,688 bytes per workgroup.
 struct alignas(128) NoiseLds {
+    float2 gxy[kGradEntries];  // entry j: gradient (j - 1) & 15, components x and y
+    float2 gz[kGradEntries];   // entry j: (z component, unused)
     uint8_t perm[kPermPaddedBytes];
-    uint8_t pad[kGradPad];
-    float2 gxy[kGradEntries];  // entry j: gradient (j - 1) & 15
-    float gz[kGradEntries];
 };
-static_assert(offsetof(NoiseLds, gxy) % 128 == 0, "the gradient block starts on a bank row");
+static_assert(offsetof(NoiseLds, gz) == 128 && offsetof(NoiseLds, perm) == 256,
+              "gradient blocks on bank rows, ahead of the permutation table");
 
 // Cooperative copy of the tables into LDS (all threads of the block call it).
 __device__ __forceinline__ void load_noise_tables(NoiseLds& lds, const uint8_t* __restrict__ perm,
@@ -48,7 +49,7 @@ __device__ __forceinline__ void load_noise_tables(NoiseLds& lds, const uint8_t* 
     for (int i = threadIdx.x; i < kGradEntries; i += blockDim.x) {
         const float4 g = grads[i];
         lds.gxy[i] = make_float2(g.x, g.y);
-        lds.gz[i] = g.z;
+        lds.gz[i] = make_float2(g.z, 0.f);
     }
     __syncthreads();
 }
@@ -123,7 +124,7 @@ __device__ __forceinline__ float corner3(const NoiseLds& T, int texel, int zrow,
     const int j = T.perm[zrow + w] & 15;
     const float2 g = T.gxy[j];
     const float t = r * r;
-    return (t * t) * fmaf(T.gz[j], fz, fmaf(g.y, fy, g.x * fx));
+    return (t * t) * fmaf(T.gz[j].x, fz, fmaf(g.y, fy, g.x * fx));
 }
 
 // snoise3, cl/simplex.cl:159-230 (simplex ranking of :74-97).
