@@ -388,9 +388,30 @@ __global__ __launch_bounds__(1024) void k_scan(KArgs a) {
 // Edge compaction (CompactEdges, cl/density_field.cl:80-92): the edge id is
 // ((x | y << S | z << 2S) << 2) | axis (cl/density_field.cl:63,74).
 // ---------------------------------------------------------------------------
-// One lane per hermite row writes the ids of the row's edges in slot order (x,
-// then axis).  Consecutive lanes take rows of consecutive z at the same y, so a
-// horizontal surface keeps every lane of a wave busy.
+// The ids of every hermite row's edges in slot order (x, then axis), and each edge's
+// crossing position (y-edges first, then x and z edges).  A wave takes 64 consecutive
+// rows r = y + H z and loads their words together (lane j: row j).  Where its live rows
+// are dense (kDenseRow edges or more on average) it walks them in order with lane x
+// holding position x of the row: an edge's rank in the row is the popcount of the three
+// edge words below its lane (lower x) plus the lower axes at its own x, so each store
+// instruction writes one contiguous stretch of the lists.  Where they are sparse, each
+// lane walks its own row's few edges (fewer, cheaper iterations; their stores scatter
+// over the rows' stretches).  With the per-lane walk alone, lines of the clipmap batch's
+// dense rows left L2 half-written: 47 bytes reached HBM per 8 bytes of list, and the
+// stage took 3.3 ms.  Both walks write the same lists.
+constexpr int kDenseRow = 16;
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int j) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, j);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), j);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 __global__ __launch_bounds__(256) void k_edge_compact(KArgs a) {
     const int chunk = blockIdx.y;
     const Geometry& G = a.G;
@@ -401,37 +422,84 @@ __global__ __launch_bounds__(256) void k_edge_compact(KArgs a) {
     int* ids = slot_ptr<int>(a, chunk, a.L.e_ids);
     int* order = slot_ptr<int>(a, chunk, a.L.e_order);
     const int S = G.shift, cap = a.caps.edges, ey_total = a.counters[chunk].edges_y;
-    for (int t = thread_in_chunk(); t < G.edge_rows; t += threads_per_chunk()) {
-        // row r = y + H z: consecutive lanes read consecutive rows and write adjacent
-        // stretches of the list (0.12 -> 0.08 ms against lanes a column of rows apart)
-        const int r = t;
-        const int z = div_magic(t, G.magic_h), y = t - z * G.H;
-        if (count[r] == 0) continue;
-        const uint64_t ex = E[4 * r], ey = E[4 * r + 1], ez = E[4 * r + 2], hi = E[4 * r + 3];
-        const int row_id = (y << S) | (z << (2 * S));
-        int k = base[r], ky = base_y[r];  // edges and y-edges before this one
-        // rank k in id order; crossing position: y-edges first, then x and z edges
-        auto put = [&](int id, bool is_y) {
-            if (k < cap) {
-                ids[k] = id;
-                const int pos = is_y ? ky : ey_total + (k - ky);
-                if (pos < cap) order[pos] = k;
-            }
-            ++k;
-            ky += is_y ? 1 : 0;
-        };
-        uint64_t any = ex | ey | ez;
-        while (any) {
-            const int x = (int)__builtin_ctzll(any);
-            any &= any - 1ull;
-            const int id = (x | row_id) << 2;
-            if ((ex >> x) & 1ull) put(id, false);
-            if ((ey >> x) & 1ull) put(id | 1, true);
-            if ((ez >> x) & 1ull) put(id | 2, false);
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lanemask_lt(lane);
+    // rank k in id order; crossing position: y-edges first, then x and z edges
+    auto put = [&](int k, int ky, int id, bool is_y) {
+        if (k < cap) {
+            ids[k] = id;
+            const int pos = is_y ? ky : ey_total + (k - ky);
+            if (pos < cap) order[pos] = k;
         }
-        const int id64 = (64 | row_id) << 2;  // x = 64 (N = 64 only)
-        for (int axis = 0; axis < 3; ++axis)
-            if ((hi >> axis) & 1ull) put(id64 | axis, axis == 1);
+    };
+    const int groups = (G.edge_rows + 63) >> 6;
+    for (int g = wave_in_chunk(); g < groups; g += waves_per_chunk()) {
+        const int rl = (g << 6) + lane;
+        int cnt = 0, kb = 0, kyb = 0;
+        uint64_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+        if (rl < G.edge_rows) {
+            cnt = count[rl];
+            kb = base[rl];
+            kyb = base_y[rl];
+            w0 = E[4 * rl];
+            w1 = E[4 * rl + 1];
+            w2 = E[4 * rl + 2];
+            w3 = E[4 * rl + 3];
+        }
+        uint64_t live = __ballot(cnt != 0);
+        if (!live) continue;
+        if (wave_sum(cnt) < kDenseRow * __popcll(live)) {  // sparse rows: one lane per row
+            if (cnt == 0) continue;
+            const int z = div_magic(rl, G.magic_h), y = rl - z * G.H;
+            const int row_id = (y << S) | (z << (2 * S));
+            int k = kb, ky = kyb;
+            uint64_t any = w0 | w1 | w2;
+            while (any) {
+                const int x = (int)__builtin_ctzll(any);
+                any &= any - 1ull;
+                const int id = (x | row_id) << 2;
+                if ((w0 >> x) & 1ull) put(k++, ky, id, false);
+                if ((w1 >> x) & 1ull) put(k++, ky++, id | 1, true);
+                if ((w2 >> x) & 1ull) put(k++, ky, id | 2, false);
+            }
+            const int id64 = (64 | row_id) << 2;  // x = 64 (N = 64 only)
+            for (int axis = 0; axis < 3; ++axis) {
+                if (!((w3 >> axis) & 1ull)) continue;
+                put(k++, ky, id64 | axis, axis == 1);
+                ky += axis == 1 ? 1 : 0;
+            }
+            continue;
+        }
+        while (live) {
+            const int j = (int)__builtin_ctzll(live);
+            live &= live - 1ull;
+            const int r = (g << 6) + j;
+            const int z = div_magic(r, G.magic_h), y = r - z * G.H;
+            const int row_id = (y << S) | (z << (2 * S));
+            const uint64_t ex = readlane64(w0, j), ey = readlane64(w1, j), ez = readlane64(w2, j);
+            const uint64_t hi = readlane64(w3, j);
+            const int k0 = __builtin_amdgcn_readlane(kb, j), ky0 = __builtin_amdgcn_readlane(kyb, j);
+            const int bx = (int)((ex >> lane) & 1ull), by = (int)((ey >> lane) & 1ull), bz = (int)((ez >> lane) & 1ull);
+            int k = k0 + __popcll(ex & lt) + __popcll(ey & lt) + __popcll(ez & lt);
+            int ky = ky0 + __popcll(ey & lt);
+            const int id = (lane | row_id) << 2;
+            if (bx) put(k, ky, id, false);
+            k += bx;
+            if (by) put(k, ky, id | 1, true);
+            k += by;
+            ky += by;
+            if (bz) put(k, ky, id | 2, false);
+            if (hi != 0 && lane == 0) {  // x = 64 (N = 64 only): after every x < 64
+                int kh = k0 + __popcll(ex) + __popcll(ey) + __popcll(ez), kyh = ky0 + __popcll(ey);
+                const int id64 = (64 | row_id) << 2;
+                for (int axis = 0; axis < 3; ++axis) {
+                    if (!((hi >> axis) & 1ull)) continue;
+                    put(kh, kyh, id64 | axis, axis == 1);
+                    ++kh;
+                    kyh += axis == 1 ? 1 : 0;
+                }
+            }
+        }
     }
 }
 
@@ -1187,10 +1255,15 @@ __global__ __launch_bounds__(256) void k_voxel_rows(KArgs a) {
     }
 }
 
-// One lane per voxel segment writes the octree codes of its active voxels in
-// linear order; the corner materials, edge mask and dominant material are found
-// by the QEF kernel, which reads the corners anyway.  For N = 64 a segment is a
-// voxel row and consecutive lanes take consecutive z.
+// The octree codes of every segment's active voxels in linear order; the corner
+// materials, edge mask and dominant material are found by the QEF kernel, which reads
+// the corners anyway.  A wave takes 64 consecutive segments and loads their masks and
+// bases together (lane j: segment j).  Where its live segments are dense (kDenseRow
+// voxels or more on average) it walks them in order with lane b holding the segment's
+// voxel b: its rank is the popcount of the mask below b, so each segment's codes are one
+// contiguous store.  Where they are sparse, each lane walks its own segment (one lane
+// per segment alone scattered the dense segments' stores: 4.4 bytes reached HBM per
+// byte of codes on the clipmap batch).  Both walks write the same codes.
 __global__ __launch_bounds__(256) void k_voxel_compact(KArgs a) {
     const int chunk = blockIdx.y;
     const Geometry& G = a.G;
@@ -1198,16 +1271,41 @@ __global__ __launch_bounds__(256) void k_voxel_compact(KArgs a) {
     const int* vbase = slot_ptr<const int>(a, chunk, a.L.v_base);
     int* vcode = slot_ptr<int>(a, chunk, a.L.v_code);
     const int cap = a.caps.voxels;
-    for (int t = thread_in_chunk(); t < G.voxel_segs; t += threads_per_chunk()) {
-        const int seg = G.N == 64 ? (t >> 6) + ((t & 63) << 6) : t;
-        uint64_t m = vmask[seg];
-        int i = vbase[seg];
-        while (m && i < cap) {
-            const int v = (seg << 6) + (int)__builtin_ctzll(m);
-            m &= m - 1ull;
-            int x, y, z;
-            decode_voxel(G, v, x, y, z);
-            vcode[i++] = (int)code_for_position(x, y, z, G.depth);
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lanemask_lt(lane);
+    const int groups = (G.voxel_segs + 63) >> 6;
+    for (int g = wave_in_chunk(); g < groups; g += waves_per_chunk()) {
+        const int sl = (g << 6) + lane;
+        uint64_t ml = 0;
+        int bl = 0;
+        if (sl < G.voxel_segs) {
+            ml = vmask[sl];
+            bl = vbase[sl];
+        }
+        uint64_t live = __ballot(ml != 0);
+        if (!live) continue;
+        if (wave_sum(__popcll(ml)) < kDenseRow * __popcll(live)) {  // sparse: one lane per segment
+            int i = bl;
+            while (ml && i < cap) {
+                const int v = (sl << 6) + (int)__builtin_ctzll(ml);
+                ml &= ml - 1ull;
+                int x, y, z;
+                decode_voxel(G, v, x, y, z);
+                vcode[i++] = (int)code_for_position(x, y, z, G.depth);
+            }
+            continue;
+        }
+        while (live) {
+            const int j = (int)__builtin_ctzll(live);
+            live &= live - 1ull;
+            const int seg = (g << 6) + j;
+            const uint64_t m = readlane64(ml, j);
+            const int i = __builtin_amdgcn_readlane(bl, j) + __popcll(m & lt);
+            if (((m >> lane) & 1ull) && i < cap) {
+                int x, y, z;
+                decode_voxel(G, (seg << 6) + lane, x, y, z);
+                vcode[i] = (int)code_for_position(x, y, z, G.depth);
+            }
         }
     }
 }
