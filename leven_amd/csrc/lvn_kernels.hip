@@ -1830,21 +1830,41 @@ __global__ __launch_bounds__(256) void k_emit(KArgs a) {
         int qi = qb + __popcll(b0 & lt) + 2 * __popcll(b1 & lt);
         // The nine neighbour lookups (three per axis) are loaded together, before any is
         // used: one memory round trip instead of one per axis and step.  An index is
-        // clamped into the grid, so an axis without a quad reads a valid, unused word.
-        const int last = G.N * G.N * G.N - 1;
+        // clamped into the grid, so an axis without a quad reads a valid, unused word (a
+        // quad's neighbours all lie in the grid: quad_flag excludes the last layers).
         uint64_t nw[3][3];
         int nb[3][3], nl[3][3];
+        if (G.N == 64) {
+            // A segment is the voxel row (y, z): the nine neighbours (kEdgeNodeOffsets) lie
+            // in the rows (y, z), (y + 1, z), (y, z + 1) and (y + 1, z + 1), at x or x + 1.
+            const int r = p.y + 64 * p.z, rmax = 64 * 64 - 1;
+            const int r10 = min(r + 1, rmax), r01 = min(r + 64, rmax), r11 = min(r + 65, rmax);
+            const uint64_t m00 = vm[r], m10 = vm[r10], m01 = vm[r01], m11 = vm[r11];
+            const int b00 = vb[r], b10 = vb[r10], b01 = vb[r01], b11 = vb[r11];
+            const int x0 = p.x, x1 = (p.x + 1) & 63;
+            nw[0][0] = m01; nb[0][0] = b01; nl[0][0] = x0;  // (0, 0, 1)
+            nw[0][1] = m10; nb[0][1] = b10; nl[0][1] = x0;  // (0, 1, 0)
+            nw[0][2] = m11; nb[0][2] = b11; nl[0][2] = x0;  // (0, 1, 1)
+            nw[1][0] = m00; nb[1][0] = b00; nl[1][0] = x1;  // (1, 0, 0)
+            nw[1][1] = m01; nb[1][1] = b01; nl[1][1] = x0;  // (0, 0, 1)
+            nw[1][2] = m01; nb[1][2] = b01; nl[1][2] = x1;  // (1, 0, 1)
+            nw[2][0] = m10; nb[2][0] = b10; nl[2][0] = x0;  // (0, 1, 0)
+            nw[2][1] = m00; nb[2][1] = b00; nl[2][1] = x1;  // (1, 0, 0)
+            nw[2][2] = m10; nb[2][2] = b10; nl[2][2] = x1;  // (1, 1, 0)
+        } else {
+            const int last = G.N * G.N * G.N - 1;
 #pragma unroll
-        for (int axis = 0; axis < 3; ++axis) {
+            for (int axis = 0; axis < 3; ++axis) {
 #pragma unroll
-            for (int n = 1; n < 4; ++n) {
-                const int x = p.x + kEdgeNodeOffsets[axis][n][0];
-                const int y = p.y + kEdgeNodeOffsets[axis][n][1];
-                const int z = p.z + kEdgeNodeOffsets[axis][n][2];
-                const int i = min(x + G.N * (y + G.N * z), last);
-                nw[axis][n - 1] = vm[i >> 6];
-                nb[axis][n - 1] = vb[i >> 6];
-                nl[axis][n - 1] = i & 63;
+                for (int n = 1; n < 4; ++n) {
+                    const int x = p.x + kEdgeNodeOffsets[axis][n][0];
+                    const int y = p.y + kEdgeNodeOffsets[axis][n][1];
+                    const int z = p.z + kEdgeNodeOffsets[axis][n][2];
+                    const int i = min(x + G.N * (y + G.N * z), last);
+                    nw[axis][n - 1] = vm[i >> 6];
+                    nb[axis][n - 1] = vb[i >> 6];
+                    nl[axis][n - 1] = i & 63;
+                }
             }
         }
 #pragma unroll
@@ -1863,13 +1883,13 @@ __global__ __launch_bounds__(256) void k_emit(KArgs a) {
             } else if (qi < a.caps.quads) {
                 const int edge = axis * 4 + 3;
                 const int flip = ((m >> kEdgeMap[edge][0]) & 1) != 0 ? 1 : 0;  // ProcessEdge, cl/octree.cl:335-372
-                int* out = tris + 6 * qi;
+                int3* out = reinterpret_cast<int3*>(tris + 6 * qi);  // two 12-byte stores
                 if (flip) {
-                    out[0] = nodes[0]; out[1] = nodes[3]; out[2] = nodes[1];
-                    out[3] = nodes[0]; out[4] = nodes[2]; out[5] = nodes[3];
+                    out[0] = make_int3(nodes[0], nodes[3], nodes[1]);
+                    out[1] = make_int3(nodes[0], nodes[2], nodes[3]);
                 } else {
-                    out[0] = nodes[0]; out[1] = nodes[1]; out[2] = nodes[3];
-                    out[3] = nodes[0]; out[4] = nodes[3]; out[5] = nodes[2];
+                    out[0] = make_int3(nodes[0], nodes[1], nodes[3]);
+                    out[1] = make_int3(nodes[0], nodes[3], nodes[2]);
                 }
             }
             ++qi;
