@@ -1797,8 +1797,15 @@ __constant__ int kEdgeNodeOffsets[3][4][3] = {{{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, 
 
 // Quads (GenerateMesh + CompactMeshTriangles, cl/octree.cl:335-464) in (voxel,
 // axis) order, and seam nodes (ExtractSeamNodeInfo, cl/octree.cl:529-551) in
-// voxel order.
-__global__ __launch_bounds__(256) void k_emit(KArgs a) {
+// voxel order.  A segment's quads fill one contiguous run of the index buffer
+// (at most 3 per voxel): each wave builds its run in LDS and copies it out with
+// 8-byte stores of consecutive addresses, so every store instruction writes whole
+// lines instead of 64 scattered 12-byte pieces.
+constexpr int kEmitWaves = 4;                  // waves per k_emit workgroup
+constexpr int kEmitStageInts = 64 * 3 * 6;     // one segment's quads, 6 indices each
+__global__ __launch_bounds__(64 * kEmitWaves) void k_emit(KArgs a) {
+    __shared__ int2 stage_all[kEmitWaves][kEmitStageInts / 2];
+    int* stage = reinterpret_cast<int*>(stage_all[threadIdx.x >> 6]);
     const int chunk = blockIdx.y;
     const Geometry& G = a.G;
     const int V = a.counters[chunk].voxels;
@@ -1825,84 +1832,101 @@ __global__ __launch_bounds__(256) void k_emit(KArgs a) {
         const int nq = __popc(qbits);
         const uint64_t b0 = __ballot(nq & 1), b1 = __ballot(nq & 2);
         const bool seam = (sm >> lane) & 1ull;
-        if (vi >= V || (!qbits && !seam)) continue;
-        const int3 p = position_for_code((uint32_t)code, G.depth);
-        int qi = qb + __popcll(b0 & lt) + 2 * __popcll(b1 & lt);
-        // The nine neighbour lookups (three per axis) are loaded together, before any is
-        // used: one memory round trip instead of one per axis and step.  An index is
-        // clamped into the grid, so an axis without a quad reads a valid, unused word (a
-        // quad's neighbours all lie in the grid: quad_flag excludes the last layers).
-        uint64_t nw[3][3];
-        int nb[3][3], nl[3][3];
-        if (G.N == 64) {
-            // A segment is the voxel row (y, z): the nine neighbours (kEdgeNodeOffsets) lie
-            // in the rows (y, z), (y + 1, z), (y, z + 1) and (y + 1, z + 1), at x or x + 1.
-            const int r = p.y + 64 * p.z, rmax = 64 * 64 - 1;
-            const int r10 = min(r + 1, rmax), r01 = min(r + 64, rmax), r11 = min(r + 65, rmax);
-            const uint64_t m00 = vm[r], m10 = vm[r10], m01 = vm[r01], m11 = vm[r11];
-            const int b00 = vb[r], b10 = vb[r10], b01 = vb[r01], b11 = vb[r11];
-            const int x0 = p.x, x1 = (p.x + 1) & 63;
-            nw[0][0] = m01; nb[0][0] = b01; nl[0][0] = x0;  // (0, 0, 1)
-            nw[0][1] = m10; nb[0][1] = b10; nl[0][1] = x0;  // (0, 1, 0)
-            nw[0][2] = m11; nb[0][2] = b11; nl[0][2] = x0;  // (0, 1, 1)
-            nw[1][0] = m00; nb[1][0] = b00; nl[1][0] = x1;  // (1, 0, 0)
-            nw[1][1] = m01; nb[1][1] = b01; nl[1][1] = x0;  // (0, 0, 1)
-            nw[1][2] = m01; nb[1][2] = b01; nl[1][2] = x1;  // (1, 0, 1)
-            nw[2][0] = m10; nb[2][0] = b10; nl[2][0] = x0;  // (0, 1, 0)
-            nw[2][1] = m00; nb[2][1] = b00; nl[2][1] = x1;  // (1, 0, 0)
-            nw[2][2] = m10; nb[2][2] = b10; nl[2][2] = x1;  // (1, 1, 0)
-        } else {
-            const int last = G.N * G.N * G.N - 1;
-#pragma unroll
+        const int run = __popcll(b0) + 2 * __popcll(b1);  // the segment's quads
+        if (vi < V && (qbits || seam)) {
+            const int3 p = position_for_code((uint32_t)code, G.depth);
+            int qi = __popcll(b0 & lt) + 2 * __popcll(b1 & lt);  // within the segment's run
+            // The nine neighbour lookups (three per axis) are loaded together, before any is
+            // used: one memory round trip instead of one per axis and step.  An index is
+            // clamped into the grid, so an axis without a quad reads a valid, unused word (a
+            // quad's neighbours all lie in the grid: quad_flag excludes the last layers).
+            uint64_t nw[3][3];
+            int nb[3][3], nl[3][3];
+            if (G.N == 64) {
+                // A segment is the voxel row (y, z): the nine neighbours (kEdgeNodeOffsets) lie
+                // in the rows (y, z), (y + 1, z), (y, z + 1) and (y + 1, z + 1), at x or x + 1.
+                const int r = p.y + 64 * p.z, rmax = 64 * 64 - 1;
+                const int r10 = min(r + 1, rmax), r01 = min(r + 64, rmax), r11 = min(r + 65, rmax);
+                const uint64_t m00 = vm[r], m10 = vm[r10], m01 = vm[r01], m11 = vm[r11];
+                const int b00 = vb[r], b10 = vb[r10], b01 = vb[r01], b11 = vb[r11];
+                const int x0 = p.x, x1 = (p.x + 1) & 63;
+                nw[0][0] = m01; nb[0][0] = b01; nl[0][0] = x0;  // (0, 0, 1)
+                nw[0][1] = m10; nb[0][1] = b10; nl[0][1] = x0;  // (0, 1, 0)
+                nw[0][2] = m11; nb[0][2] = b11; nl[0][2] = x0;  // (0, 1, 1)
+                nw[1][0] = m00; nb[1][0] = b00; nl[1][0] = x1;  // (1, 0, 0)
+                nw[1][1] = m01; nb[1][1] = b01; nl[1][1] = x0;  // (0, 0, 1)
+                nw[1][2] = m01; nb[1][2] = b01; nl[1][2] = x1;  // (1, 0, 1)
+                nw[2][0] = m10; nb[2][0] = b10; nl[2][0] = x0;  // (0, 1, 0)
+                nw[2][1] = m00; nb[2][1] = b00; nl[2][1] = x1;  // (1, 0, 0)
+                nw[2][2] = m10; nb[2][2] = b10; nl[2][2] = x1;  // (1, 1, 0)
+            } else {
+                const int last = G.N * G.N * G.N - 1;
+                #pragma unroll
+                for (int axis = 0; axis < 3; ++axis) {
+                    #pragma unroll
+                    for (int n = 1; n < 4; ++n) {
+                        const int x = p.x + kEdgeNodeOffsets[axis][n][0];
+                        const int y = p.y + kEdgeNodeOffsets[axis][n][1];
+                        const int z = p.z + kEdgeNodeOffsets[axis][n][2];
+                        const int i = min(x + G.N * (y + G.N * z), last);
+                        nw[axis][n - 1] = vm[i >> 6];
+                        nb[axis][n - 1] = vb[i >> 6];
+                        nl[axis][n - 1] = i & 63;
+                    }
+                }
+            }
+            #pragma unroll
             for (int axis = 0; axis < 3; ++axis) {
-#pragma unroll
+                if (!((qbits >> axis) & 1)) continue;
+                int nodes[4];
+                nodes[0] = vi;
+                bool ok = true;
+                #pragma unroll
                 for (int n = 1; n < 4; ++n) {
-                    const int x = p.x + kEdgeNodeOffsets[axis][n][0];
-                    const int y = p.y + kEdgeNodeOffsets[axis][n][1];
-                    const int z = p.z + kEdgeNodeOffsets[axis][n][2];
-                    const int i = min(x + G.N * (y + G.N * z), last);
-                    nw[axis][n - 1] = vm[i >> 6];
-                    nb[axis][n - 1] = vb[i >> 6];
-                    nl[axis][n - 1] = i & 63;
+                    nodes[n] = dense_index_of(nw[axis][n - 1], nb[axis][n - 1], nl[axis][n - 1]);
+                    ok = ok && nodes[n] >= 0;
                 }
-            }
-        }
-#pragma unroll
-        for (int axis = 0; axis < 3; ++axis) {
-            if (!((qbits >> axis) & 1)) continue;
-            int nodes[4];
-            nodes[0] = vi;
-            bool ok = true;
-#pragma unroll
-            for (int n = 1; n < 4; ++n) {
-                nodes[n] = dense_index_of(nw[axis][n - 1], nb[axis][n - 1], nl[axis][n - 1]);
-                ok = ok && nodes[n] >= 0;
-            }
-            if (!ok) {  // cannot happen: all four voxels around a sign-changing interior edge are active
-                atomicOr(&a.counters[chunk].status, kStatusLookupMiss);
-            } else if (qi < a.caps.quads) {
-                const int edge = axis * 4 + 3;
-                const int flip = ((m >> kEdgeMap[edge][0]) & 1) != 0 ? 1 : 0;  // ProcessEdge, cl/octree.cl:335-372
-                int3* out = reinterpret_cast<int3*>(tris + 6 * qi);  // two 12-byte stores
-                if (flip) {
-                    out[0] = make_int3(nodes[0], nodes[3], nodes[1]);
-                    out[1] = make_int3(nodes[0], nodes[2], nodes[3]);
+                int3* out = reinterpret_cast<int3*>(stage + 6 * qi);  // two 12-byte LDS stores
+                if (!ok) {  // cannot happen: all four voxels around a sign-changing interior edge are active
+                    atomicOr(&a.counters[chunk].status, kStatusLookupMiss);
+                    out[0] = out[1] = make_int3(0, 0, 0);
                 } else {
-                    out[0] = make_int3(nodes[0], nodes[1], nodes[3]);
-                    out[1] = make_int3(nodes[0], nodes[3], nodes[2]);
+                    const int edge = axis * 4 + 3;
+                    const int flip = ((m >> kEdgeMap[edge][0]) & 1) != 0 ? 1 : 0;  // ProcessEdge, cl/octree.cl:335-372
+                    if (flip) {
+                        out[0] = make_int3(nodes[0], nodes[3], nodes[1]);
+                        out[1] = make_int3(nodes[0], nodes[2], nodes[3]);
+                    } else {
+                        out[0] = make_int3(nodes[0], nodes[1], nodes[3]);
+                        out[1] = make_int3(nodes[0], nodes[3], nodes[2]);
+                    }
+                }
+                ++qi;
+            }
+            if (seam) {
+                const int si = sbase[seg] + __popcll(sm & lt);
+                if (si < a.caps.seams) {
+                    const float4 pos = verts[3 * vi], nrm = verts[3 * vi + 1];
+                    reinterpret_cast<int4*>(seams)[3 * si] = make_int4(p.x, p.y, p.z, m);
+                    seams[3 * si + 1] = pos;
+                    seams[3 * si + 2] = nrm;
                 }
             }
-            ++qi;
         }
-        if (seam) {
-            const int si = sbase[seg] + __popcll(sm & lt);
-            if (si < a.caps.seams) {
-                const float4 pos = verts[3 * vi], nrm = verts[3 * vi + 1];
-                reinterpret_cast<int4*>(seams)[3 * si] = make_int4(p.x, p.y, p.z, m);
-                seams[3 * si + 1] = pos;
-                seams[3 * si + 2] = nrm;
-            }
-        }
+        // The run's LDS stores are complete before any lane of the wave reads them back.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // Copy the run to quads [qb, qb + run), clipped at the quad capacity; 6 indices
+        // per quad make it a whole number of 8-byte pairs at an 8-byte aligned address.
+        const int keep = max(0, min(run, a.caps.quads - qb));
+        int2* dst = reinterpret_cast<int2*>(tris + 6 * (long)qb);
+        const int2* src = reinterpret_cast<const int2*>(stage);
+        for (int i = lane; i < 3 * keep; i += 64) dst[i] = src[i];
+        // ... and every lane has read the run before the next segment overwrites it.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -2035,7 +2059,8 @@ void launch_qef(const KArgs& a, int chunks, hipStream_t s) {
         hipLaunchKernelGGL(k_qef<false>, grid, dim3(256), 0, s, a);
 }
 void launch_emit(const KArgs& a, int chunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_emit, dim3(chunk_blocks(chunks, a.caps.voxels, a.voxel_blocks), chunks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_emit, dim3(chunk_blocks(chunks, a.caps.voxels, a.voxel_blocks), chunks), dim3(64 * kEmitWaves), 0,
+                       s, a);
 }
 
 const char* const kStageNames[kNumStages] = {
