@@ -31,7 +31,15 @@ FLOPS_SNOISE2 = 49
 FLOPS_SNOISE3 = 87
 
 
+# leven's Terrain (cl/noise.cl:205-223) as height term: RidgedMultiFractal over 1 + 7 snoise2
+# (9 operations per octave, 4 around them), fBm over 4 and over 2 snoise2 (5 per octave, 2 to
+# scale the point), 13 for the blend; times 900 and y - H
+FLOPS_LEVEN_TERRAIN = (8 * FLOPS_SNOISE2 + 7 * 9 + 4) + (4 * (FLOPS_SNOISE2 + 5) + 2) + (2 * (FLOPS_SNOISE2 + 5) + 2) + 13
+
+
 def flops_height(desc):
+    if desc.kind == 2:
+        return FLOPS_LEVEN_TERRAIN + 2
     return 3 + (FLOPS_SNOISE2 + 5) * desc.height_octaves
 
 
@@ -617,13 +625,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="clipmap", choices=["clipmap", "terrain_512"],
-                    help="the headline workload (the other is reported beside it)")
+    ap.add_argument("--workload", default="clipmap_leven", choices=["clipmap_leven", "clipmap", "terrain_512"],
+                    help="the headline workload (the others are reported beside it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
-    ap.add_argument("--cpu-sample", type=int, default=64, help="clipmap chunks in the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=64, help="cave clipmap chunks in the CPU baseline sample")
+    ap.add_argument("--cpu-sample-leven", type=int, default=4096,
+                    help="chunks of the leven-terrain clipmap in its CPU baseline sample (capped at all of them)")
     ap.add_argument("--plain-chunks", type=int, default=0, help="chunks for the plain form (0 = all)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="length of the served request stream (0 = skip)")
-    ap.add_argument("--only", default="", help="comma-separated legs to run (clipmap, terrain_512, in_range, "
+    ap.add_argument("--only", default="", help="comma-separated legs to run (clipmap_leven, clipmap, terrain_512, "
+                                                 "in_range, "
                                                  "precision, seams, simplify, serving, plain_cpu); empty = all")
     ap.add_argument("--no-seams", action="store_true", help="skip seam stitching of the clipmap")
     ap.add_argument("--no-precision", action="store_true", help="skip the precision-form batch")
@@ -644,6 +655,15 @@ def main():
 
     threads = max(1, min(16, os.cpu_count() or 1))
     legs = {}
+    # ---- the 6-LOD clipmap over the reference's own terrain field (DensityFunc, cl/noise.cl:225-266):
+    # ~1,900 chunks at voxel sizes 1..32 with 0 to ~10k active voxels each, 256 brushes on the surface
+    lev = workloads.clipmap_leven()
+    if want("clipmap_leven"):
+        legs["clipmap_leven"], _ = batched_leg(lev, args, args.steps, 0, 0, plain=True)
+        if not args.no_cpu:
+            legs["clipmap_leven"]["cpu_baseline"] = cpu_sample(lev, args.cpu_sample_leven, threads)
+            legs["clipmap_leven"]["batched_over_cpu"] = round(legs["clipmap_leven"]["chunks_per_s"] /
+                                                              legs["clipmap_leven"]["cpu_baseline"]["value"], 1)
     # ---- the 6-LOD clipmap: ~1,900 chunks at voxel sizes 1..32, full per-chunk capacities
     clip = workloads.clipmap()
     cb, nodes = None, None
@@ -694,12 +714,16 @@ def main():
     if head is None:  # a partial run (--only) without the headline leg
         print(json.dumps({"metric": METRIC, "partial": True, **legs}), flush=True)
         return
-    hw = clip if args.workload == "clipmap" else workloads.terrain_512()
+    hw = {"clipmap_leven": lev, "clipmap": clip}.get(args.workload) or workloads.terrain_512()
     roof = dict(head["roofline"])
     roof["note"] = ("FP32 vector-ALU bound (neither HBM nor MFMA): achieved counts the method's field "
                     "evaluations (23 per edge, 65^3 per chunk), executed_tflops what the kernels evaluate after "
                     "the cave cut and shared heights; counters in profiles/kernels_" +
-                    ("clipmap" if args.workload == "clipmap" else "terrain512") + ".md")
+                    {"clipmap_leven": "clipmap_leven", "clipmap": "clipmap"}.get(args.workload, "terrain512") + ".md")
+    fields = {"clipmap_leven": "y - 900 Terrain(x, z) (the reference's DensityFunc: ridged 7 + fBm 4 + fBm 2 "
+                               "octaves of snoise2)",
+              "clipmap": "max(y - 24 fbm2, fbm3 - 0.3), 5+5 octaves",
+              "terrain_512": "max(y - 24 fbm2, fbm3 - 0.3), 5+5 octaves"}
     out = {
         "metric": METRIC,
         "value": head["chunks_per_s"],
@@ -715,8 +739,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": hw["name"], "chunks": len(hw["sizes"]), "voxels_per_chunk": 64,
                    "lods": "voxel sizes 1..32 (6-LOD clipmap selection around a seeded camera)"
-                   if args.workload == "clipmap" else "LOD0",
-                   "field": "max(y - 24 fbm2, fbm3 - 0.3), 5+5 octaves", "csg_ops": int(len(hw["ops"])),
+                   if args.workload != "terrain_512" else "LOD0",
+                   "field": fields[args.workload], "csg_ops": int(len(hw["ops"])),
                    "form": "batched (one launch per stage over all chunks, hipGraph)"},
         "plain_chunks_per_s": head.get("plain", {}).get("chunks_per_s"),
         "batched_over_plain": head.get("batched_over_plain"),

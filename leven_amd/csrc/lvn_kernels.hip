@@ -1187,22 +1187,28 @@ __device__ __forceinline__ int corner_materials(const KArgs& a, const uint8_t* m
     return corners;
 }
 
-// FindDominantMaterial with its insertion sort, cl/octree.cl:78-138.
+// FindDominantMaterial, cl/octree.cl:78-138.  The reference sorts the eight corner
+// materials by insertion; any correct sort yields the same sequence, and Batcher's
+// odd-even merge network (19 compare-exchanges, min/max pairs) does it without the
+// insertion sort's data-dependent loops, which diverge across a wave.
+__device__ __forceinline__ void sort_exchange(int& a, int& b) {
+    const int lo = min(a, b), hi = max(a, b);
+    a = lo;
+    b = hi;
+}
+
 __device__ __forceinline__ int dominant_material(const int* m) {
     int d[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) d[i] = m[i];
-#pragma unroll
-    for (int i = 1; i < 8; ++i) {
-        const int tmp = d[i];
-        int j = i;
-        while (j >= 1 && tmp < d[j - 1]) {
-            d[j] = d[j - 1];
-            --j;
-        }
-        d[j] = tmp;
-    }
+    sort_exchange(d[0], d[1]); sort_exchange(d[2], d[3]); sort_exchange(d[4], d[5]); sort_exchange(d[6], d[7]);
+    sort_exchange(d[0], d[2]); sort_exchange(d[1], d[3]); sort_exchange(d[4], d[6]); sort_exchange(d[5], d[7]);
+    sort_exchange(d[1], d[2]); sort_exchange(d[5], d[6]);
+    sort_exchange(d[0], d[4]); sort_exchange(d[1], d[5]); sort_exchange(d[2], d[6]); sort_exchange(d[3], d[7]);
+    sort_exchange(d[2], d[4]); sort_exchange(d[3], d[5]);
+    sort_exchange(d[1], d[2]); sort_exchange(d[3], d[4]); sort_exchange(d[5], d[6]);
     int current = d[0], count = 1, maxCount = 0, maxMaterial = 0;
+#pragma unroll
     for (int i = 1; i < 8; ++i) {
         const int v = d[i];
         if (v == kAir || v == kNone) continue;
@@ -1395,26 +1401,30 @@ __device__ __forceinline__ void qef_fast(const KArgs& a, int chunk, const ChunkD
         ke[e] = k >= 0 && k < a.caps.edges ? k : -1;
     }
 
+    // The edge's point is a0 + (a1 - a0) t per component (cl/octree.cl:266-272), scaled.  Its
+    // end corners differ by exactly 1 along the edge's axis and by exactly 0 across it, and t
+    // is a finite crossing in [0, 1], so (a1 - a0) t is t along the axis and +0 across it:
+    // the across components are fscale * a0 (six per voxel, formed once) and the w lane is
+    // +0, which leaves the dot product's fourth term +0 and makes the w sums count edges.
+    const float sx0 = fscale * (float)p.x, sx1 = fscale * (float)(p.x + 1);
+    const float sy0 = fscale * (float)p.y, sy1 = fscale * (float)(p.y + 1);
+    const float sz0 = fscale * (float)p.z, sz1 = fscale * (float)(p.z + 1);
     float ata0 = 0.f, ata1 = 0.f, ata2 = 0.f, ata3 = 0.f, ata4 = 0.f, ata5 = 0.f;
     Q4 atb = {0.f, 0.f, 0.f, 0.f}, mass = {0.f, 0.f, 0.f, 0.f}, nsum = {0.f, 0.f, 0.f, 0.f};
+    int count = 0;
 #pragma unroll
     for (int e = 0; e < 12; ++e) {
         if (ke[e] < 0) continue;
-        const int e0 = kEdgeMap[e][0], e1 = kEdgeMap[e][1];
+        const int e0 = kEdgeMap[e][0], axis = e >> 2;
         const int hx = p.x + ((e0 >> 2) & 1), hy = p.y + ((e0 >> 1) & 1), hz = p.z + (e0 & 1);
         const float4 d = ed[ke[e]];
-        const float a0x = (float)hx, a0y = (float)hy, a0z = (float)hz;
-        const float a1x = (float)(p.x + ((e1 >> 2) & 1)), a1y = (float)(p.y + ((e1 >> 1) & 1)),
-                    a1z = (float)(p.z + (e1 & 1));
         Q4 pos, n;
-        pos.x = fscale * (a0x + (a1x - a0x) * d.w);
-        pos.y = fscale * (a0y + (a1y - a0y) * d.w);
-        pos.z = fscale * (a0z + (a1z - a0z) * d.w);
-        pos.w = fscale * (0.f + (0.f - 0.f) * d.w);
+        pos.x = axis == 0 ? fscale * ((float)hx + d.w) : ((e0 >> 2) & 1) ? sx1 : sx0;
+        pos.y = axis == 1 ? fscale * ((float)hy + d.w) : ((e0 >> 1) & 1) ? sy1 : sy0;
+        pos.z = axis == 2 ? fscale * ((float)hz + d.w) : (e0 & 1) ? sz1 : sz0;
         n.x = d.x;
         n.y = d.y;
         n.z = d.z;
-        n.w = 0.f;
         // qef_add_point, cl/qef.cl:170-191
         ata0 = ata0 + n.x * n.x;
         ata1 = ata1 + n.x * n.y;
@@ -1422,19 +1432,20 @@ __device__ __forceinline__ void qef_fast(const KArgs& a, int chunk, const ChunkD
         ata3 = ata3 + n.y * n.y;
         ata4 = ata4 + n.y * n.z;
         ata5 = ata5 + n.z * n.z;
-        const float b = dot4(pos, n);
+        const float b = (((pos.x * n.x + pos.y * n.y) + pos.z * n.z) + 0.f);
         atb.x = atb.x + n.x * b;
         atb.y = atb.y + n.y * b;
         atb.z = atb.z + n.z * b;
         mass.x = mass.x + pos.x;
         mass.y = mass.y + pos.y;
         mass.z = mass.z + pos.z;
-        mass.w = mass.w + 1.f;
         nsum.x = nsum.x + n.x;
         nsum.y = nsum.y + n.y;
         nsum.z = nsum.z + n.z;
-        nsum.w = (nsum.w + n.w) + 1.f;
+        ++count;
     }
+    mass.w = (float)count;  // the sums of 1 per edge
+    nsum.w = (float)count;
     // qef_create_from_points tail (cl/qef.cl:302), then qef_solve's divide by max(w, 1) (:244).
     {
         const float w = mass.w;

@@ -9,14 +9,15 @@ computes them.
   plain_cpu     one 64^3 chunk of a 2D simplex-fBm heightfield (6 octaves)
   terrain_512   8x8x8 chunks of heightfield + 3D cave noise with 256 brushes
   precision     the terrain_512 field with 1024 Y-rotated boxes (FP16 + precise QEF)
-  clipmap       6-LOD clipmap selection around a seeded camera (~2000 chunks)
+  clipmap       6-LOD clipmap selection around a seeded camera (~2000 chunks), terrain_512's field
+  clipmap_leven the same selection around a camera on the reference's own terrain field
   serving       Poisson request stream along a camera path with 5% CSG edits
 """
 import math
 
 import numpy as np
 
-from .abi import (CSG_OP_DTYPE, CTX_PRECISE_QEF, FIELD_HEIGHTFIELD, FIELD_TERRAIN_CAVES,
+from .abi import (CSG_OP_DTYPE, CTX_PRECISE_QEF, FIELD_HEIGHTFIELD, FIELD_LEVEN_TERRAIN, FIELD_TERRAIN_CAVES,
                   MATERIAL_AIR, field_desc)
 
 LEAF_SIZE_SCALE = 4
@@ -150,6 +151,44 @@ def clipmap(seed=4):
     ops, aabbs = make_ops(256, seed=2)
     return dict(name="clipmap", desc=terrain_desc(), seed=2, default_material=0, mins=mins, sizes=sizes,
                 ops=ops, aabbs=aabbs, flags=0, camera=cam)
+
+
+def leven_desc():
+    """The reference's own field, DensityFunc = y - 900 Terrain(x, z) (cl/noise.cl:205-266)."""
+    return field_desc(FIELD_LEVEN_TERRAIN)
+
+
+LEVEN_SEED = 93923590
+# Camera of clipmap_leven in world units: x, z drawn as clipmap()'s (seed 4); y sits 16 voxels
+# above leven's terrain there, whose surface near the camera is the plane
+# h = 575.5 + 0.30 (x - cx) + 0.40 (z - cz) voxels to within ~10 voxels over +-96 voxels
+# (the field sampled at 3 x 3 points 64 voxels apart).
+LEVEN_CAMERA = (907.379, 4.0 * 575.53 + 64.0, 23.199)
+LEVEN_SURFACE = (575.53, 0.30, 0.40)
+
+
+def clipmap_leven(seed=6):
+    """The 6-LOD clipmap selection (~1,900 chunks at voxel sizes 1..32) around a camera just
+    above the reference's own terrain field, with 256 brushes on the surface near the camera
+    (radii 2-12 voxels, spheres and boxes, add/subtract 50/50)."""
+    cam = np.array(LEVEN_CAMERA, np.float64)
+    R = LOD_MAX_NODE_SIZE
+    from . import mesher
+    cm = mesher.Clipmap((-14 * R, -R, -14 * R), (14 * R, R, 14 * R))
+    mins, sizes = cm.update(cam)["selected"]
+    cm.close()
+    # brushes within +-96 voxels of the camera, centred on the surface plane +-12 voxels
+    cx, cz = cam[0] / LEAF_SIZE_SCALE, cam[2] / LEAF_SIZE_SCALE
+    ops, _ = make_ops(256, seed=seed, extent=192, y_range=(-12.0, 12.0))
+    h0, sx, sz = LEVEN_SURFACE
+    for op in ops:
+        x = op["origin"][0] + cx - 96.0
+        z = op["origin"][2] + cz - 96.0
+        y = op["origin"][1] + h0 + sx * (x - cx) + sz * (z - cz)
+        op["origin"][:3] = [x, y, z]
+    aabbs = np.stack([op_aabb(op) for op in ops])
+    return dict(name="clipmap_leven", desc=leven_desc(), seed=LEVEN_SEED, default_material=0, mins=mins,
+                sizes=sizes, ops=ops, aabbs=aabbs, flags=0, camera=cam)
 
 
 def serving(seed=5, duration=2.0, rate=4000.0, edit_fraction=0.05, speed_voxels=10.0):

@@ -182,3 +182,21 @@ def test_rejects_bad_arguments():
     with pytest.raises(mesher.LvnError):  # results must cover the to-mesh list
         cm.commit(np.zeros((1, 3), np.int32))
     cm.close()
+
+
+def test_leven_clipmap_workload(oracle):
+    """The leven-terrain clipmap workload: the selection around its camera (restated rule),
+    a camera 16 voxels above the reference's terrain field, and brushes centred within 12
+    voxels of its surface plane, so they cut the surface (checked against the oracle's
+    field, density = y - H in voxels)."""
+    w = workloads.clipmap_leven()
+    want = workloads.clipmap_select(w["camera"], (-14 * R, -R, -14 * R), (14 * R, R, 14 * R))
+    assert sorted(_pairs(w["mins"], w["sizes"])) == sorted(want)
+    assert 1800 <= len(w["sizes"]) <= 2100 and sorted(set((w["sizes"] // C0).tolist())) == [1, 2, 4, 8, 16, 32]
+    perm = oracle.perm_table(w["seed"])
+    cam = w["camera"] / workloads.LEAF_SIZE_SCALE
+    assert 15.0 <= oracle.density(w["desc"], perm, *cam) <= 17.0
+    # a brush whose centre lies within its own extent of the surface cuts it
+    d = np.array([oracle.density(w["desc"], perm, *(op["origin"][:3] - 0.5)) for op in w["ops"]])
+    ext = np.array([op["dims"][:3].max() for op in w["ops"]])
+    assert (np.abs(d) <= 24.0).all() and (np.abs(d) <= ext).mean() > 0.5

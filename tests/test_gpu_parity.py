@@ -353,6 +353,50 @@ def test_batched_mixed_lods_match_oracle(oracle):
     mesher.clear_csg_ops()
 
 
+def test_leven_clipmap_batch_matches_oracle(oracle):
+    """The whole 6-LOD clipmap over the reference's own terrain field (DensityFunc) in one
+    batch; per LOD the densest chunk, the two chunks with the most overlapping brushes and
+    an empty chunk are bit-identical to the oracle, and every chunk stays within 0..60k
+    active voxels."""
+    w = workloads.clipmap_leven()
+    setup_field(w["desc"], w["seed"])
+    for op, bb in zip(w["ops"], w["aabbs"]):
+        mesher.store_csg_op(op, bb[:3], bb[3:])
+    n = len(w["sizes"])
+    b = mesher.Batch(64, n, edge_cap=0, voxel_cap=0)
+    b.set_chunks(w["mins"], w["sizes"])
+    b.run()
+    b.wait()
+    counts = b.counts()
+    assert (counts[:, 4] == 0).all()
+    assert counts[:, 1].max() <= 60000
+    lods = np.log2(w["sizes"] // 256).astype(int)
+    nops = np.array([len(_ops_for_chunk(w, mn, int(size))) for mn, size in zip(w["mins"], w["sizes"])])
+    pick = []
+    for lod in range(6):
+        sel = np.where(lods == lod)[0]
+        assert len(sel), f"no chunk of LOD {lod}"
+        pick.append(int(sel[np.argmax(counts[sel, 1])]))
+        pick += [int(i) for i in sel[np.argsort(-nops[sel], kind="stable")[:2]] if nops[i] > 0]
+        empty = sel[counts[sel, 1] == 0]
+        if len(empty):
+            pick.append(int(empty[len(empty) // 2]))
+    pick = sorted(set(pick))
+    perm = oracle.perm_table(w["seed"])
+    for i in pick:
+        mn, size = w["mins"][i], int(w["sizes"][i])
+        c = oracle.Chunk(64).generate(w["desc"], perm, mn, size, ops=_ops_for_chunk(w, mn, size)).build()
+        v, t, s = b.get_mesh(i)
+        assert (t == c.triangles()).all(), f"chunk {i} (size {size}) triangles"
+        assert (v.view(np.uint32) == c.vertices().view(np.uint32)).all(), f"chunk {i} vertices"
+        assert (s.view(np.uint32) == c.seams().view(np.uint32)).all(), f"chunk {i} seams"
+    print(f"\nleven clipmap: {n} chunks, {len(pick)} checked, active voxels up to {counts[:, 1].max()}, "
+          f"{int((nops > 0).sum())} chunks under brushes")
+    assert (nops[pick] > 0).sum() >= 4 and counts[pick, 1].max() > 1000
+    b.close()
+    mesher.clear_csg_ops()
+
+
 @pytest.mark.parametrize("n", [16, 32, 64])
 def test_height_lines_match_oracle(oracle, n):
     """x/z-edge heights evaluated once per line (k_height_lines, `height_lines` 2) and per
