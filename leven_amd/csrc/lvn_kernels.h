@@ -35,7 +35,8 @@ struct KArgs {
     int row_blocks;          // workgroups over the batch for the bookkeeping kernels
     int voxel_blocks;        // workgroups over the batch for the per-voxel kernels
     int cross_lanes;         // lanes per edge in the crossing stage (1, 2, 4, 8; 16 = sample-parallel), 0 = by batch size
-    int pad3;
+    int dens_cols;           // 1: a field without the cave term stores no density; crossings form
+                             // their end samples as y - H from the column heights (h_cols)
 };
 
 // Global work-queue heads (queue[1]: the edge-crossing items; the density and

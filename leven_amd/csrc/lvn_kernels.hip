@@ -134,7 +134,7 @@ __device__ __forceinline__ void density_column(const KArgs& a, const NoiseLds& T
     const float wx = (float)(x * cd.scale + cd.off[0]);
     const float wz = (float)(z * cd.scale + cd.off[2]);
     const float H = height_term(a.F, T, wx, wz);
-    if (seg == 0) slot_ptr<float>(a, chunk, a.L.h_cols)[col] = H;  // read by the sample-parallel crossing
+    if (seg == 0) slot_ptr<float>(a, chunk, a.L.h_cols)[col] = H;  // read by the crossing kernels
     float* dens = slot_ptr<float>(a, chunk, a.L.density);
     __half* dens16 = slot_ptr<__half>(a, chunk, a.L.density);
     unsigned caves = 0;
@@ -145,7 +145,7 @@ __device__ __forceinline__ void density_column(const KArgs& a, const NoiseLds& T
         const int idx = field_index(G, x, y, z);
         if (a.fp16)
             dens16[idx] = to_half_keep_sign(d);
-        else
+        else if (!a.dens_cols)
             dens[idx] = d;
         const uint8_t m = d < 0.f ? (uint8_t)a.defmat : (uint8_t)kAir;
         mat[idx] = m;
@@ -282,16 +282,18 @@ __global__ __launch_bounds__(256) void k_solid_rows(KArgs a) {
             if (w0 + j < words_total) stage[wave][j] = mat[w0 + j];
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        // The block's rows are 2 * rows consecutive words of S: lane 2k keeps row k's low
+        // word and lane 2k + 1 its high word, and one store writes them all.
         const int rows = min(kRowsPerBlock, G.field_rows - b * kRowsPerBlock);
+        uint64_t word = 0;
         for (int k = 0; k < rows; ++k) {
             const bool s0 = lane < F && bytes[k * F + lane] != kAir;
             const bool s1 = lane + 64 < F && bytes[k * F + lane + 64] != kAir;
             const uint64_t lo = __ballot(s0), hi = __ballot(s1);
-            if (lane == 0) {
-                S[2 * (b * kRowsPerBlock + k)] = lo;
-                S[2 * (b * kRowsPerBlock + k) + 1] = hi;
-            }
+            if (lane == 2 * k) word = lo;
+            if (lane == 2 * k + 1) word = hi;
         }
+        if (lane < 2 * rows) S[2 * b * kRowsPerBlock + lane] = word;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     }
@@ -718,6 +720,25 @@ __device__ __forceinline__ float4 brush_crossing(const KArgs& a, const ChunkDesc
 // One compacted edge.  An edge with an endpoint whose material a brush changed
 // takes brush data (src/compute_csg.cpp:97-217); otherwise the density crossing
 // (fresh field) or its data from the cached edge list.
+// The end samples of edge (x, y, z, axis), field indices i0 and i1: the stored field, or,
+// for a field without the cave term whose density a batch does not store (dens_cols),
+// y - H with H from the column heights: density_column's expression, bit for bit (its
+// sample is combine(wy - H) = wy - H for such a field).
+__device__ __forceinline__ float2 edge_ends(const KArgs& a, int chunk, const ChunkDesc& cd, int x, int y, int z, int axis,
+                                           int i0, int i1) {
+    if (a.dens_cols) {
+        const float* hc = slot_ptr<const float>(a, chunk, a.L.h_cols);
+        const int F = a.G.F;
+        const float H0 = hc[x + F * z];
+        const float H1 = axis == 1 ? H0 : hc[(x + (axis == 0 ? 1 : 0)) + F * (z + (axis == 2 ? 1 : 0))];
+        const float wy0 = (float)(y * cd.scale + cd.off[1]);
+        const float wy1 = (float)((y + (axis == 1 ? 1 : 0)) * cd.scale + cd.off[1]);
+        return make_float2(wy0 - H0, wy1 - H1);
+    }
+    const float* dens = slot_ptr<const float>(a, chunk, a.L.density);
+    return make_float2(dens[i0], dens[i1]);
+}
+
 template <bool kCount>
 __device__ __forceinline__ void cross_edge(const KArgs& a, const NoiseLds& T, int chunk, int i) {
     const Geometry& G = a.G;
@@ -753,11 +774,11 @@ __device__ __forceinline__ void cross_edge(const KArgs& a, const NoiseLds& T, in
             atomicOr(&a.counters[chunk].status, kStatusLookupMiss);
         }
     } else {
-        const float* dens = slot_ptr<float>(a, chunk, a.L.density);
         const bool ends = !a.fp16;
+        const float2 de = ends ? edge_ends(a, chunk, cd, x, y, z, axis, i0, i1) : make_float2(0.f, 0.f);
         const float* hl = slot_ptr<const float>(a, chunk, a.L.h_lines) + hline_index(G.H, axis == 2 ? 1 : 0, x, z);
         unsigned nh = 0, nc = 0;
-        out = density_crossing(a, T, cd, x, y, z, axis, ends, ends ? dens[i0] : 0.f, ends ? dens[i1] : 0.f, hl, nh, nc);
+        out = density_crossing(a, T, cd, x, y, z, axis, ends, de.x, de.y, hl, nh, nc);
         if constexpr (kCount) {
             atomicAdd(&a.work[2], (unsigned long long)nh);
             atomicAdd(&a.work[3], (unsigned long long)nc);
@@ -873,8 +894,8 @@ __device__ __forceinline__ void cross_edge_group(const KArgs& a, const NoiseLds&
     const FieldParams& Fp = a.F;
     const bool caves = Fp.kind == LVN_FIELD_TERRAIN_CAVES;
     const bool ends = !a.fp16;
-    const float* dens = slot_ptr<float>(a, chunk, a.L.density);
-    const float d_start = ends ? dens[i0] : 0.f, d_end = ends ? dens[i1] : 0.f;
+    const float2 de = ends ? edge_ends(a, chunk, cd, x, y, z, axis, i0, i1) : make_float2(0.f, 0.f);
+    const float d_start = de.x, d_end = de.y;
     const int w0x = x * cd.scale + cd.off[0], w0y = y * cd.scale + cd.off[1], w0z = z * cd.scale + cd.off[2];
     const float p0x = (float)w0x, p0y = (float)w0y, p0z = (float)w0z;
     const float p1x = (float)(w0x + (axis == 0 ? cd.scale : 0));
@@ -1073,9 +1094,9 @@ __global__ __attribute__((amdgpu_waves_per_eu(6, 8))) __launch_bounds__(kCrossTh
                     exyz = x | (y << 8) | (z << 16) | (axis << 24);
                     eH = axis == 1 ? slot_ptr<const float>(a, chunk, a.L.h_cols)[x + G.F * z] : 0.f;
                     if (ends) {
-                        const float* dens = slot_ptr<const float>(a, chunk, a.L.density);
-                        eS = dens[i0];
-                        eE = dens[i1];
+                        const float2 de = edge_ends(a, chunk, cd, x, y, z, axis, i0, i1);
+                        eS = de.x;
+                        eE = de.y;
                     }
                 }
             }

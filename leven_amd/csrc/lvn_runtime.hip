@@ -656,8 +656,15 @@ static lvn_tuning effective_tuning(const Batch* b) {
     return t;
 }
 
+// A batch exports meshes, never its density field: for a field without the cave term
+// it leaves the density unstored (KArgs::dens_cols).
+static int density_from_columns(const KArgs& a) {
+    return a.F.kind != LVN_FIELD_TERRAIN_CAVES && !a.fp16 ? 1 : 0;
+}
+
 static KArgs batch_args(const Batch* b, const lvn_tuning& t) {
     KArgs a = b->ctx.kargs();
+    a.dens_cols = density_from_columns(a);
     a.ysplit = t.density_ysplit;
     a.persist_per_cu = t.persistent_per_cu;
     a.density_per_cu = t.density_per_cu;

@@ -6,12 +6,13 @@ fractal parameters (octaves 1-8, frequencies, lacunarities away from 2 so octave
 inexact, gains, amplitudes, cave threshold), a noise seed of either sign, a chunk size
 (N = 16, 32 or 64 voxels; a second sweep N = 4 and 8) and LOD (0-4), a position with the surface inside the chunk, and
 0-6 brushes (spheres and Y-rotated boxes, add and subtract, several materials) through
-the chunk (needs a GPU).
+the chunk (needs a GPU).  Each case is also meshed by a one-chunk batch from the stored
+ops and compared with the oracle's mesh.
 """
 import numpy as np
 import pytest
 
-from leven_amd import abi, mesher
+from leven_amd import abi, mesher, workloads
 from leven_amd.abi import CSG_OP_DTYPE, MATERIAL_AIR
 from test_gpu_parity import compare_stages
 
@@ -84,6 +85,22 @@ def _run_case(oracle, c):
     ch = oracle.Chunk(c["n"]).generate(c["desc"], perm, mn, c["size"], ops=ops).build()
     compare_stages(g, ch, c["n"], c["size"])
     ctx.close()
+    # The batched form meshes the same chunk from the stored ops; for the fields without
+    # the cave term it stores no density and takes the edges' end samples from the
+    # column heights.
+    for op in ops:
+        bb = workloads.op_aabb(op)
+        mesher.store_csg_op(op, bb[:3], bb[3:])
+    b = mesher.Batch(c["n"], 1, edge_cap=0, voxel_cap=0)
+    b.set_chunks(np.array([mn], np.int32), np.array([c["size"]], np.int32))
+    b.run()
+    b.wait()
+    v, t, s = b.get_mesh(0)
+    b.close()
+    mesher.clear_csg_ops()
+    assert (t == ch.triangles()).all(), "batched triangles"
+    assert (v.view(np.uint32) == ch.vertices().view(np.uint32)).all(), "batched vertices"
+    assert (s.view(np.uint32) == ch.seams().view(np.uint32)).all(), "batched seams"
 
 
 @pytest.mark.parametrize("k", range(24))
