@@ -849,18 +849,29 @@ __global__ __attribute__((amdgpu_waves_per_eu(6, 8))) __launch_bounds__(kCrossTh
     load_noise_tables(T, a.perm, a.grads);
     const int total = a.item_prefix[chunks];
     const int lane = threadIdx.x & 63;
-    // one global queue: items are long (64 edges x ~21 field evaluations), and sharing
-    // them within workgroups instead measured slower (clipmap 93.2 -> 94.5 ms)
+    // One global queue (sharing items within workgroups instead measured slower: clipmap
+    // 93.2 -> 94.5 ms).  A grab takes up to 4 consecutive items, as many as still leave
+    // every wave 8 grabs: one counter serves ~77,000 grabs per ms, which items of a few
+    // microseconds (a heightfield's y-edges) would otherwise wait on, and the chunk of
+    // an item is found by one search per grab, the grab's later items walking forward.
+    const int waves = (int)(gridDim.x * (blockDim.x >> 6));
+    const int per_grab = max(1, min(4, total / (8 * waves)));
     for (;;) {
-        const int item = next_item(&a.queue[1]);
-        if (item >= total) break;
-        int lo = 0, hi = chunks;  // last chunk with item_prefix[c] <= item
+        int first = 0;
+        if (lane == 0) first = atomicAdd(&a.queue[1], per_grab);
+        first = __shfl(first, 0);
+        if (first >= total) break;
+        const int end = min(first + per_grab, total);
+        int lo = 0, hi = chunks;  // last chunk with item_prefix[c] <= first
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if (a.item_prefix[mid] <= item) lo = mid; else hi = mid;
+            if (a.item_prefix[mid] <= first) lo = mid; else hi = mid;
         }
-        const int i = edge_of_item(a, lo, item - a.item_prefix[lo], lane, 64);
-        if (i >= 0) cross_edge<kCount>(a, T, lo, i);
+        for (int item = first; item < end; ++item) {
+            while (a.item_prefix[lo + 1] <= item) ++lo;  // item < item_prefix[chunks] stops it
+            const int i = edge_of_item(a, lo, item - a.item_prefix[lo], lane, 64);
+            if (i >= 0) cross_edge<kCount>(a, T, lo, i);
+        }
     }
 }
 
