@@ -37,6 +37,9 @@ struct KArgs {
     int cross_lanes;         // lanes per edge in the crossing stage (1, 2, 4, 8; 16 = sample-parallel), 0 = by batch size
     int dens_cols;           // 1: a field without the cave term stores no density; crossings form
                              // their end samples as y - H from the column heights (h_cols)
+    int mat_free;            // 1: a fresh chunk without brushes stores no materials; the density
+                             // pass writes its solid bits as planes, which k_solid_rows turns
+                             // into rows and the QEF reads corners from
 };
 
 // Global work-queue heads (queue[1]: the edge-crossing items; the density and

@@ -103,15 +103,21 @@ __constant__ int kEdgeMap[12][2] = {{0, 4}, {1, 5}, {2, 6}, {3, 7}, {0, 2}, {1, 
 // Density + default material (GenerateDefaultField, cl/density_field.cl:11-37)
 // ---------------------------------------------------------------------------
 // One thread per (x, z) column segment: the height term depends on x and z
-// only, so it is evaluated once and reused down the column.
+// only, so it is evaluated once and reused down the column.  A wave takes 64
+// consecutive columns (group g, column x + F z = 64 g + lane) over one y-segment.
+__device__ __forceinline__ bool chunk_mat_free(const KArgs& a, const ChunkDesc& cd) {
+    return a.mat_free && cd.op_count == 0 && cd.mode != kModeCached;
+}
+
 template <bool kCount>
-__device__ __forceinline__ void density_column(const KArgs& a, const NoiseLds& T, int chunk, int t) {
+__device__ __forceinline__ void density_column(const KArgs& a, const NoiseLds& T, int chunk, int seg, int g) {
     const Geometry& G = a.G;
     const int F = G.F;
+    const int lane = threadIdx.x & 63;
+    const bool valid = g * 64 + lane < F * F;     // the last group is partial
+    const int col = valid ? g * 64 + lane : F * F - 1;
     const int seg_len = (F + a.ysplit - 1) / a.ysplit;
-    const int seg = div_magic(div_magic(t, G.magic_f), G.magic_f);
-    const int col = t - seg * F * F;
-    const ChunkDesc& cd = a.chunks[chunk];
+    const ChunkDesc cd = a.chunks[chunk];  // a copy: the loops below store through pointers that could alias it
     const int z = div_magic(col, G.magic_f);
     const int x = col - z * F;
     const int y0 = seg * seg_len;
@@ -123,7 +129,7 @@ __device__ __forceinline__ void density_column(const KArgs& a, const NoiseLds& T
     const bool keep_orig = cd.op_count > 0;
     if (cd.mode == kModeCached) {  // a cached field: its materials, no density
         const uint8_t* old = reinterpret_cast<const uint8_t*>(cd.old_mat);
-        for (int y = y0; y < y1; ++y) {
+        for (int y = y0; y < y1 && valid; ++y) {
             const int idx = field_index(G, x, y, z);
             const uint8_t m = old[idx];
             mat[idx] = m;
@@ -131,30 +137,50 @@ __device__ __forceinline__ void density_column(const KArgs& a, const NoiseLds& T
         }
         return;
     }
+    // A chunk without brushes in a batch stores no materials (uniform over the wave): its
+    // "not air" bits go to the planes, one ballot per y, and lane j keeps the word of
+    // y = y0 + 64 k + j until the wave stores them together.
+    const bool planes = chunk_mat_free(a, cd);
+    uint64_t* P = slot_ptr<uint64_t>(a, chunk, a.L.planes) + (long)g * F;
     const float wx = (float)(x * cd.scale + cd.off[0]);
     const float wz = (float)(z * cd.scale + cd.off[2]);
     const float H = height_term(a.F, T, wx, wz);
-    if (seg == 0) slot_ptr<float>(a, chunk, a.L.h_cols)[col] = H;  // read by the crossing kernels
+    if (seg == 0 && valid) slot_ptr<float>(a, chunk, a.L.h_cols)[col] = H;  // read by the crossing kernels
     float* dens = slot_ptr<float>(a, chunk, a.L.density);
     __half* dens16 = slot_ptr<__half>(a, chunk, a.L.density);
     unsigned caves = 0;
+    uint64_t word = 0;
     for (int y = y0; y < y1; ++y) {
         const float wy = (float)(y * cd.scale + cd.off[1]);
         const float d = combine(a.F, T, wy - H, wx, wy, wz);
         if constexpr (kCount) caves += (a.F.kind == LVN_FIELD_TERRAIN_CAVES && !((wy - H) > a.F.cave_cut)) ? 1u : 0u;
         const int idx = field_index(G, x, y, z);
-        if (a.fp16)
-            dens16[idx] = to_half_keep_sign(d);
-        else if (!a.dens_cols)
-            dens[idx] = d;
         const uint8_t m = d < 0.f ? (uint8_t)a.defmat : (uint8_t)kAir;
-        mat[idx] = m;
-        if (keep_orig) orig[idx] = m;
+        if (valid) {
+            if (a.fp16)
+                dens16[idx] = to_half_keep_sign(d);
+            else if (!a.dens_cols)
+                dens[idx] = d;
+            if (!planes) {
+                mat[idx] = m;
+                if (keep_orig) orig[idx] = m;
+            }
+        }
+        if (planes) {
+            const uint64_t bits = __ballot(valid && m != (uint8_t)kAir);
+            const int j = (y - y0) & 63;
+            if (lane == j) word = bits;
+            if (j == 63 || y == y1 - 1) {
+                if (lane <= j) P[y - j + lane] = word;
+            }
+        }
     }
     if constexpr (kCount) {
-        atomicAdd(&a.work[0], 1ull);
-        atomicAdd(&a.work[1], (unsigned long long)caves);
-        atomicAdd(&a.work[5], (unsigned long long)(y1 > y0 ? y1 - y0 : 0));
+        if (valid) {
+            atomicAdd(&a.work[0], 1ull);
+            atomicAdd(&a.work[1], (unsigned long long)caves);
+            atomicAdd(&a.work[5], (unsigned long long)(y1 > y0 ? y1 - y0 : 0));
+        }
     }
 }
 
@@ -192,16 +218,16 @@ __global__ __launch_bounds__(1024) void k_density(KArgs a, int chunks) {
     __shared__ int next_local;
     if (threadIdx.x == 0) next_local = 0;
     load_noise_tables(T, a.perm, a.grads);  // ends with a barrier
-    const int per_chunk = a.G.F * a.G.F * a.ysplit;
-    const int items_per_chunk = (per_chunk + 63) >> 6;
+    const int groups = a.G.col_groups;
+    const int items_per_chunk = groups * a.ysplit;  // (y-segment, column group) pairs
     const int total = items_per_chunk * chunks;
-    const int lane = threadIdx.x & 63;
     for (;;) {
         const int item = wg_next_item(&next_local, total);
         if (item < 0) break;
         const int chunk = item / items_per_chunk;
-        const int t = (item - chunk * items_per_chunk) * 64 + lane;
-        if (t < per_chunk) density_column<kCount>(a, T, chunk, t);
+        const int r = item - chunk * items_per_chunk;
+        const int seg = r / groups;
+        density_column<kCount>(a, T, chunk, seg, r - seg * groups);
     }
 }
 
@@ -218,7 +244,7 @@ constexpr int kCsgSlabs = 8;
 
 __global__ __launch_bounds__(256) void k_csg(KArgs a) {
     const int chunk = blockIdx.y;
-    const ChunkDesc& cd = a.chunks[chunk];
+    const ChunkDesc cd = a.chunks[chunk];  // a copy: the loops below store through pointers that could alias it
     const int F = a.G.F;
     const int zs = (F + kCsgSlabs - 1) / kCsgSlabs;
     const int z0 = (int)blockIdx.x * zs, z1 = min(F, z0 + zs);  // this workgroup's slab
@@ -271,6 +297,25 @@ __global__ __launch_bounds__(256) void k_solid_rows(KArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t* mat = slot_ptr<const uint32_t>(a, chunk, a.L.mat);
     uint64_t* S = slot_ptr<uint64_t>(a, chunk, a.L.solid);
+    if (chunk_mat_free(a, a.chunks[chunk])) {  // uniform over the workgroup
+        // Row (y, z) is bits [F z, F z + F) of plane y: a 64-bit window of two or three
+        // consecutive group words, then F - 64 bits above it (none when F <= 64).
+        const uint64_t* P = slot_ptr<const uint64_t>(a, chunk, a.L.planes);
+        const uint64_t lo_mask = F >= 64 ? ~0ull : ((1ull << F) - 1ull);
+        const uint64_t hi_mask = F > 64 ? ((1ull << (F - 64)) - 1ull) : 0ull;
+        for (int r = thread_in_chunk(); r < G.field_rows; r += threads_per_chunk()) {
+            const int z = div_magic(r, G.magic_f), y = r - z * F;
+            const int t0 = F * z, g0 = t0 >> 6, sh = t0 & 63;
+            const uint64_t w0 = P[g0 * F + y];
+            const uint64_t w1 = g0 + 1 < G.col_groups ? P[(g0 + 1) * F + y] : 0ull;
+            const uint64_t w2 = g0 + 2 < G.col_groups ? P[(g0 + 2) * F + y] : 0ull;
+            const uint64_t lo = sh ? (w0 >> sh) | (w1 << (64 - sh)) : w0;
+            const uint64_t hi = sh ? (w1 >> sh) | (w2 << (64 - sh)) : w1;
+            S[2 * r] = lo & lo_mask;
+            S[2 * r + 1] = hi & hi_mask;
+        }
+        return;
+    }
     const uint8_t* bytes = reinterpret_cast<const uint8_t*>(stage[wave]);
     const int blocks = (G.field_rows + kRowsPerBlock - 1) / kRowsPerBlock;
     const int words_total = G.field_samples >> 2;  // F is even, so F^3 is a multiple of 8
@@ -561,7 +606,7 @@ __global__ __launch_bounds__(1024) void k_height_lines(KArgs a, int chunks) {
         const int chunk = item / per_chunk;
         const int r = item - chunk * per_chunk;
         const int sel = r / H, z = r - sel * H;
-        const ChunkDesc& cd = a.chunks[chunk];
+        const ChunkDesc cd = a.chunks[chunk];  // a copy: the sample loop's stores could alias it
         if (cd.mode == kModeCached) continue;  // uniform over the wave
         const uint64_t* E = slot_ptr<const uint64_t>(a, chunk, a.L.e_rows);
         const int word = sel == 0 ? 0 : 2;  // x-edge or z-edge bits of x < 64
@@ -1219,6 +1264,21 @@ __device__ __forceinline__ int corner_materials(const KArgs& a, const uint8_t* m
     return corners;
 }
 
+// The same from the solid rows, for a chunk that stores no materials: a sample's material
+// is then the default material where it is not air (density < 0) and air elsewhere.
+__device__ __forceinline__ int corner_materials_rows(const KArgs& a, const uint64_t* S, int x, int y, int z, int* cm) {
+    const int F = a.G.F;
+    int corners = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int cx = x + ((i >> 2) & 1), s = (y + ((i >> 1) & 1)) + F * (z + (i & 1));
+        const int bit = (int)((S[2 * s + (cx >> 6)] >> (cx & 63)) & 1ull);
+        cm[i] = bit ? a.defmat : kAir;
+        corners |= bit << i;
+    }
+    return corners;
+}
+
 // FindDominantMaterial, cl/octree.cl:78-138.  The reference sorts the eight corner
 // materials by insertion; any correct sort yields the same sequence, and Batcher's
 // odd-even merge network (19 compare-exchanges, min/max pairs) does it without the
@@ -1769,6 +1829,8 @@ __global__ __launch_bounds__(256) void k_qef(KArgs a) {
     const ChunkDesc cd = a.chunks[chunk];
     const int* vcode = slot_ptr<const int>(a, chunk, a.L.v_code);
     const uint8_t* mat = slot_ptr<const uint8_t>(a, chunk, a.L.mat);
+    const uint64_t* S = slot_ptr<const uint64_t>(a, chunk, a.L.solid);
+    const bool rows = chunk_mat_free(a, cd);  // corners from the solid rows
     int* vemask = slot_ptr<int>(a, chunk, a.L.v_emask);
     int* vmat = slot_ptr<int>(a, chunk, a.L.v_mat);
     float4* verts = slot_ptr<float4>(a, chunk, a.L.verts);
@@ -1783,7 +1845,8 @@ __global__ __launch_bounds__(256) void k_qef(KArgs a) {
         if (vi < V) {
             const int3 p = position_for_code((uint32_t)vcode[vi], G.depth);
             int cm[8];
-            const int corners = corner_materials(a, mat, p.x, p.y, p.z, cm);
+            const int corners = rows ? corner_materials_rows(a, S, p.x, p.y, p.z, cm)
+                                     : corner_materials(a, mat, p.x, p.y, p.z, cm);
             int emask = 0;
 #pragma unroll
             for (int e = 0; e < 12; ++e)
@@ -1999,7 +2062,7 @@ static unsigned persistent_blocks(const KArgs& a, int per_cu, long items) {
 }
 
 void launch_density(const KArgs& a, int chunks, hipStream_t s) {
-    const long items = (((long)a.G.F * a.G.F * a.ysplit + 63) / 64) * chunks;
+    const long items = (long)a.G.col_groups * a.ysplit * chunks;
     // One workgroup per CU by default: at k_density's 93 SGPRs a SIMD holds the 4 waves
     // of one 1024-thread workgroup but not the 8 of two, so a second workgroup would
     // only start once the first has emptied the queue (it then loads its tables and

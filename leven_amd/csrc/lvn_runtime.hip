@@ -191,6 +191,7 @@ Geometry make_geometry(int n) {
     G.voxel_segs = std::max(1, G.voxels / 64);
     G.magic_h = (uint32_t)((((uint64_t)1 << 32) + G.H - 1) / G.H);
     G.magic_f = (uint32_t)((((uint64_t)1 << 32) + G.F - 1) / G.F);
+    G.col_groups = (G.F * G.F + 63) / 64;
     return G;
 }
 
@@ -217,6 +218,7 @@ SlotLayout make_layout(const Geometry& G, const Caps& c) {
     L.mat = take(fs);
     L.orig = take(fs);
     L.solid = take((uint64_t)G.field_rows * 16);
+    L.planes = take((uint64_t)G.col_groups * G.F * 8);
     L.e_rows = take((uint64_t)G.edge_rows * 32);
     L.e_count = take((uint64_t)G.edge_rows * 4);
     L.e_base = take((uint64_t)G.edge_rows * 4);
@@ -657,7 +659,8 @@ static lvn_tuning effective_tuning(const Batch* b) {
 }
 
 // A batch exports meshes, never its density field: for a field without the cave term
-// it leaves the density unstored (KArgs::dens_cols).
+// it leaves the density unstored (KArgs::dens_cols), and chunks without brushes leave
+// their materials unstored (KArgs::mat_free).
 static int density_from_columns(const KArgs& a) {
     return a.F.kind != LVN_FIELD_TERRAIN_CAVES && !a.fp16 ? 1 : 0;
 }
@@ -665,6 +668,7 @@ static int density_from_columns(const KArgs& a) {
 static KArgs batch_args(const Batch* b, const lvn_tuning& t) {
     KArgs a = b->ctx.kargs();
     a.dens_cols = density_from_columns(a);
+    a.mat_free = 1;
     a.ysplit = t.density_ysplit;
     a.persist_per_cu = t.persistent_per_cu;
     a.density_per_cu = t.density_per_cu;

@@ -369,6 +369,7 @@ int Server::launch(Lane& L, const std::vector<Job>& batch, const std::vector<int
     // served chunks export meshes only: a field without the cave term leaves its density
     // unstored (KArgs::dens_cols)
     a.dens_cols = a.F.kind != LVN_FIELD_TERRAIN_CAVES && !a.fp16 ? 1 : 0;
+    a.mat_free = 1;  // and chunks without brushes their materials
     // zero fields: the automatic choice by batch size
     a.ysplit = t.density_ysplit ? t.density_ysplit : n >= 32 ? 1 : 6;
     a.height_lines = t.height_lines ? (t.height_lines == 2 ? 1 : 0) : (n >= kHeightLinesMinChunks ? 1 : 0);

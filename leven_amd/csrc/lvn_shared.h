@@ -59,6 +59,7 @@ struct Geometry {
     int voxel_segs;      // N^3 / 64 (>= 1)
     uint32_t magic_h;    // ceil(2^32 / H): n / H == umulhi(n, magic_h) for n < 2^21
     uint32_t magic_f;    // ceil(2^32 / F)
+    int col_groups;      // ceil(F^2 / 64): groups of 64 field columns (x, z), column x + F z
 };
 
 // Per-slot capacities of the compacted arrays.
@@ -75,6 +76,8 @@ struct SlotLayout {
     uint64_t slot_bytes;
     uint64_t density, mat, orig;
     uint64_t solid;                       // field_rows x 2 words: bit x = sample (x, y, z) is not air
+    uint64_t planes;                      // col_groups x F words, [g][y] bit c: column 64 g + c of plane y
+                                          // is not air (chunks whose materials are not stored)
     uint64_t e_rows, e_count, e_base, e_ids, e_data;  // e_rows: edge_rows x {x, y, z, x = 64} words
     uint64_t e_count_y, e_base_y, e_order;  // y-edges per row and their scan; crossing order (y-edges first)
     uint64_t h_lines;                     // heights at k/16 (k = 1..15) along x- and z-edge lines, 2 x H x H x 15 f32
