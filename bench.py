@@ -453,6 +453,10 @@ def batched_leg(wl, args, steps, edge_cap, voxel_cap, plain=True, keep=False):
     work = batch.work()
     d_flops, c_flops = stage_flops(work, desc)
     dom = max(("density", "edge_cross"), key=lambda k: stage_ms[k])
+    # the crossing stage's kernel for the tuned lanes per edge (1, 2/4/8, 16 = sample-parallel)
+    lanes = tuning.get("cross_lanes", 1) if isinstance(tuning, dict) else 1
+    dom_kernel = ("k_density" if dom == "density" else
+                  "k_edge_cross_sp" if lanes == 16 else "k_edge_cross_group" if lanes in (2, 4, 8) else "k_edge_cross")
     a_d, a_c = algorithmic_flops(n, work["edges"], desc)
     dom_flops = a_d if dom == "density" else a_c
     exec_flops = d_flops if dom == "density" else c_flops
@@ -469,9 +473,9 @@ def batched_leg(wl, args, steps, edge_cap, voxel_cap, plain=True, keep=False):
                       "max_active_voxels": int(counts[:, 1].max()), "max_edges": int(counts[:, 0].max())},
            "per_lod": per_lod,
            "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()}, "work": work,
-           "roofline": {"bound": "valu", "kernel": "k_" + dom, "achieved": round(achieved, 3),
+           "roofline": {"bound": "valu", "kernel": dom_kernel, "achieved": round(achieved, 3),
                         "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                        "traffic": hbm_traffic("k_" + dom, wl["name"]), "flops_per_launch": int(dom_flops),
+                        "traffic": hbm_traffic(dom_kernel, wl["name"]), "flops_per_launch": int(dom_flops),
                         "flops_unit": "23 full field evaluations per edge" if dom == "edge_cross"
                         else "65^3 full field evaluations per chunk",
                         "launch_ms": round(stage_ms[dom], 4), "executed_flops_per_launch": int(exec_flops),
