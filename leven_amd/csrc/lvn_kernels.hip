@@ -205,15 +205,19 @@ __device__ __forceinline__ int wg_next_item(int* next, int total) {
     return j < mine ? (int)blockIdx.x + (int)gridDim.x * j : -1;
 }
 
-// Persistent: the grid fills the machine once (one 1024-thread workgroup per CU by
-// default: at this kernel's 93 SGPRs a SIMD holds the 4 waves of one, not the 8 of two;
-// the LDS tables would allow two), each workgroup loads the noise tables once, and its
+// Persistent: the grid fills the machine once (one or two 1024-thread workgroups per
+// CU, a launch-shape parameter), each workgroup loads the noise tables once, and its
 // waves take 64-column items from the workgroup's share (wg_next_item: 1.01-1.04 ms ->
 // 0.99-1.00 ms against one global queue).
 // kCount adds the work counters of lvn_batch_work (a separate instantiation, so
 // the timed kernels carry no counting code).
+// At most 80 SGPRs and 64 VGPRs: two 1024-thread workgroups then fit a CU (8 waves per
+// SIMD; some SGPRs spill into VGPR lanes), which the tuner takes where the pass is a
+// chain of height-term lookups per column (the reference's terrain field: 0.69 -> 0.57 ms
+// on its clipmap batch); the cave field's pass takes the same time either way.
 template <bool kCount>
-__global__ __launch_bounds__(1024) void k_density(KArgs a, int chunks) {
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_num_sgpr(80), amdgpu_waves_per_eu(8, 8))) void k_density(
+    KArgs a, int chunks) {
     __shared__ NoiseLds T;
     __shared__ int next_local;
     if (threadIdx.x == 0) next_local = 0;
@@ -2063,10 +2067,8 @@ static unsigned persistent_blocks(const KArgs& a, int per_cu, long items) {
 
 void launch_density(const KArgs& a, int chunks, hipStream_t s) {
     const long items = (long)a.G.col_groups * a.ysplit * chunks;
-    // One workgroup per CU by default: at k_density's 93 SGPRs a SIMD holds the 4 waves
-    // of one 1024-thread workgroup but not the 8 of two, so a second workgroup would
-    // only start once the first has emptied the queue (it then loads its tables and
-    // exits, 0.1 ms of tail on a 512-chunk batch).
+    // One workgroup per CU by default (density_per_cu); the tuner tries two, which the
+    // kernel's register caps let a CU hold at once.
     const unsigned blocks = persistent_blocks(a, a.density_per_cu, items);
     if (a.work)
         hipLaunchKernelGGL(k_density<true>, dim3(blocks), dim3(1024), 0, s, a, chunks);
